@@ -1,0 +1,225 @@
+// Stockham FFT staged through LDS, for gfx950.
+//
+// A workgroup owns LINES one-dimensional transforms of length N that sit in
+// LDS as two ping-pong buffers.  Each radix-R stage reads R elements spaced
+// N/R apart, applies twiddles exp(S*2*pi*i*k*r/(Ns*R)) (k = j mod Ns), does
+// an R-point DFT in registers and writes them Ns apart from
+// (j/Ns)*Ns*R + k (Stockham autosort: natural order in, natural order out).
+//
+// Two LDS layouts:
+//   CONTIG  element (line, idx) at line*N + idx     - lines contiguous in
+//           global memory (the z axis of the flat z-fastest grid);
+//   TILE    element (line, idx) at idx*LINES + line - LINES neighbouring
+//           z columns of a strided axis (y or x), so global loads are rows
+//           of LINES consecutive elements.
+// The transform length is a template constant for the hot sizes (the
+// compiler then folds all index arithmetic) and a runtime plan otherwise.
+#pragma once
+#include "common.hpp"
+
+namespace derek {
+
+constexpr int kMaxStages = 16;
+
+struct FftPlan {
+  int n;
+  int ns;  // number of stages, -1 if n has a prime factor outside {2,3,5,7,11,13}
+  int r[kMaxStages];
+};
+
+constexpr FftPlan make_plan(int n) {
+  FftPlan p{n, 0, {}};
+  int m = n;
+  if (n < 1) {
+    p.ns = -1;
+    return p;
+  }
+  while (m % 8 == 0 && p.ns < kMaxStages) { p.r[p.ns++] = 8; m /= 8; }
+  while (m % 4 == 0 && p.ns < kMaxStages) { p.r[p.ns++] = 4; m /= 4; }
+  const int order[6] = {3, 5, 2, 7, 11, 13};
+  for (int q = 0; q < 6; ++q) {
+    const int f = order[q];
+    while (m % f == 0 && p.ns < kMaxStages) { p.r[p.ns++] = f; m /= f; }
+  }
+  if (m != 1) p.ns = -1;
+  return p;
+}
+
+// ---------------------------------------------------------------- R-point DFTs
+// X_k = sum_j v_j exp(S*2*pi*i*j*k/R), in place.
+template <int S, typename C> __device__ __forceinline__ void dft2(C* v) {
+  C a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+
+template <int S, typename C> __device__ __forceinline__ void dft3(C* v) {
+  using R = decltype(C::x);
+  const R h = (R)0.86602540378443864676372317075294;  // sin(2pi/3)
+  C t = cadd(v[1], v[2]);
+  C d = csub(v[1], v[2]);
+  C m = csub(v[0], cscale(t, (R)0.5));
+  C s = cmul_si<S>(cscale(d, h));
+  v[0] = cadd(v[0], t);
+  v[1] = cadd(m, s);
+  v[2] = csub(m, s);
+}
+
+template <int S, typename C> __device__ __forceinline__ void dft4(C* v) {
+  C a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+  C c = cadd(v[1], v[3]), d = cmul_si<S>(csub(v[1], v[3]));
+  v[0] = cadd(a, c);
+  v[2] = csub(a, c);
+  v[1] = cadd(b, d);
+  v[3] = csub(b, d);
+}
+
+template <int S, typename C> __device__ __forceinline__ void dft5(C* v) {
+  using R = decltype(C::x);
+  const R c1 = (R)0.30901699437494742410229341718282;   // cos(2pi/5)
+  const R c2 = (R)-0.80901699437494742410229341718282;  // cos(4pi/5)
+  const R s1 = (R)0.95105651629515357211643933337938;   // sin(2pi/5)
+  const R s2 = (R)0.58778525229247312916870595463907;   // sin(4pi/5)
+  C p1 = cadd(v[1], v[4]), m1 = csub(v[1], v[4]);
+  C p2 = cadd(v[2], v[3]), m2 = csub(v[2], v[3]);
+  C a1 = cadd(v[0], cadd(cscale(p1, c1), cscale(p2, c2)));
+  C a2 = cadd(v[0], cadd(cscale(p1, c2), cscale(p2, c1)));
+  C b1 = cmul_si<S>(cadd(cscale(m1, s1), cscale(m2, s2)));
+  C b2 = cmul_si<S>(csub(cscale(m1, s2), cscale(m2, s1)));
+  v[0] = cadd(v[0], cadd(p1, p2));
+  v[1] = cadd(a1, b1);
+  v[4] = csub(a1, b1);
+  v[2] = cadd(a2, b2);
+  v[3] = csub(a2, b2);
+}
+
+template <int S, typename C> __device__ __forceinline__ void dft8(C* v) {
+  using R = decltype(C::x);
+  const R h = (R)0.70710678118654752440084436210485;
+  C e[4] = {v[0], v[2], v[4], v[6]};
+  C o[4] = {v[1], v[3], v[5], v[7]};
+  dft4<S>(e);
+  dft4<S>(o);
+  // w^1 = (1 + S i)/sqrt2, w^2 = S i, w^3 = (-1 + S i)/sqrt2
+  C o1 = cscale(cadd(o[1], cmul_si<S>(o[1])), h);
+  C o2 = cmul_si<S>(o[2]);
+  C o3 = cscale(csub(cmul_si<S>(o[3]), o[3]), h);
+  v[0] = cadd(e[0], o[0]);
+  v[4] = csub(e[0], o[0]);
+  v[1] = cadd(e[1], o1);
+  v[5] = csub(e[1], o1);
+  v[2] = cadd(e[2], o2);
+  v[6] = csub(e[2], o2);
+  v[3] = cadd(e[3], o3);
+  v[7] = csub(e[3], o3);
+}
+
+// Twiddle table entry exp(S*2*pi*i*t/N); the table holds the S = -1 values.
+template <int S, typename C> __device__ __forceinline__ C twid(const C* tw, int t) {
+  C w = tw[t];
+  return S > 0 ? cconj(w) : w;
+}
+
+// Odd primes 7, 11, 13: direct O(R^2) sum with table roots of unity.
+template <int R, int S, typename C> __device__ __forceinline__ void dft_direct(C* v, const C* tw, int N) {
+  C out[R];
+  const int step = N / R;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    C acc = v[0];
+#pragma unroll
+    for (int j = 1; j < R; ++j) acc = cadd(acc, cmul(v[j], twid<S>(tw, ((j * k) % R) * step)));
+    out[k] = acc;
+  }
+#pragma unroll
+  for (int k = 0; k < R; ++k) v[k] = out[k];
+}
+
+template <int R, int S, typename C> __device__ __forceinline__ void dftR(C* v, const C* tw, int N) {
+  if constexpr (R == 2) dft2<S>(v);
+  else if constexpr (R == 3) dft3<S>(v);
+  else if constexpr (R == 4) dft4<S>(v);
+  else if constexpr (R == 5) dft5<S>(v);
+  else if constexpr (R == 8) dft8<S>(v);
+  else dft_direct<R, S>(v, tw, N);
+}
+
+// ---------------------------------------------------------------- one stage
+template <bool TILE, int LINES>
+__device__ __forceinline__ int lds_addr(int line, int idx, int N) {
+  return TILE ? idx * LINES + line : line * N + idx;
+}
+
+template <int R, int S, bool TILE, int LINES, int NT, typename C>
+__device__ __forceinline__ void fft_stage(const C* __restrict__ src, C* __restrict__ dst, int Ns, int N,
+                                          int lines, const C* __restrict__ tw) {
+  const int NR = N / R;
+  const int total = lines * NR;
+  const int tstride = N / (Ns * R);
+  for (int t = threadIdx.x; t < total; t += NT) {
+    int line, j;
+    if (TILE) { line = t % LINES; j = t / LINES; }
+    else { line = t / NR; j = t - line * NR; }
+    C v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = src[lds_addr<TILE, LINES>(line, j + r * NR, N)];
+    const int k = j % Ns;
+    if (Ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twid<S>(tw, k * r * tstride));
+    }
+    dftR<R, S>(v, tw, N);
+    const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[lds_addr<TILE, LINES>(line, base + r * Ns, N)] = v[r];
+  }
+}
+
+// Compile-time plan: stages unrolled by recursion.
+template <int S, bool TILE, int LINES, int NT, int N, int STAGE, int NS, typename C>
+__device__ __forceinline__ C* fft_stages_ct(C* src, C* dst, const C* tw, int lines) {
+  constexpr FftPlan p = make_plan(N);
+  if constexpr (STAGE >= p.ns) {
+    return src;
+  } else {
+    constexpr int R = p.r[STAGE];
+    fft_stage<R, S, TILE, LINES, NT>(src, dst, NS, N, lines, tw);
+    __syncthreads();
+    return fft_stages_ct<S, TILE, LINES, NT, N, STAGE + 1, NS * R>(dst, src, tw, lines);
+  }
+}
+
+// Runs the transform on buf0 (buf1 is scratch); returns the buffer holding
+// the result.  NC > 0: compile-time length NC; NC == 0: runtime plan.
+template <int S, bool TILE, int LINES, int NT, int NC, typename C>
+__device__ __forceinline__ C* fft_lds(C* buf0, C* buf1, const C* tw, const FftPlan& plan, int lines) {
+  if constexpr (NC > 0) {
+    return fft_stages_ct<S, TILE, LINES, NT, NC, 0, 1>(buf0, buf1, tw, lines);
+  } else {
+    C* src = buf0;
+    C* dst = buf1;
+    int Ns = 1;
+    const int N = plan.n;
+    for (int s = 0; s < plan.ns; ++s) {
+      const int R = plan.r[s];
+      switch (R) {
+        case 2: fft_stage<2, S, TILE, LINES, NT>(src, dst, Ns, N, lines, tw); break;
+        case 3: fft_stage<3, S, TILE, LINES, NT>(src, dst, Ns, N, lines, tw); break;
+        case 4: fft_stage<4, S, TILE, LINES, NT>(src, dst, Ns, N, lines, tw); break;
+        case 5: fft_stage<5, S, TILE, LINES, NT>(src, dst, Ns, N, lines, tw); break;
+        case 7: fft_stage<7, S, TILE, LINES, NT>(src, dst, Ns, N, lines, tw); break;
+        case 8: fft_stage<8, S, TILE, LINES, NT>(src, dst, Ns, N, lines, tw); break;
+        case 11: fft_stage<11, S, TILE, LINES, NT>(src, dst, Ns, N, lines, tw); break;
+        default: fft_stage<13, S, TILE, LINES, NT>(src, dst, Ns, N, lines, tw); break;
+      }
+      __syncthreads();
+      C* t = src;
+      src = dst;
+      dst = t;
+      Ns *= R;
+    }
+    return src;
+  }
+}
+
+}  // namespace derek

@@ -1,0 +1,618 @@
+// FFT passes, plane-wave <-> grid maps, XC/Hartree/mixing and Davidson
+// pointwise kernels for gfx950.
+#include <mutex>
+
+#include "launch.hpp"
+
+namespace derek {
+
+constexpr int NT = 256;          // threads per workgroup for the FFT passes
+constexpr int TILE_LINES = 16;   // z columns per strided-axis tile
+
+// ------------------------------------------------------------------ FFT kernels
+// Contiguous lines (z axis).  Each workgroup owns `lpb` lines.
+template <typename Rl, int S, int NC>
+__global__ __launch_bounds__(NT) void k_fft_contig(cx_t<Rl>* __restrict__ data, int64_t nlines, int lpb,
+                                                   FftPlan plan, const cx_t<Rl>* __restrict__ tw) {
+  using C = cx_t<Rl>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = NC > 0 ? NC : plan.n;
+  C* twl = reinterpret_cast<C*>(smem);
+  C* b0 = twl + N;
+  C* b1 = b0 + (size_t)lpb * N;
+  for (int t = threadIdx.x; t < N; t += NT) twl[t] = tw[t];
+  const int64_t line0 = (int64_t)blockIdx.x * lpb;
+  const int64_t left = nlines - line0;
+  const int lines = left < lpb ? (int)left : lpb;
+  const C* src = data + line0 * N;
+  const int ne = lines * N;
+  for (int e = threadIdx.x; e < ne; e += NT) b0[e] = src[e];
+  __syncthreads();
+  C* res = fft_lds<S, false, 1, NT, NC>(b0, b1, twl, plan, lines);
+  C* dst = data + line0 * N;
+  for (int e = threadIdx.x; e < ne; e += NT) dst[e] = res[e];
+}
+
+// Strided axis, tiles of TILE_LINES neighbouring inner indices.
+// MODE 0: one transform of sign S.  MODE 1: inverse, *vloc*vscale, forward.
+template <typename Rl, int S, int NC, int MODE>
+__global__ __launch_bounds__(NT) void k_fft_tile(cx_t<Rl>* __restrict__ data, int64_t stride, int64_t tiles,
+                                                 FftPlan plan, const cx_t<Rl>* __restrict__ tw,
+                                                 const Rl* __restrict__ vloc, Rl vscale) {
+  using C = cx_t<Rl>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = NC > 0 ? NC : plan.n;
+  C* twl = reinterpret_cast<C*>(smem);
+  C* b0 = twl + N;
+  C* b1 = b0 + (size_t)TILE_LINES * N;
+  for (int t = threadIdx.x; t < N; t += NT) twl[t] = tw[t];
+  const int64_t o = blockIdx.x / tiles;
+  const int64_t i0 = (blockIdx.x - o * tiles) * TILE_LINES;
+  const int64_t rem = stride - i0;
+  const int lines = rem < TILE_LINES ? (int)rem : TILE_LINES;
+  C* base = data + o * (int64_t)N * stride + i0;
+  const int ne = N * TILE_LINES;
+  for (int e = threadIdx.x; e < ne; e += NT) {
+    const int a = e / TILE_LINES, l = e - a * TILE_LINES;
+    b0[e] = l < lines ? base[(int64_t)a * stride + l] : cmk<C>(0, 0);
+  }
+  __syncthreads();
+  C* res;
+  if constexpr (MODE == 0) {
+    res = fft_lds<S, true, TILE_LINES, NT, NC>(b0, b1, twl, plan, TILE_LINES);
+  } else {
+    res = fft_lds<+1, true, TILE_LINES, NT, NC>(b0, b1, twl, plan, TILE_LINES);
+    C* other = res == b0 ? b1 : b0;
+    for (int e = threadIdx.x; e < ne; e += NT) {
+      const int a = e / TILE_LINES, l = e - a * TILE_LINES;
+      const Rl v = l < lines ? vloc[(int64_t)a * stride + i0 + l] * vscale : (Rl)0;
+      res[e] = cscale(res[e], v);
+    }
+    __syncthreads();
+    res = fft_lds<-1, true, TILE_LINES, NT, NC>(res, other, twl, plan, TILE_LINES);
+  }
+  for (int e = threadIdx.x; e < ne; e += NT) {
+    const int a = e / TILE_LINES, l = e - a * TILE_LINES;
+    if (l < lines) base[(int64_t)a * stride + l] = res[e];
+  }
+}
+
+// Density: inverse transform along the strided axis for every outer block
+// (band), accumulating occ[o] |psi|^2 per point.
+template <typename Rl, int NC>
+__global__ __launch_bounds__(NT) void k_fft_tile_density(const cx_t<Rl>* __restrict__ data, int64_t nouter,
+                                                         int64_t stride, int64_t tiles, FftPlan plan,
+                                                         const cx_t<Rl>* __restrict__ tw,
+                                                         const double* __restrict__ occ,
+                                                         double* __restrict__ rho, double scale, int keep) {
+  using C = cx_t<Rl>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = NC > 0 ? NC : plan.n;
+  C* twl = reinterpret_cast<C*>(smem);
+  C* b0 = twl + N;
+  C* b1 = b0 + (size_t)TILE_LINES * N;
+  double* acc = reinterpret_cast<double*>(b1 + (size_t)TILE_LINES * N);
+  for (int t = threadIdx.x; t < N; t += NT) twl[t] = tw[t];
+  const int64_t i0 = (int64_t)blockIdx.x * TILE_LINES;
+  const int64_t rem = stride - i0;
+  const int lines = rem < TILE_LINES ? (int)rem : TILE_LINES;
+  const int ne = N * TILE_LINES;
+  for (int e = threadIdx.x; e < ne; e += NT) acc[e] = 0.0;
+  for (int64_t o = 0; o < nouter; ++o) {
+    const double w = occ[o];
+    if (w == 0.0) continue;
+    const C* base = data + o * (int64_t)N * stride + i0;
+    __syncthreads();
+    for (int e = threadIdx.x; e < ne; e += NT) {
+      const int a = e / TILE_LINES, l = e - a * TILE_LINES;
+      b0[e] = l < lines ? base[(int64_t)a * stride + l] : cmk<C>(0, 0);
+    }
+    __syncthreads();
+    C* res = fft_lds<+1, true, TILE_LINES, NT, NC>(b0, b1, twl, plan, TILE_LINES);
+    for (int e = threadIdx.x; e < ne; e += NT) {
+      const double re = (double)res[e].x, im = (double)res[e].y;
+      acc[e] += w * (re * re + im * im);
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < ne; e += NT) {
+    const int a = e / TILE_LINES, l = e - a * TILE_LINES;
+    if (l < lines) {
+      double* p = rho + (int64_t)a * stride + i0 + l;
+      *p = (keep ? *p : 0.0) + scale * acc[e];
+    }
+  }
+}
+
+template <typename K> static void allow_lds(K kern, size_t bytes) {
+  // gfx950 has 160 KB of LDS per CU; kernels above 64 KB must opt in.
+  if (bytes > 65536) DEREK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
+static void check_lds(size_t bytes, int n) {
+  DEREK_REQUIRE(bytes <= 160 * 1024, "FFT length " + std::to_string(n) + " exceeds the LDS budget");
+}
+
+template <typename Rl, int S, int NC>
+static void launch_contig(cx_t<Rl>* data, int64_t nlines, const Axis<Rl>& ax, hipStream_t s) {
+  const int N = ax.n;
+  int lpb = 2048 / N;
+  if (lpb < 1) lpb = 1;
+  const size_t lds = sizeof(cx_t<Rl>) * ((size_t)N + 2 * (size_t)lpb * N);
+  check_lds(lds, N);
+  auto kern = k_fft_contig<Rl, S, NC>;
+  allow_lds(kern, lds);
+  const int64_t blocks = ceil_div(nlines, lpb);
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(NT), lds, s, data, nlines, lpb, ax.plan, ax.tw);
+  DEREK_HIP(hipGetLastError());
+}
+
+template <typename Rl, int S, int NC, int MODE>
+static void launch_tile(cx_t<Rl>* data, int64_t nouter, int64_t stride, const Axis<Rl>& ax, const Rl* vloc,
+                        Rl vscale, hipStream_t s) {
+  const int N = ax.n;
+  const size_t lds = sizeof(cx_t<Rl>) * ((size_t)N + 2 * (size_t)TILE_LINES * N);
+  check_lds(lds, N);
+  auto kern = k_fft_tile<Rl, S, NC, MODE>;
+  allow_lds(kern, lds);
+  const int64_t tiles = ceil_div(stride, TILE_LINES);
+  hipLaunchKernelGGL(kern, dim3((unsigned)(nouter * tiles)), dim3(NT), lds, s, data, stride, tiles, ax.plan,
+                     ax.tw, vloc, vscale);
+  DEREK_HIP(hipGetLastError());
+}
+
+template <typename Rl, int NC>
+static void launch_density(const cx_t<Rl>* data, int64_t nouter, int64_t stride, const Axis<Rl>& ax,
+                           const double* occ, double* rho, double scale, int keep, hipStream_t s) {
+  const int N = ax.n;
+  const size_t lds = sizeof(cx_t<Rl>) * ((size_t)N + 2 * (size_t)TILE_LINES * N) +
+                     sizeof(double) * (size_t)TILE_LINES * N;
+  check_lds(lds, N);
+  auto kern = k_fft_tile_density<Rl, NC>;
+  allow_lds(kern, lds);
+  const int64_t tiles = ceil_div(stride, TILE_LINES);
+  hipLaunchKernelGGL(kern, dim3((unsigned)tiles), dim3(NT), lds, s, data, nouter, stride, tiles, ax.plan, ax.tw,
+                     occ, rho, scale, keep);
+  DEREK_HIP(hipGetLastError());
+}
+
+// Lengths with a compile-time plan (the benchmark grids); others run the
+// runtime plan.
+#define DEREK_FFT_DISPATCH(n, CALL) \
+  switch (n) {                      \
+    case 32: CALL(32); break;       \
+    case 72: CALL(72); break;       \
+    case 108: CALL(108); break;     \
+    case 144: CALL(144); break;     \
+    default: CALL(0); break;        \
+  }
+
+template <typename R> void fft_z(cx_t<R>* data, int64_t nlines, const Axis<R>& ax, int sign, hipStream_t s) {
+  if (sign < 0) {
+#define C_(NC) launch_contig<R, -1, NC>(data, nlines, ax, s)
+    DEREK_FFT_DISPATCH(ax.n, C_)
+#undef C_
+  } else {
+#define C_(NC) launch_contig<R, +1, NC>(data, nlines, ax, s)
+    DEREK_FFT_DISPATCH(ax.n, C_)
+#undef C_
+  }
+}
+
+template <typename R>
+void fft_strided(cx_t<R>* data, int64_t nouter, int64_t stride, const Axis<R>& ax, int sign, hipStream_t s) {
+  if (sign < 0) {
+#define C_(NC) launch_tile<R, -1, NC, 0>(data, nouter, stride, ax, nullptr, (R)0, s)
+    DEREK_FFT_DISPATCH(ax.n, C_)
+#undef C_
+  } else {
+#define C_(NC) launch_tile<R, +1, NC, 0>(data, nouter, stride, ax, nullptr, (R)0, s)
+    DEREK_FFT_DISPATCH(ax.n, C_)
+#undef C_
+  }
+}
+
+template <typename R>
+void fft_strided_vmul(cx_t<R>* data, int64_t nouter, int64_t stride, const Axis<R>& ax, const R* vloc, R vscale,
+                      hipStream_t s) {
+#define C_(NC) launch_tile<R, +1, NC, 1>(data, nouter, stride, ax, vloc, vscale, s)
+  DEREK_FFT_DISPATCH(ax.n, C_)
+#undef C_
+}
+
+template <typename R>
+void fft_strided_density(const cx_t<R>* data, int64_t nouter, int64_t stride, const Axis<R>& ax,
+                         const double* occ, double* rho, double scale, int keep, hipStream_t s) {
+#define C_(NC) launch_density<R, NC>(data, nouter, stride, ax, occ, rho, scale, keep, s)
+  DEREK_FFT_DISPATCH(ax.n, C_)
+#undef C_
+}
+
+template void fft_z<double>(double2*, int64_t, const Axis<double>&, int, hipStream_t);
+template void fft_z<float>(float2*, int64_t, const Axis<float>&, int, hipStream_t);
+template void fft_strided<double>(double2*, int64_t, int64_t, const Axis<double>&, int, hipStream_t);
+template void fft_strided<float>(float2*, int64_t, int64_t, const Axis<float>&, int, hipStream_t);
+template void fft_strided_vmul<double>(double2*, int64_t, int64_t, const Axis<double>&, const double*, double,
+                                       hipStream_t);
+template void fft_strided_vmul<float>(float2*, int64_t, int64_t, const Axis<float>&, const float*, float,
+                                      hipStream_t);
+template void fft_strided_density<double>(const double2*, int64_t, int64_t, const Axis<double>&, const double*,
+                                          double*, double, int, hipStream_t);
+template void fft_strided_density<float>(const float2*, int64_t, int64_t, const Axis<float>&, const double*,
+                                         double*, double, int, hipStream_t);
+
+// ------------------------------------------------------------------ maps
+static unsigned grid1d(int64_t n, int per = 256) {
+  int64_t b = ceil_div(n, per);
+  if (b > (int64_t)1 << 30) b = (int64_t)1 << 30;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+template <typename C> __global__ void k_zero(C* p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = cmk<C>(0, 0);
+}
+template <typename R> void zero(cx_t<R>* p, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero<cx_t<R>>, dim3(grid1d(n)), dim3(256), 0, s, p, n);
+  DEREK_HIP(hipGetLastError());
+}
+template void zero<double>(double2*, int64_t, hipStream_t);
+template void zero<float>(float2*, int64_t, hipStream_t);
+
+template <typename C>
+__global__ void k_scatter(const C* __restrict__ psi, int64_t ldpsi, const int32_t* __restrict__ idx, int64_t npw,
+                          C* __restrict__ grid, int64_t N) {
+  const int b = blockIdx.y;
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < npw; g += (int64_t)gridDim.x * blockDim.x)
+    grid[b * N + idx[g]] = psi[b * ldpsi + g];
+}
+template <typename R>
+void scatter(const cx_t<R>* psi, int64_t ldpsi, int nb, const int32_t* idx, int64_t npw, cx_t<R>* grid, int64_t N,
+             hipStream_t s) {
+  hipLaunchKernelGGL(k_scatter<cx_t<R>>, dim3(grid1d(npw), nb), dim3(256), 0, s, psi, ldpsi, idx, npw, grid, N);
+  DEREK_HIP(hipGetLastError());
+}
+template void scatter<double>(const double2*, int64_t, int, const int32_t*, int64_t, double2*, int64_t, hipStream_t);
+template void scatter<float>(const float2*, int64_t, int, const int32_t*, int64_t, float2*, int64_t, hipStream_t);
+
+template <typename C>
+__global__ void k_gather_kin(const C* __restrict__ grid, int64_t N, const int32_t* __restrict__ idx,
+                             const double* __restrict__ kin, const C* __restrict__ psi, int64_t ldpsi,
+                             C* __restrict__ hpsi, int64_t ldh, int64_t npw) {
+  using R = decltype(C::x);
+  const int b = blockIdx.y;
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < npw;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const C v = grid[b * N + idx[g]];
+    const C p = psi[b * ldpsi + g];
+    const R t = (R)kin[g];
+    hpsi[b * ldh + g] = cmk<C>(v.x + t * p.x, v.y + t * p.y);
+  }
+}
+template <typename R>
+void gather_kin(const cx_t<R>* grid, int64_t N, const int32_t* idx, const double* kin, const cx_t<R>* psi,
+                int64_t ldpsi, cx_t<R>* hpsi, int64_t ldh, int nb, int64_t npw, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_kin<cx_t<R>>, dim3(grid1d(npw), nb), dim3(256), 0, s, grid, N, idx, kin, psi, ldpsi,
+                     hpsi, ldh, npw);
+  DEREK_HIP(hipGetLastError());
+}
+template void gather_kin<double>(const double2*, int64_t, const int32_t*, const double*, const double2*, int64_t,
+                                 double2*, int64_t, int, int64_t, hipStream_t);
+template void gather_kin<float>(const float2*, int64_t, const int32_t*, const double*, const float2*, int64_t,
+                                float2*, int64_t, int, int64_t, hipStream_t);
+
+// ------------------------------------------------------------------ LDA XC
+// Slater exchange + Perdew-Zunger 1981 correlation, spin-unpolarised, with
+// libxc's screening (skip n < 1e-15; exchange zero when n/2 <= 1e-15).
+__device__ __forceinline__ void lda_point(double n, double& eps, double& v) {
+  eps = 0.0;
+  v = 0.0;
+  const double thr = 1e-15;
+  if (!(n >= thr)) return;
+  if (n / 2.0 > thr) {
+    const double cx = -0.75 * 0.98474502184269654115;  // (3/pi)^(1/3)
+    const double n13 = cbrt(n);
+    eps = cx * n13;
+    v = (4.0 / 3.0) * cx * n13;
+  }
+  const double rs = cbrt(3.0 / (4.0 * 3.141592653589793238462643383279502884197 * n));
+  double ec, vc;
+  if (rs >= 1.0) {
+    const double g = -0.1423, b1 = 1.0529, b2 = 0.3334;
+    const double sr = sqrt(rs);
+    const double den = 1.0 + b1 * sr + b2 * rs;
+    ec = g / den;
+    vc = ec * (1.0 + (7.0 / 6.0) * b1 * sr + (4.0 / 3.0) * b2 * rs) / den;
+  } else {
+    const double a = 0.0311, b = -0.048, c = 0.0020, d = -0.0116;
+    const double lr = log(rs);
+    ec = a * lr + b + c * rs * lr + d * rs;
+    vc = a * lr + (b - a / 3.0) + (2.0 / 3.0) * c * rs * lr + ((2.0 * d - c) / 3.0) * rs;
+  }
+  eps += ec;
+  v += vc;
+}
+
+constexpr int RED_NT = 256;
+constexpr int RED_BLOCKS = 1024;
+
+__device__ __forceinline__ double block_sum(double x, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = x;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += sh[w];
+  }
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(RED_NT) void k_lda(const double* __restrict__ rho, double* __restrict__ vxc,
+                                                double* __restrict__ partial, int64_t n) {
+  __shared__ double sh[RED_NT / 64];
+  double e = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)RED_NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_NT) {
+    double eps, v;
+    const double r = rho[i];
+    lda_point(r, eps, v);
+    vxc[i] = v;
+    e += r * eps;
+  }
+  const double t = block_sum(e, sh);
+  if (threadIdx.x == 0 && partial) partial[blockIdx.x] = t;
+}
+
+int lda_partials(int64_t n) { return (int)std::min<int64_t>(RED_BLOCKS, ceil_div(n, RED_NT)); }
+
+void lda_xc(const double* rho, double* vxc, double* partial, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_lda, dim3(lda_partials(n)), dim3(RED_NT), 0, s, rho, vxc, partial, n);
+  DEREK_HIP(hipGetLastError());
+}
+
+__global__ void k_lda_eps(const double* __restrict__ rho, double* __restrict__ eps, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double e, v;
+    lda_point(rho[i], e, v);
+    eps[i] = e;
+  }
+}
+void lda_eps(const double* rho, double* eps, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_lda_eps, dim3(grid1d(n)), dim3(256), 0, s, rho, eps, n);
+  DEREK_HIP(hipGetLastError());
+}
+
+__global__ void k_real_to_cx(const double* __restrict__ in, double2* __restrict__ out, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = make_double2(in[i], 0.0);
+}
+void real_to_cx(const double* in, double2* out, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_real_to_cx, dim3(grid1d(n)), dim3(256), 0, s, in, out, n);
+  DEREK_HIP(hipGetLastError());
+}
+
+__global__ void k_cx_real(const double2* __restrict__ in, double* __restrict__ out, double scale, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = in[i].x * scale;
+}
+void cx_real(const double2* in, double* out, double scale, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_cx_real, dim3(grid1d(n)), dim3(256), 0, s, in, out, scale, n);
+  DEREK_HIP(hipGetLastError());
+}
+
+__global__ void k_gspace_scale(double2* __restrict__ g, const double* __restrict__ gsq, double q0sq, double scale,
+                               int mode, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double q = gsq[i];
+    double f;
+    if (mode == 0) f = q > 0.0 ? 4.0 * 3.141592653589793238462643383279502884197 / q * scale : 0.0;
+    else f = q > 0.0 ? q / (q + q0sq) * scale : 0.0;
+    g[i] = make_double2(g[i].x * f, g[i].y * f);
+  }
+}
+void gspace_scale(double2* g, const double* gsq, double q0sq, double scale, int mode, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_gspace_scale, dim3(grid1d(n)), dim3(256), 0, s, g, gsq, q0sq, scale, mode, n);
+  DEREK_HIP(hipGetLastError());
+}
+
+__global__ void k_add3(const double* a, const double* b, const double* c, double* out, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double v = a ? a[i] : 0.0;
+    if (b) v += b[i];
+    if (c) v += c[i];
+    out[i] = v;
+  }
+}
+void add3(const double* a, const double* b, const double* c, double* out, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_add3, dim3(grid1d(n)), dim3(256), 0, s, a, b, c, out, n);
+  DEREK_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(RED_NT) void k_dot(const double* __restrict__ a, const double* __restrict__ b,
+                                                double* __restrict__ partial, int64_t n) {
+  __shared__ double sh[RED_NT / 64];
+  double e = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)RED_NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_NT)
+    e += a[i] * b[i];
+  const double t = block_sum(e, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+int dot_nblocks(int64_t n) { return (int)std::min<int64_t>(RED_BLOCKS, ceil_div(n, RED_NT)); }
+void dot_partials(const double* a, const double* b, double* partial, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_dot, dim3(dot_nblocks(n)), dim3(RED_NT), 0, s, a, b, partial, n);
+  DEREK_HIP(hipGetLastError());
+}
+
+struct LinComb {
+  const double* x[16];
+  double c[16];
+  int m;
+};
+__global__ void k_lincomb(double* __restrict__ out, LinComb lc, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double v = 0.0;
+    for (int j = 0; j < lc.m; ++j) v += lc.c[j] * lc.x[j][i];
+    out[i] = v;
+  }
+}
+void lincomb(double* out, const double* const* xs, const double* c, int m, int64_t n, hipStream_t s) {
+  DEREK_REQUIRE(m >= 1 && m <= 16, "lincomb: 1..16 vectors");
+  LinComb lc{};
+  for (int j = 0; j < m; ++j) {
+    lc.x[j] = xs[j];
+    lc.c[j] = c[j];
+  }
+  lc.m = m;
+  hipLaunchKernelGGL(k_lincomb, dim3(grid1d(n)), dim3(256), 0, s, out, lc, n);
+  DEREK_HIP(hipGetLastError());
+}
+
+__global__ void k_axpy(double* y, const double* x, double a, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] += a * x[i];
+}
+void axpy(double* y, const double* x, double a, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpy, dim3(grid1d(n)), dim3(256), 0, s, y, x, a, n);
+  DEREK_HIP(hipGetLastError());
+}
+__global__ void k_sub(double* out, const double* a, const double* b, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = a[i] - b[i];
+}
+void sub(double* out, const double* a, const double* b, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_sub, dim3(grid1d(n)), dim3(256), 0, s, out, a, b, n);
+  DEREK_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ Davidson helpers
+template <typename C>
+__global__ __launch_bounds__(RED_NT) void k_band_sums(const C* __restrict__ x, const C* __restrict__ hx, int64_t ld,
+                                                      const double* __restrict__ eps,
+                                                      const double* __restrict__ kin, int64_t npw,
+                                                      double* __restrict__ partial, int nchunk) {
+  __shared__ double sh[RED_NT / 64];
+  const int b = blockIdx.y, c = blockIdx.x;
+  const double e = eps[b];
+  double sk = 0.0, sr = 0.0;
+  for (int64_t g = c * (int64_t)RED_NT + threadIdx.x; g < npw; g += (int64_t)nchunk * RED_NT) {
+    const C p = x[b * ld + g], h = hx[b * ld + g];
+    const double px = p.x, py = p.y;
+    sk += kin[g] * (px * px + py * py);
+    const double rx = (double)h.x - e * px, ry = (double)h.y - e * py;
+    sr += rx * rx + ry * ry;
+  }
+  const double tk = block_sum(sk, sh);
+  const double tr = block_sum(sr, sh);
+  if (threadIdx.x == 0) {
+    partial[((int64_t)b * nchunk + c) * 2 + 0] = tk;
+    partial[((int64_t)b * nchunk + c) * 2 + 1] = tr;
+  }
+}
+template <typename R>
+void band_sums(const cx_t<R>* x, const cx_t<R>* hx, int64_t ld, const double* eps, const double* kin, int nb,
+               int64_t npw, double* partial, int nchunk, hipStream_t s) {
+  hipLaunchKernelGGL(k_band_sums<cx_t<R>>, dim3(nchunk, nb), dim3(RED_NT), 0, s, x, hx, ld, eps, kin, npw, partial,
+                     nchunk);
+  DEREK_HIP(hipGetLastError());
+}
+template void band_sums<double>(const double2*, const double2*, int64_t, const double*, const double*, int, int64_t,
+                                double*, int, hipStream_t);
+template void band_sums<float>(const float2*, const float2*, int64_t, const double*, const double*, int, int64_t,
+                               double*, int, hipStream_t);
+
+template <typename C>
+__global__ void k_tpa(const C* __restrict__ x, const C* __restrict__ hx, int64_t ld, const double* __restrict__ eps,
+                      const double* __restrict__ ekin, const int32_t* __restrict__ act,
+                      const double* __restrict__ kin, int64_t npw, C* __restrict__ w, int64_t ldw) {
+  using R = decltype(C::x);
+  const int j = blockIdx.y;
+  const int b = act[j];
+  const double e = eps[b], ik = 1.0 / fmax(ekin[b], 0.1);  // <T>_n floored at 0.1 Ha
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < npw; g += (int64_t)gridDim.x * blockDim.x) {
+    const C p = x[b * ld + g], h = hx[b * ld + g];
+    const double t = kin[g] * ik;
+    const double pp = 27.0 + t * (18.0 + t * (12.0 + 8.0 * t));
+    const double k = pp / (pp + 16.0 * t * t * t * t);
+    w[j * ldw + g] = cmk<C>((R)(k * ((double)h.x - e * (double)p.x)), (R)(k * ((double)h.y - e * (double)p.y)));
+  }
+}
+template <typename R>
+void tpa_residual(const cx_t<R>* x, const cx_t<R>* hx, int64_t ld, const double* eps, const double* ekin,
+                  const int32_t* act, int nact, const double* kin, int64_t npw, cx_t<R>* w, int64_t ldw,
+                  hipStream_t s) {
+  if (nact == 0) return;
+  hipLaunchKernelGGL(k_tpa<cx_t<R>>, dim3(grid1d(npw), nact), dim3(256), 0, s, x, hx, ld, eps, ekin, act, kin, npw,
+                     w, ldw);
+  DEREK_HIP(hipGetLastError());
+}
+template void tpa_residual<double>(const double2*, const double2*, int64_t, const double*, const double*,
+                                   const int32_t*, int, const double*, int64_t, double2*, int64_t, hipStream_t);
+template void tpa_residual<float>(const float2*, const float2*, int64_t, const double*, const double*,
+                                  const int32_t*, int, const double*, int64_t, float2*, int64_t, hipStream_t);
+
+template <typename C> __global__ void k_zero_pad(C* x, int64_t ld, int64_t npw) {
+  const int b = blockIdx.y;
+  for (int64_t g = npw + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < ld; g += (int64_t)gridDim.x * blockDim.x)
+    x[b * ld + g] = cmk<C>(0, 0);
+}
+template <typename R> void zero_pad(cx_t<R>* x, int64_t ld, int64_t npw, int nb, hipStream_t s) {
+  if (ld <= npw || nb == 0) return;
+  hipLaunchKernelGGL(k_zero_pad<cx_t<R>>, dim3(grid1d(ld - npw), nb), dim3(256), 0, s, x, ld, npw);
+  DEREK_HIP(hipGetLastError());
+}
+template void zero_pad<double>(double2*, int64_t, int64_t, int, hipStream_t);
+template void zero_pad<float>(float2*, int64_t, int64_t, int, hipStream_t);
+
+// ------------------------------------------------------------------ initial wavefunctions
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__global__ void k_init_psi(double2* __restrict__ psi, int64_t ld, int64_t npw, const double* __restrict__ kin,
+                           uint64_t seed, int k) {
+  const int b = blockIdx.y;
+  const uint64_t G = 0x9E3779B97F4A7C15ull;
+  const uint64_t stream = (1ull << 48) | ((uint64_t)k << 24) | (uint64_t)b;
+  const uint64_t key = mix64((seed * G) ^ mix64(stream + G));
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < npw; g += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t z1 = mix64(key + (uint64_t)(2 * g + 1) * G);
+    const uint64_t z2 = mix64(key + (uint64_t)(2 * g + 2) * G);
+    const double u1 = (double)(z1 >> 11) * (1.0 / 9007199254740992.0);
+    const double u2 = (double)(z2 >> 11) * (1.0 / 9007199254740992.0);
+    const double r = sqrt(-2.0 * log(1.0 - u1));
+    const double t = 2.0 * 3.141592653589793238462643383279502884197 * u2;
+    const double sc = 1.0 / (1.0 + kin[g]);
+    psi[b * ld + g] = make_double2(r * cos(t) * sc, r * sin(t) * sc);
+  }
+}
+void init_psi(double2* psi, int64_t ld, int nb, int64_t npw, const double* kin, uint64_t seed, int k,
+              hipStream_t s) {
+  hipLaunchKernelGGL(k_init_psi, dim3(grid1d(npw), nb), dim3(256), 0, s, psi, ld, npw, kin, seed, k);
+  DEREK_HIP(hipGetLastError());
+}
+
+template <typename D, typename S> __global__ void k_convert(D* dst, const S* src, int64_t n) {
+  using RD = decltype(D::x);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = cmk<D>((RD)src[i].x, (RD)src[i].y);
+}
+template <typename Rd, typename Rs> void convert(cx_t<Rd>* dst, const cx_t<Rs>* src, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL((k_convert<cx_t<Rd>, cx_t<Rs>>), dim3(grid1d(n)), dim3(256), 0, s, dst, src, n);
+  DEREK_HIP(hipGetLastError());
+}
+template void convert<float, double>(float2*, const double2*, int64_t, hipStream_t);
+template void convert<double, float>(double2*, const float2*, int64_t, hipStream_t);
+
+__global__ void k_convert_real_f(float* dst, const double* src, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = (float)src[i];
+}
+void convert_real_f(float* dst, const double* src, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_convert_real_f, dim3(grid1d(n)), dim3(256), 0, s, dst, src, n);
+  DEREK_HIP(hipGetLastError());
+}
+
+}  // namespace derek

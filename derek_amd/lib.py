@@ -1,0 +1,334 @@
+"""ctypes binding of ``libderek_hip.so`` (the C-ABI in include/derek_hip.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (make in
+derek_amd/csrc).  There is no fallback: if the library is missing every
+call raises, naming the build step.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libderek_hip.so")
+
+DEREK_OK, DEREK_ERR_ARG, DEREK_ERR_HIP, DEREK_ERR_NUMERIC, DEREK_ERR_STATE = 0, 1, 2, 3, 4
+DEREK_FP64, DEREK_MIXED = 0, 1
+
+# every symbol declared in include/derek_hip.h
+EXPORTS = (
+    "derek_hip_version", "derek_hip_last_error",
+    "derek_basis_grid_dims", "derek_basis_inv_cell", "derek_basis_g_map", "derek_basis_real_map",
+    "derek_basis_kpoints", "derek_basis_sphere", "derek_basis_gsq", "derek_gaussian_pot",
+    "derek_hip_init", "derek_hip_destroy", "derek_hip_set_basis", "derek_hip_set_gsq",
+    "derek_hip_set_vloc", "derek_hip_set_vext",
+    "derek_hip_fft3d", "derek_hip_apply_h", "derek_hip_overlap", "derek_hip_rotate",
+    "derek_hip_density", "derek_hip_lda", "derek_hip_hartree",
+    "derek_hip_scf_init", "derek_hip_scf_step", "derek_hip_scf_polish", "derek_hip_get_eigenvalues",
+    "derek_hip_get_occupations", "derek_hip_get_density", "derek_hip_get_psi", "derek_hip_put_psi",
+    "derek_hip_apply_h_resident", "derek_hip_sync", "derek_hip_stream",
+)
+
+
+class DerekError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Dims(C.Structure):
+    _fields_ = [("ngrid", C.c_int32 * 3), ("nk", C.c_int32), ("nbands", C.c_int32),
+                ("precision", C.c_int32), ("volume", C.c_double), ("max_npw", C.c_int64)]
+
+
+class ScfParams(C.Structure):
+    _fields_ = [("nelec", C.c_double), ("kT", C.c_double), ("mix_beta", C.c_double),
+                ("mix_q0", C.c_double), ("n_david_first", C.c_int32), ("n_david", C.c_int32),
+                ("history", C.c_int32), ("reserved", C.c_int32), ("seed", C.c_uint64)]
+
+
+class ScfReport(C.Structure):
+    _fields_ = [("iscf", C.c_int32), ("ndav", C.c_int32), ("etot", C.c_double), ("e_band", C.c_double),
+                ("e_hartree", C.c_double), ("e_xc", C.c_double), ("e_dc", C.c_double),
+                ("minus_ts", C.c_double), ("fermi", C.c_double), ("drho", C.c_double),
+                ("max_res", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_lib = None
+
+
+def load():
+    """Load the library once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() (make -C derek_amd/csrc)")
+        _lib = C.CDLL(LIB_PATH)
+        for name in EXPORTS:
+            getattr(_lib, name).restype = C.c_int32
+    return _lib
+
+
+def last_error():
+    buf = C.create_string_buffer(4096)
+    load().derek_hip_last_error(buf, C.c_int64(len(buf)))
+    return buf.value.decode(errors="replace")
+
+
+def check(code):
+    if code != DEREK_OK:
+        raise DerekError(code, last_error())
+
+
+def ptr(a, ctype=C.c_double):
+    return a.ctypes.data_as(C.POINTER(ctype)) if a is not None else None
+
+
+def _c128(a):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    return a, a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+# ---------------------------------------------------------------- discretisation (host)
+def grid_dims(cell, ecut, g_fine_scale):
+    cell9 = np.asfortranarray(np.asarray(cell, dtype=np.float64)).ravel(order="F")
+    s = (C.c_int32 * 3)()
+    f = (C.c_int32 * 3)()
+    check(load().derek_basis_grid_dims(ptr(cell9), C.c_double(ecut), C.c_double(g_fine_scale), s, f))
+    return list(s), list(f)
+
+
+def inv_cell(cell):
+    cell9 = np.asarray(cell, dtype=np.float64).ravel(order="F").copy()
+    out = np.zeros(9)
+    check(load().derek_basis_inv_cell(ptr(cell9), ptr(out)))
+    return out.reshape(3, 3, order="F")
+
+
+def g_map(inv, dims):
+    d = (C.c_int32 * 3)(*dims)
+    n = int(np.prod(dims))
+    g = np.zeros((n, 3), order="F")
+    m = np.zeros((n, 3), dtype=np.int32, order="F")
+    inv9 = np.asarray(inv, dtype=np.float64).ravel(order="F").copy()
+    check(load().derek_basis_g_map(ptr(inv9), d, g.ctypes.data_as(C.POINTER(C.c_double)),
+                                   m.ctypes.data_as(C.POINTER(C.c_int32))))
+    return g, m
+
+
+def real_map(cell, dims):
+    d = (C.c_int32 * 3)(*dims)
+    n = int(np.prod(dims))
+    r = np.zeros((n, 3), order="F")
+    f = np.zeros((n, 3), order="F")
+    cell9 = np.asarray(cell, dtype=np.float64).ravel(order="F").copy()
+    check(load().derek_basis_real_map(ptr(cell9), d, r.ctypes.data_as(C.POINTER(C.c_double)),
+                                      f.ctypes.data_as(C.POINTER(C.c_double))))
+    return r, f
+
+
+def kpoints(mp):
+    m = (C.c_int32 * 3)(*mp)
+    nk = C.c_int32()
+    check(load().derek_basis_kpoints(m, None, C.byref(nk)))
+    out = np.zeros((nk.value, 3), order="F")
+    check(load().derek_basis_kpoints(m, out.ctypes.data_as(C.POINTER(C.c_double)), C.byref(nk)))
+    return out
+
+
+def sphere(inv, dims, kfrac, ecut):
+    d = (C.c_int32 * 3)(*dims)
+    inv9 = np.asarray(inv, dtype=np.float64).ravel(order="F").copy()
+    kf = np.ascontiguousarray(kfrac, dtype=np.float64)
+    n = C.c_int64()
+    check(load().derek_basis_sphere(ptr(inv9), d, ptr(kf), C.c_double(ecut), C.c_int64(0), C.byref(n), None, None))
+    idx = np.zeros(n.value, dtype=np.int32)
+    kin = np.zeros(n.value)
+    check(load().derek_basis_sphere(ptr(inv9), d, ptr(kf), C.c_double(ecut), C.c_int64(n.value), C.byref(n),
+                                    idx.ctypes.data_as(C.POINTER(C.c_int32)), ptr(kin)))
+    return idx, kin
+
+
+def gsq(inv, dims):
+    d = (C.c_int32 * 3)(*dims)
+    inv9 = np.asarray(inv, dtype=np.float64).ravel(order="F").copy()
+    out = np.zeros(int(np.prod(dims)))
+    check(load().derek_basis_gsq(ptr(inv9), d, ptr(out)))
+    return out
+
+
+def gaussian_pot(cell, dims, centres, heights, widths, cos_amp=0.0):
+    d = (C.c_int32 * 3)(*dims)
+    cell9 = np.asarray(cell, dtype=np.float64).ravel(order="F").copy()
+    cen = np.ascontiguousarray(np.asarray(centres, dtype=np.float64).reshape(-1, 3))
+    h = np.ascontiguousarray(heights, dtype=np.float64)
+    w = np.ascontiguousarray(widths, dtype=np.float64)
+    v = np.zeros(int(np.prod(dims)))
+    check(load().derek_gaussian_pot(ptr(cell9), d, C.c_int32(len(h)), ptr(cen) if len(h) else None,
+                                    ptr(h) if len(h) else None, ptr(w) if len(h) else None,
+                                    C.c_double(cos_amp), ptr(v)))
+    return v
+
+
+# ---------------------------------------------------------------- context
+class Context:
+    """Owns one derek_ctx (all device memory of one SCF)."""
+
+    def __init__(self, ngrid, nk, nbands, volume, max_npw, precision=DEREK_FP64):
+        self.lib = load()
+        d = Dims()
+        d.ngrid = (C.c_int32 * 3)(*ngrid)
+        d.nk, d.nbands, d.precision = nk, nbands, precision
+        d.volume, d.max_npw = volume, max_npw
+        self.h = C.c_void_p()
+        check(self.lib.derek_hip_init(C.byref(self.h), C.byref(d)))
+        self.ngrid, self.N = list(ngrid), int(np.prod(ngrid))
+        self.nk, self.nbands, self.precision = nk, nbands, precision
+        self.npw = [0] * nk
+
+    def close(self):
+        if self.h:
+            check(self.lib.derek_hip_destroy(self.h))
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_basis(self, k, fft_index, kin, weight):
+        idx = np.ascontiguousarray(fft_index, dtype=np.int32)
+        kn = np.ascontiguousarray(kin, dtype=np.float64)
+        check(self.lib.derek_hip_set_basis(self.h, C.c_int32(k), C.c_int64(len(idx)),
+                                           idx.ctypes.data_as(C.POINTER(C.c_int32)), ptr(kn), C.c_double(weight)))
+        self.npw[k] = len(idx)
+
+    def set_gsq(self, g):
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        check(self.lib.derek_hip_set_gsq(self.h, ptr(g)))
+
+    def set_vloc(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        check(self.lib.derek_hip_set_vloc(self.h, ptr(v)))
+
+    def set_vext(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        check(self.lib.derek_hip_set_vext(self.h, ptr(v)))
+
+    def fft3d(self, grid, direction):
+        a = np.array(grid, dtype=np.complex128, copy=True)
+        shape = a.shape
+        a = np.ascontiguousarray(a.reshape(-1, self.N))
+        check(self.lib.derek_hip_fft3d(self.h, a.ctypes.data_as(C.POINTER(C.c_double)),
+                                       C.c_int32(a.shape[0]), C.c_int32(direction)))
+        return a.reshape(shape)
+
+    def apply_h(self, k, psi):
+        """psi: (nb, npw_k) complex (each band contiguous) -> H psi."""
+        p, pp = _c128(np.atleast_2d(psi))
+        out = np.zeros_like(p)
+        check(self.lib.derek_hip_apply_h(self.h, C.c_int32(k), C.c_int32(p.shape[0]), pp,
+                                         out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def overlap(self, X, Y):
+        """S[i,j] = <X_i|Y_j> for band-stacked X (m, npw), Y (n, npw)."""
+        x, xp = _c128(np.atleast_2d(X))
+        if Y is X:
+            y, yp = x, xp  # same pointer: Hermitian kernel
+        else:
+            y, yp = _c128(np.atleast_2d(Y))
+        m, n, npw = x.shape[0], y.shape[0], x.shape[1]
+        S = np.zeros((n, m), dtype=np.complex128)  # column-major m x n == C-order (n, m)
+        check(self.lib.derek_hip_overlap(self.h, C.c_int64(npw), C.c_int32(m), C.c_int32(n), xp, yp,
+                                         S.ctypes.data_as(C.POINTER(C.c_double))))
+        return S.T.copy()
+
+    def rotate(self, X, U):
+        """Z_j = sum_i U[i,j] X_i; X (m, npw), U (m, n) -> (n, npw)."""
+        x, xp = _c128(np.atleast_2d(X))
+        m, npw = x.shape
+        n = U.shape[1]
+        u, up = _c128(np.asarray(U).T)  # column-major m x n
+        Z = np.zeros((n, npw), dtype=np.complex128)
+        check(self.lib.derek_hip_rotate(self.h, C.c_int64(npw), C.c_int32(m), C.c_int32(n), xp, up,
+                                        Z.ctypes.data_as(C.POINTER(C.c_double))))
+        return Z
+
+    def density(self, k, psi, occ, rho=None):
+        p, pp = _c128(np.atleast_2d(psi))
+        o = np.ascontiguousarray(occ, dtype=np.float64)
+        acc = rho is not None
+        r = np.array(rho, dtype=np.float64, copy=True) if acc else np.zeros(self.N)
+        check(self.lib.derek_hip_density(self.h, C.c_int32(k), C.c_int32(p.shape[0]), pp, ptr(o), ptr(r),
+                                         C.c_int32(1 if acc else 0)))
+        return r
+
+    def lda(self, rho):
+        r = np.ascontiguousarray(rho, dtype=np.float64)
+        e = np.zeros_like(r)
+        v = np.zeros_like(r)
+        check(self.lib.derek_hip_lda(self.h, C.c_int64(len(r)), ptr(r), ptr(e), ptr(v)))
+        return e, v
+
+    def hartree(self, rho):
+        r = np.ascontiguousarray(rho, dtype=np.float64)
+        v = np.zeros_like(r)
+        eh = C.c_double()
+        check(self.lib.derek_hip_hartree(self.h, ptr(r), ptr(v), C.byref(eh)))
+        return v, eh.value
+
+    # ---- resident SCF
+    def scf_init(self, nelec, kT=0.0, beta=0.5, q0=1.0, n_david_first=4, n_david=2, history=8, seed=1):
+        p = ScfParams(nelec=nelec, kT=kT, mix_beta=beta, mix_q0=q0, n_david_first=n_david_first,
+                      n_david=n_david, history=history, reserved=0, seed=seed)
+        check(self.lib.derek_hip_scf_init(self.h, C.byref(p)))
+
+    def scf_step(self):
+        r = ScfReport()
+        check(self.lib.derek_hip_scf_step(self.h, C.byref(r)))
+        return r.as_dict()
+
+    def scf_polish(self, max_iter=100, res_tol=1e-9):
+        r = C.c_double()
+        check(self.lib.derek_hip_scf_polish(self.h, C.c_int32(max_iter), C.c_double(res_tol), C.byref(r)))
+        return r.value
+
+    def eigenvalues(self):
+        e = np.zeros((self.nk, self.nbands))
+        check(self.lib.derek_hip_get_eigenvalues(self.h, ptr(e)))
+        return e
+
+    def occupations(self):
+        e = np.zeros((self.nk, self.nbands))
+        check(self.lib.derek_hip_get_occupations(self.h, ptr(e)))
+        return e
+
+    def get_density(self):
+        r = np.zeros(self.N)
+        check(self.lib.derek_hip_get_density(self.h, ptr(r)))
+        return r
+
+    def get_psi(self, k):
+        out = np.zeros((self.nbands, self.npw[k]), dtype=np.complex128)
+        check(self.lib.derek_hip_get_psi(self.h, C.c_int32(k), out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def put_psi(self, k, psi):
+        p, pp = _c128(psi)
+        check(self.lib.derek_hip_put_psi(self.h, C.c_int32(k), pp))
+
+    def apply_h_resident(self, k, nb):
+        check(self.lib.derek_hip_apply_h_resident(self.h, C.c_int32(k), C.c_int32(nb)))
+
+    def sync(self):
+        check(self.lib.derek_hip_sync(self.h))
+
+    def stream(self):
+        s = C.c_void_p()
+        check(self.lib.derek_hip_stream(self.h, C.byref(s)))
+        return s.value

@@ -1,0 +1,174 @@
+/*
+ * derek_hip.h - C-ABI of the MI355X (gfx950) Kohn-Sham SCF hot path for DEReK.
+ *
+ * Every entry point returns an int32 status: 0 (DEREK_OK) or one of the
+ * DEREK_ERR_* codes; derek_hip_last_error() copies the message of the last
+ * failure of the calling thread.  Arguments are plain pointers and sizes.
+ * Host arrays are Fortran-compatible: column-major, contiguous, complex
+ * values interleaved (re, im) as complex(C_DOUBLE_COMPLEX).  No device
+ * pointer crosses the boundary: the context owns all GPU memory.
+ *
+ * Grids are flat with z fastest and x slowest, exactly the layout the
+ * reference's fft_3d reshapes around FFTW (Source/fft.f90:258-273).
+ * dir = +1 is FFTW_FORWARD (exp(-iGr), grids_real2recip), dir = -1 is
+ * FFTW_BACKWARD (exp(+iGr), grids_recip2real); both unnormalised.
+ *
+ * Not thread-safe: one context per process and host thread, calls are
+ * ordered on the context's HIP stream.
+ */
+#ifndef DEREK_HIP_H
+#define DEREK_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DEREK_OK 0
+#define DEREK_ERR_ARG 1     /* invalid argument (checked on the host before any launch) */
+#define DEREK_ERR_HIP 2     /* HIP runtime / device failure */
+#define DEREK_ERR_NUMERIC 3 /* e.g. overlap matrix not positive definite */
+#define DEREK_ERR_STATE 4   /* call out of order (e.g. basis not set) */
+
+#define DEREK_FP64 0  /* complex128 wavefunctions, FP64 everywhere */
+#define DEREK_MIXED 1 /* complex64 FFT/H|psi>/density, BF16-in FP32-acc overlaps, FP64 small dense + energies */
+
+typedef struct derek_ctx derek_ctx;
+
+typedef struct {
+  int32_t ngrid[3];  /* fine FFT grid (fine_ngx, fine_ngy, fine_ngz), basis.f90:93-105 */
+  int32_t nk;        /* number of k-points, io.f90:2008 */
+  int32_t nbands;    /* bands per k-point */
+  int32_t precision; /* DEREK_FP64 or DEREK_MIXED */
+  double volume;     /* cell volume in bohr^3, io.f90:1976 */
+  int64_t max_npw;   /* largest plane-wave count over the k-points */
+} derek_dims;
+
+typedef struct {
+  double nelec;          /* electrons (spin-degenerate, occupation 2 per band) */
+  double kT;             /* Fermi-Dirac smearing in Ha (0: fixed occupations) */
+  double mix_beta;       /* Pulay/Kerker step, default 0.5 */
+  double mix_q0;         /* Kerker wavevector in 1/bohr, default 1.0 */
+  int32_t n_david_first; /* Davidson steps in the first SCF step, default 4 */
+  int32_t n_david;       /* Davidson steps per later SCF step, default 2 */
+  int32_t history;       /* Pulay history, default 8 (max 16) */
+  int32_t reserved;
+  uint64_t seed;         /* initial-wavefunction seed */
+} derek_scf_params;
+
+typedef struct {
+  int32_t iscf;     /* step counter, starts at 1 */
+  int32_t ndav;     /* Davidson steps done in this SCF step (first k) */
+  double etot;      /* E_band - int V_Hxc,in rho_out + E_H + E_xc - TS (Ha) */
+  double e_band;
+  double e_hartree;
+  double e_xc;
+  double e_dc;      /* int (V_H,in + V_xc,in) rho_out */
+  double minus_ts;
+  double fermi;
+  double drho;      /* || rho_out - rho_in ||_2 (bohr^-3/2) */
+  double max_res;   /* largest Davidson residual norm */
+} derek_scf_report;
+
+/* ---- library ------------------------------------------------------------ */
+int32_t derek_hip_version(void);
+int32_t derek_hip_last_error(char* buf, int64_t len);
+
+/* ---- discretisation (host; restates Source/basis.f90 and Source/io.f90) -- */
+/* Standard and fine grid sizes: basis.f90:87-105 with basis_prime_fact (:257-309).
+ * cell is the 3x3 lattice, column-major as current_structure%cell. */
+int32_t derek_basis_grid_dims(const double cell[9], double ecut, double g_fine_scale, int32_t std_dims[3],
+                              int32_t fine_dims[3]);
+/* transpose(twopi*io_inv(cell)), io.f90:1954 (orthorhombic cells). */
+int32_t derek_basis_inv_cell(const double cell[9], double inv_cell[9]);
+/* grid_points / fine_grid_points (N,3) and the integer indices (N,3), basis.f90:124-149. */
+int32_t derek_basis_g_map(const double inv_cell[9], const int32_t dims[3], double* g, int32_t* m);
+/* real_*grid_points and *frac_points (N,3), basis.f90:155-188. */
+int32_t derek_basis_real_map(const double cell[9], const int32_t dims[3], double* r, double* frac);
+/* kpt_scf_list (nk,3), io.f90:1992-2035.  *nk receives the count. */
+int32_t derek_basis_kpoints(const int32_t mp[3], double* kfrac, int32_t* nk);
+/* Plane waves 1/2|G+k|^2 <= ecut in fine-map order: FFT-bin index in the flat
+ * z-fastest layout and kinetic energy.  Outputs may be NULL (count only);
+ * at most cap entries are written. */
+int32_t derek_basis_sphere(const double inv_cell[9], const int32_t dims[3], const double kfrac[3], double ecut,
+                           int64_t cap, int64_t* npw, int32_t* fft_index, double* kin);
+/* |G|^2 of every FFT bin of the grid (flat layout). */
+int32_t derek_basis_gsq(const double inv_cell[9], const int32_t dims[3], double* gsq);
+/* External potential of $DATA gaussian_pot wells (io.f90:1888-1922; the
+ * reference's pot_external_pot sets it to zero, pot.f90:165-166) at the
+ * real_fine_grid_points: V = sum_i h_i sum_R exp(-|r-c_i-R|^2/(2 w_i^2))
+ * + cos_amp*cos(2 pi x/L).  centres are fractional (3,nwells) column-major. */
+int32_t derek_gaussian_pot(const double cell[9], const int32_t dims[3], int32_t nwells, const double* centres,
+                           const double* heights, const double* widths, double cos_amp, double* v);
+
+/* ---- context -------------------------------------------------------------- */
+/* Allocates the device state (replaces fft_init's plans, fft.f90:139-219). */
+int32_t derek_hip_init(derek_ctx** ctx, const derek_dims* dims);
+int32_t derek_hip_destroy(derek_ctx* ctx);
+/* Plane-wave set of k-point k (0-based) and its weight. */
+int32_t derek_hip_set_basis(derek_ctx* ctx, int32_t k, int64_t npw, const int32_t* fft_index, const double* kin,
+                            double weight);
+int32_t derek_hip_set_gsq(derek_ctx* ctx, const double* gsq);
+/* Total local potential used by derek_hip_apply_h (N_g doubles). */
+int32_t derek_hip_set_vloc(derek_ctx* ctx, const double* v);
+/* External potential used by the SCF (N_g doubles). */
+int32_t derek_hip_set_vext(derek_ctx* ctx, const double* v);
+
+/* ---- per-call operations on host buffers ---------------------------------- */
+/* In-place 3D FFT of howmany grids (complex, N_g each); replaces fft_3d,
+ * fft.f90:222-303.  dir must be +1 or -1. */
+int32_t derek_hip_fft3d(derek_ctx* ctx, double* grid, int32_t howmany, int32_t dir);
+/* H|psi> for nb bands of k-point k: hpsi = 1/2|G+k|^2 psi + FFT[V psi(r)]/N.
+ * psi, hpsi: npw_k x nb complex.  Replaces pot_apply_wfn (pot.f90:344-382)
+ * plus the kinetic term. */
+int32_t derek_hip_apply_h(derek_ctx* ctx, int32_t k, int32_t nb, const double* psi, double* hpsi);
+/* S = X^H Y (m x n complex); X: npw x m, Y: npw x n (X == Y with m == n
+ * computes only the upper tiles of the Hermitian result).  Replaces the
+ * wave_dot_* stubs (wave.f90:929-1036) with the full band-band overlap. */
+int32_t derek_hip_overlap(derek_ctx* ctx, int64_t npw, int32_t m, int32_t n, const double* X, const double* Y,
+                          double* S);
+/* Z = X U; X: npw x m, U: m x n, Z: npw x n (complex). */
+int32_t derek_hip_rotate(derek_ctx* ctx, int64_t npw, int32_t m, int32_t n, const double* X, const double* U,
+                         double* Z);
+/* rho(r) (accumulate ? += : =) weight_k * sum_b occ[b] |psi_b(r)|^2 / volume.
+ * Replaces density_calculate (density.f90:101-131). */
+int32_t derek_hip_density(derek_ctx* ctx, int32_t k, int32_t nb, const double* psi, const double* occ, double* rho,
+                          int32_t accumulate);
+/* LDA (Slater + Perdew-Zunger 81), spin-unpolarised, per point; the same
+ * arguments as libxc's xc_f03_lda_exc_vxc(p, np, rho, zk, vrho)
+ * (ExLib/libxc/src/libxc_master.F90:1168-1176). */
+int32_t derek_hip_lda(derek_ctx* ctx, int64_t np, const double* rho, double* exc, double* vxc);
+/* V_H(r) from rho(r) (V_H(G) = 4 pi rho(G)/|G|^2, G = 0 dropped) and
+ * E_H = 1/2 int V_H rho.  Needs derek_hip_set_gsq. */
+int32_t derek_hip_hartree(derek_ctx* ctx, const double* rho, double* vh, double* eh);
+
+/* ---- resident SCF (electronic_scf body, electronic.f90:86-175) ----------- */
+/* Random orthonormal wavefunctions, uniform density nelec/volume. */
+int32_t derek_hip_scf_init(derek_ctx* ctx, const derek_scf_params* params);
+/* One SCF step: V_in from rho_in, Davidson for every k, occupations,
+ * rho_out, energies, Pulay mixing.  The caller tests convergence
+ * (|E_i - E_{i-1}| < energy_tol, electronic.f90:156-165). */
+int32_t derek_hip_scf_step(derek_ctx* ctx, derek_scf_report* report);
+/* Davidson on the last step's potential, up to max_iter steps per k-point or
+ * until every residual norm is below res_tol (eigenvalues for reporting). */
+int32_t derek_hip_scf_polish(derek_ctx* ctx, int32_t max_iter, double res_tol, double* max_res);
+/* eigenvalues (nbands, nk) column-major, Ha */
+int32_t derek_hip_get_eigenvalues(derek_ctx* ctx, double* eig);
+int32_t derek_hip_get_occupations(derek_ctx* ctx, double* occ);
+/* rho_out of the last step (N_g doubles) */
+int32_t derek_hip_get_density(derek_ctx* ctx, double* rho);
+/* resident wavefunctions of k (npw_k x nbands complex, FP64 on the host) */
+int32_t derek_hip_get_psi(derek_ctx* ctx, int32_t k, double* psi);
+int32_t derek_hip_put_psi(derek_ctx* ctx, int32_t k, const double* psi);
+/* H applied to the resident wavefunctions of k (for timing; result kept on device). */
+int32_t derek_hip_apply_h_resident(derek_ctx* ctx, int32_t k, int32_t nb);
+/* Block until all work on the context's stream is done. */
+int32_t derek_hip_sync(derek_ctx* ctx);
+/* Host-visible stream handle (hipStream_t) for event timing. */
+int32_t derek_hip_stream(derek_ctx* ctx, void** stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DEREK_HIP_H */

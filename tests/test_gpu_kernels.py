@@ -1,0 +1,184 @@
+"""GPU kernels through the C-ABI against the oracle on the same seeded inputs.
+
+FP64: relative error <= 1e-12 (FFT/H|psi>/density/Hartree/GEMM: sums of
+O(N) terms in double).  Mixed mode (complex64 FFT and H|psi>): <= 5e-5.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from derek_amd import config, lib
+from derek_amd.scf import Setup
+from oracle import fft as offt
+from oracle import ks as oks
+from oracle import xc as oxc
+from tests.conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def rnd_c(rng, *shape):
+    return rng.normal(size=shape) + 1j * rng.normal(size=shape)
+
+
+@pytest.mark.parametrize("dims", [(32, 32, 32), (8, 12, 10), (14, 22, 26), (15, 27, 20), (72, 72, 72),
+                                  (108, 108, 108), (144, 144, 144)])
+@pytest.mark.parametrize("precision", [lib.DEREK_FP64, lib.DEREK_MIXED])
+def test_fft3d(dims, precision):
+    print("fft3d", dims, precision, flush=True)
+    N = int(np.prod(dims))
+    howmany = 2 if N < 200000 else 1
+    ctx = lib.Context(dims, 1, 1, 1.0, 1, precision)
+    rng = np.random.default_rng(N)
+    a = rnd_c(rng, howmany, N)
+    tol = 1e-12 if precision == lib.DEREK_FP64 else 5e-5
+    for d in (1, -1):
+        got = ctx.fft3d(a, d)
+        ref = offt.fft3d(a, dims, d)
+        assert rel_err(got, ref) < tol, (dims, d)
+    ctx.close()
+
+
+def test_fft3d_roundtrip_and_bad_direction():
+    dims = (108, 108, 108)
+    N = int(np.prod(dims))
+    ctx = lib.Context(dims, 1, 1, 1.0, 1)
+    a = rnd_c(np.random.default_rng(0), N)
+    back = ctx.fft3d(ctx.fft3d(a, 1), -1)
+    assert rel_err(back, N * a) < 1e-12
+    with pytest.raises(lib.DerekError) as e:
+        ctx.fft3d(a, 0)
+    assert e.value.code == lib.DEREK_ERR_ARG
+    ctx.close()
+
+
+def make_setup(spec):
+    su = Setup(spec)
+    return su
+
+
+@pytest.mark.parametrize("name", ["small", "small_k", "cfg1"])
+@pytest.mark.parametrize("precision", [lib.DEREK_FP64, lib.DEREK_MIXED])
+def test_apply_h(name, precision):
+    print("apply_h", name, precision, flush=True)
+    if name == "small":
+        spec = config.small_system(seed=3)
+    elif name == "small_k":
+        spec = config.small_system(seed=4, mp=(2, 1, 2), ecut=5.0)
+    else:
+        spec = config.benchmark(1)
+    su = make_setup(spec)
+    ctx = su.context(precision, nbands=11)
+    rng = np.random.default_rng(7)
+    v = su.vext + 0.3 * rng.normal(size=su.N)
+    ctx.set_vloc(v)
+    tol = 1e-12 if precision == lib.DEREK_FP64 else 5e-5
+    for k in range(su.nk):
+        psi = rnd_c(rng, 11, su.npw[k])
+        got = ctx.apply_h(k, psi)
+        ref = oks.apply_h(psi, su.fft_index[k], su.kin[k], v, su.dims)
+        assert rel_err(got, ref) < tol, k
+    ctx.close()
+
+
+def test_apply_h_state_errors():
+    su = make_setup(config.small_system(seed=3))
+    ctx = su.context()
+    psi = np.ones((1, su.npw[0]), dtype=complex)
+    with pytest.raises(lib.DerekError) as e:
+        ctx.apply_h(0, psi)
+    assert e.value.code == lib.DEREK_ERR_STATE
+    with pytest.raises(lib.DerekError) as e:
+        ctx.set_basis(0, np.array([0, su.N], dtype=np.int32), np.zeros(2), 1.0)
+    assert e.value.code == lib.DEREK_ERR_ARG
+    ctx.close()
+
+
+def test_apply_h_hermitian_full_size():
+    """cfg2 size (108^3 grid, 62669 plane waves): <x|Hy> = conj(<y|Hx>)."""
+    su = make_setup(config.benchmark(2))
+    ctx = su.context(nbands=2)
+    ctx.set_vloc(su.vext)
+    rng = np.random.default_rng(11)
+    xy = rnd_c(rng, 2, su.npw[0])
+    h = ctx.apply_h(0, xy)
+    a = np.vdot(xy[0], h[1])
+    b = np.conj(np.vdot(xy[1], h[0]))
+    assert abs(a - b) < 1e-11 * abs(a)
+    # kinetic-only check: V = 0 gives H = diag(1/2|G|^2) exactly
+    ctx.set_vloc(np.zeros(su.N))
+    h0 = ctx.apply_h(0, xy)
+    assert rel_err(h0, su.kin[0][None, :] * xy) < 1e-13
+    ctx.close()
+
+
+@pytest.mark.parametrize("m,n,npw", [(10, 10, 1503), (37, 53, 777), (160, 160, 5003), (80, 80, 62669)])
+def test_overlap(m, n, npw):
+    rng = np.random.default_rng(m * 1000 + n)
+    ctx = lib.Context((8, 8, 8), 1, 1, 1.0, 1)
+    X = rnd_c(rng, m, npw)
+    Y = rnd_c(rng, n, npw)
+    assert rel_err(ctx.overlap(X, Y), np.conj(X) @ Y.T) < 1e-13
+    if m == n:
+        S = ctx.overlap(X, X)
+        assert rel_err(S, np.conj(X) @ X.T) < 1e-13
+        assert np.array_equal(S, S.conj().T)
+    ctx.close()
+
+
+@pytest.mark.parametrize("m,n,npw", [(10, 10, 1503), (37, 53, 777), (160, 80, 4003)])
+def test_rotate(m, n, npw):
+    rng = np.random.default_rng(m + n)
+    ctx = lib.Context((8, 8, 8), 1, 1, 1.0, 1)
+    X = rnd_c(rng, m, npw)
+    U = rnd_c(rng, m, n)
+    assert rel_err(ctx.rotate(X, U), U.T @ X) < 1e-13
+    ctx.close()
+
+
+@pytest.mark.parametrize("precision", [lib.DEREK_FP64, lib.DEREK_MIXED])
+def test_density(precision):
+    spec = config.benchmark(1)
+    su = make_setup(spec)
+    ctx = su.context(precision)
+    rng = np.random.default_rng(5)
+    psi = rnd_c(rng, 10, su.npw[0]) / np.sqrt(su.npw[0])
+    occ = np.array([2.0, 2.0, 2.0, 2.0, 1.0, 0.0, 0.5, 0.0, 0.0, 0.25])
+    got = ctx.density(0, psi, occ)
+    ref = oks.density([psi], [occ], [1.0], [su.fft_index[0]], su.dims, su.volume)
+    tol = 1e-12 if precision == lib.DEREK_FP64 else 5e-5
+    assert rel_err(got, ref) < tol
+    got2 = ctx.density(0, psi, occ, rho=got)
+    assert rel_err(got2, 2 * ref) < tol
+    ctx.close()
+
+
+def test_lda_matches_libxc_fixture_and_oracle():
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "libxc_lda.json")))
+    ctx = lib.Context((8, 8, 8), 1, 1, 1.0, 1)
+    rho = np.array(fx["regression"]["lda_x.BrOH"]["rho"])
+    e, v = ctx.lda(rho)
+    ex = np.array(fx["regression"]["lda_x.BrOH"]["exc"]) + np.array(fx["regression"]["lda_c_pz.BrOH"]["exc"])
+    vx = np.array(fx["regression"]["lda_x.BrOH"]["vxc"]) + np.array(fx["regression"]["lda_c_pz.BrOH"]["vxc"])
+    assert np.max(np.abs(e - ex) / (1 + np.abs(ex))) < 1e-13
+    assert np.max(np.abs(v - vx) / (1 + np.abs(vx))) < 1e-13
+    r = np.concatenate([np.geomspace(1e-16, 1e3, 5000), [0.0, 1e-15, 1.5e-15, 3 / (4 * np.pi)]])
+    e, v = ctx.lda(r)
+    eo, vo = oxc.lda_xc(r)
+    assert np.max(np.abs(e - eo) / (1e-300 + np.abs(eo) + (eo == 0))) < 1e-13
+    assert np.max(np.abs(v - vo) / (1e-300 + np.abs(vo) + (vo == 0))) < 1e-13
+    ctx.close()
+
+
+def test_hartree():
+    su = make_setup(config.benchmark(1))
+    ctx = su.context()
+    rng = np.random.default_rng(9)
+    rho = 0.01 + 0.001 * rng.random(su.N)
+    vh, eh = ctx.hartree(rho)
+    vr, er = oks.hartree(rho, su.gsq, su.dims, su.volume)
+    assert rel_err(vh, vr) < 1e-12
+    assert abs(eh - er) < 1e-12 * abs(er)
+    ctx.close()

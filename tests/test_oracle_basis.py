@@ -1,0 +1,94 @@
+"""Reference discretisation restated in the oracle (Source/basis.f90,
+Source/io.f90): known grid sizes, enumeration order and ranges, k-points."""
+import numpy as np
+import pytest
+
+from derek_amd import config
+from oracle import basis
+
+
+def smooth(n):
+    for p in (2, 3, 5, 7, 11, 13):
+        while n % p == 0:
+            n //= p
+    return n == 1
+
+
+def test_prime_fact_bruteforce():
+    for n in range(1, 400):
+        m = n
+        while not smooth(m):
+            m += 1
+        assert basis.prime_fact(n) == m
+
+
+@pytest.mark.parametrize("i,std,fine", [(1, 15, 32), (2, 50, 108), (4, 33, 72), (5, 70, 144)])
+def test_benchmark_grids(i, std, fine):
+    sp = config.benchmark(i)
+    s, f = basis.grid_dims(np.diag(sp.lengths), sp.ecut, sp.g_fine_scale)
+    assert s == [std] * 3 and f == [fine] * 3
+    # default g_fine_scale = 2.0 gives the reference's own fine grids
+    _, f2 = basis.grid_dims(np.diag(sp.lengths), sp.ecut, 2.0)
+    assert f2 == [{1: 30, 2: 99, 4: 65, 5: 140}[i]] * 3
+
+
+def test_g_map_order_and_values():
+    cell = np.diag([10.0, 12.0, 9.0])
+    inv = basis.inv_cell_orthorhombic(cell)
+    dims = [4, 5, 3]
+    G, M = basis.g_map(inv, dims)
+    assert list(M[0]) == [-2, -2, -1] and list(M[1]) == [-2, -2, 0] and list(M[-1]) == [1, 2, 1]
+    assert M.shape == (60, 3)
+    # z fastest, then y, then x
+    assert list(M[3]) == [-2, -1, -1]
+    for a, L in enumerate((10.0, 12.0, 9.0)):
+        c = (2.0 * np.pi) * (1.0 / L)
+        assert np.all(G[:, a] == c * M[:, a])
+
+
+def test_real_map_last_point_is_one():
+    cell = np.diag([10.0, 10.0, 10.0])
+    R, F = basis.real_map(cell, [3, 4, 5])
+    assert list(F[0]) == [1 / 3, 1 / 4, 1 / 5]
+    assert list(F[-1]) == [1.0, 1.0, 1.0]
+    assert np.all(R == 10.0 * F)
+
+
+def test_kpoints_mp():
+    k = basis.kpoint_grid((2, 1, 4))
+    assert k.shape == (8, 3)
+    assert list(k[0]) == [-0.25, 0.0, -0.375]
+    assert list(k[1]) == [-0.25, 0.0, -0.125]
+    assert list(k[-1]) == [0.25, 0.0, 0.375]
+    assert np.allclose(k.sum(axis=0), 0.0)
+
+
+def test_fft_bin_roundtrip():
+    dims = [6, 5, 4]
+    G, M = basis.g_map(basis.inv_cell_orthorhombic(np.diag([5.0, 6.0, 7.0])), dims)
+    b = basis.fft_bin(M, dims)
+    assert sorted(b.tolist()) == list(range(120))
+
+
+@pytest.mark.parametrize("i,npw", [(1, 1503), (2, 62669)])
+def test_sphere_counts(i, npw):
+    sp = config.benchmark(i)
+    cell = np.diag(sp.lengths)
+    inv = basis.inv_cell_orthorhombic(cell)
+    _, dims = basis.grid_dims(cell, sp.ecut, sp.g_fine_scale)
+    G, M = basis.g_map(inv, dims)
+    idx, kin, _ = basis.sphere(G, M, dims, np.zeros(3), sp.ecut)
+    assert len(idx) == npw
+    assert np.all(kin <= sp.ecut)
+    assert len(set(idx.tolist())) == npw
+
+
+def test_sphere_kpoint_counts_cfg4():
+    sp = config.benchmark(4)
+    cell = np.diag(sp.lengths)
+    inv = basis.inv_cell_orthorhombic(cell)
+    _, dims = basis.grid_dims(cell, sp.ecut, sp.g_fine_scale)
+    G, M = basis.g_map(inv, dims)
+    ks = basis.kpoint_grid(sp.mp)
+    counts = [len(basis.sphere(G, M, dims, basis.kcart(inv, kf), sp.ecut)[0]) for kf in ks[:6]]
+    assert all(17400 < c < 17600 for c in counts)

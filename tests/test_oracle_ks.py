@@ -1,0 +1,100 @@
+"""Kohn-Sham pieces of the oracle against brute force and closed forms."""
+import numpy as np
+
+from derek_amd import config
+from oracle import basis, fft, ks
+from tests.conftest import oracle_system
+
+
+def tiny():
+    return oracle_system(config.small_system(seed=5, length=6.0, ecut=2.5, g=2.0, nwells=2, nbands=4))
+
+
+def test_apply_h_matches_dense_matrix():
+    s = tiny()
+    idx, kin, M = s.fft_index[0], s.kin[0], s.Mk[0]
+    assert 20 < len(idx) < 400
+    rng = np.random.default_rng(1)
+    v = rng.normal(size=s.N)
+    H = ks.apply_h_dense(M, kin, v, s.dims)
+    assert np.allclose(H, H.conj().T, atol=1e-13)
+    psi = rng.normal(size=(3, len(idx))) + 1j * rng.normal(size=(3, len(idx)))
+    assert np.allclose(ks.apply_h(psi, idx, kin, v, s.dims), psi @ H.T, atol=1e-12)
+
+
+def test_constant_potential_shift():
+    s = tiny()
+    idx, kin = s.fft_index[0], s.kin[0]
+    rng = np.random.default_rng(2)
+    psi = rng.normal(size=(2, len(idx))) + 1j * rng.normal(size=(2, len(idx)))
+    out = ks.apply_h(psi, idx, kin, np.full(s.N, 0.37), s.dims)
+    assert np.allclose(out, (kin + 0.37)[None, :] * psi, atol=1e-13)
+
+
+def grid_coords(s):
+    n = np.array(s.dims)
+    I = np.stack(np.meshgrid(*[np.arange(d) for d in s.dims], indexing="ij"), -1).reshape(-1, 3)
+    return I / n * np.array(s.lengths)
+
+
+def test_hartree_cosine_mode():
+    s = tiny()
+    r = grid_coords(s)
+    L = s.lengths[0]
+    G = 2 * np.pi / L
+    rho = 0.01 + 0.003 * np.cos(G * r[:, 0]) + 0.002 * np.cos(2 * G * r[:, 1])
+    vh, eh = ks.hartree(rho, s.gsq, s.dims, s.volume)
+    ref = 4 * np.pi * (0.003 / G ** 2 * np.cos(G * r[:, 0]) + 0.002 / (2 * G) ** 2 * np.cos(2 * G * r[:, 1]))
+    assert np.allclose(vh, ref, atol=1e-12)
+    e_ref = 0.5 * 4 * np.pi * (0.003 ** 2 / G ** 2 + 0.002 ** 2 / (2 * G) ** 2) * s.volume / 2
+    assert abs(eh - e_ref) < 1e-12 * max(1, e_ref)
+
+
+def test_density_integrates_and_plane_wave_is_uniform():
+    s = tiny()
+    idx = s.fft_index[0]
+    X = s.initial_psi(0)
+    occ = np.array([2.0, 2.0, 1.0, 0.5])
+    rho = ks.density([X], [occ], [1.0], [idx], s.dims, s.volume)
+    assert abs(rho.sum() * s.volume / s.N - occ.sum()) < 1e-12
+    assert rho.min() >= 0
+    pw = np.zeros((1, len(idx)), dtype=complex)
+    pw[0, 7] = 1.0
+    r1 = ks.density([pw], [[2.0]], [1.0], [idx], s.dims, s.volume)
+    assert np.allclose(r1, 2.0 / s.volume, atol=1e-14)
+
+
+def test_overlap_rotate_loops():
+    rng = np.random.default_rng(4)
+    X = rng.normal(size=(3, 17)) + 1j * rng.normal(size=(3, 17))
+    Y = rng.normal(size=(4, 17)) + 1j * rng.normal(size=(4, 17))
+    S = ks.overlap(X, Y)
+    for i in range(3):
+        for j in range(4):
+            assert abs(S[i, j] - sum(np.conj(X[i, g]) * Y[j, g] for g in range(17))) < 1e-12
+    U = rng.normal(size=(3, 2)) + 1j * rng.normal(size=(3, 2))
+    Z = ks.rotate(X, U)
+    for j in range(2):
+        assert np.allclose(Z[j], sum(U[i, j] * X[i] for i in range(3)))
+
+
+def test_fermi_occupations():
+    eig = np.array([[-1.0, -0.5, 0.0, 0.1], [-0.9, -0.4, 0.05, 0.2]])
+    w = np.array([0.5, 0.5])
+    occ, mu, mts = ks.fermi_occupations(eig, w, 4.0, 0.01)
+    assert abs(np.sum(w[:, None] * occ) - 4.0) < 1e-10
+    assert -0.4 < mu < 0.0 and mts <= 0.0
+    occ0, _, mts0 = ks.fermi_occupations(eig, w, 4.0, 0.0)
+    assert np.all(occ0[:, :2] == 2.0) and np.all(occ0[:, 2:] == 0.0) and mts0 == 0.0
+
+
+def test_gaussian_wells_periodic_and_centred():
+    cell = np.diag([8.0, 8.0, 8.0])
+    R, _ = basis.real_map(cell, [16, 16, 16])
+    v = ks.gaussian_wells(R, [8.0] * 3, [[0.5, 0.5, 0.5]], [-2.0], [1.0])
+    i = np.argmin(v)
+    assert np.allclose(R[i], [4.0, 4.0, 4.0]) and abs(v[i] + 2.0) < 1e-12
+    # periodic image sum: a well at the corner is symmetric about the boundary
+    v2 = ks.gaussian_wells(R, [8.0] * 3, [[0.0, 0.0, 0.0]], [-1.0], [1.5])
+    a = v2.reshape(16, 16, 16)
+    assert np.allclose(a[0, 15, 15], a[14, 15, 15])  # x = 0.5 and x = 7.5 are mirror images about 8
