@@ -73,6 +73,74 @@ int64_t round_up(int64_t a, int64_t b) { return ceil_div(a, b) * b; }
 
 }  // namespace
 
+// Per-kernel-class timers: HIP events recorded on the context's stream around
+// each launch while enabled, resolved lazily; algorithmic bytes and flops of
+// each launch are accumulated beside the times.
+enum KernelClass {
+  KC_ZERO = 0,
+  KC_SCATTER,
+  KC_FFT_Z,
+  KC_FFT_Y,
+  KC_FFT_X_VMUL,
+  KC_GATHER_KIN,
+  KC_FFT_X_DENSITY,
+  KC_OVERLAP,
+  KC_ROTATE,
+  KC_FFT_X,
+  KC_COUNT
+};
+
+struct Profiler {
+  struct Pending {
+    int cls;
+    hipEvent_t a, b;
+    double bytes, flops;
+  };
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  std::vector<Pending> pending;
+  int64_t count[KC_COUNT] = {};
+  double ms[KC_COUNT] = {}, bytes[KC_COUNT] = {}, flops[KC_COUNT] = {};
+  hipEvent_t take() {
+    if (pool.empty()) {
+      hipEvent_t e;
+      DEREK_HIP(hipEventCreate(&e));
+      return e;
+    }
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  void resolve() {
+    for (auto& p : pending) {
+      DEREK_HIP(hipEventSynchronize(p.b));
+      float t = 0.f;
+      DEREK_HIP(hipEventElapsedTime(&t, p.a, p.b));
+      count[p.cls] += 1;
+      ms[p.cls] += t;
+      bytes[p.cls] += p.bytes;
+      flops[p.cls] += p.flops;
+      pool.push_back(p.a);
+      pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+  void reset() {
+    resolve();
+    for (int i = 0; i < KC_COUNT; ++i) {
+      count[i] = 0;
+      ms[i] = bytes[i] = flops[i] = 0.0;
+    }
+  }
+  ~Profiler() {
+    for (auto& p : pending) {
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+
 struct KState {
   int64_t npw = 0, ld = 0;
   int32_t* idx = nullptr;
@@ -132,6 +200,19 @@ struct derek_ctx {
   double* rho_target = nullptr;  // density accumulation target
   void set_rho_target(double* p) { rho_target = p; }
   double* rho_out_target() const { return rho_target; }
+  Profiler prof;
+  template <typename F> void timed(int cls, double bytes, double flops, F&& launch) {
+    if (!prof.on) {
+      launch();
+      return;
+    }
+    hipEvent_t a = prof.take(), b = prof.take();
+    DEREK_HIP(hipEventRecord(a, s));
+    launch();
+    DEREK_HIP(hipEventRecord(b, s));
+    prof.pending.push_back({cls, a, b, bytes, flops});
+    if (prof.pending.size() > 4096) prof.resolve();
+  }
 
   ~derek_ctx();
   template <typename R> const Axis<R>* axes() const;
@@ -204,27 +285,37 @@ void derek_ctx::require_basis(int k) const {
 template <typename R> void derek_ctx::fft3d(cx_t<R>* g, int howmany, int dir) {
   const int sign = dir == 1 ? -1 : +1;
   const Axis<R>* ax = axes<R>();
-  fft_z<R>(g, (int64_t)howmany * dims[0] * dims[1], ax[2], sign, s);
-  fft_strided<R>(g, (int64_t)howmany * dims[0], dims[2], ax[1], sign, s);
-  fft_strided<R>(g, howmany, (int64_t)dims[1] * dims[2], ax[0], sign, s);
+  const double pass = 2.0 * (double)howmany * N * sizeof(cx_t<R>);
+  timed(KC_FFT_Z, pass, 0, [&] { fft_z<R>(g, (int64_t)howmany * dims[0] * dims[1], ax[2], sign, s); });
+  timed(KC_FFT_Y, pass, 0, [&] { fft_strided<R>(g, (int64_t)howmany * dims[0], dims[2], ax[1], sign, s); });
+  timed(KC_FFT_X, pass, 0, [&] { fft_strided<R>(g, howmany, (int64_t)dims[1] * dims[2], ax[0], sign, s); });
 }
 
+// Algorithmic bytes per launch (compulsory reads + writes of each kernel's
+// own inputs and outputs) are passed to the profiler with every launch.
 template <typename R>
 void derek_ctx::apply_h(int k, const cx_t<R>* psi, int64_t ldp, cx_t<R>* hpsi, int64_t ldh, int nbands) {
   const KState& K = ks[k];
   const Axis<R>* ax = axes<R>();
   cx_t<R>* g = grid<R>();
   const R* v = vlocal<R>();
+  const double sc = sizeof(cx_t<R>);
   for (int b0 = 0; b0 < nbands; b0 += batch) {
     const int nbb = std::min(batch, nbands - b0);
-    zero<R>(g, nbb * N, s);
-    scatter<R>(psi + b0 * ldp, ldp, nbb, K.idx, K.npw, g, N, s);
-    fft_z<R>(g, (int64_t)nbb * dims[0] * dims[1], ax[2], +1, s);
-    fft_strided<R>(g, (int64_t)nbb * dims[0], dims[2], ax[1], +1, s);
-    fft_strided_vmul<R>(g, nbb, (int64_t)dims[1] * dims[2], ax[0], v, (R)(1.0 / (double)N), s);
-    fft_strided<R>(g, (int64_t)nbb * dims[0], dims[2], ax[1], -1, s);
-    fft_z<R>(g, (int64_t)nbb * dims[0] * dims[1], ax[2], -1, s);
-    gather_kin<R>(g, N, K.idx, K.kin, psi + b0 * ldp, ldp, hpsi + b0 * ldh, ldh, nbb, K.npw, s);
+    const double grid_b = (double)nbb * N * sc, pw_b = (double)nbb * K.npw * sc;
+    timed(KC_ZERO, grid_b, 0, [&] { zero<R>(g, nbb * N, s); });
+    timed(KC_SCATTER, 2 * pw_b + 4.0 * K.npw, 0,
+          [&] { scatter<R>(psi + b0 * ldp, ldp, nbb, K.idx, K.npw, g, N, s); });
+    timed(KC_FFT_Z, 2 * grid_b, 0, [&] { fft_z<R>(g, (int64_t)nbb * dims[0] * dims[1], ax[2], +1, s); });
+    timed(KC_FFT_Y, 2 * grid_b, 0, [&] { fft_strided<R>(g, (int64_t)nbb * dims[0], dims[2], ax[1], +1, s); });
+    timed(KC_FFT_X_VMUL, 2 * grid_b + (double)N * sizeof(R), 0, [&] {
+      fft_strided_vmul<R>(g, nbb, (int64_t)dims[1] * dims[2], ax[0], v, (R)(1.0 / (double)N), s);
+    });
+    timed(KC_FFT_Y, 2 * grid_b, 0, [&] { fft_strided<R>(g, (int64_t)nbb * dims[0], dims[2], ax[1], -1, s); });
+    timed(KC_FFT_Z, 2 * grid_b, 0, [&] { fft_z<R>(g, (int64_t)nbb * dims[0] * dims[1], ax[2], -1, s); });
+    timed(KC_GATHER_KIN, 3 * pw_b + 12.0 * K.npw, 0, [&] {
+      gather_kin<R>(g, N, K.idx, K.kin, psi + b0 * ldp, ldp, hpsi + b0 * ldh, ldh, nbb, K.npw, s);
+    });
   }
 }
 
@@ -233,14 +324,19 @@ void derek_ctx::density(int k, const cx_t<R>* psi, int64_t ldp, int nbands, cons
   const KState& K = ks[k];
   const Axis<R>* ax = axes<R>();
   cx_t<R>* g = grid<R>();
+  const double sc = sizeof(cx_t<R>);
   for (int b0 = 0; b0 < nbands; b0 += batch) {
     const int nbb = std::min(batch, nbands - b0);
-    zero<R>(g, nbb * N, s);
-    scatter<R>(psi + b0 * ldp, ldp, nbb, K.idx, K.npw, g, N, s);
-    fft_z<R>(g, (int64_t)nbb * dims[0] * dims[1], ax[2], +1, s);
-    fft_strided<R>(g, (int64_t)nbb * dims[0], dims[2], ax[1], +1, s);
-    fft_strided_density<R>(g, nbb, (int64_t)dims[1] * dims[2], ax[0], occw + b0, rho_out_target(), 1.0,
-                           (keep || b0 > 0) ? 1 : 0, s);
+    const double grid_b = (double)nbb * N * sc, pw_b = (double)nbb * K.npw * sc;
+    const int kp = (keep || b0 > 0) ? 1 : 0;
+    timed(KC_ZERO, grid_b, 0, [&] { zero<R>(g, nbb * N, s); });
+    timed(KC_SCATTER, 2 * pw_b + 4.0 * K.npw, 0,
+          [&] { scatter<R>(psi + b0 * ldp, ldp, nbb, K.idx, K.npw, g, N, s); });
+    timed(KC_FFT_Z, 2 * grid_b, 0, [&] { fft_z<R>(g, (int64_t)nbb * dims[0] * dims[1], ax[2], +1, s); });
+    timed(KC_FFT_Y, 2 * grid_b, 0, [&] { fft_strided<R>(g, (int64_t)nbb * dims[0], dims[2], ax[1], +1, s); });
+    timed(KC_FFT_X_DENSITY, grid_b + (1 + kp) * 8.0 * N, 0, [&] {
+      fft_strided_density<R>(g, nbb, (int64_t)dims[1] * dims[2], ax[0], occw + b0, rho_out_target(), 1.0, kp, s);
+    });
   }
 }
 
@@ -290,12 +386,14 @@ template <>
 void derek_ctx::overlap<double>(const double2* X, int64_t ldx, const double2* Y, int64_t ldy, int m, int n,
                                 int64_t npw, double2* C, int ldc, bool herm) {
   double2* w = ovwork.get<double2>(overlap_work_elems(m, n, npw));
-  overlap_f64(X, ldx, Y, ldy, m, n, npw, C, ldc, w, herm, s);
+  timed(KC_OVERLAP, (double)(m + n) * npw * 16.0, 8.0 * npw * m * n,
+        [&] { overlap_f64(X, ldx, Y, ldy, m, n, npw, C, ldc, w, herm, s); });
 }
 template <>
 void derek_ctx::rotate<double>(const double2* X, int64_t ldx, const double2* U, int ldu, int m, int n, int64_t npw,
                                double2* Z, int64_t ldz) {
-  rotate_f64(X, ldx, U, ldu, m, n, npw, Z, ldz, s);
+  timed(KC_ROTATE, (double)(m + n) * npw * 16.0, 8.0 * npw * m * n,
+        [&] { rotate_f64(X, ldx, U, ldu, m, n, npw, Z, ldz, s); });
 }
 
 template <typename R>
@@ -1024,6 +1122,34 @@ int32_t derek_hip_scf_step(derek_ctx* c, derek_scf_report* rep) {
     rep->max_res = maxres;
     c->mix();
     c->sync();
+  });
+}
+
+int32_t derek_hip_profile_enable(derek_ctx* c, int32_t on) {
+  return guard([&] {
+    DEREK_REQUIRE(c, "null context");
+    c->prof.resolve();
+    c->prof.on = on != 0;
+  });
+}
+
+int32_t derek_hip_profile_reset(derek_ctx* c) {
+  return guard([&] {
+    DEREK_REQUIRE(c, "null context");
+    c->prof.reset();
+  });
+}
+
+int32_t derek_hip_profile_read(derek_ctx* c, int32_t cls, int64_t* launches, double* total_ms, double* bytes,
+                               double* flops) {
+  return guard([&] {
+    DEREK_REQUIRE(c, "null context");
+    DEREK_REQUIRE(cls >= 0 && cls < KC_COUNT, "unknown kernel class");
+    c->prof.resolve();
+    if (launches) *launches = c->prof.count[cls];
+    if (total_ms) *total_ms = c->prof.ms[cls];
+    if (bytes) *bytes = c->prof.bytes[cls];
+    if (flops) *flops = c->prof.flops[cls];
   });
 }
 

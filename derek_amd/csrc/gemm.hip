@@ -10,7 +10,7 @@
 //
 // MFMA f64 16x16x4 operand layout (lane l of a 64-wide wave):
 //   A (16x4): A[l%16][l/16]      B (4x16): B[l/16][l%16]
-//   D (16x16): register r holds D[4*(l/16)+r][l%16]
+//   D (16x16): register r holds D[l/16 + 4*r][l%16]
 #include "launch.hpp"
 
 namespace derek {
@@ -131,7 +131,7 @@ __global__ __launch_bounds__(256) void k_overlap_f64(const double2* __restrict__
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int i = i0 + wi * 32 + a * 16 + 4 * (lane >> 4) + r;
+        const int i = i0 + wi * 32 + a * 16 + (lane >> 4) + 4 * r;
         const int j = j0 + wj * 32 + b * 16 + (lane & 15);
         if (i < g.m && j < g.n) P[(int64_t)j * g.m + i] = make_double2(cr[a][b][r], ci[a][b][r]);
       }
@@ -141,7 +141,9 @@ __global__ void k_overlap_reduce(const double2* __restrict__ part, OvGeom g, dou
   const int64_t mn = (int64_t)g.m * g.n;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < mn; e += (int64_t)gridDim.x * blockDim.x) {
     const int i = (int)(e % g.m), j = (int)(e / g.m);
-    bool mirror = g.herm && (i / OV_T) > (j / OV_T);
+    // Hermitian: the strict lower triangle is the conjugate of the upper one
+    // (exactly Hermitian output, real diagonal)
+    const bool mirror = g.herm && i > j;
     const int64_t src = mirror ? (int64_t)i * g.m + j : (int64_t)j * g.m + i;
     double sr = 0.0, si = 0.0;
     for (int c = 0; c < g.nchunk; ++c) {
@@ -149,6 +151,7 @@ __global__ void k_overlap_reduce(const double2* __restrict__ part, OvGeom g, dou
       sr += v.x;
       si += v.y;
     }
+    if (g.herm && i == j) si = 0.0;
     C[(int64_t)j * ldc + i] = mirror ? make_double2(sr, -si) : make_double2(sr, si);
   }
 }
@@ -223,7 +226,7 @@ __global__ __launch_bounds__(256) void k_rotate_f64(const double2* __restrict__ 
     if (j >= n) continue;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int64_t gg = g0 + 4 * (lane >> 4) + r;
+      const int64_t gg = g0 + (lane >> 4) + 4 * r;
       if (gg < npw) Z[(int64_t)j * ldz + gg] = make_double2(cr[b][r], ci[b][r]);
     }
   }

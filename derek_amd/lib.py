@@ -27,7 +27,11 @@ EXPORTS = (
     "derek_hip_scf_init", "derek_hip_scf_step", "derek_hip_scf_polish", "derek_hip_get_eigenvalues",
     "derek_hip_get_occupations", "derek_hip_get_density", "derek_hip_get_psi", "derek_hip_put_psi",
     "derek_hip_apply_h_resident", "derek_hip_sync", "derek_hip_stream",
+    "derek_hip_profile_enable", "derek_hip_profile_reset", "derek_hip_profile_read",
 )
+
+KERNEL_CLASSES = ("zero", "scatter", "fft_z", "fft_y", "fft_x_vmul", "gather_kin", "fft_x_density",
+                  "overlap", "rotate", "fft_x")
 
 
 class DerekError(RuntimeError):
@@ -324,6 +328,23 @@ class Context:
 
     def apply_h_resident(self, k, nb):
         check(self.lib.derek_hip_apply_h_resident(self.h, C.c_int32(k), C.c_int32(nb)))
+
+    def profile(self, on=True):
+        check(self.lib.derek_hip_profile_enable(self.h, C.c_int32(1 if on else 0)))
+
+    def profile_reset(self):
+        check(self.lib.derek_hip_profile_reset(self.h))
+
+    def profile_read(self):
+        """{class name: (launches, total_ms, algorithmic bytes, flops)}"""
+        out = {}
+        for i, name in enumerate(KERNEL_CLASSES):
+            n = C.c_int64()
+            ms, by, fl = C.c_double(), C.c_double(), C.c_double()
+            check(self.lib.derek_hip_profile_read(self.h, C.c_int32(i), C.byref(n), C.byref(ms), C.byref(by),
+                                                  C.byref(fl)))
+            out[name] = (n.value, ms.value, by.value, fl.value)
+        return out
 
     def sync(self):
         check(self.lib.derek_hip_sync(self.h))

@@ -163,6 +163,17 @@ int32_t derek_hip_get_psi(derek_ctx* ctx, int32_t k, double* psi);
 int32_t derek_hip_put_psi(derek_ctx* ctx, int32_t k, const double* psi);
 /* H applied to the resident wavefunctions of k (for timing; result kept on device). */
 int32_t derek_hip_apply_h_resident(derek_ctx* ctx, int32_t k, int32_t nb);
+/* Per-kernel-class timers: while enabled, HIP events on the context's
+ * stream bracket every launch; read returns launches, total milliseconds and
+ * the algorithmic bytes / flops of those launches.  Classes: 0 zero-fill,
+ * 1 plane-wave scatter, 2 FFT z pass, 3 FFT y pass, 4 FFT x pass fused with
+ * the V(r) multiply, 5 gather + kinetic, 6 FFT x pass fused with the density
+ * accumulation, 7 overlap GEMM, 8 rotation GEMM, 9 FFT x pass.
+ * (The reference's equivalent is its trace/profile module, trace.f90:515-667.) */
+int32_t derek_hip_profile_enable(derek_ctx* ctx, int32_t on);
+int32_t derek_hip_profile_reset(derek_ctx* ctx);
+int32_t derek_hip_profile_read(derek_ctx* ctx, int32_t cls, int64_t* launches, double* total_ms, double* bytes,
+                               double* flops);
 /* Block until all work on the context's stream is done. */
 int32_t derek_hip_sync(derek_ctx* ctx);
 /* Host-visible stream handle (hipStream_t) for event timing. */

@@ -31,3 +31,8 @@ def rel_err(a, b):
     a = np.asarray(a)
     b = np.asarray(b)
     return float(np.max(np.abs(a - b)) / max(1e-300, float(np.max(np.abs(b)))))
+
+
+def assert_close(got, ref, tol, what=""):
+    e = rel_err(got, ref)
+    assert e < tol, f"{what} relative error {e:.3e} > {tol:.1e}"
