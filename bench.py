@@ -93,9 +93,11 @@ def main():
         ctx.close()
         ctx = su.context()
         res = gscf.run(spec, max_scf=60, energy_tol=1e-6, setup=su, ctx=ctx, log=None)
+        t_dense = sum(h["t_dense"] for h in res["history"])
         scf_info = {"fp64_time_to_scf_s": round(res["wall"], 4), "fp64_scf_steps": res["iterations"],
                     "fp64_ms_per_scf_step": round(res["wall"] / res["iterations"] * 1e3, 2),
-                    "etot_ha": res["etot"], "converged": res["converged"]}
+                    "host_dense_share": round(t_dense / res["wall"], 3),
+                    "kT_ha": spec.kT, "etot_ha": res["etot"], "converged": res["converged"]}
         print(f"# FP64 SCF: {res['iterations']} steps, {res['wall']:.3f} s, E = {res['etot']:.10f}",
               file=sys.stderr, flush=True)
     ctx.close()

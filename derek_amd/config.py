@@ -118,11 +118,13 @@ def benchmark(i):
     if i in (1, 3):
         return SystemSpec("cfg1", 10.0, 10.0, 2.15, (1, 1, 1), 8, 10, nwells=4, seed=1)
     if i == 2:
-        return SystemSpec("cfg2", 20.0, 30.0, 2.15, (1, 1, 1), 128, 80, nwells=16, cos_amp=0.2, seed=2)
+        # metallic: the gap between bands 64 and 65 closes, fixed occupations
+        # slosh; a 0.002 Ha Fermi-Dirac smearing (16 kT below the top band) converges
+        return SystemSpec("cfg2", 20.0, 30.0, 2.15, (1, 1, 1), 128, 80, kT=0.002, nwells=16, cos_amp=0.2, seed=2)
     if i == 4:
         return SystemSpec("cfg4", 16.0, 20.0, 2.2, (4, 4, 4), 32, 24, kT=0.01, nwells=8, seed=4)
     if i == 5:
-        return SystemSpec("cfg5", 30.0, 25.0, 2.125, (2, 2, 2), 512, 300, nwells=64, seed=5)
+        return SystemSpec("cfg5", 30.0, 25.0, 2.125, (2, 2, 2), 512, 300, kT=0.002, nwells=64, seed=5)
     raise ValueError("benchmark systems are numbered 1..5")
 
 

@@ -2,15 +2,18 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
 #include <vector>
+#include <chrono>
 
 #include "../../include/derek_hip.h"
 #include "basis.hpp"
 #include "dense.hpp"
 #include "launch.hpp"
+#include "pw_fft.hpp"
 
 using namespace derek;
 using cd = std::complex<double>;
@@ -77,15 +80,16 @@ int64_t round_up(int64_t a, int64_t b) { return ceil_div(a, b) * b; }
 // each launch while enabled, resolved lazily; algorithmic bytes and flops of
 // each launch are accumulated beside the times.
 enum KernelClass {
-  KC_ZERO = 0,
-  KC_SCATTER,
-  KC_FFT_Z,
-  KC_FFT_Y,
-  KC_FFT_X_VMUL,
-  KC_GATHER_KIN,
-  KC_FFT_X_DENSITY,
+  KC_STICKS_Z_INV = 0,
+  KC_PLANES_Y_INV,
+  KC_PLANES_X_VMUL,
+  KC_PLANES_Y_FWD,
+  KC_STICKS_Z_FWD,
+  KC_PLANES_X_DENSITY,
   KC_OVERLAP,
   KC_ROTATE,
+  KC_FFT_Z,
+  KC_FFT_Y,
   KC_FFT_X,
   KC_COUNT
 };
@@ -147,6 +151,8 @@ struct KState {
   double* kin = nullptr;
   double weight = 0.0;
   bool set = false;
+  PwMaps maps;                  // sphere maps (device pointers into mapbuf)
+  int32_t* mapbuf = nullptr;
   // SCF wavefunction blocks, 2*nbands columns of leading dimension ld:
   // Y = [X | W], HY = [HX | HW]; Y2/HY2 receive rotations.
   void* Y = nullptr;
@@ -174,8 +180,13 @@ struct derek_ctx {
   float* vloc32 = nullptr;
   bool gsq_set = false, vloc_set = false, vext_set = false;
   int batch = 8;
-  double2* grid64 = nullptr;
-  float2* grid32 = nullptr;
+  // pruned-FFT work buffers for one band batch (sized for the largest k)
+  int64_t st_elems = 0, pl_elems = 0;
+  double2 *S64 = nullptr, *P64 = nullptr;
+  float2 *S32 = nullptr, *P32 = nullptr;
+  template <typename R> cx_t<R>* sticks();
+  template <typename R> cx_t<R>* planes();
+  void ensure_pw_buffers();
   double2* cgrid = nullptr;
   DevBuf tA, tB, tC, tD, ovwork, partial;
   // SCF
@@ -196,6 +207,7 @@ struct derek_ctx {
   double2* d_H = nullptr;
   double2* d_U = nullptr;
   double fermi = 0.0, minus_ts = 0.0;
+  double t_dense = 0.0;  // host seconds in the subspace solves of the current step
   bool have_rho_out = false;
   double* rho_target = nullptr;  // density accumulation target
   void set_rho_target(double* p) { rho_target = p; }
@@ -216,7 +228,6 @@ struct derek_ctx {
 
   ~derek_ctx();
   template <typename R> const Axis<R>* axes() const;
-  template <typename R> cx_t<R>* grid() const;
   template <typename R> const R* vlocal() const;
 
   template <typename R> void fft3d(cx_t<R>* g, int howmany, int dir);
@@ -242,9 +253,32 @@ struct derek_ctx {
 
 template <> const Axis<double>* derek_ctx::axes<double>() const { return ax64; }
 template <> const Axis<float>* derek_ctx::axes<float>() const { return ax32; }
-template <> double2* derek_ctx::grid<double>() const { return grid64; }
-template <> float2* derek_ctx::grid<float>() const { return grid32; }
 template <> const double* derek_ctx::vlocal<double>() const { return vloc; }
+template <> double2* derek_ctx::sticks<double>() { return S64; }
+template <> float2* derek_ctx::sticks<float>() { return S32; }
+template <> double2* derek_ctx::planes<double>() { return P64; }
+template <> float2* derek_ctx::planes<float>() { return P32; }
+
+// (Re)allocate the stick and plane buffers when a k-point needs more room.
+void derek_ctx::ensure_pw_buffers() {
+  int64_t st = 0, pl = 0;
+  for (const KState& k : ks)
+    if (k.set) {
+      st = std::max<int64_t>(st, (int64_t)k.maps.nst * dims[2]);
+      pl = std::max<int64_t>(pl, (int64_t)k.maps.nax * dims[1] * dims[2]);
+    }
+  st *= batch;
+  pl *= batch;
+  if (st > st_elems || pl > pl_elems) {
+    for (void* p : {(void*)S64, (void*)P64, (void*)S32, (void*)P32}) dfree(p);
+    S64 = dmalloc<double2>(st);
+    P64 = dmalloc<double2>(pl);
+    S32 = dmalloc<float2>(st);
+    P32 = dmalloc<float2>(pl);
+    st_elems = st;
+    pl_elems = pl;
+  }
+}
 template <> const float* derek_ctx::vlocal<float>() const { return vloc32; }
 
 derek_ctx::~derek_ctx() {
@@ -256,12 +290,14 @@ derek_ctx::~derek_ctx() {
   for (auto& k : ks) {
     dfree(k.idx);
     dfree(k.kin);
+    dfree(k.mapbuf);
     dfree(k.Y);
     dfree(k.HY);
     dfree(k.Y2);
     dfree(k.HY2);
   }
-  for (void* p : {(void*)gsq, (void*)vext, (void*)vloc, (void*)vloc32, (void*)grid64, (void*)grid32, (void*)cgrid,
+  for (void* p : {(void*)S64, (void*)P64, (void*)S32, (void*)P32}) dfree(p);
+  for (void* p : {(void*)gsq, (void*)vext, (void*)vloc, (void*)vloc32, (void*)cgrid,
                   (void*)rho_in, (void*)rho_out, (void*)vh, (void*)vxc, (void*)vhxc, (void*)tmp1, (void*)tmp2,
                   (void*)d_eps, (void*)d_ekin, (void*)d_occw, (void*)d_act, (void*)d_S, (void*)d_H, (void*)d_U})
     dfree(p);
@@ -296,25 +332,26 @@ template <typename R> void derek_ctx::fft3d(cx_t<R>* g, int howmany, int dir) {
 template <typename R>
 void derek_ctx::apply_h(int k, const cx_t<R>* psi, int64_t ldp, cx_t<R>* hpsi, int64_t ldh, int nbands) {
   const KState& K = ks[k];
+  const PwMaps& m = K.maps;
   const Axis<R>* ax = axes<R>();
-  cx_t<R>* g = grid<R>();
+  cx_t<R>* S = sticks<R>();
+  cx_t<R>* P = planes<R>();
   const R* v = vlocal<R>();
   const double sc = sizeof(cx_t<R>);
+  const int64_t nyz = (int64_t)dims[1] * dims[2];
   for (int b0 = 0; b0 < nbands; b0 += batch) {
     const int nbb = std::min(batch, nbands - b0);
-    const double grid_b = (double)nbb * N * sc, pw_b = (double)nbb * K.npw * sc;
-    timed(KC_ZERO, grid_b, 0, [&] { zero<R>(g, nbb * N, s); });
-    timed(KC_SCATTER, 2 * pw_b + 4.0 * K.npw, 0,
-          [&] { scatter<R>(psi + b0 * ldp, ldp, nbb, K.idx, K.npw, g, N, s); });
-    timed(KC_FFT_Z, 2 * grid_b, 0, [&] { fft_z<R>(g, (int64_t)nbb * dims[0] * dims[1], ax[2], +1, s); });
-    timed(KC_FFT_Y, 2 * grid_b, 0, [&] { fft_strided<R>(g, (int64_t)nbb * dims[0], dims[2], ax[1], +1, s); });
-    timed(KC_FFT_X_VMUL, 2 * grid_b + (double)N * sizeof(R), 0, [&] {
-      fft_strided_vmul<R>(g, nbb, (int64_t)dims[1] * dims[2], ax[0], v, (R)(1.0 / (double)N), s);
-    });
-    timed(KC_FFT_Y, 2 * grid_b, 0, [&] { fft_strided<R>(g, (int64_t)nbb * dims[0], dims[2], ax[1], -1, s); });
-    timed(KC_FFT_Z, 2 * grid_b, 0, [&] { fft_z<R>(g, (int64_t)nbb * dims[0] * dims[1], ax[2], -1, s); });
-    timed(KC_GATHER_KIN, 3 * pw_b + 12.0 * K.npw, 0, [&] {
-      gather_kin<R>(g, N, K.idx, K.kin, psi + b0 * ldp, ldp, hpsi + b0 * ldh, ldh, nbb, K.npw, s);
+    const double pw_b = (double)nbb * K.npw * sc;
+    const double st_b = (double)nbb * m.nst * dims[2] * sc;
+    const double pl_b = (double)nbb * m.nax * nyz * sc;
+    timed(KC_STICKS_Z_INV, pw_b + 12.0 * K.npw + st_b, 0,
+          [&] { sticks_z_inv<R>(psi + b0 * ldp, ldp, nbb, m, ax, dims[2], S, s); });
+    timed(KC_PLANES_Y_INV, st_b + pl_b, 0, [&] { planes_y<R>(S, P, nbb, m, ax, dims[1], dims[2], +1, nullptr, s); });
+    timed(KC_PLANES_X_VMUL, 2 * pl_b + (double)N * sizeof(R), 0,
+          [&] { planes_x_vmul<R>(P, nbb, m, ax, nyz, v, (R)(1.0 / (double)N), s); });
+    timed(KC_PLANES_Y_FWD, st_b + pl_b, 0, [&] { planes_y<R>(nullptr, P, nbb, m, ax, dims[1], dims[2], -1, S, s); });
+    timed(KC_STICKS_Z_FWD, st_b + 2 * pw_b + 20.0 * K.npw, 0, [&] {
+      sticks_z_fwd<R>(S, nbb, m, ax, dims[2], psi + b0 * ldp, ldp, K.kin, hpsi + b0 * ldh, ldh, s);
     });
   }
 }
@@ -322,21 +359,23 @@ void derek_ctx::apply_h(int k, const cx_t<R>* psi, int64_t ldp, cx_t<R>* hpsi, i
 template <typename R>
 void derek_ctx::density(int k, const cx_t<R>* psi, int64_t ldp, int nbands, const double* occw, bool keep) {
   const KState& K = ks[k];
+  const PwMaps& m = K.maps;
   const Axis<R>* ax = axes<R>();
-  cx_t<R>* g = grid<R>();
+  cx_t<R>* S = sticks<R>();
+  cx_t<R>* P = planes<R>();
   const double sc = sizeof(cx_t<R>);
+  const int64_t nyz = (int64_t)dims[1] * dims[2];
   for (int b0 = 0; b0 < nbands; b0 += batch) {
     const int nbb = std::min(batch, nbands - b0);
-    const double grid_b = (double)nbb * N * sc, pw_b = (double)nbb * K.npw * sc;
+    const double pw_b = (double)nbb * K.npw * sc;
+    const double st_b = (double)nbb * m.nst * dims[2] * sc;
+    const double pl_b = (double)nbb * m.nax * nyz * sc;
     const int kp = (keep || b0 > 0) ? 1 : 0;
-    timed(KC_ZERO, grid_b, 0, [&] { zero<R>(g, nbb * N, s); });
-    timed(KC_SCATTER, 2 * pw_b + 4.0 * K.npw, 0,
-          [&] { scatter<R>(psi + b0 * ldp, ldp, nbb, K.idx, K.npw, g, N, s); });
-    timed(KC_FFT_Z, 2 * grid_b, 0, [&] { fft_z<R>(g, (int64_t)nbb * dims[0] * dims[1], ax[2], +1, s); });
-    timed(KC_FFT_Y, 2 * grid_b, 0, [&] { fft_strided<R>(g, (int64_t)nbb * dims[0], dims[2], ax[1], +1, s); });
-    timed(KC_FFT_X_DENSITY, grid_b + (1 + kp) * 8.0 * N, 0, [&] {
-      fft_strided_density<R>(g, nbb, (int64_t)dims[1] * dims[2], ax[0], occw + b0, rho_out_target(), 1.0, kp, s);
-    });
+    timed(KC_STICKS_Z_INV, pw_b + 12.0 * K.npw + st_b, 0,
+          [&] { sticks_z_inv<R>(psi + b0 * ldp, ldp, nbb, m, ax, dims[2], S, s); });
+    timed(KC_PLANES_Y_INV, st_b + pl_b, 0, [&] { planes_y<R>(S, P, nbb, m, ax, dims[1], dims[2], +1, nullptr, s); });
+    timed(KC_PLANES_X_DENSITY, pl_b + (1 + kp) * 8.0 * N, 0,
+          [&] { planes_x_density<R>(P, nbb, m, ax, nyz, occw + b0, rho_out_target(), kp, s); });
   }
 }
 
@@ -435,8 +474,12 @@ template <typename R> int derek_ctx::davidson(int k, int n_iter, double res_tol)
   overlap<R>(Y, ld, HY, ld, nb, nb, npw, d_S, nb, false);
   DEREK_HIP(hipMemcpyAsync(Srr.data(), d_S, Srr.size() * sizeof(cd), hipMemcpyDeviceToHost, s));
   sync();
-  dense::hermitize(nb, Srr.data(), nb);
-  dense::heev(nb, Srr.data(), nb, eps, Zrr.data(), nb);
+  {
+    const auto t0 = std::chrono::steady_clock::now();
+    dense::hermitize(nb, Srr.data(), nb);
+    dense::heev(nb, Srr.data(), nb, eps, Zrr.data(), nb);
+    t_dense += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
   DEREK_HIP(hipMemcpyAsync(d_U, Zrr.data(), Zrr.size() * sizeof(cd), hipMemcpyHostToDevice, s));
   rotate<R>(Y, ld, d_U, nb, nb, nb, npw, Y2, ld);
   rotate<R>(HY, ld, d_U, nb, nb, nb, npw, HY2, ld);
@@ -472,6 +515,7 @@ template <typename R> int derek_ctx::davidson(int k, int n_iter, double res_tol)
         S[i + (size_t)j * m] *= d[i] * d[j];
         H[i + (size_t)j * m] *= d[i] * d[j];
       }
+    const auto td0 = std::chrono::steady_clock::now();
     dense::hermitize(m, S.data(), m);
     dense::hermitize(m, H.data(), m);
     // drop corrections numerically dependent on the vectors before them
@@ -484,7 +528,9 @@ template <typename R> int derek_ctx::davidson(int k, int n_iter, double res_tol)
         Sa[i + (size_t)j * na] = S[a[i] + (size_t)a[j] * m];
         Ha[i + (size_t)j * na] = H[a[i] + (size_t)a[j] * m];
       }
-    if (!dense::hegv_lowest(na, Sa.data(), Ha.data(), na, nb, w.data(), Ca.data(), na)) break;
+    const bool ok = dense::hegv_lowest(na, Sa.data(), Ha.data(), na, nb, w.data(), Ca.data(), na);
+    t_dense += std::chrono::duration<double>(std::chrono::steady_clock::now() - td0).count();
+    if (!ok) break;
     std::fill(C.begin(), C.end(), cd(0.0));
     for (int j = 0; j < nb; ++j)
       for (int i = 0; i < na; ++i) C[a[i] + (size_t)j * m] = Ca[i + (size_t)j * na] * d[a[i]];
@@ -794,14 +840,10 @@ int32_t derek_hip_init(derek_ctx** out, const derek_dims* d) {
     c->vloc = dmalloc<double>(c->N);
     c->vloc32 = dmalloc<float>(c->N);
     c->cgrid = dmalloc<double2>(c->N);
-    // band batch for the FFT pipelines: keep the batch grids under ~512 MB
-    const size_t per = (size_t)c->N * (c->prec == DEREK_FP64 ? 16 : 8);
-    c->batch = (int)std::max<size_t>(1, std::min<size_t>(8, (512ull << 20) / per));
-    if (c->prec == DEREK_FP64) c->grid64 = dmalloc<double2>((size_t)c->batch * c->N);
-    else c->grid32 = dmalloc<float2>((size_t)c->batch * c->N);
-    // the host-buffer entry points run in FP64 or FP32 per the context; both grids
-    if (!c->grid64) c->grid64 = dmalloc<double2>((size_t)c->batch * c->N);
-    if (!c->grid32) c->grid32 = dmalloc<float2>((size_t)c->batch * c->N);
+    // bands per launch of the pruned FFT pipeline (stick + plane buffers are
+    // allocated for this many bands when the basis is set)
+    c->batch = 16;
+    if (const char* b = std::getenv("DEREK_BAND_BATCH")) c->batch = std::max(1, std::atoi(b));
     *out = c.release();
   });
 }
@@ -829,8 +871,36 @@ int32_t derek_hip_set_basis(derek_ctx* c, int32_t k, int64_t npw, const int32_t*
     K.kin = dmalloc<double>(npw);
     DEREK_HIP(hipMemcpy(K.idx, idx, npw * sizeof(int32_t), hipMemcpyHostToDevice));
     DEREK_HIP(hipMemcpy(K.kin, kin, npw * sizeof(double), hipMemcpyHostToDevice));
+    // sphere maps for the pruned FFT pipeline
+    const PwMapsHost h = build_pw_maps(idx, npw, c->dims[0], c->dims[1], c->dims[2]);
+    dfree(K.mapbuf);
+    const size_t nmap = 3 * (size_t)npw + 2 * (size_t)h.nst + 1 + 2 * (size_t)h.nax + 1;
+    std::vector<int32_t> flat;
+    flat.reserve(nmap);
+    for (const std::vector<int32_t>* v : {&h.perm, &h.pw_z, &h.pw_st, &h.st_pw, &h.st_y, &h.pl_st, &h.ax_x})
+      flat.insert(flat.end(), v->begin(), v->end());
+    K.mapbuf = dmalloc<int32_t>(flat.size());
+    DEREK_HIP(hipMemcpy(K.mapbuf, flat.data(), flat.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    int32_t* q = K.mapbuf;
+    K.maps.npw = h.npw;
+    K.maps.nst = h.nst;
+    K.maps.nax = h.nax;
+    K.maps.perm = q;
+    q += npw;
+    K.maps.pw_z = q;
+    q += npw;
+    K.maps.pw_st = q;
+    q += npw;
+    K.maps.st_pw = q;
+    q += h.nst + 1;
+    K.maps.st_y = q;
+    q += h.nst;
+    K.maps.pl_st = q;
+    q += h.nax + 1;
+    K.maps.ax_x = q;
     K.weight = weight;
     K.set = true;
+    c->ensure_pw_buffers();
   });
 }
 
@@ -1067,6 +1137,8 @@ int32_t derek_hip_scf_step(derek_ctx* c, derek_scf_report* rep) {
     DEREK_REQUIRE(c && rep, "null argument");
     if (!c->scf_ready) throw StateError("scf_step before scf_init");
     c->iscf += 1;
+    const auto tstep0 = std::chrono::steady_clock::now();
+    c->t_dense = 0.0;
     const int nb = c->nb;
     const double dv = c->volume / (double)c->N;
     // V_in = V_ext + V_H[rho_in] + V_xc[rho_in]
@@ -1120,8 +1192,10 @@ int32_t derek_hip_scf_step(derek_ctx* c, derek_scf_report* rep) {
     rep->etot = e_band - e_dc + eh + exc + mts;
     rep->drho = drho;
     rep->max_res = maxres;
+    rep->t_dense = c->t_dense;
     c->mix();
     c->sync();
+    rep->t_step = std::chrono::duration<double>(std::chrono::steady_clock::now() - tstep0).count();
   });
 }
 

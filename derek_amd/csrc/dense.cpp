@@ -8,6 +8,10 @@
 
 #include "common.hpp"
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 namespace derek {
 namespace dense {
 
@@ -65,9 +69,15 @@ void inv_lower(int n, cd* L, int lda) {
 }
 
 // Implicit QL with Wilkinson shifts on a real symmetric tridiagonal matrix
-// (diagonal d, sub-diagonal e[i] = T(i+1,i), e[n-1] = 0); Givens rotations are
-// accumulated into the columns of the real matrix Z (column-major, ldz).
-static void tridiag_ql(int n, double* d, double* e, double* Z, int ldz) {
+// (diagonal d, sub-diagonal e[i] = T(i+1,i), e[n-1] = 0).  Eigenvalues are
+// left in d; every Givens rotation (plane i,i+1, cosine c, sine s) is
+// recorded so that the eigenvectors can be formed afterwards row by row.
+struct Givens {
+  int i;
+  double c, s;
+};
+
+static void tridiag_ql(int n, double* d, double* e, std::vector<Givens>* rot) {
   for (int l = 0; l < n; ++l) {
     int iter = 0;
     int m;
@@ -101,13 +111,7 @@ static void tridiag_ql(int n, double* d, double* e, double* Z, int ldz) {
           p = s * r;
           d[i + 1] = g + p;
           g = c * r - b;
-          double* zi = Z + (size_t)i * ldz;
-          double* zj = Z + (size_t)(i + 1) * ldz;
-          for (int k = 0; k < n; ++k) {
-            const double fz = zj[k];
-            zj[k] = s * zi[k] + c * fz;
-            zi[k] = c * zi[k] - s * fz;
-          }
+          if (rot) rot->push_back({i, c, s});
         }
         if (deflated) continue;
         d[l] -= p;
@@ -118,8 +122,21 @@ static void tridiag_ql(int n, double* d, double* e, double* Z, int ldz) {
   }
 }
 
-void heev(int n, cd* A, int lda, double* w, cd* Z, int ldz) {
-  if (n <= 0) return;
+static int nthreads(int work_rows) {
+#ifdef _OPENMP
+  int t = std::min(16, omp_get_max_threads());
+  return std::max(1, std::min(t, work_rows / 16));
+#else
+  (void)work_rows;
+  return 1;
+#endif
+}
+
+void heev(int n, cd* A, int lda, double* w, cd* Z, int ldz) { heev_lowest(n, A, lda, n, w, Z, ldz); }
+
+void heev_lowest(int n, cd* A, int lda, int nev, double* w, cd* Z, int ldz) {
+  if (n <= 0 || nev <= 0) return;
+  nev = std::min(nev, n);
   // Householder reduction A <- H_k A H_k, H_k = I - 2 v v^H acting on rows/cols k+1..n-1.
   std::vector<cd> V((size_t)n * n, cd(0.0));  // v_k stored in column k (rows k+1..)
   std::vector<cd> sub(n, cd(0.0));            // T(k+1,k)
@@ -145,17 +162,20 @@ void heev(int n, cd* A, int lda, double* w, cd* Z, int ldz) {
     const double ivn = 1.0 / std::sqrt(vn2);
     for (int i = k + 1; i < n; ++i) v[i] *= ivn;
     sub[k] = alpha;
-    // u = A22 v
-    for (int i = k + 1; i < n; ++i) u[i] = 0.0;
-    for (int j = k + 1; j < n; ++j) {
-      const cd vj = v[j];
-      const cd* Aj = A + (size_t)j * lda;
-      for (int i = k + 1; i < n; ++i) u[i] += Aj[i] * vj;
+    const int nt = nthreads(n - k - 1);
+    // u = A22 v, row i = sum_j A(i,j) v(j) = sum_j conj(A(j,i)) v(j) (Hermitian: read column i)
+#pragma omp parallel for num_threads(nt) schedule(static) if (nt > 1)
+    for (int i = k + 1; i < n; ++i) {
+      const cd* Ai = A + (size_t)i * lda;
+      cd acc = 0.0;
+      for (int j = k + 1; j < n; ++j) acc += std::conj(Ai[j]) * v[j];
+      u[i] = acc;
     }
     double gam = 0.0;
     for (int i = k + 1; i < n; ++i) gam += (std::conj(v[i]) * u[i]).real();
     for (int i = k + 1; i < n; ++i) wv[i] = 2.0 * u[i] - 2.0 * gam * v[i];
     // A22 -= v w^H + w v^H
+#pragma omp parallel for num_threads(nt) schedule(static) if (nt > 1)
     for (int j = k + 1; j < n; ++j) {
       const cd cw = std::conj(wv[j]), cv = std::conj(v[j]);
       cd* Aj = A + (size_t)j * lda;
@@ -174,27 +194,58 @@ void heev(int n, cd* A, int lda, double* w, cd* Z, int ldz) {
     e[k] = a;
     ph[k + 1] = a > 0.0 ? ph[k] * (sub[k] / a) : ph[k];
   }
-  std::vector<double> Zr((size_t)n * n, 0.0);
-  for (int i = 0; i < n; ++i) Zr[i + (size_t)i * n] = 1.0;
-  tridiag_ql(n, d.data(), e.data(), Zr.data(), n);
-  // sort ascending
+  std::vector<Givens> rot;
+  rot.reserve((size_t)3 * n * n / 2 + 16);
+  tridiag_ql(n, d.data(), e.data(), &rot);
   std::vector<int> ord(n);
   for (int i = 0; i < n; ++i) ord[i] = i;
   std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return d[a] < d[b]; });
-  for (int j = 0; j < n; ++j) {
+  // Eigenvectors of the real tridiagonal: start from the identity and replay
+  // the rotations; each thread owns a block of rows (the column-pair updates
+  // of one rotation stay contiguous and vectorise).
+  std::vector<double> Zf((size_t)n * n, 0.0);  // column-major n x n
+  for (int i = 0; i < n; ++i) Zf[i + (size_t)i * n] = 1.0;
+  const int ntr = nthreads(n / 2);
+#pragma omp parallel num_threads(ntr) if (ntr > 1)
+  {
+#ifdef _OPENMP
+    const int t = omp_get_thread_num(), T = omp_get_num_threads();
+#else
+    const int t = 0, T = 1;
+#endif
+    const int r0 = (int)((int64_t)n * t / T), r1 = (int)((int64_t)n * (t + 1) / T);
+    for (const Givens& g : rot) {
+      double* zi = Zf.data() + (size_t)g.i * n;
+      double* zj = zi + n;
+      const double c = g.c, sn = g.s;
+      for (int r = r0; r < r1; ++r) {
+        const double fz = zj[r];
+        zj[r] = sn * zi[r] + c * fz;
+        zi[r] = c * zi[r] - sn * fz;
+      }
+    }
+  }
+  std::vector<double> Zr((size_t)nev * n);  // column j = eigenvector ord[j]
+  for (int j = 0; j < nev; ++j) std::copy(Zf.begin() + (size_t)ord[j] * n, Zf.begin() + (size_t)(ord[j] + 1) * n,
+                                          Zr.begin() + (size_t)j * n);
+  for (int j = 0; j < nev; ++j) {
     w[j] = d[ord[j]];
-    const double* zs = Zr.data() + (size_t)ord[j] * n;
     cd* zd = Z + (size_t)j * ldz;
+    const double* zs = Zr.data() + (size_t)j * n;
     for (int i = 0; i < n; ++i) zd[i] = ph[i] * zs[i];
   }
-  // back-transform: Z <- H_0 H_1 ... H_{n-3} Z
-  for (int k = n - 3; k >= 0; --k) {
-    const cd* v = V.data() + (size_t)k * n;
-    bool any = false;
-    for (int i = k + 1; i < n; ++i) any = any || v[i] != cd(0.0);
-    if (!any) continue;
-    for (int j = 0; j < n; ++j) {
-      cd* zj = Z + (size_t)j * ldz;
+  // back-transform: Z <- H_0 H_1 ... H_{n-3} Z, column by column
+  const int ntc = nthreads(nev * 4);
+#pragma omp parallel for num_threads(ntc) schedule(static) if (ntc > 1)
+  for (int j = 0; j < nev; ++j) {
+    cd* zj = Z + (size_t)j * ldz;
+    for (int k = n - 3; k >= 0; --k) {
+      const cd* v = V.data() + (size_t)k * n;
+      if (v[k + 1] == cd(0.0)) {
+        bool any = false;
+        for (int i = k + 2; i < n && !any; ++i) any = v[i] != cd(0.0);
+        if (!any) continue;
+      }
       cd s = 0.0;
       for (int i = k + 1; i < n; ++i) s += std::conj(v[i]) * zj[i];
       s *= 2.0;
@@ -232,6 +283,8 @@ bool hegv_lowest(int n, cd* S, cd* H, int ld, int nev, double* w, cd* C, int ldc
   inv_lower(n, S, ld);  // S <- L^-1
   // T = L^-1 H ; A = T L^-H
   std::vector<cd> T((size_t)n * n, cd(0.0)), A((size_t)n * n, cd(0.0));
+  const int ntp = nthreads(n * 4);
+#pragma omp parallel for num_threads(ntp) schedule(static) if (ntp > 1)
   for (int j = 0; j < n; ++j) {
     cd* Tj = T.data() + (size_t)j * n;
     for (int k = 0; k < n; ++k) {
@@ -241,6 +294,7 @@ bool hegv_lowest(int n, cd* S, cd* H, int ld, int nev, double* w, cd* C, int ldc
     }
   }
   // A(:,j) = sum_k T(:,k) conj(Linv(j,k)), Linv(j,k) nonzero for k <= j
+#pragma omp parallel for num_threads(ntp) schedule(static) if (ntp > 1)
   for (int j = 0; j < n; ++j) {
     cd* Aj = A.data() + (size_t)j * n;
     for (int k = 0; k <= j; ++k) {
@@ -249,12 +303,13 @@ bool hegv_lowest(int n, cd* S, cd* H, int ld, int nev, double* w, cd* C, int ldc
       for (int i = 0; i < n; ++i) Aj[i] += Tk[i] * c;
     }
   }
-  std::vector<double> ww(n);
-  std::vector<cd> Z((size_t)n * n);
-  heev(n, A.data(), n, ww.data(), Z.data(), n);
-  // C = L^-H Z(:, :nev); (L^-H)(i,k) = conj(Linv(k,i)), nonzero for k >= i
+  std::vector<double> ww(nev);
+  std::vector<cd> Z((size_t)n * nev);
+  heev_lowest(n, A.data(), n, nev, ww.data(), Z.data(), n);
+  // C = L^-H Z; (L^-H)(i,k) = conj(Linv(k,i)), nonzero for k >= i
+  const int nt = nthreads(nev * 4);
+#pragma omp parallel for num_threads(nt) schedule(static) if (nt > 1)
   for (int j = 0; j < nev; ++j) {
-    w[j] = ww[j];
     const cd* zj = Z.data() + (size_t)j * n;
     cd* cj = C + (size_t)j * ldc;
     for (int i = 0; i < n; ++i) {
@@ -263,6 +318,7 @@ bool hegv_lowest(int n, cd* S, cd* H, int ld, int nev, double* w, cd* C, int ldc
       cj[i] = s;
     }
   }
+  for (int j = 0; j < nev; ++j) w[j] = ww[j];
   return true;
 }
 

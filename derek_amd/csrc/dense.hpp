@@ -20,6 +20,8 @@ void inv_lower(int n, cd* L, int lda);
 // Eigenvalues ascending in w, orthonormal eigenvectors in the columns of Z.
 // A is destroyed.  Throws derek::Error if QL does not converge.
 void heev(int n, cd* A, int lda, double* w, cd* Z, int ldz);
+// Lowest nev eigenpairs only (Z: n x nev).
+void heev_lowest(int n, cd* A, int lda, int nev, double* w, cd* Z, int ldz);
 // Lowest nev solutions of H c = lambda S c with C^H S C = I.  S and H are
 // destroyed.  Returns false if S is not positive definite.
 bool hegv_lowest(int n, cd* S, cd* H, int ld, int nev, double* w, cd* C, int ldc);

@@ -30,8 +30,8 @@ EXPORTS = (
     "derek_hip_profile_enable", "derek_hip_profile_reset", "derek_hip_profile_read",
 )
 
-KERNEL_CLASSES = ("zero", "scatter", "fft_z", "fft_y", "fft_x_vmul", "gather_kin", "fft_x_density",
-                  "overlap", "rotate", "fft_x")
+KERNEL_CLASSES = ("sticks_z_inv", "planes_y_inv", "planes_x_vmul", "planes_y_fwd", "sticks_z_fwd", "planes_x_density",
+                  "overlap", "rotate", "fft_z", "fft_y", "fft_x")
 
 
 class DerekError(RuntimeError):
@@ -55,7 +55,7 @@ class ScfReport(C.Structure):
     _fields_ = [("iscf", C.c_int32), ("ndav", C.c_int32), ("etot", C.c_double), ("e_band", C.c_double),
                 ("e_hartree", C.c_double), ("e_xc", C.c_double), ("e_dc", C.c_double),
                 ("minus_ts", C.c_double), ("fermi", C.c_double), ("drho", C.c_double),
-                ("max_res", C.c_double)]
+                ("max_res", C.c_double), ("t_dense", C.c_double), ("t_step", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -69,6 +69,8 @@ typedef struct {
   double fermi;
   double drho;      /* || rho_out - rho_in ||_2 (bohr^-3/2) */
   double max_res;   /* largest Davidson residual norm */
+  double t_dense;   /* seconds spent in the host FP64 subspace solves (Cholesky, eigensolves) */
+  double t_step;    /* wall seconds of the step */
 } derek_scf_report;
 
 /* ---- library ------------------------------------------------------------ */
