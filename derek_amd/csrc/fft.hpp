@@ -27,21 +27,58 @@ struct FftPlan {
   int r[kMaxStages];
 };
 
+// Radices with a register DFT; composite ones (6, 9, 12, 16) are built from
+// the prime-power kernels with internal twiddles.
+constexpr int kRadices[12] = {16, 12, 9, 8, 6, 5, 4, 3, 2, 7, 11, 13};
+
+// Depth-first search for the factorisation with the fewest stages, ties
+// broken by the largest smallest factor (balanced stages).
+constexpr void plan_search(int m, int depth, int* cur, int* best, int& best_len, int& best_min) {
+  if (m == 1) {
+    int mn = 1 << 30;
+    for (int i = 0; i < depth; ++i) mn = cur[i] < mn ? cur[i] : mn;
+    if (depth < best_len || (depth == best_len && mn > best_min)) {
+      best_len = depth;
+      best_min = mn;
+      for (int i = 0; i < depth; ++i) best[i] = cur[i];
+    }
+    return;
+  }
+  if (depth + 1 > best_len || depth >= kMaxStages) return;
+  for (int q = 0; q < 12; ++q) {
+    const int r = kRadices[q];
+    if (m % r == 0) {
+      cur[depth] = r;
+      plan_search(m / r, depth + 1, cur, best, best_len, best_min);
+    }
+  }
+}
+
 constexpr FftPlan make_plan(int n) {
   FftPlan p{n, 0, {}};
-  int m = n;
   if (n < 1) {
     p.ns = -1;
     return p;
   }
-  while (m % 8 == 0 && p.ns < kMaxStages) { p.r[p.ns++] = 8; m /= 8; }
-  while (m % 4 == 0 && p.ns < kMaxStages) { p.r[p.ns++] = 4; m /= 4; }
-  const int order[6] = {3, 5, 2, 7, 11, 13};
-  for (int q = 0; q < 6; ++q) {
-    const int f = order[q];
-    while (m % f == 0 && p.ns < kMaxStages) { p.r[p.ns++] = f; m /= f; }
+  int cur[kMaxStages] = {};
+  int best[kMaxStages] = {};
+  int best_len = kMaxStages + 1, best_min = 0;
+  plan_search(n, 0, cur, best, best_len, best_min);
+  if (best_len > kMaxStages) {
+    p.ns = -1;
+    return p;
   }
-  if (m != 1) p.ns = -1;
+  p.ns = best_len;
+  // larger radix first
+  for (int i = 0; i < best_len; ++i) p.r[i] = best[i];
+  for (int i = 0; i < best_len; ++i)
+    for (int j = i + 1; j < best_len; ++j)
+      if (p.r[j] > p.r[i]) {
+        const int t = p.r[i];
+        p.r[i] = p.r[j];
+        p.r[j] = t;
+      }
+  if (n == 1) p.ns = 0;
   return p;
 }
 
@@ -135,12 +172,44 @@ template <int R, int S, typename C> __device__ __forceinline__ void dft_direct(C
   for (int k = 0; k < R; ++k) v[k] = out[k];
 }
 
+template <int R, int S, typename C> __device__ __forceinline__ void dftR(C* v, const C* tw, int N);
+
+// Composite R = A*B in registers: n = B*a + b, k = ka + A*kb;
+// B DFTs of length A, twiddles exp(S*2*pi*i*b*ka/R), A DFTs of length B.
+template <int A, int B, int S, typename C> __device__ __forceinline__ void dft_comp(C* v, const C* tw, int N) {
+  constexpr int R = A * B;
+  const int step = N / R;
+  C y[B][A];
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    C t[A];
+#pragma unroll
+    for (int a = 0; a < A; ++a) t[a] = v[B * a + b];
+    dftR<A, S>(t, tw, N);
+#pragma unroll
+    for (int ka = 0; ka < A; ++ka) y[b][ka] = (b * ka) % R ? cmul(t[ka], twid<S>(tw, ((b * ka) % R) * step)) : t[ka];
+  }
+#pragma unroll
+  for (int ka = 0; ka < A; ++ka) {
+    C t[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) t[b] = y[b][ka];
+    dftR<B, S>(t, tw, N);
+#pragma unroll
+    for (int kb = 0; kb < B; ++kb) v[ka + A * kb] = t[kb];
+  }
+}
+
 template <int R, int S, typename C> __device__ __forceinline__ void dftR(C* v, const C* tw, int N) {
   if constexpr (R == 2) dft2<S>(v);
   else if constexpr (R == 3) dft3<S>(v);
   else if constexpr (R == 4) dft4<S>(v);
   else if constexpr (R == 5) dft5<S>(v);
   else if constexpr (R == 8) dft8<S>(v);
+  else if constexpr (R == 6) dft_comp<3, 2, S>(v, tw, N);
+  else if constexpr (R == 9) dft_comp<3, 3, S>(v, tw, N);
+  else if constexpr (R == 12) dft_comp<4, 3, S>(v, tw, N);
+  else if constexpr (R == 16) dft_comp<4, 4, S>(v, tw, N);
   else dft_direct<R, S>(v, tw, N);
 }
 
@@ -207,9 +276,13 @@ __device__ __forceinline__ C* fft_lds(C* buf0, C* buf1, const C* tw, const FftPl
         case 3: fft_stage<3, S, TILE, LINES, NT>(src, dst, Ns, N, lines, tw); break;
         case 4: fft_stage<4, S, TILE, LINES, NT>(src, dst, Ns, N, lines, tw); break;
         case 5: fft_stage<5, S, TILE, LINES, NT>(src, dst, Ns, N, lines, tw); break;
+        case 6: fft_stage<6, S, TILE, LINES, NT>(src, dst, Ns, N, lines, tw); break;
         case 7: fft_stage<7, S, TILE, LINES, NT>(src, dst, Ns, N, lines, tw); break;
         case 8: fft_stage<8, S, TILE, LINES, NT>(src, dst, Ns, N, lines, tw); break;
+        case 9: fft_stage<9, S, TILE, LINES, NT>(src, dst, Ns, N, lines, tw); break;
         case 11: fft_stage<11, S, TILE, LINES, NT>(src, dst, Ns, N, lines, tw); break;
+        case 12: fft_stage<12, S, TILE, LINES, NT>(src, dst, Ns, N, lines, tw); break;
+        case 16: fft_stage<16, S, TILE, LINES, NT>(src, dst, Ns, N, lines, tw); break;
         default: fft_stage<13, S, TILE, LINES, NT>(src, dst, Ns, N, lines, tw); break;
       }
       __syncthreads();

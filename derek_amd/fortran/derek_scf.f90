@@ -1,0 +1,250 @@
+!-------------------------------------------------------------------------------
+! derek_scf: Fortran SCF driver on the gfx950 library.
+!
+! Usage: derek_scf <seed> [-c]     reads <seed>.info
+!   -c   check mode (like the reference's `derek -c`, io.f90:1074-1080):
+!        parse the input, build the basis, report the grids and stop (no GPU).
+!
+! Input: the reference's `key : value` lines (io.f90:360-831; `!`/`#`
+! comments) and the $DATA lattice / gaussian_pot blocks (io.f90:1765-1935),
+! for the keys the SCF needs: cut_off_energy, g_fine_scale, kpt_mp_grid,
+! n_electrons, electronic_temp (Kelvin, io.f90:340), energy_tol, max_scf,
+! unit_energy / unit_length (Ha and bohr only: their factors are exactly 1),
+! plus nbands, cos_pot_amp, wfn_seed and precision.
+! The loop, its table and the stop test |E_i - E_{i-1}| < energy_tol follow
+! electronic_scf (electronic.f90:86-175).
+!-------------------------------------------------------------------------------
+program derek_scf
+  use iso_c_binding
+  use derek_hip_m
+  implicit none
+
+  real(c_double), parameter :: kelvin_ha = 3.1668115634556076d-6  ! k_B / E_h (CODATA 2018)
+  character(len=256) :: seed, arg, line, key, val
+  real(c_double) :: cell(3, 3), inv_cell(3, 3), ecut = 200.0d0, g_fine = 2.0d0, temp_k = 0.0d0
+  real(c_double) :: energy_tol = 1.0d-7, cos_amp = 0.0d0, nelec = 1.0d0
+  integer(c_int32_t) :: mp(3) = (/1, 1, 1/), max_scf = 40, nbands = 0, prec = DEREK_FP64
+  integer(c_int64_t) :: wfn_seed = 1
+  real(c_double), allocatable :: centres(:, :), heights(:), widths(:), kfrac(:, :), kin(:), gsq(:), vext(:)
+  real(c_double), allocatable :: eig(:, :)
+  integer(c_int32_t), allocatable :: fft_index(:)
+  integer(c_int32_t) :: std_dims(3), fine_dims(3), nk, k, nwells = 0, iscf
+  integer(c_int64_t) :: npw, max_npw, ngrid
+  logical :: check_only = .false., have_cell = .false.
+  integer :: u, ios, i, n
+  type(c_ptr) :: ctx
+  type(derek_dims) :: dims
+  type(derek_scf_params) :: par
+  type(derek_scf_report) :: rep
+  real(c_double) :: e_old, t0, t1
+  logical :: converged
+
+  call get_command_argument(1, seed)
+  if (len_trim(seed) == 0) then
+     write(*, *) 'usage: derek_scf <seed> [-c]'
+     error stop 2
+  end if
+  do i = 2, command_argument_count()
+     call get_command_argument(i, arg)
+     if (trim(arg) == '-c') check_only = .true.
+  end do
+  cell = 0.0d0
+  allocate(centres(3, 0), heights(0), widths(0))
+
+  open(newunit=u, file=trim(seed)//'.info', status='old', action='read', iostat=ios)
+  if (ios /= 0) then
+     write(*, *) "Error: file '"//trim(seed)//".info' does not exist."
+     error stop 1
+  end if
+  do
+     read(u, '(a)', iostat=ios) line
+     if (ios /= 0) exit
+     line = adjustl(line)
+     if (len_trim(line) == 0) cycle
+     if (line(1:1) == '!' .or. line(1:1) == '#') cycle
+     if (index(lower(line), '$data') == 1) then
+        call read_block(u, lower(line))
+        cycle
+     end if
+     i = scan(line, ':=')
+     if (i == 0) cycle
+     key = lower(adjustl(line(1:i-1)))
+     val = adjustl(line(i+1:))
+     select case (trim(key))
+     case ('cut_off_energy'); read(val, *) ecut
+     case ('g_fine_scale'); read(val, *) g_fine
+     case ('kpt_mp_grid'); read(val, *) mp
+     case ('n_electrons'); read(val, *) nelec
+     case ('nbands'); read(val, *) nbands
+     case ('electronic_temp'); read(val, *) temp_k
+     case ('energy_tol'); read(val, *) energy_tol
+     case ('max_scf'); read(val, *) max_scf
+     case ('cos_pot_amp'); read(val, *) cos_amp
+     case ('wfn_seed'); read(val, *) wfn_seed
+     case ('precision')
+        if (trim(lower(val)) == 'mixed') prec = DEREK_MIXED
+     case ('unit_energy')
+        if (trim(val) /= 'Ha') call fail('only unit_energy : Ha is supported by this driver')
+     case ('unit_length')
+        if (trim(val) /= 'bohr') call fail('only unit_length : bohr is supported by this driver')
+     case default
+        continue
+     end select
+  end do
+  close(u)
+  if (.not. have_cell) call fail(' No lattice provided')
+  if (nbands <= 0) nbands = int(nelec / 2.0d0) + 4
+
+  call derek_check(derek_basis_grid_dims(cell, ecut, g_fine, std_dims, fine_dims), 'basis_grid_dims')
+  call derek_check(derek_basis_inv_cell(cell, inv_cell), 'basis_inv_cell')
+  nk = mp(1) * mp(2) * mp(3)
+  if (nk <= 0) call fail(' invalid number of kpoints')
+  allocate(kfrac(nk, 3))
+  call derek_check(derek_basis_kpoints(mp, kfrac, nk), 'basis_kpoints')
+  ngrid = int(fine_dims(1), c_int64_t) * fine_dims(2) * fine_dims(3)
+  max_npw = 0
+  do k = 1, nk
+     call derek_check(derek_basis_sphere(inv_cell, fine_dims, kfrac(k, :), ecut, 0_c_int64_t, npw, fft_dummy(), &
+          kin_dummy()), 'basis_sphere')
+     max_npw = max(max_npw, npw)
+  end do
+  write(*, '(a,3i5)') ' Standard FFT grid      :', std_dims
+  write(*, '(a,3i5)') ' Fine FFT grid          :', fine_dims
+  write(*, '(a,i8)') ' k-points               :', nk
+  write(*, '(a,i8)') ' max plane waves per k  :', max_npw
+  write(*, '(a,i8)') ' bands                  :', nbands
+  if (check_only) stop
+
+  dims%ngrid = fine_dims
+  dims%nk = nk
+  dims%nbands = nbands
+  dims%precision = prec
+  dims%volume = cell(1, 1) * cell(2, 2) * cell(3, 3)
+  dims%max_npw = max_npw
+  call derek_check(derek_hip_init(ctx, dims), 'hip_init')
+  allocate(fft_index(max_npw), kin(max_npw), gsq(ngrid), vext(ngrid), eig(nbands, nk))
+  do k = 1, nk
+     call derek_check(derek_basis_sphere(inv_cell, fine_dims, kfrac(k, :), ecut, max_npw, npw, fft_index, kin), &
+          'basis_sphere')
+     call derek_check(derek_hip_set_basis(ctx, k - 1, npw, fft_index, kin, 1.0d0 / nk), 'hip_set_basis')
+  end do
+  call derek_check(derek_basis_gsq(inv_cell, fine_dims, gsq), 'basis_gsq')
+  call derek_check(derek_hip_set_gsq(ctx, gsq), 'hip_set_gsq')
+  n = size(heights)
+  call derek_check(derek_gaussian_pot(cell, fine_dims, n, centres, heights, widths, cos_amp, vext), &
+       'gaussian_pot')
+  call derek_check(derek_hip_set_vext(ctx, vext), 'hip_set_vext')
+
+  par%nelec = nelec
+  par%kT = temp_k * kelvin_ha
+  par%mix_beta = 0.5d0
+  par%mix_q0 = 1.0d0
+  par%n_david_first = 4
+  par%n_david = 2
+  par%history = 8
+  par%reserved = 0
+  par%seed = wfn_seed
+  call derek_check(derek_hip_scf_init(ctx, par), 'hip_scf_init')
+
+  write(*, *) '+--------+-------------------+-------------------+---------+ <-- SCF'
+  write(*, *) '|  SCF   |      Energy       |   Energy Change   |  Time   | <-- SCF'
+  write(*, *) '|  Step  |       (Ha)        |        (Ha)       |   (s)   | <-- SCF'
+  write(*, *) '+--------+-------------------+-------------------+---------+ <-- SCF'
+  call cpu_time(t0)
+  converged = .false.
+  e_old = 0.0d0
+  do iscf = 1, max_scf
+     call derek_check(derek_hip_scf_step(ctx, rep), 'hip_scf_step')
+     call cpu_time(t1)
+     if (iscf == 1) then
+        write(*, 11) iscf, rep%etot, t1 - t0
+     else
+        write(*, 12) iscf, rep%etot, rep%etot - e_old, t1 - t0
+        if (abs(rep%etot - e_old) < energy_tol) converged = .true.
+     end if
+     e_old = rep%etot
+     if (converged) exit
+  end do
+  write(*, *) '+--------+-------------------+-------------------+---------+ <-- SCF'
+  call derek_check(derek_hip_get_eigenvalues(ctx, eig), 'hip_get_eigenvalues')
+  write(*, '(a,f20.12)') ' Final energy (Ha)      :', rep%etot
+  write(*, '(a,f20.12)') ' Fermi level (Ha)       :', rep%fermi
+  write(*, '(a,l2)') ' Converged              :', converged
+  write(*, '(a,4f14.8)') ' lowest eigenvalues (Ha):', eig(1:min(4, nbands), 1)
+  call derek_check(derek_hip_destroy(ctx), 'hip_destroy')
+11 format(1x,'|',T6,I05,T11,"|",T15,ES16.8E3,T31,"|",T51,'|',T53,f8.2,T61,'| <-- SCF')
+12 format(1x,'|',T6,I05,T11,"|",T15,ES16.8E3,T31,"|",T35,ES16.8E3,T51,'|',T53,f8.2,T61,'| <-- SCF')
+
+contains
+
+  function lower(s) result(t)
+    character(*), intent(in) :: s
+    character(len=len(s)) :: t
+    integer :: j
+    t = s
+    do j = 1, len(s)
+       if (t(j:j) >= 'A' .and. t(j:j) <= 'Z') t(j:j) = achar(iachar(t(j:j)) + 32)
+    end do
+  end function lower
+
+  subroutine fail(msg)
+    character(*), intent(in) :: msg
+    write(*, *) 'Error: '//trim(msg)
+    error stop 1
+  end subroutine fail
+
+  function fft_dummy() result(p)
+    integer(c_int32_t) :: p(1)
+    p = 0
+  end function fft_dummy
+
+  function kin_dummy() result(p)
+    real(c_double) :: p(1)
+    p = 0.0d0
+  end function kin_dummy
+
+  ! $DATA <type> ... $ENDDATA <type>; lattice rows are cell(row, :)
+  ! (io.f90:1859-1878); gaussian_pot rows are cx cy cz height width.
+  subroutine read_block(unit, head)
+    integer, intent(in) :: unit
+    character(*), intent(in) :: head
+    character(len=256) :: l, kind
+    integer :: r, st
+    real(c_double) :: row(5)
+    real(c_double), allocatable :: tc(:, :), th(:), tw(:)
+    read(head(6:), *) kind
+    r = 0
+    do
+       read(unit, '(a)', iostat=st) l
+       if (st /= 0) call fail('Block '//trim(kind)//' not closed')
+       l = adjustl(l)
+       if (index(lower(l), '$enddata') == 1) exit
+       if (len_trim(l) == 0 .or. l(1:1) == '!' .or. l(1:1) == '#') cycle
+       r = r + 1
+       select case (trim(kind))
+       case ('lattice')
+          if (r > 3) call fail('LATTICE must have 3 lines')
+          read(l, *) cell(r, 1), cell(r, 2), cell(r, 3)
+          have_cell = .true.
+       case ('gaussian_pot')
+          read(l, *) row
+          allocate(tc(3, r), th(r), tw(r))
+          if (r > 1) then
+             tc(:, 1:r-1) = centres
+             th(1:r-1) = heights
+             tw(1:r-1) = widths
+          end if
+          tc(:, r) = row(1:3)
+          th(r) = row(4)
+          tw(r) = row(5)
+          call move_alloc(tc, centres)
+          call move_alloc(th, heights)
+          call move_alloc(tw, widths)
+       case default
+          call fail(' Unknown block type '//trim(kind))
+       end select
+    end do
+    if (trim(kind) == 'lattice' .and. r /= 3) call fail('LATTICE must have 3 lines')
+  end subroutine read_block
+
+end program derek_scf

@@ -23,7 +23,7 @@ def rnd_c(rng, *shape):
     return rng.normal(size=shape) + 1j * rng.normal(size=shape)
 
 
-@pytest.mark.parametrize("dims", [(32, 32, 32), (8, 12, 10), (14, 22, 26), (15, 27, 20), (72, 72, 72),
+@pytest.mark.parametrize("dims", [(32, 32, 32), (8, 12, 10), (14, 22, 26), (15, 27, 20), (30, 16, 24), (72, 72, 72),
                                   (108, 108, 108), (144, 144, 144)])
 @pytest.mark.parametrize("precision", [lib.DEREK_FP64, lib.DEREK_MIXED])
 def test_fft3d(dims, precision):
