@@ -12,6 +12,7 @@
 #include "../../include/derek_hip.h"
 #include "basis.hpp"
 #include "dense.hpp"
+#include "dense_gpu.hpp"
 #include "launch.hpp"
 #include "pw_fft.hpp"
 
@@ -206,6 +207,20 @@ struct derek_ctx {
   double2* d_S = nullptr;
   double2* d_H = nullptr;
   double2* d_U = nullptr;
+  // GPU subspace-solver scratch (sized for m = 2 nbands)
+  double* d_dsc = nullptr;
+  double* d_w = nullptr;
+  double* d_hwork = nullptr;
+  int* d_sact = nullptr;
+  int* d_na = nullptr;
+  double2 *d_Lb = nullptr, *d_Lc = nullptr, *d_Hc = nullptr, *d_Linv = nullptr, *d_LinvH = nullptr, *d_T = nullptr,
+          *d_A = nullptr, *d_Zs = nullptr, *d_Ca = nullptr;
+  int dense_cap = 0;
+  int sub_cap = 0;  // columns d_S / d_H / d_U can hold (sub_cap x sub_cap)
+  void ensure_sub(int m);
+  void ensure_dense(int m, int nev);
+  void rr_gpu(int n, double* w_host);
+  bool gen_gpu(int m, int nev, double* w_host);
   double fermi = 0.0, minus_ts = 0.0;
   double t_dense = 0.0;  // host seconds in the subspace solves of the current step
   bool have_rho_out = false;
@@ -300,6 +315,9 @@ derek_ctx::~derek_ctx() {
   for (void* p : {(void*)gsq, (void*)vext, (void*)vloc, (void*)vloc32, (void*)cgrid,
                   (void*)rho_in, (void*)rho_out, (void*)vh, (void*)vxc, (void*)vhxc, (void*)tmp1, (void*)tmp2,
                   (void*)d_eps, (void*)d_ekin, (void*)d_occw, (void*)d_act, (void*)d_S, (void*)d_H, (void*)d_U})
+    dfree(p);
+  for (void* p : {(void*)d_dsc, (void*)d_w, (void*)d_hwork, (void*)d_sact, (void*)d_na, (void*)d_Lb, (void*)d_Lc,
+                  (void*)d_Hc, (void*)d_Linv, (void*)d_LinvH, (void*)d_T, (void*)d_A, (void*)d_Zs, (void*)d_Ca})
     dfree(p);
   for (double* p : h_rin) dfree(p);
   for (double* p : h_res) dfree(p);
@@ -469,23 +487,18 @@ template <typename R> int derek_ctx::davidson(int k, int n_iter, double res_tol)
   double* rn = res.data() + (size_t)k * nb;
   std::vector<double> ekin(nb);
   apply_h<R>(k, Y, ld, HY, ld, nb);
-  // Rayleigh-Ritz
-  std::vector<cd> Srr((size_t)nb * nb), Zrr((size_t)nb * nb);
+  // Rayleigh-Ritz (GPU FP64 eigensolver)
   overlap<R>(Y, ld, HY, ld, nb, nb, npw, d_S, nb, false);
-  DEREK_HIP(hipMemcpyAsync(Srr.data(), d_S, Srr.size() * sizeof(cd), hipMemcpyDeviceToHost, s));
-  sync();
   {
     const auto t0 = std::chrono::steady_clock::now();
-    dense::hermitize(nb, Srr.data(), nb);
-    dense::heev(nb, Srr.data(), nb, eps, Zrr.data(), nb);
+    rr_gpu(nb, eps);
     t_dense += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
-  DEREK_HIP(hipMemcpyAsync(d_U, Zrr.data(), Zrr.size() * sizeof(cd), hipMemcpyHostToDevice, s));
   rotate<R>(Y, ld, d_U, nb, nb, nb, npw, Y2, ld);
   rotate<R>(HY, ld, d_U, nb, nb, nb, npw, HY2, ld);
   std::swap(Y, Y2);
   std::swap(HY, HY2);
-  band_stats<R>(k, Y, HY, eps, ekin.data(), rn);  // synchronises: Zrr may be released after this
+  band_stats<R>(k, Y, HY, eps, ekin.data(), rn);
   int it = 0;
   for (; it < n_iter; ++it) {
     double rmax = 0.0;
@@ -504,37 +517,11 @@ template <typename R> int derek_ctx::davidson(int k, int n_iter, double res_tol)
     const int m = nb + nact;
     overlap<R>(Y, ld, Y, ld, m, m, npw, d_S, m, true);
     overlap<R>(Y, ld, HY, ld, m, m, npw, d_H, m, false);
-    std::vector<cd> S((size_t)m * m), H((size_t)m * m), C((size_t)m * nb);
-    DEREK_HIP(hipMemcpyAsync(S.data(), d_S, S.size() * sizeof(cd), hipMemcpyDeviceToHost, s));
-    DEREK_HIP(hipMemcpyAsync(H.data(), d_H, H.size() * sizeof(cd), hipMemcpyDeviceToHost, s));
-    sync();
-    std::vector<double> d(m), w(nb);
-    for (int i = 0; i < m; ++i) d[i] = 1.0 / std::sqrt(S[i + (size_t)i * m].real());
-    for (int j = 0; j < m; ++j)
-      for (int i = 0; i < m; ++i) {
-        S[i + (size_t)j * m] *= d[i] * d[j];
-        H[i + (size_t)j * m] *= d[i] * d[j];
-      }
+    std::vector<double> w(nb);
     const auto td0 = std::chrono::steady_clock::now();
-    dense::hermitize(m, S.data(), m);
-    dense::hermitize(m, H.data(), m);
-    // drop corrections numerically dependent on the vectors before them
-    const std::vector<int> a = dense::cholesky_active(m, S.data(), m, 1e-8);
-    const int na = (int)a.size();
-    if (na < nb) break;
-    std::vector<cd> Sa((size_t)na * na), Ha((size_t)na * na), Ca((size_t)na * nb);
-    for (int j = 0; j < na; ++j)
-      for (int i = 0; i < na; ++i) {
-        Sa[i + (size_t)j * na] = S[a[i] + (size_t)a[j] * m];
-        Ha[i + (size_t)j * na] = H[a[i] + (size_t)a[j] * m];
-      }
-    const bool ok = dense::hegv_lowest(na, Sa.data(), Ha.data(), na, nb, w.data(), Ca.data(), na);
+    const bool ok = gen_gpu(m, nb, w.data());  // C (m x nb) in d_U
     t_dense += std::chrono::duration<double>(std::chrono::steady_clock::now() - td0).count();
     if (!ok) break;
-    std::fill(C.begin(), C.end(), cd(0.0));
-    for (int j = 0; j < nb; ++j)
-      for (int i = 0; i < na; ++i) C[a[i] + (size_t)j * m] = Ca[i + (size_t)j * na] * d[a[i]];
-    DEREK_HIP(hipMemcpyAsync(d_U, C.data(), C.size() * sizeof(cd), hipMemcpyHostToDevice, s));
     rotate<R>(Y, ld, d_U, m, m, nb, npw, Y2, ld);
     rotate<R>(HY, ld, d_U, m, m, nb, npw, HY2, ld);
     std::swap(Y, Y2);
@@ -549,21 +536,85 @@ template <typename R> int derek_ctx::davidson(int k, int n_iter, double res_tol)
   return it;
 }
 
+void derek_ctx::ensure_sub(int m) {
+  if (m <= sub_cap) return;
+  dfree(d_S);
+  dfree(d_H);
+  dfree(d_U);
+  d_S = dmalloc<double2>((size_t)m * m);
+  d_H = dmalloc<double2>((size_t)m * m);
+  d_U = dmalloc<double2>((size_t)m * m);
+  sub_cap = m;
+}
+
+void derek_ctx::ensure_dense(int m, int nev) {
+  if (m <= dense_cap) return;
+  for (void* p : {(void*)d_dsc, (void*)d_w, (void*)d_hwork, (void*)d_sact, (void*)d_na, (void*)d_Lb, (void*)d_Lc,
+                  (void*)d_Hc, (void*)d_Linv, (void*)d_LinvH, (void*)d_T, (void*)d_A, (void*)d_Zs, (void*)d_Ca})
+    dfree(p);
+  const size_t mm = (size_t)m * m;
+  d_dsc = dmalloc<double>(m);
+  d_w = dmalloc<double>(m);
+  d_hwork = dmalloc<double>(heev_work_doubles(m, m));
+  d_sact = dmalloc<int>(m);
+  d_na = dmalloc<int>(1);
+  for (double2** q : {&d_Lb, &d_Lc, &d_Hc, &d_Linv, &d_LinvH, &d_T, &d_A, &d_Zs, &d_Ca}) *q = dmalloc<double2>(mm);
+  dense_cap = m;
+  (void)nev;
+}
+
+// Eigenpairs of the Hermitian n x n matrix in d_S: vectors to d_U, values to w_host.
+void derek_ctx::rr_gpu(int n, double* w_host) {
+  ensure_dense(n, n);
+  hermitize_gpu(d_S, n, s);
+  heev_gpu(d_S, n, 0, n, n, d_w, 0, d_U, n, 0, d_hwork, 0, 1, s);
+  DEREK_HIP(hipMemcpyAsync(w_host, d_w, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  sync();
+}
+
+// Lowest nev solutions of d_H c = w d_S c (m x m): C (m x nev) to d_U, values to
+// w_host.  Columns numerically dependent on earlier ones are dropped first.
+bool derek_ctx::gen_gpu(int m, int nev, double* w_host) {
+  ensure_dense(m, nev);
+  hermitize_gpu(d_S, m, s);
+  hermitize_gpu(d_H, m, s);
+  hegv_gpu_prepare(d_S, m, d_dsc, s);
+  chol_drop_gpu(d_S, m, 1e-8, d_T, d_Lb, d_sact, d_na, s);
+  int na = 0;
+  DEREK_HIP(hipMemcpyAsync(&na, d_na, sizeof(int), hipMemcpyDeviceToHost, s));
+  sync();
+  if (na < nev) return false;
+  chol_compact_gpu(d_Lb, m, d_sact, d_na, d_H, m, d_dsc, d_Lc, d_Hc, s);
+  trinv_gpu(d_Lc, d_na, na, d_Linv, s);
+  rotate_f64(d_Linv, na, d_Hc, na, na, na, na, d_T, na, s);  // T = L^-1 H
+  conj_transpose_gpu(d_Linv, d_na, na, d_LinvH, s);
+  rotate_f64(d_T, na, d_LinvH, na, na, na, na, d_A, na, s);  // A = L^-1 H L^-H
+  hermitize_gpu(d_A, na, s);
+  heev_gpu(d_A, na, 0, na, nev, d_w, 0, d_Zs, na, 0, d_hwork, 0, 1, s);
+  rotate_f64(d_LinvH, na, d_Zs, na, na, nev, na, d_Ca, na, s);  // C_active = L^-H Z
+  expand_rows_gpu(d_Ca, d_na, d_sact, d_dsc, m, nev, d_U, s);
+  DEREK_HIP(hipMemcpyAsync(w_host, d_w, nev * sizeof(double), hipMemcpyDeviceToHost, s));
+  sync();
+  return true;
+}
+
 template <typename R> void derek_ctx::orthonormalise(int k) {
   KState& K = ks[k];
   auto Y = reinterpret_cast<cx_t<R>*>(K.Y);
   auto Y2 = reinterpret_cast<cx_t<R>*>(K.Y2);
+  ensure_dense(nb, nb);
   overlap<R>(Y, K.ld, Y, K.ld, nb, nb, K.npw, d_S, nb, true);
-  std::vector<cd> S((size_t)nb * nb), U((size_t)nb * nb, cd(0.0));
-  DEREK_HIP(hipMemcpyAsync(S.data(), d_S, S.size() * sizeof(cd), hipMemcpyDeviceToHost, s));
+  // S' = D S D = L L^H (unit diagonal), X <- X D L^-H
+  hegv_gpu_prepare(d_S, nb, d_dsc, s);
+  chol_drop_gpu(d_S, nb, 0.0, d_T, d_Lb, d_sact, d_na, s);
+  int na = 0;
+  DEREK_HIP(hipMemcpyAsync(&na, d_na, sizeof(int), hipMemcpyDeviceToHost, s));
   sync();
-  dense::hermitize(nb, S.data(), nb);
-  if (!dense::cholesky(nb, S.data(), nb)) throw NumericError("initial wavefunctions are linearly dependent");
-  dense::inv_lower(nb, S.data(), nb);
-  // U = L^-H
-  for (int j = 0; j < nb; ++j)
-    for (int i = 0; i < nb; ++i) U[i + (size_t)j * nb] = std::conj(S[j + (size_t)i * nb]);
-  DEREK_HIP(hipMemcpyAsync(d_U, U.data(), U.size() * sizeof(cd), hipMemcpyHostToDevice, s));
+  if (na < nb) throw NumericError("initial wavefunctions are linearly dependent");
+  chol_compact_gpu(d_Lb, nb, d_sact, d_na, d_S, nb, d_dsc, d_Lc, d_Hc, s);
+  trinv_gpu(d_Lc, d_na, nb, d_Linv, s);
+  conj_transpose_gpu(d_Linv, d_na, nb, d_LinvH, s);
+  expand_rows_gpu(d_LinvH, d_na, d_sact, d_dsc, nb, nb, d_U, s);
   rotate<R>(Y, K.ld, d_U, nb, nb, nb, K.npw, Y2, K.ld);
   std::swap(K.Y, K.Y2);
   sync();
@@ -1090,12 +1141,7 @@ int32_t derek_hip_scf_init(derek_ctx* c, const derek_scf_params* p) {
     c->iscf = 0;
     const int nb = c->nb;
     const int m2 = 2 * nb;
-    dfree(c->d_S);
-    dfree(c->d_H);
-    dfree(c->d_U);
-    c->d_S = dmalloc<double2>((size_t)m2 * m2);
-    c->d_H = dmalloc<double2>((size_t)m2 * m2);
-    c->d_U = dmalloc<double2>((size_t)m2 * m2);
+    c->ensure_sub(m2);
     for (double** q : {&c->d_eps, &c->d_ekin, &c->d_occw}) {
       dfree(*q);
       *q = dmalloc<double>(nb > 0 ? (size_t)c->nk * nb : 1);
@@ -1196,6 +1242,37 @@ int32_t derek_hip_scf_step(derek_ctx* c, derek_scf_report* rep) {
     c->mix();
     c->sync();
     rep->t_step = std::chrono::duration<double>(std::chrono::steady_clock::now() - tstep0).count();
+  });
+}
+
+int32_t derek_hip_heev(derek_ctx* c, int32_t n, int32_t nev, const double* A, double* w, double* Z) {
+  return guard([&] {
+    DEREK_REQUIRE(c && A && w && Z, "null argument");
+    DEREK_REQUIRE(n >= 1 && n <= 2048 && nev >= 1 && nev <= n, "need 1 <= nev <= n <= 2048");
+    c->ensure_dense(n, nev);
+    double2* dA = c->tA.get<double2>((size_t)n * n);
+    double2* dZ = c->tB.get<double2>((size_t)n * nev);
+    DEREK_HIP(hipMemcpyAsync(dA, A, (size_t)n * n * sizeof(double2), hipMemcpyHostToDevice, c->s));
+    hermitize_gpu(dA, n, c->s);
+    heev_gpu(dA, n, 0, n, nev, c->d_w, 0, dZ, n, 0, c->d_hwork, 0, 1, c->s);
+    DEREK_HIP(hipMemcpyAsync(w, c->d_w, nev * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    DEREK_HIP(hipMemcpyAsync(Z, dZ, (size_t)n * nev * sizeof(double2), hipMemcpyDeviceToHost, c->s));
+    c->sync();
+  });
+}
+
+int32_t derek_hip_hegv(derek_ctx* c, int32_t m, int32_t nev, const double* S, const double* H, double* w, double* C) {
+  return guard([&] {
+    DEREK_REQUIRE(c && S && H && w && C, "null argument");
+    DEREK_REQUIRE(m >= 1 && m <= 640 && nev >= 1 && nev <= m, "need 1 <= nev <= m <= 640");
+    const int cap = std::max(m, 2 * c->nb);
+    c->ensure_sub(cap);
+    c->ensure_dense(cap, nev);
+    DEREK_HIP(hipMemcpyAsync(c->d_S, S, (size_t)m * m * sizeof(double2), hipMemcpyHostToDevice, c->s));
+    DEREK_HIP(hipMemcpyAsync(c->d_H, H, (size_t)m * m * sizeof(double2), hipMemcpyHostToDevice, c->s));
+    if (!c->gen_gpu(m, nev, w)) throw NumericError("overlap has fewer than nev independent columns");
+    DEREK_HIP(hipMemcpyAsync(C, c->d_U, (size_t)m * nev * sizeof(double2), hipMemcpyDeviceToHost, c->s));
+    c->sync();
   });
 }
 

@@ -1,0 +1,751 @@
+// FP64 dense subspace solvers on the GPU, one 1024-thread workgroup per
+// matrix (blockIdx.x selects the matrix of a batch).
+//
+// k_heev: lowest nev eigenpairs of a Hermitian n x n matrix.
+//   1. Householder reduction to real tridiagonal form, the matrix kept in
+//      global memory (L2 resident): per column, a norm reduction, the
+//      reflector in LDS, u = A22 v with one wavefront per column (coalesced
+//      column reads), gamma = v^H u, and the rank-2 update A22 -= v w^H + w v^H.
+//      The reflectors overwrite the eliminated columns.
+//   2. Phases that make the sub-diagonal real and non-negative.
+//   3. Sturm-sequence multisection for the lowest nev eigenvalues, one
+//      wavefront each (64 points per step).
+//   4. Inverse iteration for the eigenvectors of the tridiagonal matrix with
+//      partial-pivoting LU; eigenvalues closer than 1e-5 |T| form a cluster
+//      whose vectors are computed by one thread in order and re-orthogonalised
+//      (modified Gram-Schmidt) against the earlier members.
+//   5. Back-transformation by the reflectors, one wavefront per vector.
+// k_chol_drop: Cholesky of a unit-diagonal Hermitian overlap that skips every
+//   column whose remaining pivot is below tol (dependent on earlier columns).
+// k_trinv: inverse of a lower-triangular matrix, one wavefront per column.
+#include <cfloat>
+
+#include "dense_gpu.hpp"
+
+namespace derek {
+
+constexpr int DT = 1024;  // threads per workgroup
+constexpr int DW = DT / 64;
+
+__device__ __forceinline__ double2 zmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 zconjmul(double2 a, double2 b) {  // conj(a) * b
+  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+
+__device__ __forceinline__ double wave_sum(double x) {
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+  return __shfl(x, 0, 64);
+}
+
+// Sum over the workgroup; every thread gets the result.
+__device__ double block_sum_all(double x, double* red) {
+  x = wave_sum(x);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = x;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < DW; ++w) t += red[w];
+  return t;
+}
+
+__device__ double block_min_all(double x, double* red) {
+  for (int o = 32; o > 0; o >>= 1) x = fmin(x, __shfl_down(x, o, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = x;
+  __syncthreads();
+  double t = red[0];
+  for (int w = 1; w < DW; ++w) t = fmin(t, red[w]);
+  return t;
+}
+
+__device__ double block_max_all(double x, double* red) {
+  for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_down(x, o, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = x;
+  __syncthreads();
+  double t = red[0];
+  for (int w = 1; w < DW; ++w) t = fmax(t, red[w]);
+  return t;
+}
+
+// number of eigenvalues of the tridiagonal (d, e) smaller than x
+__device__ int sturm_count(const double* d, const double* e2, int n, double x, double pivmin) {
+  int c = 0;
+  double q = d[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  if (q < 0.0) ++c;
+  for (int i = 1; i < n; ++i) {
+    q = d[i] - x - e2[i - 1] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    if (q < 0.0) ++c;
+  }
+  return c;
+}
+
+// Solve (T - lambda I) x = b by LU with partial pivoting; T has diagonal d and
+// off-diagonal e.  u0/u1/u2/l/pv are scratch of length n; b is overwritten by x.
+__device__ void tri_solve(const double* d, const double* e, int n, double lambda, double pivmin, double* u0,
+                          double* u1, double* u2, double* l, double* pv, double* b) {
+  double cd = d[0] - lambda, cs = n > 1 ? e[0] : 0.0, cs2 = 0.0;
+  for (int i = 0; i + 1 < n; ++i) {
+    const double c = e[i];
+    const double a1 = d[i + 1] - lambda;
+    const double b1 = i + 2 < n ? e[i + 1] : 0.0;
+    if (fabs(cd) >= fabs(c)) {
+      const double piv = fabs(cd) < pivmin ? copysign(pivmin, cd == 0.0 ? 1.0 : cd) : cd;
+      const double li = c / piv;
+      u0[i] = piv;
+      u1[i] = cs;
+      u2[i] = cs2;
+      l[i] = li;
+      pv[i] = 0.0;
+      cd = a1 - li * cs;
+      cs = b1 - li * cs2;
+      cs2 = 0.0;
+    } else {
+      const double li = cd / c;
+      u0[i] = c;
+      u1[i] = a1;
+      u2[i] = b1;
+      l[i] = li;
+      pv[i] = 1.0;
+      cd = cs - li * a1;
+      cs = cs2 - li * b1;
+      cs2 = 0.0;
+    }
+  }
+  u0[n - 1] = fabs(cd) < pivmin ? copysign(pivmin, cd == 0.0 ? 1.0 : cd) : cd;
+  for (int i = 0; i + 1 < n; ++i) {
+    if (pv[i] != 0.0) {
+      const double t = b[i];
+      b[i] = b[i + 1];
+      b[i + 1] = t;
+    }
+    b[i + 1] -= l[i] * b[i];
+  }
+  b[n - 1] /= u0[n - 1];
+  if (n > 1) b[n - 2] = (b[n - 2] - u1[n - 2] * b[n - 1]) / u0[n - 2];
+  for (int i = n - 3; i >= 0; --i) b[i] = (b[i] - u1[i] * b[i + 1] - u2[i] * b[i + 2]) / u0[i];
+}
+
+// Per-matrix workspace layout (doubles), see heev_work_doubles.
+struct HeevWork {
+  double2* sub;  // [n]  T(k+1,k) before the phase change
+  double2* ph;   // [n]  phases
+  int* skip;     // [n]  1 if column k needed no reflector
+  double* d;     // [n]  diagonal
+  double* e;     // [n]  |sub|
+  double* lam;   // [nev]
+  int* cstart;   // [nev + 1]
+  int* ncl;      // [1]
+  double* xt;    // [n * nev] tridiagonal eigenvectors
+  __device__ __host__ HeevWork(double* w, int n, int nev) {
+    sub = reinterpret_cast<double2*>(w);
+    ph = sub + n;
+    d = reinterpret_cast<double*>(ph + n);
+    e = d + n;
+    lam = e + n;
+    xt = lam + nev;
+    skip = reinterpret_cast<int*>(xt + (size_t)n * nev);
+    cstart = skip + n;
+    ncl = cstart + nev + 1;
+  }
+};
+
+int64_t heev_work_doubles(int n, int nev) {
+  return 4 * (int64_t)n + 2 * (int64_t)n + nev + (int64_t)n * nev + (n + nev + 2 + 1) / 2 + 8;
+}
+
+// ---- 1. Householder tridiagonalisation + phases (one workgroup per matrix)
+__global__ __launch_bounds__(DT) void k_tridiag(double2* __restrict__ Aall, int lda, int64_t strideA, int n, int nev,
+                                               double* __restrict__ work, int64_t strideWork) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double2* v = reinterpret_cast<double2*>(smem);  // [n]
+  double2* u = v + n;                              // [n]
+  double* red = reinterpret_cast<double*>(u + n);  // [DW]
+  __shared__ int s_skip;
+  __shared__ double s_scal[4];
+  const int mat = blockIdx.x;
+  double2* A = Aall + mat * strideA;
+  HeevWork W(work + mat * strideWork, n, nev);
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  for (int k = 0; k + 2 < n; ++k) {
+    double part = 0.0;
+    for (int i = k + 1 + tid; i < n; i += DT) {
+      const double2 a = A[i + (int64_t)k * lda];
+      part += a.x * a.x + a.y * a.y;
+    }
+    const double xn2 = block_sum_all(part, red);
+    const double2 x0 = A[(k + 1) + (int64_t)k * lda];
+    const double ax0 = sqrt(x0.x * x0.x + x0.y * x0.y);
+    if (tid == 0) {
+      const double rest = xn2 - ax0 * ax0;
+      s_skip = (xn2 == 0.0 || rest <= 1e-300) ? 1 : 0;
+      W.skip[k] = s_skip;
+      if (s_skip) {
+        W.sub[k] = x0;
+      } else {
+        const double xn = sqrt(xn2);
+        const double2 phs = ax0 > 0.0 ? make_double2(x0.x / ax0, x0.y / ax0) : make_double2(1.0, 0.0);
+        const double2 alpha = make_double2(-phs.x * xn, -phs.y * xn);
+        const double2 v0 = make_double2(x0.x - alpha.x, x0.y - alpha.y);
+        const double vn2 = rest + v0.x * v0.x + v0.y * v0.y;
+        W.sub[k] = alpha;
+        s_scal[0] = 1.0 / sqrt(vn2);
+        s_scal[1] = v0.x;
+        s_scal[2] = v0.y;
+      }
+    }
+    __syncthreads();
+    if (s_skip) continue;
+    const double iv = s_scal[0];
+    for (int i = k + 1 + tid; i < n; i += DT) {
+      double2 a = i == k + 1 ? make_double2(s_scal[1], s_scal[2]) : A[i + (int64_t)k * lda];
+      v[i] = make_double2(a.x * iv, a.y * iv);
+    }
+    __syncthreads();
+    // u_i = sum_j A(i,j) v_j = sum_j conj(A(j,i)) v_j  (column i), one wave per column
+    for (int i = k + 1 + wid; i < n; i += DW) {
+      double sr = 0.0, si = 0.0;
+      const double2* col = A + (int64_t)i * lda;
+      for (int j = k + 1 + lane; j < n; j += 64) {
+        const double2 t = zconjmul(col[j], v[j]);
+        sr += t.x;
+        si += t.y;
+      }
+      sr = wave_sum(sr);
+      si = wave_sum(si);
+      if (lane == 0) u[i] = make_double2(sr, si);
+    }
+    __syncthreads();
+    double gp = 0.0;
+    for (int i = k + 1 + tid; i < n; i += DT) gp += zconjmul(v[i], u[i]).x;
+    const double gam = block_sum_all(gp, red);
+    for (int i = k + 1 + tid; i < n; i += DT)
+      u[i] = make_double2(2.0 * u[i].x - 2.0 * gam * v[i].x, 2.0 * u[i].y - 2.0 * gam * v[i].y);
+    __syncthreads();
+    // A22 -= v w^H + w v^H  (w held in u)
+    const int m = n - k - 1;
+    const int64_t mm = (int64_t)m * m;
+    for (int64_t q = tid; q < mm; q += DT) {
+      const int i = k + 1 + (int)(q % m), j = k + 1 + (int)(q / m);
+      const double2 vi = v[i], wi = u[i], vj = v[j], wj = u[j];
+      double2* p = A + i + (int64_t)j * lda;
+      const double2 t1 = zmul(vi, make_double2(wj.x, -wj.y));
+      const double2 t2 = zmul(wi, make_double2(vj.x, -vj.y));
+      *p = make_double2(p->x - t1.x - t2.x, p->y - t1.y - t2.y);
+    }
+    __syncthreads();
+    for (int i = k + 1 + tid; i < n; i += DT) A[i + (int64_t)k * lda] = v[i];  // keep the reflector
+    __syncthreads();
+  }
+  if (tid == 0) {
+    if (n >= 2) {
+      W.sub[n - 2] = A[(n - 1) + (int64_t)(n - 2) * lda];
+      W.skip[n - 2] = 1;
+    }
+    W.skip[n - 1] = 1;
+  }
+  for (int i = tid; i < n; i += DT) W.d[i] = A[i + (int64_t)i * lda].x;
+  __syncthreads();
+  // phases: T' = D^H T D real with off-diagonal |sub|
+  if (tid == 0) {
+    double2 p = make_double2(1.0, 0.0);
+    W.ph[0] = p;
+    for (int k = 0; k + 1 < n; ++k) {
+      const double a = sqrt(W.sub[k].x * W.sub[k].x + W.sub[k].y * W.sub[k].y);
+      W.e[k] = a;
+      if (a > 0.0) p = zmul(p, make_double2(W.sub[k].x / a, W.sub[k].y / a));
+      W.ph[k + 1] = p;
+    }
+    W.e[n - 1] = 0.0;
+  }
+}
+
+// ---- 2. eigenvalues by multisection: one wavefront per eigenvalue; the 64
+// lanes count eigenvalues below 64 interior points, so every step narrows the
+// bracket 65-fold.  Then the clusters (one thread).
+constexpr int BSW = 4;  // wavefronts (eigenvalues) per workgroup
+
+__global__ __launch_bounds__(BSW * 64) void k_bisect(int n, int nev, double* __restrict__ work, int64_t strideWork) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* dd = reinterpret_cast<double*>(smem);  // [n]
+  double* e2 = dd + n;                            // [n]
+  const int mat = blockIdx.y;
+  HeevWork W(work + mat * strideWork, n, nev);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double glo = 1e300, ghi = -1e300, emax = 0.0;
+  for (int i = threadIdx.x; i < n; i += BSW * 64) {
+    const double ei = W.e[i];
+    dd[i] = W.d[i];
+    e2[i] = ei * ei;
+  }
+  __syncthreads();
+  // Gershgorin bounds (every wave computes them; they are cheap)
+  for (int i = lane; i < n; i += 64) {
+    const double r = (i > 0 ? sqrt(e2[i - 1]) : 0.0) + (i + 1 < n ? sqrt(e2[i]) : 0.0);
+    glo = fmin(glo, dd[i] - r);
+    ghi = fmax(ghi, dd[i] + r);
+    if (i + 1 < n) emax = fmax(emax, e2[i]);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    glo = fmin(glo, __shfl_down(glo, o, 64));
+    ghi = fmax(ghi, __shfl_down(ghi, o, 64));
+    emax = fmax(emax, __shfl_down(emax, o, 64));
+  }
+  glo = __shfl(glo, 0, 64);
+  ghi = __shfl(ghi, 0, 64);
+  emax = __shfl(emax, 0, 64);
+  const double tnorm = fmax(fabs(glo), fabs(ghi));
+  const double pivmin = fmax(DBL_MIN * fmax(1.0, emax), 1e-300);
+  const double span = ghi - glo;
+  glo -= 2.0 * DBL_EPSILON * tnorm * n + 1e-300 + 1e-12 * span;
+  ghi += 2.0 * DBL_EPSILON * tnorm * n + 1e-300 + 1e-12 * span;
+  const int t = blockIdx.x * BSW + wid;
+  if (t >= nev) return;
+  // invariant: count(lo) <= t < count(hi); the points increase with the lane
+  // and the Sturm count is monotone in x, so the lanes with count <= t form a
+  // prefix 0..p: x_p is the new lo and x_(p+1) (or hi) the new hi
+  double lo = glo, hi = ghi;
+  for (int it = 0; it < 64; ++it) {
+    if (hi - lo <= 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + pivmin) break;
+    const double x = fmin(fmax(lo + (hi - lo) * ((double)(lane + 1) / 65.0), lo), hi);
+    const int cnt = sturm_count(dd, e2, n, x, pivmin);
+    const uint64_t below = __ballot(cnt <= t);
+    const int p = below ? 63 - __clzll(below) : -1;
+    const double newlo = p >= 0 ? __shfl(x, p, 64) : lo;
+    const double newhi = p < 63 ? __shfl(x, p + 1, 64) : hi;
+    if (!(newlo > lo) && !(newhi < hi)) break;
+    lo = newlo;
+    hi = newhi;
+  }
+  if (lane == 0) W.lam[t] = 0.5 * (lo + hi);
+}
+
+__global__ void k_clusters(int n, int nev, double* __restrict__ work, int64_t strideWork) {
+  const int mat = blockIdx.x;
+  HeevWork W(work + mat * strideWork, n, nev);
+  if (threadIdx.x != 0) return;
+  double tnorm = 0.0;
+  for (int i = 0; i < n; ++i) tnorm = fmax(tnorm, fabs(W.d[i]) + 2.0 * (i + 1 < n ? fabs(W.e[i]) : 0.0));
+  // eigenvalues closer than 1e-5 |T| are computed in order and re-orthogonalised
+  const double ortol = 1e-5 * fmax(tnorm, 1e-300);
+  int nc = 0;
+  for (int t = 0; t < nev; ++t)
+    if (t == 0 || W.lam[t] - W.lam[t - 1] > ortol) W.cstart[nc++] = t;
+  W.cstart[nc] = nev;
+  *W.ncl = nc;
+}
+
+// ---- 3. inverse iteration: one thread per cluster, factors and vector in LDS
+constexpr int IT = 64;  // threads per inverse-iteration workgroup
+
+__global__ __launch_bounds__(IT) void k_inviter(int n, int nev, int cpw, double* __restrict__ work,
+                                               int64_t strideWork) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* dd = reinterpret_cast<double*>(smem);  // [n]
+  double* ee = dd + n;                            // [n]
+  const int mat = blockIdx.y;
+  HeevWork W(work + mat * strideWork, n, nev);
+  for (int i = threadIdx.x; i < n; i += IT) {
+    dd[i] = W.d[i];
+    ee[i] = W.e[i];
+  }
+  __syncthreads();
+  const int c = blockIdx.x * cpw + threadIdx.x;
+  if (threadIdx.x >= cpw || c >= *W.ncl) return;
+  double* mine = ee + n + (size_t)threadIdx.x * 6 * n;
+  double* u0 = mine;
+  double* u1 = u0 + n;
+  double* u2 = u1 + n;
+  double* lv = u2 + n;
+  double* pv = lv + n;
+  double* x = pv + n;
+  double emax = 0.0, tnorm = 0.0;
+  for (int i = 0; i < n; ++i) {
+    emax = fmax(emax, ee[i] * ee[i]);
+    tnorm = fmax(tnorm, fabs(dd[i]) + 2.0 * ee[i]);
+  }
+  const double pivmin = fmax(DBL_MIN * fmax(1.0, emax), 1e-300);
+  const int t0 = W.cstart[c], t1 = W.cstart[c + 1];
+  double prev = -1e300;
+  for (int t = t0; t < t1; ++t) {
+    double lmb = W.lam[t];
+    const double sep = 10.0 * DBL_EPSILON * fmax(tnorm, 1e-300);
+    if (t > t0 && lmb - prev < sep) lmb = prev + sep;
+    prev = lmb;
+    uint64_t h = 0x9E3779B97F4A7C15ull * (uint64_t)(t + 1);
+    for (int i = 0; i < n; ++i) {
+      h ^= h >> 31;
+      h *= 0xBF58476D1CE4E5B9ull;
+      h ^= h >> 29;
+      x[i] = ((double)(h >> 11) * (1.0 / 9007199254740992.0)) - 0.5;
+    }
+    for (int it = 0; it < 3; ++it) {
+      tri_solve(dd, ee, n, lmb, pivmin, u0, u1, u2, lv, pv, x);
+      for (int q = t0; q < t; ++q) {  // orthogonalise against the earlier cluster members
+        const double* y = W.xt + (size_t)q * n;
+        double sdot = 0.0;
+        for (int i = 0; i < n; ++i) sdot += y[i] * x[i];
+        for (int i = 0; i < n; ++i) x[i] -= sdot * y[i];
+      }
+      double nr = 0.0;
+      for (int i = 0; i < n; ++i) nr += x[i] * x[i];
+      const double inr = nr > 0.0 ? 1.0 / sqrt(nr) : 0.0;
+      for (int i = 0; i < n; ++i) x[i] *= inr;
+    }
+    double* out = W.xt + (size_t)t * n;
+    for (int i = 0; i < n; ++i) out[i] = x[i];
+  }
+}
+
+// ---- 4. back-transformation, one wavefront per eigenvector
+constexpr int BW = 16;  // waves per back-transform workgroup
+
+__global__ __launch_bounds__(BW * 64) void k_backtransform(const double2* __restrict__ Aall, int lda,
+                                                          int64_t strideA, int n, int nev,
+                                                          double* __restrict__ work, int64_t strideWork,
+                                                          double* __restrict__ Wall, int64_t strideW,
+                                                          double2* __restrict__ Zall, int ldz, int64_t strideZ) {
+  const int mat = blockIdx.y;
+  const double2* A = Aall + mat * strideA;
+  HeevWork W(work + mat * strideWork, n, nev);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int t = blockIdx.x * BW + wid;
+  if (t >= nev) return;
+  double2* z = Zall + mat * strideZ + (int64_t)t * ldz;
+  const double* xs = W.xt + (size_t)t * n;
+  for (int i = lane; i < n; i += 64) z[i] = make_double2(W.ph[i].x * xs[i], W.ph[i].y * xs[i]);
+  if (lane == 0) Wall[mat * strideW + t] = W.lam[t];
+  for (int k = n - 3; k >= 0; --k) {
+    if (W.skip[k]) continue;
+    const double2* vk = A + (int64_t)k * lda;  // reflector in rows k+1..n-1
+    double sr = 0.0, si = 0.0;
+    for (int i = k + 1 + lane; i < n; i += 64) {
+      const double2 p = zconjmul(vk[i], z[i]);
+      sr += p.x;
+      si += p.y;
+    }
+    sr = 2.0 * wave_sum(sr);
+    si = 2.0 * wave_sum(si);
+    for (int i = k + 1 + lane; i < n; i += 64) {
+      const double2 p = zmul(vk[i], make_double2(sr, si));
+      z[i] = make_double2(z[i].x - p.x, z[i].y - p.y);
+    }
+  }
+}
+
+void heev_gpu(double2* A, int lda, int64_t strideA, int n, int nev, double* Wv, int64_t strideW, double2* Z, int ldz,
+              int64_t strideZ, double* work, int64_t strideWork, int batch, hipStream_t s) {
+  DEREK_REQUIRE(n >= 1 && nev >= 1 && nev <= n, "heev: need 1 <= nev <= n");
+  DEREK_REQUIRE(n <= 2048, "heev: n must be <= 2048");
+  {
+    const size_t lds = sizeof(double2) * 2 * (size_t)n + sizeof(double) * DW;
+    hipLaunchKernelGGL(k_tridiag, dim3(batch), dim3(DT), lds, s, A, lda, strideA, n, nev, work, strideWork);
+    DEREK_HIP(hipGetLastError());
+  }
+  {
+    const size_t lds = sizeof(double) * 2 * (size_t)n;
+    hipLaunchKernelGGL(k_bisect, dim3((unsigned)ceil_div(nev, BSW), batch), dim3(BSW * 64), lds, s, n, nev, work,
+                       strideWork);
+    DEREK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_clusters, dim3(batch), dim3(64), 0, s, n, nev, work, strideWork);
+    DEREK_HIP(hipGetLastError());
+  }
+  {
+    // clusters per workgroup: LDS holds d, e and 6 n doubles per cluster thread
+    const int cpw = std::max(1, std::min(IT, (int)((60 * 1024 / sizeof(double) - 2 * (size_t)n) / (6 * (size_t)n))));
+    const size_t lds = sizeof(double) * (2 * (size_t)n + 6 * (size_t)n * cpw);
+    DEREK_REQUIRE(lds <= 64 * 1024, "heev: inverse-iteration scratch exceeds the LDS budget");
+    dim3 grid((unsigned)ceil_div(nev, cpw), batch);
+    hipLaunchKernelGGL(k_inviter, grid, dim3(IT), lds, s, n, nev, cpw, work, strideWork);
+    DEREK_HIP(hipGetLastError());
+  }
+  {
+    dim3 grid((unsigned)ceil_div(nev, BW), batch);
+    hipLaunchKernelGGL(k_backtransform, grid, dim3(BW * 64), 0, s, A, lda, strideA, n, nev, work, strideWork, Wv,
+                       strideW, Z, ldz, strideZ);
+    DEREK_HIP(hipGetLastError());
+  }
+}
+
+// ---------------------------------------------------------------- Cholesky with dropping
+// Blocked right-looking on a work copy Wm of S (m x m, column-major, lower
+// triangle used).  A panel of nb columns is staged in LDS and factored there:
+// column j is kept if its updated pivot exceeds tol * S_jj; a kept column
+// becomes L's column c (stored row-major in Lb: Lb[i*m + c]) and updates the
+// rest of the panel; a dropped column is zeroed and contributes nothing.
+// The panel then updates the trailing lower triangle in one pass (one
+// wavefront per column, rows over the lanes).
+__global__ __launch_bounds__(DT) void k_chol_drop(const double2* __restrict__ S, int m, int nb, double tol,
+                                                  double2* __restrict__ Wm, double2* __restrict__ Lb,
+                                                  int* __restrict__ act, int* __restrict__ na) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double2* P = reinterpret_cast<double2*>(smem);  // [nb][m]: P[jj*m + i]
+  double* sd = reinterpret_cast<double*>(P + (size_t)nb * m);  // [m] diagonal of S
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int64_t mm = (int64_t)m * m;
+  for (int64_t q = tid; q < mm; q += DT) Wm[q] = S[q];
+  for (int i = tid; i < m; i += DT) sd[i] = S[i + (int64_t)i * m].x;
+  __syncthreads();
+  int c = 0;
+  for (int p0 = 0; p0 < m; p0 += nb) {
+    const int pn = min(nb, m - p0), rows = m - p0;
+    for (int q = tid; q < pn * rows; q += DT) {
+      const int jj = q / rows, i = p0 + q % rows;
+      P[jj * m + i] = Wm[i + (int64_t)(p0 + jj) * m];
+    }
+    __syncthreads();
+    for (int jj = 0; jj < pn; ++jj) {
+      const int j = p0 + jj;
+      double2* pc = P + jj * m;
+      const double d = pc[j].x;
+      if (d > tol * sd[j]) {
+        const double lj = sqrt(d), idj = 1.0 / lj;
+        for (int i = j + 1 + tid; i < m; i += DT) {
+          const double2 l = make_double2(pc[i].x * idj, pc[i].y * idj);
+          pc[i] = l;
+          Lb[(int64_t)i * m + c] = l;
+        }
+        if (tid == 0) {
+          Lb[(int64_t)j * m + c] = make_double2(lj, 0.0);
+          act[c] = j;
+        }
+        __syncthreads();
+        // rest of the panel: P(i,kk) -= l_i conj(l_(p0+kk)), i >= p0+kk
+        for (int kk = jj + 1 + wid; kk < pn; kk += DW) {
+          const double2 lk = pc[p0 + kk];
+          double2* pk = P + kk * m;
+          for (int i = p0 + kk + lane; i < m; i += 64) {
+            const double2 li = pc[i];
+            pk[i] = make_double2(pk[i].x - (li.x * lk.x + li.y * lk.y), pk[i].y - (li.y * lk.x - li.x * lk.y));
+          }
+        }
+        __syncthreads();
+        ++c;
+      } else {
+        // only column jj's own rows are written; the next column reads
+        // column jj+1, and the trailing update waits for a barrier
+        for (int i = j + tid; i < m; i += DT) pc[i] = make_double2(0.0, 0.0);
+      }
+    }
+    __syncthreads();
+    // trailing update W(i,k) -= sum_jj P(i,jj) conj(P(k,jj)), i >= k >= p0+pn
+    for (int k = p0 + pn + wid; k < m; k += DW) {
+      double2* wk = Wm + (int64_t)k * m;
+      for (int i = k + lane; i < m; i += 64) {
+        double2 acc = wk[i];
+        for (int jj = 0; jj < pn; ++jj) {
+          const double2 li = P[jj * m + i], lk = P[jj * m + k];
+          acc.x -= li.x * lk.x + li.y * lk.y;
+          acc.y -= li.y * lk.x - li.x * lk.y;
+        }
+        wk[i] = acc;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) *na = c;
+}
+
+// Compact: Lc (na x na, column-major, ld na) from the row-major build buffer
+// (rows act[], kept columns), Hc = D H D restricted to act (na x na).
+__global__ void k_chol_compact(const double2* __restrict__ Lb, int m, const int* __restrict__ act,
+                               const int* __restrict__ na_p, const double2* __restrict__ H, int ldh,
+                               const double* __restrict__ dsc, double2* __restrict__ Lc, double2* __restrict__ Hc) {
+  const int na = *na_p;
+  const int64_t nn = (int64_t)na * na;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nn; q += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(q % na), c = (int)(q / na);
+    Lc[q] = r >= c ? Lb[(int64_t)act[r] * m + c] : make_double2(0.0, 0.0);
+    const int ar = act[r], ac = act[c];
+    const double2 h = H[ar + (int64_t)ac * ldh];
+    const double sc = dsc[ar] * dsc[ac];
+    Hc[q] = make_double2(h.x * sc, h.y * sc);
+  }
+}
+
+// Inverse of a lower-triangular na x na matrix (column-major, ld na), one
+// wavefront per column j: forward substitution on e_j with the solution and
+// the current column of L in registers (row i lives in lane i % 64, slot
+// i / 64); the pivot and x_k are read across lanes with readlane, and the
+// next column of L is loaded while the current one is applied.
+constexpr int TR = 10;  // rows per lane: na <= 640
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__global__ __launch_bounds__(64) void k_trinv(const double2* __restrict__ L, const int* __restrict__ na_p,
+                                              double2* __restrict__ Linv) {
+  const int na = *na_p;
+  const int j = blockIdx.x, lane = threadIdx.x;
+  if (j >= na) return;
+  double2 x[TR], cur[TR];
+#pragma unroll
+  for (int t = 0; t < TR; ++t) {
+    const int i = lane + 64 * t;
+    x[t] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+    cur[t] = (i >= j && i < na) ? L[i + (int64_t)j * na] : make_double2(0.0, 0.0);
+  }
+  for (int k = j; k < na; ++k) {
+    double2 nxt[TR];
+#pragma unroll
+    for (int t = 0; t < TR; ++t) {
+      const int i = lane + 64 * t;
+      nxt[t] = (i > k && i < na) ? L[i + (int64_t)(k + 1) * na] : make_double2(0.0, 0.0);
+    }
+    const int tk = k >> 6, lk = k & 63;
+    double2 xo = make_double2(0.0, 0.0), dg = make_double2(1.0, 0.0);
+#pragma unroll
+    for (int t = 0; t < TR; ++t)
+      if (t == tk) {
+        xo = x[t];
+        dg = cur[t];
+      }
+    xo = make_double2(readlane_d(xo.x, lk), readlane_d(xo.y, lk));
+    dg = make_double2(readlane_d(dg.x, lk), readlane_d(dg.y, lk));
+    const double den = 1.0 / (dg.x * dg.x + dg.y * dg.y);
+    const double2 xk = make_double2((xo.x * dg.x + xo.y * dg.y) * den, (xo.y * dg.x - xo.x * dg.y) * den);
+#pragma unroll
+    for (int t = 0; t < TR; ++t) {
+      const int i = lane + 64 * t;
+      if (i > k) {
+        const double2 u = zmul(cur[t], xk);
+        x[t] = make_double2(x[t].x - u.x, x[t].y - u.y);
+      } else if (i == k) {
+        x[t] = xk;
+      }
+      cur[t] = nxt[t];
+    }
+  }
+  double2* out = Linv + (int64_t)j * na;
+#pragma unroll
+  for (int t = 0; t < TR; ++t) {
+    const int i = lane + 64 * t;
+    if (i < na) out[i] = x[t];
+  }
+}
+
+// B = A^H for an na x na matrix (column-major, ld na)
+__global__ void k_conj_transpose(const double2* __restrict__ A, const int* __restrict__ na_p, double2* __restrict__ B) {
+  const int na = *na_p;
+  const int64_t nn = (int64_t)na * na;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nn; q += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(q % na), c = (int)(q / na);
+    const double2 a = A[c + (int64_t)r * na];
+    B[q] = make_double2(a.x, -a.y);
+  }
+}
+
+// C (m x nev, ld m) = 0 except rows act[r] = dsc[act[r]] * Ca[r, :]
+__global__ void k_expand_rows(const double2* __restrict__ Ca, const int* __restrict__ na_p,
+                              const int* __restrict__ act, const double* __restrict__ dsc, int m, int nev,
+                              double2* __restrict__ C) {
+  const int na = *na_p;
+  const int64_t tot = (int64_t)m * nev;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x)
+    C[q] = make_double2(0.0, 0.0);
+  __syncthreads();
+  const int64_t t2 = (int64_t)na * nev;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < t2; q += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(q % na), c = (int)(q / na);
+    const double2 v = Ca[q];
+    const double s = dsc[act[r]];
+    C[act[r] + (int64_t)c * m] = make_double2(v.x * s, v.y * s);
+  }
+}
+
+// dsc[i] = 1/sqrt(S_ii); S <- D S D
+__global__ void k_unit_diag(double2* __restrict__ S, int m, double* __restrict__ dsc) {
+  __shared__ double sd[2048];
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    const double v = 1.0 / sqrt(S[i + (int64_t)i * m].x);
+    sd[i] = v;
+    dsc[i] = v;
+  }
+  __syncthreads();
+  const int64_t mm = (int64_t)m * m;
+  for (int64_t q = threadIdx.x; q < mm; q += blockDim.x) {
+    const int r = (int)(q % m), c = (int)(q / m);
+    const double s = sd[r] * sd[c];
+    S[q] = make_double2(S[q].x * s, S[q].y * s);
+  }
+}
+
+void hegv_gpu_prepare(double2* S, int m, double* dsc, hipStream_t s) {
+  DEREK_REQUIRE(m <= 2048, "hegv: at most 2048 columns");
+  hipLaunchKernelGGL(k_unit_diag, dim3(1), dim3(1024), 0, s, S, m, dsc);
+  DEREK_HIP(hipGetLastError());
+}
+
+void chol_drop_gpu(const double2* S, int m, double tol, double2* Wm, double2* Lbuild, int* act, int* na,
+                   hipStream_t s) {
+  constexpr size_t kLds = 60 * 1024;
+  DEREK_REQUIRE(m >= 1 && (size_t)m * (sizeof(double2) + sizeof(double)) <= kLds, "chol_drop: matrix too large");
+  const int nb = (int)std::min<size_t>(16, (kLds - sizeof(double) * m) / (sizeof(double2) * m));
+  const size_t lds = sizeof(double2) * (size_t)nb * m + sizeof(double) * m;
+  hipLaunchKernelGGL(k_chol_drop, dim3(1), dim3(DT), lds, s, S, m, nb, tol, Wm, Lbuild, act, na);
+  DEREK_HIP(hipGetLastError());
+}
+
+void chol_compact_gpu(const double2* Lbuild, int m, const int* act, const int* na, const double2* H, int ldh,
+                      const double* dsc, double2* Lc, double2* Hc, hipStream_t s) {
+  hipLaunchKernelGGL(k_chol_compact, dim3((unsigned)std::min<int64_t>(1024, ceil_div((int64_t)m * m, 256))),
+                     dim3(256), 0, s, Lbuild, m, act, na, H, ldh, dsc, Lc, Hc);
+  DEREK_HIP(hipGetLastError());
+}
+
+void trinv_gpu(const double2* L, const int* na, int m, double2* Linv, hipStream_t s) {
+  DEREK_REQUIRE(m <= 64 * TR, "trinv: matrix too large");
+  hipLaunchKernelGGL(k_trinv, dim3((unsigned)m), dim3(64), 0, s, L, na, Linv);
+  DEREK_HIP(hipGetLastError());
+}
+
+void conj_transpose_gpu(const double2* A, const int* na, int m, double2* B, hipStream_t s) {
+  hipLaunchKernelGGL(k_conj_transpose, dim3((unsigned)std::min<int64_t>(1024, ceil_div((int64_t)m * m, 256))),
+                     dim3(256), 0, s, A, na, B);
+  DEREK_HIP(hipGetLastError());
+}
+
+void expand_rows_gpu(const double2* Ca, const int* na, const int* act, const double* dsc, int m, int nev, double2* C,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_expand_rows, dim3(1), dim3(1024), 0, s, Ca, na, act, dsc, m, nev, C);
+  DEREK_HIP(hipGetLastError());
+}
+
+}  // namespace derek
+
+namespace derek {
+
+__global__ void k_hermitize(double2* __restrict__ A, int m) {
+  const int64_t mm = (int64_t)m * m;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < mm; q += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(q % m), c = (int)(q / m);
+    if (r < c) continue;
+    const double2 a = A[q], b = A[c + (int64_t)r * m];
+    if (r == c) {
+      A[q] = make_double2(a.x, 0.0);
+    } else {
+      const double2 h = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+      A[q] = h;
+      A[c + (int64_t)r * m] = make_double2(h.x, -h.y);
+    }
+  }
+}
+
+void hermitize_gpu(double2* A, int m, hipStream_t s) {
+  hipLaunchKernelGGL(k_hermitize, dim3((unsigned)std::min<int64_t>(1024, ceil_div((int64_t)m * m, 256))), dim3(256), 0,
+                     s, A, m);
+  DEREK_HIP(hipGetLastError());
+}
+
+}  // namespace derek

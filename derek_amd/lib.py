@@ -23,7 +23,7 @@ EXPORTS = (
     "derek_hip_init", "derek_hip_destroy", "derek_hip_set_basis", "derek_hip_set_gsq",
     "derek_hip_set_vloc", "derek_hip_set_vext",
     "derek_hip_fft3d", "derek_hip_apply_h", "derek_hip_overlap", "derek_hip_rotate",
-    "derek_hip_density", "derek_hip_lda", "derek_hip_hartree",
+    "derek_hip_density", "derek_hip_lda", "derek_hip_hartree", "derek_hip_heev", "derek_hip_hegv",
     "derek_hip_scf_init", "derek_hip_scf_step", "derek_hip_scf_polish", "derek_hip_get_eigenvalues",
     "derek_hip_get_occupations", "derek_hip_get_density", "derek_hip_get_psi", "derek_hip_put_psi",
     "derek_hip_apply_h_resident", "derek_hip_sync", "derek_hip_stream",
@@ -285,6 +285,32 @@ class Context:
         eh = C.c_double()
         check(self.lib.derek_hip_hartree(self.h, ptr(r), ptr(v), C.byref(eh)))
         return v, eh.value
+
+    def heev(self, A, nev=None):
+        """Lowest nev eigenpairs of Hermitian A (n x n) on the GPU."""
+        A = np.asarray(A, dtype=np.complex128)
+        n = A.shape[0]
+        nev = n if nev is None else nev
+        a = np.ascontiguousarray(A.T)  # column-major
+        w = np.zeros(nev)
+        Z = np.zeros((nev, n), dtype=np.complex128)
+        check(self.lib.derek_hip_heev(self.h, C.c_int32(n), C.c_int32(nev), a.ctypes.data_as(C.POINTER(C.c_double)),
+                                      ptr(w), Z.ctypes.data_as(C.POINTER(C.c_double))))
+        return w, Z.T.copy()
+
+    def hegv(self, S, H, nev):
+        """Lowest nev solutions of H c = w S c on the GPU; returns (w, C m x nev)."""
+        S = np.asarray(S, dtype=np.complex128)
+        H = np.asarray(H, dtype=np.complex128)
+        m = S.shape[0]
+        s_ = np.ascontiguousarray(S.T)
+        h_ = np.ascontiguousarray(H.T)
+        w = np.zeros(nev)
+        Cm = np.zeros((nev, m), dtype=np.complex128)
+        check(self.lib.derek_hip_hegv(self.h, C.c_int32(m), C.c_int32(nev), s_.ctypes.data_as(C.POINTER(C.c_double)),
+                                      h_.ctypes.data_as(C.POINTER(C.c_double)), ptr(w),
+                                      Cm.ctypes.data_as(C.POINTER(C.c_double))))
+        return w, Cm.T.copy()
 
     # ---- resident SCF
     def scf_init(self, nelec, kT=0.0, beta=0.5, q0=1.0, n_david_first=4, n_david=2, history=8, seed=1):

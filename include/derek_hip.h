@@ -69,7 +69,7 @@ typedef struct {
   double fermi;
   double drho;      /* || rho_out - rho_in ||_2 (bohr^-3/2) */
   double max_res;   /* largest Davidson residual norm */
-  double t_dense;   /* seconds spent in the host FP64 subspace solves (Cholesky, eigensolves) */
+  double t_dense;   /* seconds spent in the FP64 subspace solves (Cholesky, eigensolves), incl. waits */
   double t_step;    /* wall seconds of the step */
 } derek_scf_report;
 
@@ -144,6 +144,17 @@ int32_t derek_hip_lda(derek_ctx* ctx, int64_t np, const double* rho, double* exc
 /* V_H(r) from rho(r) (V_H(G) = 4 pi rho(G)/|G|^2, G = 0 dropped) and
  * E_H = 1/2 int V_H rho.  Needs derek_hip_set_gsq. */
 int32_t derek_hip_hartree(derek_ctx* ctx, const double* rho, double* vh, double* eh);
+
+/* ---- FP64 subspace solvers on the GPU (host buffers) ----------------------
+ * The reference calls LAPACK only for a 3x3 inverse (io.f90:2488,2494); the
+ * SCF's subspace problems (absent there) run on the device. */
+/* Lowest nev eigenpairs of a Hermitian n x n A (column-major complex):
+ * w ascending, Z n x nev. */
+int32_t derek_hip_heev(derek_ctx* ctx, int32_t n, int32_t nev, const double* A, double* w, double* Z);
+/* Lowest nev solutions of H c = w S c, C^H S C = I (S Hermitian positive
+ * semi-definite; columns of S numerically dependent on earlier ones are
+ * dropped; C rows of dropped columns are zero).  C is m x nev. */
+int32_t derek_hip_hegv(derek_ctx* ctx, int32_t m, int32_t nev, const double* S, const double* H, double* w, double* C);
 
 /* ---- resident SCF (electronic_scf body, electronic.f90:86-175) ----------- */
 /* Random orthonormal wavefunctions, uniform density nelec/volume. */
