@@ -1,16 +1,26 @@
 """Benchmark of the Kohn-Sham SCF hot path on one MI355X.
 
-Workload: the plain FP64 system of 20 bohr cubic cell, Ecut 30 Ha (108^3 FFT
-grid, 62669 plane waves at Gamma), 128 electrons, 80 bands, 16 seeded
-Gaussian wells plus a 0.2 Ha cosine term.  A step is one H|psi> over all 80
-resident bands (batched scatter, inverse 3D FFT, V(r) multiply, forward 3D
-FFT, gather + kinetic).  value = bands/s over the K timed steps with inputs
-resident in HBM.  The same run reports the roofline of the dominant kernel
-(HIP events on the library's stream, over the timed region), the
-time-to-SCF (|dE| < 1e-6 Ha) of the whole FP64 SCF, and a CPU baseline (the
-numpy oracle's H|psi> on a bounded sample, timed on this host).
+Headline workload (``value``): the plain FP64 system of a 20 bohr cubic
+cell, Ecut 30 Ha (108^3 FFT grid, 62669 plane waves at Gamma), 128
+electrons, 80 bands, 16 seeded Gaussian wells plus a 0.2 Ha cosine term.
+A step is one H|psi> over all 80 resident bands (sphere-pruned inverse 3D
+FFT, V(r) multiply, forward 3D FFT, gather + kinetic); value = bands/s over
+the K timed steps with the inputs resident in HBM.  The same run reports the
+roofline of the dominant kernel (HIP events on the library's stream over the
+timed region), the time to SCF convergence (|dE| < 1e-6 Ha) of
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-scf] [--cpu-seconds S]
+  * that system in plain FP64 and in the mixed form (complex64 bulk, FP64
+    subspace solves and energies, FP64 refinement), with dE_tot and the
+    largest eigenvalue difference between the two;
+  * the 64-k-point system (16 bohr, Ecut 20 Ha, 4x4x4 mesh, 72^3 grid, 24
+    bands, kT = 0.01 Ha) in plain FP64 (one k-point per launch, no graph),
+    FP64 batched (all k-points per launch, graph replay) and mixed + batched
+    + autotuned, with the speed-up of the last over the first;
+
+and a CPU baseline (the numpy oracle's H|psi> on a bounded sample of the
+headline system, timed on this host).
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-scf] [--scf cfg2,cfg4] [--cpu-seconds S] [--no-cpu]
 """
 import argparse
 import json
@@ -25,12 +35,49 @@ METRIC = ("time-to-SCF (|ΔE|<1e-6 Ha) mixed/batched vs FP64 GPU; H|ψ⟩ bands/
 HBM_PEAK_GBS = 8000.0
 
 
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def hpsi_rate(ctx, nb, steps, warmup):
+    """bands/s of H|psi> on the resident bands, with the per-class profile."""
+    for _ in range(warmup):
+        ctx.apply_h_resident(0, nb)
+    ctx.sync()
+    ctx.profile(True)
+    ctx.profile_reset()
+    ctx.sync()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        ctx.apply_h_resident(0, nb)
+    ctx.sync()
+    dt = time.perf_counter() - t1
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    return steps * nb / dt, dt / steps * 1e3, prof
+
+
+def scf_run(gscf, lib, spec, su, precision, **kw):
+    """Time to |dE| < 1e-6 Ha from a fresh engine (init included)."""
+    ctx = su.context(precision)
+    t0 = time.perf_counter()
+    res = gscf.run(spec, max_scf=80, energy_tol=1e-6, setup=su, ctx=ctx, log=None, **kw)
+    wall = time.perf_counter() - t0
+    out = {"time_to_scf_s": round(wall, 4), "init_s": round(res["wall_init"], 4), "steps": res["iterations"],
+           "fp32_steps": res["fp32_steps"], "ms_per_step": round(res["wall"] / res["iterations"] * 1e3, 2),
+           "etot_ha": res["etot"], "converged": res["converged"], "tuning": res["tuning"]}
+    eigs = res["eigs"]
+    ctx.close()
+    return out, eigs
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-scf", action="store_true")
+    ap.add_argument("--scf", default="cfg2,cfg4")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -50,36 +97,30 @@ def main():
     ctx = su.context()
     ctx.scf_init(spec.nelec, kT=spec.kT, seed=spec.seed)
     ctx.set_vloc(su.vext)
-    setup_s = time.perf_counter() - t0
     nb = spec.nbands
-    print(f"# setup {setup_s:.2f}s grid {su.dims} npw {su.npw[0]} bands {nb}", file=sys.stderr, flush=True)
+    log(f"# setup {time.perf_counter() - t0:.2f}s grid {su.dims} npw {su.npw[0]} bands {nb}")
+    bands_per_s, ms_per_step, prof = hpsi_rate(ctx, nb, args.steps, args.warmup)
+    log(f"# H|psi> FP64: {ms_per_step:.3f} ms per {nb} bands, {bands_per_s:.1f} bands/s")
+    ctx.close()
+    # the same H|psi> in complex64 (the mixed form's bulk)
+    ctx = su.context(lib.DEREK_MIXED)
+    ctx.scf_init(spec.nelec, kT=spec.kT, seed=spec.seed)
+    ctx.set_vloc(su.vext)
+    bps32, ms32, prof32 = hpsi_rate(ctx, nb, args.steps, args.warmup)
+    log(f"# H|psi> FP32: {ms32:.3f} ms per {nb} bands, {bps32:.1f} bands/s")
+    ctx.close()
 
-    for _ in range(args.warmup):
-        ctx.apply_h_resident(0, nb)
-    ctx.sync()
-    ctx.profile(True)
-    ctx.profile_reset()
-    ctx.sync()
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.apply_h_resident(0, nb)
-    ctx.sync()
-    t2 = time.perf_counter()
-    prof = ctx.profile_read()
-    ctx.profile(False)
-    dt = t2 - t1
-    ms_per_step = dt / args.steps * 1e3
-    bands_per_s = args.steps * nb / dt
-    print(f"# H|psi>: {ms_per_step:.3f} ms per {nb} bands, {bands_per_s:.1f} bands/s", file=sys.stderr, flush=True)
-
-    # dominant kernel of the timed region
+    # dominant kernel of the timed FP64 region
     busy = {k: v for k, v in prof.items() if v[0] > 0}
     dom = max(busy, key=lambda k: busy[k][1])
     n_l, tot_ms, by, _ = busy[dom]
     avg_s = tot_ms / n_l * 1e-3
     achieved = by / n_l / avg_s / 1e9
+    total = sum(x[1] for x in busy.values())
     kernels = {k: {"launches": v[0], "avg_us": v[1] / v[0] * 1e3, "GB_s": v[2] / (v[1] * 1e-3) / 1e9,
-                   "share": v[1] / sum(x[1] for x in busy.values())} for k, v in busy.items()}
+                   "share": v[1] / total} for k, v in busy.items()}
+    busy32 = {k: v for k, v in prof32.items() if v[0] > 0}
+    kernels32 = {k: {"avg_us": v[1] / v[0] * 1e3, "GB_s": v[2] / (v[1] * 1e-3) / 1e9} for k, v in busy32.items()}
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_cfg2.json")
     if os.path.exists(tfile):
@@ -88,19 +129,34 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                 "bytes_per_launch": by / n_l, "avg_launch_us": avg_s * 1e6}
 
-    scf_info = None
-    if not args.no_scf:
-        ctx.close()
-        ctx = su.context()
-        res = gscf.run(spec, max_scf=60, energy_tol=1e-6, setup=su, ctx=ctx, log=None)
-        t_dense = sum(h["t_dense"] for h in res["history"])
-        scf_info = {"fp64_time_to_scf_s": round(res["wall"], 4), "fp64_scf_steps": res["iterations"],
-                    "fp64_ms_per_scf_step": round(res["wall"] / res["iterations"] * 1e3, 2),
-                    "host_dense_share": round(t_dense / res["wall"], 3),
-                    "kT_ha": spec.kT, "etot_ha": res["etot"], "converged": res["converged"]}
-        print(f"# FP64 SCF: {res['iterations']} steps, {res['wall']:.3f} s, E = {res['etot']:.10f}",
-              file=sys.stderr, flush=True)
-    ctx.close()
+    scf_info = {}
+    todo = [] if args.no_scf else [x.strip() for x in args.scf.split(",") if x.strip()]
+    if "cfg2" in todo:
+        r64, e64 = scf_run(gscf, lib, spec, su, lib.DEREK_FP64, kbatch=1)
+        log(f"# cfg2 FP64: {r64['steps']} steps, {r64['time_to_scf_s']:.3f} s, E = {r64['etot_ha']:.10f}")
+        rmx, emx = scf_run(gscf, lib, spec, su, lib.DEREK_MIXED, graph=True)
+        log(f"# cfg2 mixed: {rmx['steps']} steps ({rmx['fp32_steps']} FP32), {rmx['time_to_scf_s']:.3f} s, "
+            f"E = {rmx['etot_ha']:.10f}")
+        scf_info["cfg2_fp64"] = r64
+        scf_info["cfg2_mixed"] = dict(rmx, dE_vs_fp64_ha=abs(rmx["etot_ha"] - r64["etot_ha"]),
+                                      max_deig_vs_fp64_ha=float(np.max(np.abs(emx - e64))),
+                                      speedup_vs_fp64=round(r64["time_to_scf_s"] / rmx["time_to_scf_s"], 3))
+    if "cfg4" in todo:
+        spec4 = config.benchmark(4)
+        su4 = gscf.Setup(spec4)
+        log(f"# cfg4 grid {su4.dims} nk {su4.nk} npw {min(su4.npw)}..{max(su4.npw)}")
+        p64, e_p = scf_run(gscf, lib, spec4, su4, lib.DEREK_FP64, kbatch=1)
+        log(f"# cfg4 FP64 plain: {p64['steps']} steps, {p64['time_to_scf_s']:.3f} s, E = {p64['etot_ha']:.10f}")
+        b64, e_b = scf_run(gscf, lib, spec4, su4, lib.DEREK_FP64, kbatch=0, graph=True)
+        log(f"# cfg4 FP64 batched: {b64['steps']} steps, {b64['time_to_scf_s']:.3f} s")
+        bmx, e_m = scf_run(gscf, lib, spec4, su4, lib.DEREK_MIXED, kbatch=0, graph=True, autotune=True)
+        log(f"# cfg4 mixed batched: {bmx['steps']} steps ({bmx['fp32_steps']} FP32), {bmx['time_to_scf_s']:.3f} s")
+        scf_info["cfg4_fp64_plain"] = p64
+        scf_info["cfg4_fp64_batched"] = dict(b64, dE_vs_plain_ha=abs(b64["etot_ha"] - p64["etot_ha"]),
+                                             speedup_vs_plain=round(p64["time_to_scf_s"] / b64["time_to_scf_s"], 3))
+        scf_info["cfg4_mixed_batched"] = dict(bmx, dE_vs_plain_ha=abs(bmx["etot_ha"] - p64["etot_ha"]),
+                                              max_deig_vs_plain_ha=float(np.max(np.abs(e_m - e_p))),
+                                              speedup_vs_plain=round(p64["time_to_scf_s"] / bmx["time_to_scf_s"], 3))
 
     cpu = None
     if not args.no_cpu:
@@ -127,7 +183,10 @@ def main():
         "config": {"workload": "plain FP64 H|psi>: cubic L=20 bohr, Ecut=30 Ha, 108^3 grid, 62669 plane waves, "
                                "80 bands (Gamma), 16 wells + 0.2 Ha cos", "bands": nb, "npw": su.npw[0],
                    "grid": su.dims, "parallelism": "single GPU"},
-        "roofline": roofline, "kernels": kernels, "scf": scf_info, "cpu_baseline": cpu,
+        "roofline": roofline, "kernels": kernels,
+        "hpsi_fp32": {"value": round(bps32, 2), "unit": "bands/s", "ms_per_step": round(ms32, 4),
+                      "kernels": kernels32},
+        "scf": scf_info, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 

@@ -1,0 +1,178 @@
+// The library context (derek_ctx): grid, per-k-point spheres, device
+// buffers of the pruned FFT pipeline, profiler, and the batched SCF engine.
+#pragma once
+#include <algorithm>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/derek_hip.h"
+#include "dense_gpu.hpp"
+#include "launch.hpp"
+#include "pw_fft.hpp"
+#include "scf_kernels.hpp"
+
+namespace derek {
+
+// Device buffer that grows on demand.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T> T* get(size_t n) {
+    const size_t need = std::max<size_t>(n, 1) * sizeof(T);
+    if (need > bytes) {
+      release();
+      DEREK_HIP(hipMalloc(&p, need));
+      bytes = need;
+    }
+    return reinterpret_cast<T*>(p);
+  }
+};
+
+template <typename T> T* dmalloc(size_t n) {
+  void* p = nullptr;
+  DEREK_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+  return reinterpret_cast<T*>(p);
+}
+
+inline void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+inline int64_t round_up(int64_t a, int64_t b) { return ceil_div(a, b) * b; }
+
+// Per-kernel-class timers: HIP events recorded on the context's stream around
+// each launch while enabled, resolved lazily; algorithmic bytes and flops of
+// each launch are accumulated beside the times.
+enum KernelClass {
+  KC_STICKS_Z_INV = 0,
+  KC_PLANES_Y_INV,
+  KC_PLANES_X_VMUL,
+  KC_PLANES_Y_FWD,
+  KC_STICKS_Z_FWD,
+  KC_PLANES_X_DENSITY,
+  KC_OVERLAP,
+  KC_ROTATE,
+  KC_FFT_Z,
+  KC_FFT_Y,
+  KC_FFT_X,
+  KC_DENSE,
+  KC_OTHER,
+  KC_COUNT
+};
+
+struct Profiler {
+  struct Pending {
+    int cls;
+    hipEvent_t a, b;
+    double bytes, flops;
+  };
+  bool on = false;
+  bool paused = false;  // no events while a graph is being captured
+  std::vector<hipEvent_t> pool;
+  std::vector<Pending> pending;
+  int64_t count[KC_COUNT] = {};
+  double ms[KC_COUNT] = {}, bytes[KC_COUNT] = {}, flops[KC_COUNT] = {};
+  hipEvent_t take();
+  void resolve();
+  void reset();
+  ~Profiler();
+};
+
+struct KState {
+  int64_t npw = 0;
+  int32_t* idx = nullptr;
+  double* kin = nullptr;
+  double weight = 0.0;
+  bool set = false;
+  PwMaps maps;                  // sphere maps (device pointers into mapbuf)
+  int32_t* mapbuf = nullptr;
+};
+
+struct ScfEngine;
+struct EngineDeleter {
+  void operator()(ScfEngine* e) const;  // defined with the engine (scf.hip)
+};
+
+}  // namespace derek
+
+struct derek_ctx {
+  int dims[3] = {0, 0, 0};
+  int64_t N = 0;
+  int nk = 0, nb = 0, prec = DEREK_FP64;
+  double volume = 0.0;
+  int64_t max_npw = 0;
+  hipStream_t s = nullptr;
+  derek::FftPlan plan[3];
+  double2* tw64[3] = {nullptr, nullptr, nullptr};
+  float2* tw32[3] = {nullptr, nullptr, nullptr};
+  derek::Axis<double> ax64[3];
+  derek::Axis<float> ax32[3];
+  std::vector<derek::KState> ks;
+  derek::PwMaps* d_maps = nullptr;  // device table of every k-point's maps
+  int* d_npw = nullptr;             // [nk]
+  double* d_wk = nullptr;           // [nk] k-point weights
+  int nst_max = 0, nax_max = 0;     // largest stick / plane count over the k-points
+  double* gsq = nullptr;
+  double* vext = nullptr;
+  double* vloc = nullptr;
+  float* vloc32 = nullptr;
+  bool gsq_set = false, vloc_set = false, vext_set = false;
+  // launch shapes (autotuned by the SCF engine when asked)
+  int bpw = 4;         // bands per x-pass workgroup
+  int item_cap = 64;   // (k-point, band) items per pipeline launch
+  // pruned-FFT work buffers for item_cap items
+  derek::DevBuf S64, P64, S32, P32;
+  template <typename R> derek::cx_t<R>* sticks(int items);
+  template <typename R> derek::cx_t<R>* planes(int items);
+  double2* cgrid = nullptr;
+  derek::DevBuf tA, tB, tC, tD, ovwork, partial;
+  derek::Profiler prof;
+  std::unique_ptr<derek::ScfEngine, derek::EngineDeleter> scf;
+  template <typename F> void timed(int cls, double bytes, double flops, F&& launch) {
+    if (!prof.on || prof.paused) {
+      launch();
+      return;
+    }
+    hipEvent_t a = prof.take(), b = prof.take();
+    DEREK_HIP(hipEventRecord(a, s));
+    launch();
+    DEREK_HIP(hipEventRecord(b, s));
+    prof.pending.push_back({cls, a, b, bytes, flops});
+    if (prof.pending.size() > 4096) prof.resolve();
+  }
+
+  derek_ctx();
+  ~derek_ctx();
+  template <typename R> const derek::Axis<R>* axes() const;
+  template <typename R> const R* vlocal() const;
+  void require_basis(int k) const;
+  void require_all_bases() const;
+  void sync() { DEREK_HIP(hipStreamSynchronize(s)); }
+
+  // Pipelines over the items (k, b) for k = k0..k0+nkc-1, b < nbands; psi
+  // and hpsi blocks as in PwBlock.  Launches are split so that no launch has
+  // more than item_cap items.
+  template <typename R>
+  void apply_h(int k0, int nkc, derek::PwBlock<R> psi, derek::PwBlock<R> hpsi, int nbands);
+  template <typename R>
+  void density(int k0, int nkc, derek::PwBlock<R> psi, int nbands, const double* occw, int occ_ks, double* rho,
+               bool keep);
+  template <typename R> void fft3d(derek::cx_t<R>* g, int howmany, int dir);
+  // Grid operations on device buffers (FP64); energies land in rep[slot]
+  // when rep is given (no host synchronisation).
+  void hartree(const double* rho, double* vh_out, double* rep, int slot);
+  void lda(const double* rho, double* vxc_out, double* rep, int slot);
+  void dot_into(const double* a, const double* b, int64_t n, double scale, bool root, double* rep, int slot);
+  void kerker(const double* f, double* out, double q0);
+};

@@ -21,11 +21,14 @@
 #include <cfloat>
 
 #include "dense_gpu.hpp"
+#include "launch.hpp"
 
 namespace derek {
 
 constexpr int DT = 1024;  // threads per workgroup
 constexpr int DW = DT / 64;
+// LDS of the matrix-resident tridiagonalisation (gfx950: 160 KiB per workgroup)
+constexpr size_t kTridiagLds = 156 * 1024;
 
 __device__ __forceinline__ double2 zmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -162,86 +165,127 @@ int64_t heev_work_doubles(int n, int nev) {
 }
 
 // ---- 1. Householder tridiagonalisation + phases (one workgroup per matrix)
-__global__ __launch_bounds__(DT) void k_tridiag(double2* __restrict__ Aall, int lda, int64_t strideA, int n, int nev,
-                                               double* __restrict__ work, int64_t strideWork) {
+// Wave 0 forms the reflector of the next column while it applies the
+// rank-2 update to that column (the column stays in its registers), so one
+// step costs three barriers: reflector -> v, v -> u = A v, u -> update.
+// IN_LDS: the whole matrix is copied to LDS first (n <= 99), else it is
+// updated in place in global memory (L2 resident).
+__device__ __forceinline__ void tri_reflector(int tid, int k, int n, double xn2, double2 x0, int* s_skip,
+                                              double* s_scal, HeevWork& W) {
+  if (tid != 0) return;
+  const double ax0 = sqrt(x0.x * x0.x + x0.y * x0.y);
+  const double rest = xn2 - ax0 * ax0;
+  const int skip = (xn2 == 0.0 || rest <= 1e-300) ? 1 : 0;
+  *s_skip = skip;
+  W.skip[k] = skip;
+  if (skip) {
+    W.sub[k] = x0;
+  } else {
+    const double xn = sqrt(xn2);
+    const double2 phs = ax0 > 0.0 ? make_double2(x0.x / ax0, x0.y / ax0) : make_double2(1.0, 0.0);
+    const double2 alpha = make_double2(-phs.x * xn, -phs.y * xn);
+    const double2 v0 = make_double2(x0.x - alpha.x, x0.y - alpha.y);
+    const double vn2 = rest + v0.x * v0.x + v0.y * v0.y;
+    W.sub[k] = alpha;
+    s_scal[0] = 1.0 / sqrt(vn2);
+    s_scal[1] = v0.x;
+    s_scal[2] = v0.y;
+  }
+}
+
+template <bool IN_LDS>
+__global__ __launch_bounds__(DT) void k_tridiag(double2* __restrict__ Aall, int lda_g, int64_t strideA, int n,
+                                               int nev, double* __restrict__ work, int64_t strideWork) {
   extern __shared__ __align__(16) unsigned char smem[];
   double2* v = reinterpret_cast<double2*>(smem);  // [n]
   double2* u = v + n;                              // [n]
-  double* red = reinterpret_cast<double*>(u + n);  // [DW]
-  __shared__ int s_skip;
-  __shared__ double s_scal[4];
+  double2* As = u + n;                             // [n * n] when IN_LDS
+  // reflector scalars, double-buffered by the parity of the step: step k
+  // reads slot k&1 while wave 0 writes slot (k+1)&1 for the next step
+  __shared__ int s_skip[2];
+  __shared__ double s_scal[2][4];
   const int mat = blockIdx.x;
-  double2* A = Aall + mat * strideA;
+  double2* Ag = Aall + mat * strideA;
   HeevWork W(work + mat * strideWork, n, nev);
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  for (int k = 0; k + 2 < n; ++k) {
+  double2* A = IN_LDS ? As : Ag;
+  const int lda = IN_LDS ? n : lda_g;
+  if (IN_LDS) {
+    for (int q = tid; q < n * n; q += DT) As[q] = Ag[(q % n) + (int64_t)(q / n) * lda_g];
+    __syncthreads();
+  }
+  if (n > 2 && wid == 0) {
     double part = 0.0;
-    for (int i = k + 1 + tid; i < n; i += DT) {
-      const double2 a = A[i + (int64_t)k * lda];
+    for (int i = 1 + lane; i < n; i += 64) {
+      const double2 a = A[i];
       part += a.x * a.x + a.y * a.y;
     }
-    const double xn2 = block_sum_all(part, red);
-    const double2 x0 = A[(k + 1) + (int64_t)k * lda];
-    const double ax0 = sqrt(x0.x * x0.x + x0.y * x0.y);
-    if (tid == 0) {
-      const double rest = xn2 - ax0 * ax0;
-      s_skip = (xn2 == 0.0 || rest <= 1e-300) ? 1 : 0;
-      W.skip[k] = s_skip;
-      if (s_skip) {
-        W.sub[k] = x0;
-      } else {
-        const double xn = sqrt(xn2);
-        const double2 phs = ax0 > 0.0 ? make_double2(x0.x / ax0, x0.y / ax0) : make_double2(1.0, 0.0);
-        const double2 alpha = make_double2(-phs.x * xn, -phs.y * xn);
-        const double2 v0 = make_double2(x0.x - alpha.x, x0.y - alpha.y);
-        const double vn2 = rest + v0.x * v0.x + v0.y * v0.y;
-        W.sub[k] = alpha;
-        s_scal[0] = 1.0 / sqrt(vn2);
-        s_scal[1] = v0.x;
-        s_scal[2] = v0.y;
+    part = wave_sum(part);
+    tri_reflector(tid, 0, n, part, A[1], &s_skip[0], s_scal[0], W);
+  }
+  __syncthreads();
+  for (int k = 0; k + 2 < n; ++k) {
+    const bool last = k + 3 >= n;  // no reflector for column k+1 after this step
+    const int cur = k & 1, nxt = cur ^ 1;
+    if (!s_skip[cur]) {
+      const double iv = s_scal[cur][0];
+      for (int i = k + 1 + tid; i < n; i += DT) {
+        const double2 a = i == k + 1 ? make_double2(s_scal[cur][1], s_scal[cur][2]) : A[i + (int64_t)k * lda];
+        v[i] = make_double2(a.x * iv, a.y * iv);
       }
-    }
-    __syncthreads();
-    if (s_skip) continue;
-    const double iv = s_scal[0];
-    for (int i = k + 1 + tid; i < n; i += DT) {
-      double2 a = i == k + 1 ? make_double2(s_scal[1], s_scal[2]) : A[i + (int64_t)k * lda];
-      v[i] = make_double2(a.x * iv, a.y * iv);
-    }
-    __syncthreads();
-    // u_i = sum_j A(i,j) v_j = sum_j conj(A(j,i)) v_j  (column i), one wave per column
-    for (int i = k + 1 + wid; i < n; i += DW) {
-      double sr = 0.0, si = 0.0;
-      const double2* col = A + (int64_t)i * lda;
-      for (int j = k + 1 + lane; j < n; j += 64) {
-        const double2 t = zconjmul(col[j], v[j]);
-        sr += t.x;
-        si += t.y;
+      __syncthreads();
+      // u_i = sum_j A(i,j) v_j = sum_j conj(A(j,i)) v_j  (column i), one wave per column
+      for (int i = k + 1 + wid; i < n; i += DW) {
+        double sr = 0.0, si = 0.0;
+        const double2* col = A + (int64_t)i * lda;
+        for (int j = k + 1 + lane; j < n; j += 64) {
+          const double2 t = zconjmul(col[j], v[j]);
+          sr += t.x;
+          si += t.y;
+        }
+        sr = wave_sum(sr);
+        si = wave_sum(si);
+        if (lane == 0) u[i] = make_double2(sr, si);
       }
-      sr = wave_sum(sr);
-      si = wave_sum(si);
-      if (lane == 0) u[i] = make_double2(sr, si);
+      __syncthreads();
+      // gamma = v^H u, every wave on its own; w = 2u - 2 gamma v
+      double gp = 0.0;
+      for (int i = k + 1 + lane; i < n; i += 64) gp += zconjmul(v[i], u[i]).x;
+      const double g2 = 2.0 * wave_sum(gp);
+      // A22 -= v w^H + w v^H, one wave per column; the reflector replaces column k
+      for (int j = k + 1 + wid; j < n; j += DW) {
+        const double2 vj = v[j], uj = u[j];
+        const double2 wj = make_double2(2.0 * uj.x - g2 * vj.x, 2.0 * uj.y - g2 * vj.y);
+        double2* col = A + (int64_t)j * lda;
+        double part = 0.0;
+        double2 x0 = make_double2(0.0, 0.0);
+        for (int i = k + 1 + lane; i < n; i += 64) {
+          const double2 vi = v[i], ui = u[i];
+          const double2 wi = make_double2(2.0 * ui.x - g2 * vi.x, 2.0 * ui.y - g2 * vi.y);
+          const double2 t1 = zmul(vi, make_double2(wj.x, -wj.y));
+          const double2 t2 = zmul(wi, make_double2(vj.x, -vj.y));
+          const double2 a = col[i];
+          const double2 r = make_double2(a.x - t1.x - t2.x, a.y - t1.y - t2.y);
+          col[i] = r;
+          if (i > j) part += r.x * r.x + r.y * r.y;
+          if (i == j + 1) x0 = r;
+        }
+        if (j == k + 1 && !last) {
+          part = wave_sum(part);
+          x0 = make_double2(wave_sum(x0.x), wave_sum(x0.y));  // only one lane holds it
+          tri_reflector(lane, k + 1, n, part, x0, &s_skip[nxt], s_scal[nxt], W);
+        }
+      }
+      for (int i = k + 1 + tid; i < n; i += DT) A[i + (int64_t)k * lda] = v[i];  // keep the reflector
+    } else if (wid == 0 && !last) {
+      double part = 0.0;
+      for (int i = k + 2 + lane; i < n; i += 64) {
+        const double2 a = A[i + (int64_t)(k + 1) * lda];
+        part += a.x * a.x + a.y * a.y;
+      }
+      part = wave_sum(part);
+      tri_reflector(lane, k + 1, n, part, A[(k + 2) + (int64_t)(k + 1) * lda], &s_skip[nxt], s_scal[nxt], W);
     }
-    __syncthreads();
-    double gp = 0.0;
-    for (int i = k + 1 + tid; i < n; i += DT) gp += zconjmul(v[i], u[i]).x;
-    const double gam = block_sum_all(gp, red);
-    for (int i = k + 1 + tid; i < n; i += DT)
-      u[i] = make_double2(2.0 * u[i].x - 2.0 * gam * v[i].x, 2.0 * u[i].y - 2.0 * gam * v[i].y);
-    __syncthreads();
-    // A22 -= v w^H + w v^H  (w held in u)
-    const int m = n - k - 1;
-    const int64_t mm = (int64_t)m * m;
-    for (int64_t q = tid; q < mm; q += DT) {
-      const int i = k + 1 + (int)(q % m), j = k + 1 + (int)(q / m);
-      const double2 vi = v[i], wi = u[i], vj = v[j], wj = u[j];
-      double2* p = A + i + (int64_t)j * lda;
-      const double2 t1 = zmul(vi, make_double2(wj.x, -wj.y));
-      const double2 t2 = zmul(wi, make_double2(vj.x, -vj.y));
-      *p = make_double2(p->x - t1.x - t2.x, p->y - t1.y - t2.y);
-    }
-    __syncthreads();
-    for (int i = k + 1 + tid; i < n; i += DT) A[i + (int64_t)k * lda] = v[i];  // keep the reflector
     __syncthreads();
   }
   if (tid == 0) {
@@ -252,6 +296,10 @@ __global__ __launch_bounds__(DT) void k_tridiag(double2* __restrict__ Aall, int 
     W.skip[n - 1] = 1;
   }
   for (int i = tid; i < n; i += DT) W.d[i] = A[i + (int64_t)i * lda].x;
+  if (IN_LDS) {
+    // the reflectors (strict lower triangle) are read by the back-transformation
+    for (int q = tid; q < n * n; q += DT) Ag[(q % n) + (int64_t)(q / n) * lda_g] = As[q];
+  }
   __syncthreads();
   // phases: T' = D^H T D real with off-diagonal |sub|
   if (tid == 0) {
@@ -445,8 +493,15 @@ void heev_gpu(double2* A, int lda, int64_t strideA, int n, int nev, double* Wv, 
   DEREK_REQUIRE(n >= 1 && nev >= 1 && nev <= n, "heev: need 1 <= nev <= n");
   DEREK_REQUIRE(n <= 2048, "heev: n must be <= 2048");
   {
-    const size_t lds = sizeof(double2) * 2 * (size_t)n + sizeof(double) * DW;
-    hipLaunchKernelGGL(k_tridiag, dim3(batch), dim3(DT), lds, s, A, lda, strideA, n, nev, work, strideWork);
+    const size_t lds_g = sizeof(double2) * 2 * (size_t)n;
+    const size_t lds_l = lds_g + sizeof(double2) * (size_t)n * n;
+    if (lds_l <= kTridiagLds) {
+      allow_dynamic_lds(reinterpret_cast<const void*>(k_tridiag<true>), kTridiagLds);
+      hipLaunchKernelGGL(k_tridiag<true>, dim3(batch), dim3(DT), lds_l, s, A, lda, strideA, n, nev, work, strideWork);
+    } else {
+      hipLaunchKernelGGL(k_tridiag<false>, dim3(batch), dim3(DT), lds_g, s, A, lda, strideA, n, nev, work,
+                         strideWork);
+    }
     DEREK_HIP(hipGetLastError());
   }
   {
@@ -474,22 +529,55 @@ void heev_gpu(double2* A, int lda, int64_t strideA, int n, int nev, double* Wv, 
   }
 }
 
-// ---------------------------------------------------------------- Cholesky with dropping
-// Blocked right-looking on a work copy Wm of S (m x m, column-major, lower
-// triangle used).  A panel of nb columns is staged in LDS and factored there:
-// column j is kept if its updated pivot exceeds tol * S_jj; a kept column
-// becomes L's column c (stored row-major in Lb: Lb[i*m + c]) and updates the
-// rest of the panel; a dropped column is zeroed and contributes nothing.
-// The panel then updates the trailing lower triangle in one pass (one
-// wavefront per column, rows over the lanes).
-__global__ __launch_bounds__(DT) void k_chol_drop(const double2* __restrict__ S, int m, int nb, double tol,
-                                                  double2* __restrict__ Wm, double2* __restrict__ Lb,
-                                                  int* __restrict__ act, int* __restrict__ na) {
+// ---------------------------------------------------------------- generalized problem pieces
+// All matrices m x m, column-major, ld m; matrix kz of a batch at kz*m*m
+// (blockIdx.y or blockIdx.x selects it).
+
+// dsc = 1/sqrt(diag S) (0 for a zero column), S <- D S D
+__global__ void k_unit_diag(double2* __restrict__ Sall, int m, double* __restrict__ dscall) {
   extern __shared__ __align__(16) unsigned char smem[];
-  double2* P = reinterpret_cast<double2*>(smem);  // [nb][m]: P[jj*m + i]
-  double* sd = reinterpret_cast<double*>(P + (size_t)nb * m);  // [m] diagonal of S
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  double* sd = reinterpret_cast<double*>(smem);
+  const int kz = blockIdx.y;
+  double2* S = Sall + (int64_t)kz * m * m;
+  double* dsc = dscall + (int64_t)kz * m;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    const double d = S[i + (int64_t)i * m].x;
+    const double v = d > 0.0 ? 1.0 / sqrt(d) : 0.0;
+    sd[i] = v;
+    dsc[i] = v;
+  }
+  __syncthreads();
   const int64_t mm = (int64_t)m * m;
+  for (int64_t q = threadIdx.x; q < mm; q += blockDim.x) {
+    const int r = (int)(q % m), c = (int)(q / m);
+    const double f = sd[r] * sd[c];
+    S[q] = make_double2(S[q].x * f, S[q].y * f);
+  }
+}
+
+// Blocked right-looking Cholesky of the unit-diagonal S with dropping, on a
+// work copy Wm (lower triangle used).  A panel of nb columns is staged in
+// LDS and factored there: column j is kept if its updated pivot exceeds
+// tol * S_jj; a kept column is L's column j and updates the rest of the
+// panel; a dropped column is zeroed and contributes nothing.  The panel then
+// updates the trailing lower triangle in one pass (one wavefront per column,
+// rows over the lanes).  Output: Lf (lower triangular) = L on the kept rows
+// and columns, the identity on the dropped ones, keep[j], na = #kept, ok =
+// (na >= nev).
+__global__ __launch_bounds__(DT) void k_chol_drop(const double2* __restrict__ Sall, int m, int nb, double tol,
+                                                  int nev, double2* __restrict__ Wall, double2* __restrict__ Lall,
+                                                  int* __restrict__ keepall, int* __restrict__ naall,
+                                                  int* __restrict__ okall) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double2* P = reinterpret_cast<double2*>(smem);               // [nb][m]: P[jj*m + i]
+  double* sd = reinterpret_cast<double*>(P + (size_t)nb * m);  // [m] diagonal of S
+  int* kp = reinterpret_cast<int*>(sd + m);                    // [m] keep flags
+  const int kz = blockIdx.x;
+  const int64_t mm = (int64_t)m * m;
+  const double2* S = Sall + kz * mm;
+  double2* Wm = Wall + kz * mm;
+  double2* Lf = Lall + kz * mm;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   for (int64_t q = tid; q < mm; q += DT) Wm[q] = S[q];
   for (int i = tid; i < m; i += DT) sd[i] = S[i + (int64_t)i * m].x;
   __syncthreads();
@@ -507,14 +595,15 @@ __global__ __launch_bounds__(DT) void k_chol_drop(const double2* __restrict__ S,
       const double d = pc[j].x;
       if (d > tol * sd[j]) {
         const double lj = sqrt(d), idj = 1.0 / lj;
+        double2* lcol = Lf + (int64_t)j * m;
         for (int i = j + 1 + tid; i < m; i += DT) {
           const double2 l = make_double2(pc[i].x * idj, pc[i].y * idj);
           pc[i] = l;
-          Lb[(int64_t)i * m + c] = l;
+          lcol[i] = l;
         }
         if (tid == 0) {
-          Lb[(int64_t)j * m + c] = make_double2(lj, 0.0);
-          act[c] = j;
+          lcol[j] = make_double2(lj, 0.0);
+          kp[j] = 1;
         }
         __syncthreads();
         // rest of the panel: P(i,kk) -= l_i conj(l_(p0+kk)), i >= p0+kk
@@ -532,6 +621,7 @@ __global__ __launch_bounds__(DT) void k_chol_drop(const double2* __restrict__ S,
         // only column jj's own rows are written; the next column reads
         // column jj+1, and the trailing update waits for a barrier
         for (int i = j + tid; i < m; i += DT) pc[i] = make_double2(0.0, 0.0);
+        if (tid == 0) kp[j] = 0;
       }
     }
     __syncthreads();
@@ -550,32 +640,25 @@ __global__ __launch_bounds__(DT) void k_chol_drop(const double2* __restrict__ S,
     }
     __syncthreads();
   }
-  if (tid == 0) *na = c;
-}
-
-// Compact: Lc (na x na, column-major, ld na) from the row-major build buffer
-// (rows act[], kept columns), Hc = D H D restricted to act (na x na).
-__global__ void k_chol_compact(const double2* __restrict__ Lb, int m, const int* __restrict__ act,
-                               const int* __restrict__ na_p, const double2* __restrict__ H, int ldh,
-                               const double* __restrict__ dsc, double2* __restrict__ Lc, double2* __restrict__ Hc) {
-  const int na = *na_p;
-  const int64_t nn = (int64_t)na * na;
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nn; q += (int64_t)gridDim.x * blockDim.x) {
-    const int r = (int)(q % na), c = (int)(q / na);
-    Lc[q] = r >= c ? Lb[(int64_t)act[r] * m + c] : make_double2(0.0, 0.0);
-    const int ar = act[r], ac = act[c];
-    const double2 h = H[ar + (int64_t)ac * ldh];
-    const double sc = dsc[ar] * dsc[ac];
-    Hc[q] = make_double2(h.x * sc, h.y * sc);
+  // upper triangle zero; dropped rows and columns -> identity
+  for (int k = wid; k < m; k += DW) {
+    double2* lcol = Lf + (int64_t)k * m;
+    for (int i = lane; i < m; i += 64)
+      if (i < k || !kp[i] || !kp[k]) lcol[i] = make_double2(i == k ? 1.0 : 0.0, 0.0);
+  }
+  for (int i = tid; i < m; i += DT) keepall[(int64_t)kz * m + i] = kp[i];
+  if (tid == 0) {
+    naall[kz] = c;
+    okall[kz] = c >= nev ? 1 : 0;
   }
 }
 
-// Inverse of a lower-triangular na x na matrix (column-major, ld na), one
-// wavefront per column j: forward substitution on e_j with the solution and
-// the current column of L in registers (row i lives in lane i % 64, slot
-// i / 64); the pivot and x_k are read across lanes with readlane, and the
-// next column of L is loaded while the current one is applied.
-constexpr int TR = 10;  // rows per lane: na <= 640
+// Inverse of a lower-triangular m x m matrix, one wavefront per column j:
+// forward substitution on e_j with the solution and the current column of L
+// in registers (row i lives in lane i % 64, slot i / 64); the pivot and x_k
+// are read across lanes with readlane, and the next column of L is loaded
+// while the current one is applied.
+constexpr int TR = 10;  // rows per lane: m <= 640
 
 __device__ __forceinline__ double readlane_d(double v, int l) {
   const long long b = __double_as_longlong(v);
@@ -584,24 +667,24 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-__global__ __launch_bounds__(64) void k_trinv(const double2* __restrict__ L, const int* __restrict__ na_p,
-                                              double2* __restrict__ Linv) {
-  const int na = *na_p;
+__global__ __launch_bounds__(64) void k_trinv(const double2* __restrict__ Lall, int m,
+                                              double2* __restrict__ Linvall) {
   const int j = blockIdx.x, lane = threadIdx.x;
-  if (j >= na) return;
+  const int64_t mm = (int64_t)m * m;
+  const double2* L = Lall + blockIdx.y * mm;
   double2 x[TR], cur[TR];
 #pragma unroll
   for (int t = 0; t < TR; ++t) {
     const int i = lane + 64 * t;
     x[t] = make_double2(i == j ? 1.0 : 0.0, 0.0);
-    cur[t] = (i >= j && i < na) ? L[i + (int64_t)j * na] : make_double2(0.0, 0.0);
+    cur[t] = (i >= j && i < m) ? L[i + (int64_t)j * m] : make_double2(0.0, 0.0);
   }
-  for (int k = j; k < na; ++k) {
+  for (int k = j; k < m; ++k) {
     double2 nxt[TR];
 #pragma unroll
     for (int t = 0; t < TR; ++t) {
       const int i = lane + 64 * t;
-      nxt[t] = (i > k && i < na) ? L[i + (int64_t)(k + 1) * na] : make_double2(0.0, 0.0);
+      nxt[t] = (i > k && i < m) ? L[i + (int64_t)(k + 1) * m] : make_double2(0.0, 0.0);
     }
     const int tk = k >> 6, lk = k & 63;
     double2 xo = make_double2(0.0, 0.0), dg = make_double2(1.0, 0.0);
@@ -627,124 +710,159 @@ __global__ __launch_bounds__(64) void k_trinv(const double2* __restrict__ L, con
       cur[t] = nxt[t];
     }
   }
-  double2* out = Linv + (int64_t)j * na;
+  double2* out = Linvall + blockIdx.y * mm + (int64_t)j * m;
 #pragma unroll
   for (int t = 0; t < TR; ++t) {
     const int i = lane + 64 * t;
-    if (i < na) out[i] = x[t];
+    if (i < m) out[i] = x[t];
   }
 }
 
-// B = A^H for an na x na matrix (column-major, ld na)
-__global__ void k_conj_transpose(const double2* __restrict__ A, const int* __restrict__ na_p, double2* __restrict__ B) {
-  const int na = *na_p;
-  const int64_t nn = (int64_t)na * na;
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nn; q += (int64_t)gridDim.x * blockDim.x) {
-    const int r = (int)(q % na), c = (int)(q / na);
-    const double2 a = A[c + (int64_t)r * na];
+// B = A^H
+__global__ void k_conj_transpose(const double2* __restrict__ A, int m, double2* __restrict__ B) {
+  const int64_t mm = (int64_t)m * m;
+  A += blockIdx.y * mm;
+  B += blockIdx.y * mm;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < mm; q += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(q % m), c = (int)(q / m);
+    const double2 a = A[c + (int64_t)r * m];
     B[q] = make_double2(a.x, -a.y);
   }
 }
 
-// C (m x nev, ld m) = 0 except rows act[r] = dsc[act[r]] * Ca[r, :]
-__global__ void k_expand_rows(const double2* __restrict__ Ca, const int* __restrict__ na_p,
-                              const int* __restrict__ act, const double* __restrict__ dsc, int m, int nev,
-                              double2* __restrict__ C) {
-  const int na = *na_p;
-  const int64_t tot = (int64_t)m * nev;
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x)
-    C[q] = make_double2(0.0, 0.0);
-  __syncthreads();
-  const int64_t t2 = (int64_t)na * nev;
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < t2; q += (int64_t)gridDim.x * blockDim.x) {
-    const int r = (int)(q % na), c = (int)(q / na);
-    const double2 v = Ca[q];
-    const double s = dsc[act[r]];
-    C[act[r] + (int64_t)c * m] = make_double2(v.x * s, v.y * s);
-  }
-}
-
-// dsc[i] = 1/sqrt(S_ii); S <- D S D
-__global__ void k_unit_diag(double2* __restrict__ S, int m, double* __restrict__ dsc) {
-  __shared__ double sd[2048];
-  for (int i = threadIdx.x; i < m; i += blockDim.x) {
-    const double v = 1.0 / sqrt(S[i + (int64_t)i * m].x);
-    sd[i] = v;
-    dsc[i] = v;
-  }
-  __syncthreads();
+// Hc = (D H D) on the kept rows and columns, 0 elsewhere
+__global__ void k_scale_mask(const double2* __restrict__ H, const double* __restrict__ dsc,
+                             const int* __restrict__ keep, int m, double2* __restrict__ Hc) {
   const int64_t mm = (int64_t)m * m;
-  for (int64_t q = threadIdx.x; q < mm; q += blockDim.x) {
-    const int r = (int)(q % m), c = (int)(q / m);
-    const double s = sd[r] * sd[c];
-    S[q] = make_double2(S[q].x * s, S[q].y * s);
-  }
-}
-
-void hegv_gpu_prepare(double2* S, int m, double* dsc, hipStream_t s) {
-  DEREK_REQUIRE(m <= 2048, "hegv: at most 2048 columns");
-  hipLaunchKernelGGL(k_unit_diag, dim3(1), dim3(1024), 0, s, S, m, dsc);
-  DEREK_HIP(hipGetLastError());
-}
-
-void chol_drop_gpu(const double2* S, int m, double tol, double2* Wm, double2* Lbuild, int* act, int* na,
-                   hipStream_t s) {
-  constexpr size_t kLds = 60 * 1024;
-  DEREK_REQUIRE(m >= 1 && (size_t)m * (sizeof(double2) + sizeof(double)) <= kLds, "chol_drop: matrix too large");
-  const int nb = (int)std::min<size_t>(16, (kLds - sizeof(double) * m) / (sizeof(double2) * m));
-  const size_t lds = sizeof(double2) * (size_t)nb * m + sizeof(double) * m;
-  hipLaunchKernelGGL(k_chol_drop, dim3(1), dim3(DT), lds, s, S, m, nb, tol, Wm, Lbuild, act, na);
-  DEREK_HIP(hipGetLastError());
-}
-
-void chol_compact_gpu(const double2* Lbuild, int m, const int* act, const int* na, const double2* H, int ldh,
-                      const double* dsc, double2* Lc, double2* Hc, hipStream_t s) {
-  hipLaunchKernelGGL(k_chol_compact, dim3((unsigned)std::min<int64_t>(1024, ceil_div((int64_t)m * m, 256))),
-                     dim3(256), 0, s, Lbuild, m, act, na, H, ldh, dsc, Lc, Hc);
-  DEREK_HIP(hipGetLastError());
-}
-
-void trinv_gpu(const double2* L, const int* na, int m, double2* Linv, hipStream_t s) {
-  DEREK_REQUIRE(m <= 64 * TR, "trinv: matrix too large");
-  hipLaunchKernelGGL(k_trinv, dim3((unsigned)m), dim3(64), 0, s, L, na, Linv);
-  DEREK_HIP(hipGetLastError());
-}
-
-void conj_transpose_gpu(const double2* A, const int* na, int m, double2* B, hipStream_t s) {
-  hipLaunchKernelGGL(k_conj_transpose, dim3((unsigned)std::min<int64_t>(1024, ceil_div((int64_t)m * m, 256))),
-                     dim3(256), 0, s, A, na, B);
-  DEREK_HIP(hipGetLastError());
-}
-
-void expand_rows_gpu(const double2* Ca, const int* na, const int* act, const double* dsc, int m, int nev, double2* C,
-                     hipStream_t s) {
-  hipLaunchKernelGGL(k_expand_rows, dim3(1), dim3(1024), 0, s, Ca, na, act, dsc, m, nev, C);
-  DEREK_HIP(hipGetLastError());
-}
-
-}  // namespace derek
-
-namespace derek {
-
-__global__ void k_hermitize(double2* __restrict__ A, int m) {
-  const int64_t mm = (int64_t)m * m;
+  const int kz = blockIdx.y;
+  H += kz * mm;
+  Hc += kz * mm;
+  dsc += (int64_t)kz * m;
+  keep += (int64_t)kz * m;
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < mm; q += (int64_t)gridDim.x * blockDim.x) {
     const int r = (int)(q % m), c = (int)(q / m);
-    if (r < c) continue;
-    const double2 a = A[q], b = A[c + (int64_t)r * m];
-    if (r == c) {
-      A[q] = make_double2(a.x, 0.0);
-    } else {
-      const double2 h = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
-      A[q] = h;
-      A[c + (int64_t)r * m] = make_double2(h.x, -h.y);
-    }
+    const double f = (keep[r] && keep[c]) ? dsc[r] * dsc[c] : 0.0;
+    Hc[q] = make_double2(H[q].x * f, H[q].y * f);
   }
 }
 
-void hermitize_gpu(double2* A, int m, hipStream_t s) {
-  hipLaunchKernelGGL(k_hermitize, dim3((unsigned)std::min<int64_t>(1024, ceil_div((int64_t)m * m, 256))), dim3(256), 0,
-                     s, A, m);
+// Dropped diagonal entries of A <- 2 max_i sum_j |A_ij| + 1 (above every
+// eigenvalue of the kept block), so the lowest eigenpairs are the kept ones.
+__global__ __launch_bounds__(DT) void k_big_diag(double2* __restrict__ Aall, const int* __restrict__ keepall, int m) {
+  __shared__ double red[DW];
+  const int kz = blockIdx.x;
+  double2* A = Aall + (int64_t)kz * m * m;
+  const int* keep = keepall + (int64_t)kz * m;
+  double rmax = 0.0;
+  for (int i = threadIdx.x; i < m; i += DT) {
+    double r = 0.0;
+    for (int j = 0; j < m; ++j) {
+      const double2 a = A[i + (int64_t)j * m];
+      r += sqrt(a.x * a.x + a.y * a.y);
+    }
+    rmax = fmax(rmax, r);
+  }
+  const double big = 2.0 * block_max_all(rmax, red) + 1.0;
+  for (int i = threadIdx.x; i < m; i += DT)
+    if (!keep[i]) A[i + (int64_t)i * m] = make_double2(big, 0.0);
+}
+
+// U (m x nev) = D C on the kept rows, 0 on the dropped ones; where ok == 0
+// the subspace step is void: U = [I; 0] and w keeps its previous values.
+__global__ void k_expand(const double2* __restrict__ Call, const double* __restrict__ dscall,
+                         const int* __restrict__ keepall, const int* __restrict__ okall, int m, int nev,
+                         const double* __restrict__ wnew, double* __restrict__ w, double2* __restrict__ Uall) {
+  const int kz = blockIdx.y;
+  const int ok = okall[kz];
+  const double2* C = Call + (int64_t)kz * m * m;
+  const double* dsc = dscall + (int64_t)kz * m;
+  const int* keep = keepall + (int64_t)kz * m;
+  double2* U = Uall + (int64_t)kz * m * nev;
+  const int64_t n = (int64_t)m * nev;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(q % m), c = (int)(q / m);
+    double2 v;
+    if (ok) {
+      const double f = keep[r] ? dsc[r] : 0.0;
+      const double2 x = C[r + (int64_t)c * m];
+      v = make_double2(x.x * f, x.y * f);
+    } else {
+      v = make_double2(r == c ? 1.0 : 0.0, 0.0);
+    }
+    U[q] = v;
+  }
+  if (blockIdx.x == 0 && ok)
+    for (int c = threadIdx.x; c < nev; c += blockDim.x) w[(int64_t)kz * nev + c] = wnew[(int64_t)kz * nev + c];
+}
+
+// A <- (A + A^H)/2
+__global__ void k_hermitize(double2* __restrict__ A, int m) {
+  const int64_t mm = (int64_t)m * m;
+  A += blockIdx.y * mm;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < mm; q += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(q % m), c = (int)(q / m);
+    if (r > c) continue;
+    const double2 a = A[q], b = A[c + (int64_t)r * m];
+    const double2 h = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+    A[q] = r == c ? make_double2(h.x, 0.0) : h;
+    if (r != c) A[c + (int64_t)r * m] = make_double2(h.x, -h.y);
+  }
+}
+
+static dim3 mgrid(int m, int batch) {
+  return dim3((unsigned)std::min<int64_t>(256, ceil_div((int64_t)m * m, 256)), (unsigned)batch);
+}
+
+void hermitize_gpu(double2* A, int m, int batch, hipStream_t s) {
+  hipLaunchKernelGGL(k_hermitize, mgrid(m, batch), dim3(256), 0, s, A, m);
+  DEREK_HIP(hipGetLastError());
+}
+
+void unit_diag_gpu(double2* S, int m, double* dsc, int batch, hipStream_t s) {
+  DEREK_REQUIRE(m <= 8192, "unit_diag: matrix too large");
+  // one workgroup per matrix: S is scaled in place after its diagonal is read
+  hipLaunchKernelGGL(k_unit_diag, dim3(1, batch), dim3(1024), sizeof(double) * m, s, S, m, dsc);
+  DEREK_HIP(hipGetLastError());
+}
+
+void chol_drop_gpu(const double2* S, int m, double tol, int nev, double2* Wm, double2* Lf, int* keep, int* na,
+                   int* ok, int batch, hipStream_t s) {
+  constexpr size_t kLds = 60 * 1024;
+  const size_t per_row = sizeof(double2) + sizeof(double) + sizeof(int);
+  DEREK_REQUIRE(m >= 1 && (size_t)m * per_row <= kLds, "chol_drop: matrix too large");
+  const int nb = (int)std::min<size_t>(16, (kLds - (sizeof(double) + sizeof(int)) * m) / (sizeof(double2) * m));
+  const size_t lds = sizeof(double2) * (size_t)nb * m + (sizeof(double) + sizeof(int)) * m;
+  hipLaunchKernelGGL(k_chol_drop, dim3(batch), dim3(DT), lds, s, S, m, nb, tol, nev, Wm, Lf, keep, na, ok);
+  DEREK_HIP(hipGetLastError());
+}
+
+void trinv_gpu(const double2* L, int m, double2* Linv, int batch, hipStream_t s) {
+  DEREK_REQUIRE(m <= 64 * TR, "trinv: matrix too large");
+  hipLaunchKernelGGL(k_trinv, dim3((unsigned)m, batch), dim3(64), 0, s, L, m, Linv);
+  DEREK_HIP(hipGetLastError());
+}
+
+void conj_transpose_gpu(const double2* A, int m, double2* B, int batch, hipStream_t s) {
+  hipLaunchKernelGGL(k_conj_transpose, mgrid(m, batch), dim3(256), 0, s, A, m, B);
+  DEREK_HIP(hipGetLastError());
+}
+
+void scale_mask_gpu(const double2* H, const double* dsc, const int* keep, int m, double2* Hc, int batch,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(k_scale_mask, mgrid(m, batch), dim3(256), 0, s, H, dsc, keep, m, Hc);
+  DEREK_HIP(hipGetLastError());
+}
+
+void big_diag_gpu(double2* A, const int* keep, int m, int batch, hipStream_t s) {
+  hipLaunchKernelGGL(k_big_diag, dim3(batch), dim3(DT), 0, s, A, keep, m);
+  DEREK_HIP(hipGetLastError());
+}
+
+void expand_gpu(const double2* C, const double* dsc, const int* keep, const int* ok, int m, int nev,
+                const double* wnew, double* w, double2* U, int batch, hipStream_t s) {
+  const int64_t n = (int64_t)m * nev;
+  hipLaunchKernelGGL(k_expand, dim3((unsigned)std::min<int64_t>(256, ceil_div(n, 256)), batch), dim3(256), 0, s, C,
+                     dsc, keep, ok, m, nev, wnew, w, U);
   DEREK_HIP(hipGetLastError());
 }
 

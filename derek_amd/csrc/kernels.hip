@@ -1,6 +1,6 @@
-// FFT passes, plane-wave <-> grid maps, XC/Hartree/mixing and Davidson
-// pointwise kernels for gfx950.
+// Full-grid FFT passes, LDA XC, Hartree/Kerker and pointwise kernels for gfx950.
 #include <mutex>
+#include <unordered_map>
 
 #include "launch.hpp"
 
@@ -34,11 +34,9 @@ __global__ __launch_bounds__(NT) void k_fft_contig(cx_t<Rl>* __restrict__ data, 
 }
 
 // Strided axis, tiles of TILE_LINES neighbouring inner indices.
-// MODE 0: one transform of sign S.  MODE 1: inverse, *vloc*vscale, forward.
-template <typename Rl, int S, int NC, int MODE>
+template <typename Rl, int S, int NC>
 __global__ __launch_bounds__(NT) void k_fft_tile(cx_t<Rl>* __restrict__ data, int64_t stride, int64_t tiles,
-                                                 FftPlan plan, const cx_t<Rl>* __restrict__ tw,
-                                                 const Rl* __restrict__ vloc, Rl vscale) {
+                                                 FftPlan plan, const cx_t<Rl>* __restrict__ tw) {
   using C = cx_t<Rl>;
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = NC > 0 ? NC : plan.n;
@@ -57,77 +55,27 @@ __global__ __launch_bounds__(NT) void k_fft_tile(cx_t<Rl>* __restrict__ data, in
     b0[e] = l < lines ? base[(int64_t)a * stride + l] : cmk<C>(0, 0);
   }
   __syncthreads();
-  C* res;
-  if constexpr (MODE == 0) {
-    res = fft_lds<S, true, TILE_LINES, NT, NC>(b0, b1, twl, plan, TILE_LINES);
-  } else {
-    res = fft_lds<+1, true, TILE_LINES, NT, NC>(b0, b1, twl, plan, TILE_LINES);
-    C* other = res == b0 ? b1 : b0;
-    for (int e = threadIdx.x; e < ne; e += NT) {
-      const int a = e / TILE_LINES, l = e - a * TILE_LINES;
-      const Rl v = l < lines ? vloc[(int64_t)a * stride + i0 + l] * vscale : (Rl)0;
-      res[e] = cscale(res[e], v);
-    }
-    __syncthreads();
-    res = fft_lds<-1, true, TILE_LINES, NT, NC>(res, other, twl, plan, TILE_LINES);
-  }
+  C* res = fft_lds<S, true, TILE_LINES, NT, NC>(b0, b1, twl, plan, TILE_LINES);
   for (int e = threadIdx.x; e < ne; e += NT) {
     const int a = e / TILE_LINES, l = e - a * TILE_LINES;
     if (l < lines) base[(int64_t)a * stride + l] = res[e];
   }
 }
 
-// Density: inverse transform along the strided axis for every outer block
-// (band), accumulating occ[o] |psi|^2 per point.
-template <typename Rl, int NC>
-__global__ __launch_bounds__(NT) void k_fft_tile_density(const cx_t<Rl>* __restrict__ data, int64_t nouter,
-                                                         int64_t stride, int64_t tiles, FftPlan plan,
-                                                         const cx_t<Rl>* __restrict__ tw,
-                                                         const double* __restrict__ occ,
-                                                         double* __restrict__ rho, double scale, int keep) {
-  using C = cx_t<Rl>;
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int N = NC > 0 ? NC : plan.n;
-  C* twl = reinterpret_cast<C*>(smem);
-  C* b0 = twl + N;
-  C* b1 = b0 + (size_t)TILE_LINES * N;
-  double* acc = reinterpret_cast<double*>(b1 + (size_t)TILE_LINES * N);
-  for (int t = threadIdx.x; t < N; t += NT) twl[t] = tw[t];
-  const int64_t i0 = (int64_t)blockIdx.x * TILE_LINES;
-  const int64_t rem = stride - i0;
-  const int lines = rem < TILE_LINES ? (int)rem : TILE_LINES;
-  const int ne = N * TILE_LINES;
-  for (int e = threadIdx.x; e < ne; e += NT) acc[e] = 0.0;
-  for (int64_t o = 0; o < nouter; ++o) {
-    const double w = occ[o];
-    if (w == 0.0) continue;
-    const C* base = data + o * (int64_t)N * stride + i0;
-    __syncthreads();
-    for (int e = threadIdx.x; e < ne; e += NT) {
-      const int a = e / TILE_LINES, l = e - a * TILE_LINES;
-      b0[e] = l < lines ? base[(int64_t)a * stride + l] : cmk<C>(0, 0);
-    }
-    __syncthreads();
-    C* res = fft_lds<+1, true, TILE_LINES, NT, NC>(b0, b1, twl, plan, TILE_LINES);
-    for (int e = threadIdx.x; e < ne; e += NT) {
-      const double re = (double)res[e].x, im = (double)res[e].y;
-      acc[e] += w * (re * re + im * im);
-    }
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < ne; e += NT) {
-    const int a = e / TILE_LINES, l = e - a * TILE_LINES;
-    if (l < lines) {
-      double* p = rho + (int64_t)a * stride + i0 + l;
-      *p = (keep ? *p : 0.0) + scale * acc[e];
-    }
-  }
+void allow_dynamic_lds(const void* kern, size_t bytes) {
+  static std::mutex mu;
+  static std::unordered_map<const void*, size_t> done;
+  std::lock_guard<std::mutex> lock(mu);
+  size_t& have = done[kern];
+  if (bytes <= have) return;
+  DEREK_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  have = bytes;
 }
 
 template <typename K> static void allow_lds(K kern, size_t bytes) {
-  // gfx950 has 160 KB of LDS per CU; kernels above 64 KB must opt in.
-  if (bytes > 65536) DEREK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  // gfx950 has 160 KB of LDS per CU; kernels above 64 KB must opt in (once
+  // per kernel, before any graph capture replays the launch)
+  if (bytes > 65536) allow_dynamic_lds(reinterpret_cast<const void*>(kern), bytes);
 }
 
 static void check_lds(size_t bytes, int n) {
@@ -148,32 +96,15 @@ static void launch_contig(cx_t<Rl>* data, int64_t nlines, const Axis<Rl>& ax, hi
   DEREK_HIP(hipGetLastError());
 }
 
-template <typename Rl, int S, int NC, int MODE>
-static void launch_tile(cx_t<Rl>* data, int64_t nouter, int64_t stride, const Axis<Rl>& ax, const Rl* vloc,
-                        Rl vscale, hipStream_t s) {
+template <typename Rl, int S, int NC>
+static void launch_tile(cx_t<Rl>* data, int64_t nouter, int64_t stride, const Axis<Rl>& ax, hipStream_t s) {
   const int N = ax.n;
   const size_t lds = sizeof(cx_t<Rl>) * ((size_t)N + 2 * (size_t)TILE_LINES * N);
   check_lds(lds, N);
-  auto kern = k_fft_tile<Rl, S, NC, MODE>;
+  auto kern = k_fft_tile<Rl, S, NC>;
   allow_lds(kern, lds);
   const int64_t tiles = ceil_div(stride, TILE_LINES);
-  hipLaunchKernelGGL(kern, dim3((unsigned)(nouter * tiles)), dim3(NT), lds, s, data, stride, tiles, ax.plan,
-                     ax.tw, vloc, vscale);
-  DEREK_HIP(hipGetLastError());
-}
-
-template <typename Rl, int NC>
-static void launch_density(const cx_t<Rl>* data, int64_t nouter, int64_t stride, const Axis<Rl>& ax,
-                           const double* occ, double* rho, double scale, int keep, hipStream_t s) {
-  const int N = ax.n;
-  const size_t lds = sizeof(cx_t<Rl>) * ((size_t)N + 2 * (size_t)TILE_LINES * N) +
-                     sizeof(double) * (size_t)TILE_LINES * N;
-  check_lds(lds, N);
-  auto kern = k_fft_tile_density<Rl, NC>;
-  allow_lds(kern, lds);
-  const int64_t tiles = ceil_div(stride, TILE_LINES);
-  hipLaunchKernelGGL(kern, dim3((unsigned)tiles), dim3(NT), lds, s, data, nouter, stride, tiles, ax.plan, ax.tw,
-                     occ, rho, scale, keep);
+  hipLaunchKernelGGL(kern, dim3((unsigned)(nouter * tiles)), dim3(NT), lds, s, data, stride, tiles, ax.plan, ax.tw);
   DEREK_HIP(hipGetLastError());
 }
 
@@ -203,46 +134,21 @@ template <typename R> void fft_z(cx_t<R>* data, int64_t nlines, const Axis<R>& a
 template <typename R>
 void fft_strided(cx_t<R>* data, int64_t nouter, int64_t stride, const Axis<R>& ax, int sign, hipStream_t s) {
   if (sign < 0) {
-#define C_(NC) launch_tile<R, -1, NC, 0>(data, nouter, stride, ax, nullptr, (R)0, s)
+#define C_(NC) launch_tile<R, -1, NC>(data, nouter, stride, ax, s)
     DEREK_FFT_DISPATCH(ax.n, C_)
 #undef C_
   } else {
-#define C_(NC) launch_tile<R, +1, NC, 0>(data, nouter, stride, ax, nullptr, (R)0, s)
+#define C_(NC) launch_tile<R, +1, NC>(data, nouter, stride, ax, s)
     DEREK_FFT_DISPATCH(ax.n, C_)
 #undef C_
   }
-}
-
-template <typename R>
-void fft_strided_vmul(cx_t<R>* data, int64_t nouter, int64_t stride, const Axis<R>& ax, const R* vloc, R vscale,
-                      hipStream_t s) {
-#define C_(NC) launch_tile<R, +1, NC, 1>(data, nouter, stride, ax, vloc, vscale, s)
-  DEREK_FFT_DISPATCH(ax.n, C_)
-#undef C_
-}
-
-template <typename R>
-void fft_strided_density(const cx_t<R>* data, int64_t nouter, int64_t stride, const Axis<R>& ax,
-                         const double* occ, double* rho, double scale, int keep, hipStream_t s) {
-#define C_(NC) launch_density<R, NC>(data, nouter, stride, ax, occ, rho, scale, keep, s)
-  DEREK_FFT_DISPATCH(ax.n, C_)
-#undef C_
 }
 
 template void fft_z<double>(double2*, int64_t, const Axis<double>&, int, hipStream_t);
 template void fft_z<float>(float2*, int64_t, const Axis<float>&, int, hipStream_t);
 template void fft_strided<double>(double2*, int64_t, int64_t, const Axis<double>&, int, hipStream_t);
 template void fft_strided<float>(float2*, int64_t, int64_t, const Axis<float>&, int, hipStream_t);
-template void fft_strided_vmul<double>(double2*, int64_t, int64_t, const Axis<double>&, const double*, double,
-                                       hipStream_t);
-template void fft_strided_vmul<float>(float2*, int64_t, int64_t, const Axis<float>&, const float*, float,
-                                      hipStream_t);
-template void fft_strided_density<double>(const double2*, int64_t, int64_t, const Axis<double>&, const double*,
-                                          double*, double, int, hipStream_t);
-template void fft_strided_density<float>(const float2*, int64_t, int64_t, const Axis<float>&, const double*,
-                                         double*, double, int, hipStream_t);
-
-// ------------------------------------------------------------------ maps
+// ------------------------------------------------------------------ pointwise
 static unsigned grid1d(int64_t n, int per = 256) {
   int64_t b = ceil_div(n, per);
   if (b > (int64_t)1 << 30) b = (int64_t)1 << 30;
@@ -259,48 +165,6 @@ template <typename R> void zero(cx_t<R>* p, int64_t n, hipStream_t s) {
 }
 template void zero<double>(double2*, int64_t, hipStream_t);
 template void zero<float>(float2*, int64_t, hipStream_t);
-
-template <typename C>
-__global__ void k_scatter(const C* __restrict__ psi, int64_t ldpsi, const int32_t* __restrict__ idx, int64_t npw,
-                          C* __restrict__ grid, int64_t N) {
-  const int b = blockIdx.y;
-  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < npw; g += (int64_t)gridDim.x * blockDim.x)
-    grid[b * N + idx[g]] = psi[b * ldpsi + g];
-}
-template <typename R>
-void scatter(const cx_t<R>* psi, int64_t ldpsi, int nb, const int32_t* idx, int64_t npw, cx_t<R>* grid, int64_t N,
-             hipStream_t s) {
-  hipLaunchKernelGGL(k_scatter<cx_t<R>>, dim3(grid1d(npw), nb), dim3(256), 0, s, psi, ldpsi, idx, npw, grid, N);
-  DEREK_HIP(hipGetLastError());
-}
-template void scatter<double>(const double2*, int64_t, int, const int32_t*, int64_t, double2*, int64_t, hipStream_t);
-template void scatter<float>(const float2*, int64_t, int, const int32_t*, int64_t, float2*, int64_t, hipStream_t);
-
-template <typename C>
-__global__ void k_gather_kin(const C* __restrict__ grid, int64_t N, const int32_t* __restrict__ idx,
-                             const double* __restrict__ kin, const C* __restrict__ psi, int64_t ldpsi,
-                             C* __restrict__ hpsi, int64_t ldh, int64_t npw) {
-  using R = decltype(C::x);
-  const int b = blockIdx.y;
-  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < npw;
-       g += (int64_t)gridDim.x * blockDim.x) {
-    const C v = grid[b * N + idx[g]];
-    const C p = psi[b * ldpsi + g];
-    const R t = (R)kin[g];
-    hpsi[b * ldh + g] = cmk<C>(v.x + t * p.x, v.y + t * p.y);
-  }
-}
-template <typename R>
-void gather_kin(const cx_t<R>* grid, int64_t N, const int32_t* idx, const double* kin, const cx_t<R>* psi,
-                int64_t ldpsi, cx_t<R>* hpsi, int64_t ldh, int nb, int64_t npw, hipStream_t s) {
-  hipLaunchKernelGGL(k_gather_kin<cx_t<R>>, dim3(grid1d(npw), nb), dim3(256), 0, s, grid, N, idx, kin, psi, ldpsi,
-                     hpsi, ldh, npw);
-  DEREK_HIP(hipGetLastError());
-}
-template void gather_kin<double>(const double2*, int64_t, const int32_t*, const double*, const double2*, int64_t,
-                                 double2*, int64_t, int, int64_t, hipStream_t);
-template void gather_kin<float>(const float2*, int64_t, const int32_t*, const double*, const float2*, int64_t,
-                                float2*, int64_t, int, int64_t, hipStream_t);
 
 // ------------------------------------------------------------------ LDA XC
 // Slater exchange + Perdew-Zunger 1981 correlation, spin-unpolarised, with
@@ -445,30 +309,6 @@ void dot_partials(const double* a, const double* b, double* partial, int64_t n, 
   DEREK_HIP(hipGetLastError());
 }
 
-struct LinComb {
-  const double* x[16];
-  double c[16];
-  int m;
-};
-__global__ void k_lincomb(double* __restrict__ out, LinComb lc, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    double v = 0.0;
-    for (int j = 0; j < lc.m; ++j) v += lc.c[j] * lc.x[j][i];
-    out[i] = v;
-  }
-}
-void lincomb(double* out, const double* const* xs, const double* c, int m, int64_t n, hipStream_t s) {
-  DEREK_REQUIRE(m >= 1 && m <= 16, "lincomb: 1..16 vectors");
-  LinComb lc{};
-  for (int j = 0; j < m; ++j) {
-    lc.x[j] = xs[j];
-    lc.c[j] = c[j];
-  }
-  lc.m = m;
-  hipLaunchKernelGGL(k_lincomb, dim3(grid1d(n)), dim3(256), 0, s, out, lc, n);
-  DEREK_HIP(hipGetLastError());
-}
-
 __global__ void k_axpy(double* y, const double* x, double a, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     y[i] += a * x[i];
@@ -483,114 +323,6 @@ __global__ void k_sub(double* out, const double* a, const double* b, int64_t n) 
 }
 void sub(double* out, const double* a, const double* b, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_sub, dim3(grid1d(n)), dim3(256), 0, s, out, a, b, n);
-  DEREK_HIP(hipGetLastError());
-}
-
-// ------------------------------------------------------------------ Davidson helpers
-template <typename C>
-__global__ __launch_bounds__(RED_NT) void k_band_sums(const C* __restrict__ x, const C* __restrict__ hx, int64_t ld,
-                                                      const double* __restrict__ eps,
-                                                      const double* __restrict__ kin, int64_t npw,
-                                                      double* __restrict__ partial, int nchunk) {
-  __shared__ double sh[RED_NT / 64];
-  const int b = blockIdx.y, c = blockIdx.x;
-  const double e = eps[b];
-  double sk = 0.0, sr = 0.0;
-  for (int64_t g = c * (int64_t)RED_NT + threadIdx.x; g < npw; g += (int64_t)nchunk * RED_NT) {
-    const C p = x[b * ld + g], h = hx[b * ld + g];
-    const double px = p.x, py = p.y;
-    sk += kin[g] * (px * px + py * py);
-    const double rx = (double)h.x - e * px, ry = (double)h.y - e * py;
-    sr += rx * rx + ry * ry;
-  }
-  const double tk = block_sum(sk, sh);
-  const double tr = block_sum(sr, sh);
-  if (threadIdx.x == 0) {
-    partial[((int64_t)b * nchunk + c) * 2 + 0] = tk;
-    partial[((int64_t)b * nchunk + c) * 2 + 1] = tr;
-  }
-}
-template <typename R>
-void band_sums(const cx_t<R>* x, const cx_t<R>* hx, int64_t ld, const double* eps, const double* kin, int nb,
-               int64_t npw, double* partial, int nchunk, hipStream_t s) {
-  hipLaunchKernelGGL(k_band_sums<cx_t<R>>, dim3(nchunk, nb), dim3(RED_NT), 0, s, x, hx, ld, eps, kin, npw, partial,
-                     nchunk);
-  DEREK_HIP(hipGetLastError());
-}
-template void band_sums<double>(const double2*, const double2*, int64_t, const double*, const double*, int, int64_t,
-                                double*, int, hipStream_t);
-template void band_sums<float>(const float2*, const float2*, int64_t, const double*, const double*, int, int64_t,
-                               double*, int, hipStream_t);
-
-template <typename C>
-__global__ void k_tpa(const C* __restrict__ x, const C* __restrict__ hx, int64_t ld, const double* __restrict__ eps,
-                      const double* __restrict__ ekin, const int32_t* __restrict__ act,
-                      const double* __restrict__ kin, int64_t npw, C* __restrict__ w, int64_t ldw) {
-  using R = decltype(C::x);
-  const int j = blockIdx.y;
-  const int b = act[j];
-  const double e = eps[b], ik = 1.0 / fmax(ekin[b], 0.1);  // <T>_n floored at 0.1 Ha
-  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < npw; g += (int64_t)gridDim.x * blockDim.x) {
-    const C p = x[b * ld + g], h = hx[b * ld + g];
-    const double t = kin[g] * ik;
-    const double pp = 27.0 + t * (18.0 + t * (12.0 + 8.0 * t));
-    const double k = pp / (pp + 16.0 * t * t * t * t);
-    w[j * ldw + g] = cmk<C>((R)(k * ((double)h.x - e * (double)p.x)), (R)(k * ((double)h.y - e * (double)p.y)));
-  }
-}
-template <typename R>
-void tpa_residual(const cx_t<R>* x, const cx_t<R>* hx, int64_t ld, const double* eps, const double* ekin,
-                  const int32_t* act, int nact, const double* kin, int64_t npw, cx_t<R>* w, int64_t ldw,
-                  hipStream_t s) {
-  if (nact == 0) return;
-  hipLaunchKernelGGL(k_tpa<cx_t<R>>, dim3(grid1d(npw), nact), dim3(256), 0, s, x, hx, ld, eps, ekin, act, kin, npw,
-                     w, ldw);
-  DEREK_HIP(hipGetLastError());
-}
-template void tpa_residual<double>(const double2*, const double2*, int64_t, const double*, const double*,
-                                   const int32_t*, int, const double*, int64_t, double2*, int64_t, hipStream_t);
-template void tpa_residual<float>(const float2*, const float2*, int64_t, const double*, const double*,
-                                  const int32_t*, int, const double*, int64_t, float2*, int64_t, hipStream_t);
-
-template <typename C> __global__ void k_zero_pad(C* x, int64_t ld, int64_t npw) {
-  const int b = blockIdx.y;
-  for (int64_t g = npw + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < ld; g += (int64_t)gridDim.x * blockDim.x)
-    x[b * ld + g] = cmk<C>(0, 0);
-}
-template <typename R> void zero_pad(cx_t<R>* x, int64_t ld, int64_t npw, int nb, hipStream_t s) {
-  if (ld <= npw || nb == 0) return;
-  hipLaunchKernelGGL(k_zero_pad<cx_t<R>>, dim3(grid1d(ld - npw), nb), dim3(256), 0, s, x, ld, npw);
-  DEREK_HIP(hipGetLastError());
-}
-template void zero_pad<double>(double2*, int64_t, int64_t, int, hipStream_t);
-template void zero_pad<float>(float2*, int64_t, int64_t, int, hipStream_t);
-
-// ------------------------------------------------------------------ initial wavefunctions
-__device__ __forceinline__ uint64_t mix64(uint64_t z) {
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
-__global__ void k_init_psi(double2* __restrict__ psi, int64_t ld, int64_t npw, const double* __restrict__ kin,
-                           uint64_t seed, int k) {
-  const int b = blockIdx.y;
-  const uint64_t G = 0x9E3779B97F4A7C15ull;
-  const uint64_t stream = (1ull << 48) | ((uint64_t)k << 24) | (uint64_t)b;
-  const uint64_t key = mix64((seed * G) ^ mix64(stream + G));
-  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < npw; g += (int64_t)gridDim.x * blockDim.x) {
-    const uint64_t z1 = mix64(key + (uint64_t)(2 * g + 1) * G);
-    const uint64_t z2 = mix64(key + (uint64_t)(2 * g + 2) * G);
-    const double u1 = (double)(z1 >> 11) * (1.0 / 9007199254740992.0);
-    const double u2 = (double)(z2 >> 11) * (1.0 / 9007199254740992.0);
-    const double r = sqrt(-2.0 * log(1.0 - u1));
-    const double t = 2.0 * 3.141592653589793238462643383279502884197 * u2;
-    const double sc = 1.0 / (1.0 + kin[g]);
-    psi[b * ld + g] = make_double2(r * cos(t) * sc, r * sin(t) * sc);
-  }
-}
-void init_psi(double2* psi, int64_t ld, int nb, int64_t npw, const double* kin, uint64_t seed, int k,
-              hipStream_t s) {
-  hipLaunchKernelGGL(k_init_psi, dim3(grid1d(npw), nb), dim3(256), 0, s, psi, ld, npw, kin, seed, k);
   DEREK_HIP(hipGetLastError());
 }
 

@@ -19,33 +19,17 @@ template <typename R> struct Axis {
   const cx_t<R>* tw;
 };
 
+// Allows a kernel more than 64 KiB of dynamic LDS (set once per kernel).
+void allow_dynamic_lds(const void* kern, size_t bytes);
+
 // ---- FFT passes on batches of flat grids (nx, ny, nz), z fastest.
 // Lines along z: nlines contiguous transforms of length ax.n.
 template <typename R> void fft_z(cx_t<R>* data, int64_t nlines, const Axis<R>& ax, int sign, hipStream_t s);
 // Strided axis: element (o, a, i) at o*n*stride + a*stride + i, i < stride.
 template <typename R>
 void fft_strided(cx_t<R>* data, int64_t nouter, int64_t stride, const Axis<R>& ax, int sign, hipStream_t s);
-// Fused: inverse transform along the strided axis, multiply by vloc*vscale
-// (vloc indexed a*stride + i, shared by all outer blocks), forward transform.
-template <typename R>
-void fft_strided_vmul(cx_t<R>* data, int64_t nouter, int64_t stride, const Axis<R>& ax, const R* vloc,
-                      R vscale, hipStream_t s);
-// Fused: inverse transform along the strided axis for outer = 0..nouter-1
-// (bands) and rho[a*stride+i] = keep*rho + scale * sum_o occ[o] |.|^2.
-template <typename R>
-void fft_strided_density(const cx_t<R>* data, int64_t nouter, int64_t stride, const Axis<R>& ax,
-                         const double* occ, double* rho, double scale, int keep, hipStream_t s);
-
-// ---- plane-wave <-> grid
+// ---- buffers
 template <typename R> void zero(cx_t<R>* p, int64_t n, hipStream_t s);
-template <typename R>
-void scatter(const cx_t<R>* psi, int64_t ldpsi, int nb, const int32_t* idx, int64_t npw, cx_t<R>* grid,
-             int64_t N, hipStream_t s);
-// hpsi = grid[idx] + kin * psi
-template <typename R>
-void gather_kin(const cx_t<R>* grid, int64_t N, const int32_t* idx, const double* kin, const cx_t<R>* psi,
-                int64_t ldpsi, cx_t<R>* hpsi, int64_t ldh, int nb, int64_t npw, hipStream_t s);
-
 // ---- real-space fields (FP64)
 void lda_xc(const double* rho, double* vxc, double* partial_exc, int64_t n, hipStream_t s);
 int lda_partials(int64_t n);
@@ -60,44 +44,28 @@ void add3(const double* a, const double* b, const double* c, double* out, int64_
 // partial dot products: partial[blk] = sum a*b over the block's slice
 void dot_partials(const double* a, const double* b, double* partial, int64_t n, hipStream_t s);
 int dot_nblocks(int64_t n);
-// out = sum_i c[i] * x_i  (x_i = xs[i], up to 16 vectors, coefficients on host)
-void lincomb(double* out, const double* const* xs, const double* c, int m, int64_t n, hipStream_t s);
 void axpy(double* y, const double* x, double a, int64_t n, hipStream_t s);
 void sub(double* out, const double* a, const double* b, int64_t n, hipStream_t s);
 
-// ---- Davidson helpers (per band b: column b of leading dimension ld)
-// partial[(b*nchunk + c)*2 + {0,1}] = sum over chunk c of {kin|x|^2, |hx - eps x|^2}
-template <typename R>
-void band_sums(const cx_t<R>* x, const cx_t<R>* hx, int64_t ld, const double* eps, const double* kin, int nb,
-               int64_t npw, double* partial, int nchunk, hipStream_t s);
-// w = TPA(kin/ekin_b) * (hx - eps_b x), for the active bands listed in act[0..nact)
-template <typename R>
-void tpa_residual(const cx_t<R>* x, const cx_t<R>* hx, int64_t ld, const double* eps, const double* ekin,
-                  const int32_t* act, int nact, const double* kin, int64_t npw, cx_t<R>* w, int64_t ldw,
-                  hipStream_t s);
-// zero rows npw..ld-1 of nb columns
-template <typename R> void zero_pad(cx_t<R>* x, int64_t ld, int64_t npw, int nb, hipStream_t s);
-// psi(g, b) = (N + iN)/(1 + kin) from the counter-based generator
-void init_psi(double2* psi, int64_t ld, int nb, int64_t npw, const double* kin, uint64_t seed, int k,
-              hipStream_t s);
 template <typename Rd, typename Rs>
 void convert(cx_t<Rd>* dst, const cx_t<Rs>* src, int64_t n, hipStream_t s);
 void convert_real_f(float* dst, const double* src, int64_t n, hipStream_t s);
 
-// ---- tall-skinny complex GEMMs (MFMA)
-// C (m x n, column-major, ldc) = X^H Y; X: npw x m (ldx), Y: npw x n (ldy).
-// `work` must hold overlap_work_elems(m, n, npw) double2.
-int64_t overlap_work_elems(int m, int n, int64_t npw);
-void overlap_f64(const double2* X, int64_t ldx, const double2* Y, int64_t ldy, int m, int n, int64_t npw,
-                 double2* C, int ldc, double2* work, bool hermitian, hipStream_t s);
-// Z (npw x n, ldz) = X (npw x m, ldx) * U (m x n, column-major, ldu).
-void rotate_f64(const double2* X, int64_t ldx, const double2* U, int ldu, int m, int n, int64_t npw, double2* Z,
-                int64_t ldz, hipStream_t s);
-// Mixed precision: X, Y complex64; bf16 MFMA with fp32 accumulation; C in FP64.
-int64_t overlap_work_elems_bf16(int m, int n, int64_t npw);
-void overlap_bf16(const float2* X, int64_t ldx, const float2* Y, int64_t ldy, int m, int n, int64_t npw,
-                  double2* C, int ldc, float2* work, bool hermitian, hipStream_t s);
-void rotate_f32(const float2* X, int64_t ldx, const float2* U, int ldu, int m, int n, int64_t npw, float2* Z,
-                int64_t ldz, hipStream_t s);
+// ---- tall-skinny complex GEMMs (MFMA), batched: matrix kz of a batch is at
+// X + kz*sx, Y + kz*sy, C + kz*sc ...; npwk (device, nullable) gives each
+// matrix its own row count (<= npw), npw otherwise.
+// C (m x n, column-major, ldc) = X^H Y; X: npw x m (ldx), Y: npw x n (ldy),
+// accumulated on MFMA in R, chunk partials summed in FP64.
+// `work` must hold overlap_work_elems(m, n, npw, batch) double2.
+int64_t overlap_work_elems(int m, int n, int64_t npw, int batch);
+template <typename R>
+void overlap(const cx_t<R>* X, int64_t ldx, int64_t sx, const cx_t<R>* Y, int64_t ldy, int64_t sy, int m, int n,
+             int64_t npw, const int* npwk, double2* C, int ldc, int64_t sc, int batch, double2* work,
+             bool hermitian, hipStream_t s);
+// Z (npw x n, ldz) = X (npw x m, ldx) * U (m x n, FP64, column-major, ldu),
+// U rounded to R.
+template <typename R>
+void rotate(const cx_t<R>* X, int64_t ldx, int64_t sx, const double2* U, int ldu, int64_t su, int m, int n,
+            int64_t npw, const int* npwk, cx_t<R>* Z, int64_t ldz, int64_t sz, int batch, hipStream_t s);
 
 }  // namespace derek

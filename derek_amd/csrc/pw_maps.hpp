@@ -19,6 +19,17 @@ struct PwMaps {
   const int32_t* st_y = nullptr;  // [nst] y bin of each stick
   const int32_t* pl_st = nullptr; // [nax+1] first stick of each active plane
   const int32_t* ax_x = nullptr;  // [nax] x bin of each active plane
+  const double* kin = nullptr;    // [npw] kinetic energy 1/2 |G+k|^2, caller order
+};
+
+// One launch of a pruned pipeline covers the items (k, band) for
+// k = k0 .. k0+nkc-1 and nbb bands of each.  `mt` is a device array of the
+// maps of every k-point; the stick and plane buffers hold nst_cap sticks and
+// nax_cap planes per item.
+struct PwBatch {
+  const PwMaps* mt = nullptr;
+  int k0 = 0, nkc = 1, nbb = 1;
+  int nst_cap = 0, nax_cap = 0;
 };
 
 // Host construction of the maps from flat FFT-bin indices (z fastest).

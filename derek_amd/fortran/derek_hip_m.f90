@@ -37,6 +37,11 @@ module derek_hip_m
      integer(c_int32_t) :: history
      integer(c_int32_t) :: reserved
      integer(c_int64_t) :: seed
+     real(c_double)     :: fp32_switch_tol
+     integer(c_int32_t) :: refine_steps
+     integer(c_int32_t) :: kbatch
+     integer(c_int32_t) :: use_graph
+     integer(c_int32_t) :: autotune
   end type derek_scf_params
 
   type, bind(C), public :: derek_scf_report
@@ -44,6 +49,7 @@ module derek_hip_m
      integer(c_int32_t) :: ndav
      real(c_double)     :: etot, e_band, e_hartree, e_xc, e_dc, minus_ts, fermi, drho, max_res
      real(c_double)     :: t_dense, t_step
+     integer(c_int32_t) :: fp32, refine_left
   end type derek_scf_report
 
   public :: derek_check

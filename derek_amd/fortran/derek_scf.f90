@@ -144,6 +144,11 @@ program derek_scf
   par%history = 8
   par%reserved = 0
   par%seed = wfn_seed
+  par%fp32_switch_tol = 0.0d0
+  par%refine_steps = 0
+  par%kbatch = 0
+  par%use_graph = 0
+  par%autotune = 0
   call derek_check(derek_hip_scf_init(ctx, par), 'hip_scf_init')
 
   write(*, *) '+--------+-------------------+-------------------+---------+ <-- SCF'
@@ -160,7 +165,8 @@ program derek_scf
         write(*, 11) iscf, rep%etot, t1 - t0
      else
         write(*, 12) iscf, rep%etot, rep%etot - e_old, t1 - t0
-        if (abs(rep%etot - e_old) < energy_tol) converged = .true.
+        ! the FP64 refinement of DEREK_MIXED must be complete (refine_left = 0)
+        if (abs(rep%etot - e_old) < energy_tol .and. rep%refine_left == 0) converged = .true.
      end if
      e_old = rep%etot
      if (converged) exit

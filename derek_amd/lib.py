@@ -27,11 +27,11 @@ EXPORTS = (
     "derek_hip_scf_init", "derek_hip_scf_step", "derek_hip_scf_polish", "derek_hip_get_eigenvalues",
     "derek_hip_get_occupations", "derek_hip_get_density", "derek_hip_get_psi", "derek_hip_put_psi",
     "derek_hip_apply_h_resident", "derek_hip_sync", "derek_hip_stream",
-    "derek_hip_profile_enable", "derek_hip_profile_reset", "derek_hip_profile_read",
+    "derek_hip_profile_enable", "derek_hip_profile_reset", "derek_hip_profile_read", "derek_hip_get_tuning",
 )
 
 KERNEL_CLASSES = ("sticks_z_inv", "planes_y_inv", "planes_x_vmul", "planes_y_fwd", "sticks_z_fwd", "planes_x_density",
-                  "overlap", "rotate", "fft_z", "fft_y", "fft_x")
+                  "overlap", "rotate", "fft_z", "fft_y", "fft_x", "dense", "other")
 
 
 class DerekError(RuntimeError):
@@ -48,14 +48,17 @@ class Dims(C.Structure):
 class ScfParams(C.Structure):
     _fields_ = [("nelec", C.c_double), ("kT", C.c_double), ("mix_beta", C.c_double),
                 ("mix_q0", C.c_double), ("n_david_first", C.c_int32), ("n_david", C.c_int32),
-                ("history", C.c_int32), ("reserved", C.c_int32), ("seed", C.c_uint64)]
+                ("history", C.c_int32), ("reserved", C.c_int32), ("seed", C.c_uint64),
+                ("fp32_switch_tol", C.c_double), ("refine_steps", C.c_int32), ("kbatch", C.c_int32),
+                ("use_graph", C.c_int32), ("autotune", C.c_int32)]
 
 
 class ScfReport(C.Structure):
     _fields_ = [("iscf", C.c_int32), ("ndav", C.c_int32), ("etot", C.c_double), ("e_band", C.c_double),
                 ("e_hartree", C.c_double), ("e_xc", C.c_double), ("e_dc", C.c_double),
                 ("minus_ts", C.c_double), ("fermi", C.c_double), ("drho", C.c_double),
-                ("max_res", C.c_double), ("t_dense", C.c_double), ("t_step", C.c_double)]
+                ("max_res", C.c_double), ("t_dense", C.c_double), ("t_step", C.c_double), ("fp32", C.c_int32),
+                ("refine_left", C.c_int32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -313,10 +316,22 @@ class Context:
         return w, Cm.T.copy()
 
     # ---- resident SCF
-    def scf_init(self, nelec, kT=0.0, beta=0.5, q0=1.0, n_david_first=4, n_david=2, history=8, seed=1):
+    def scf_init(self, nelec, kT=0.0, beta=0.5, q0=1.0, n_david_first=4, n_david=2, history=8, seed=1,
+                 switch_tol=0.0, refine_steps=0, kbatch=0, graph=False, autotune=False):
+        """kbatch: k-points per batched launch (0 = all); graph: replay each SCF
+        step from a hipGraph; autotune: time the launch shapes at init;
+        switch_tol / refine_steps: the FP32 -> FP64 schedule of DEREK_MIXED."""
         p = ScfParams(nelec=nelec, kT=kT, mix_beta=beta, mix_q0=q0, n_david_first=n_david_first,
-                      n_david=n_david, history=history, reserved=0, seed=seed)
+                      n_david=n_david, history=history, reserved=0, seed=seed, fp32_switch_tol=switch_tol,
+                      refine_steps=refine_steps, kbatch=kbatch, use_graph=1 if graph else 0,
+                      autotune=1 if autotune else 0)
         check(self.lib.derek_hip_scf_init(self.h, C.byref(p)))
+
+    def tuning(self):
+        """{bands_per_wg, items_per_launch, kbatch, graph}"""
+        out = (C.c_int32 * 4)()
+        check(self.lib.derek_hip_get_tuning(self.h, out))
+        return {"bands_per_wg": out[0], "items_per_launch": out[1], "kbatch": out[2], "graph": bool(out[3])}
 
     def scf_step(self):
         r = ScfReport()

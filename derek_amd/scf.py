@@ -50,15 +50,21 @@ class Setup:
 
 
 def run(spec, max_scf=40, energy_tol=1e-7, rho_tol=None, precision=lib.DEREK_FP64, log=print,
-        n_david_first=4, n_david=2, beta=0.5, q0=1.0, history=8, setup=None, ctx=None):
+        n_david_first=4, n_david=2, beta=0.5, q0=1.0, history=8, setup=None, ctx=None, kbatch=0, graph=False,
+        autotune=False, switch_tol=0.0, refine_steps=0):
     """Run the SCF; returns a dict with the final energy, eigenvalues,
-    per-step reports, wall time and the converged flag."""
+    per-step reports, wall time (init included in ``wall_init``) and the
+    converged flag.  In DEREK_MIXED mode a step may only end the run once
+    its report's ``refine_left`` is 0 (the FP64 refinement is done)."""
     su = setup if setup is not None else Setup(spec)
     own = ctx is None
     if ctx is None:
         ctx = su.context(precision)
+    ti = time.perf_counter()
     ctx.scf_init(spec.nelec, kT=spec.kT, beta=beta, q0=q0, n_david_first=n_david_first, n_david=n_david,
-                 history=history, seed=spec.seed)
+                 history=history, seed=spec.seed, kbatch=kbatch, graph=graph, autotune=autotune,
+                 switch_tol=switch_tol, refine_steps=refine_steps)
+    wall_init = time.perf_counter() - ti
     hist = []
     e_old = None
     converged = False
@@ -76,7 +82,7 @@ def run(spec, max_scf=40, energy_tol=1e-7, rho_tol=None, precision=lib.DEREK_FP6
         if log:
             des = "                 " if de is None else "%17.8E" % de
             log(" | %5d  | %17.8E | %s | %7.2f | <-- SCF" % (r["iscf"], r["etot"], des, r["time"]))
-        ok = de is not None and abs(de) < energy_tol
+        ok = de is not None and abs(de) < energy_tol and r["refine_left"] == 0
         if rho_tol is not None:
             ok = ok and r["drho"] < rho_tol
         e_old = r["etot"]
@@ -87,7 +93,8 @@ def run(spec, max_scf=40, energy_tol=1e-7, rho_tol=None, precision=lib.DEREK_FP6
     if log:
         log(" +--------+-------------------+-------------------+---------+ <-- SCF")
     out = dict(etot=hist[-1]["etot"], eigs=ctx.eigenvalues(), occ=ctx.occupations(), history=hist,
-               converged=converged, iterations=len(hist), wall=wall)
+               converged=converged, iterations=len(hist), wall=wall, wall_init=wall_init,
+               fp32_steps=sum(1 for h in hist if h["fp32"]), tuning=ctx.tuning())
     if own:
         out["rho"] = ctx.get_density()
         ctx.close()

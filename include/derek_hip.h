@@ -32,7 +32,9 @@ extern "C" {
 #define DEREK_ERR_STATE 4   /* call out of order (e.g. basis not set) */
 
 #define DEREK_FP64 0  /* complex128 wavefunctions, FP64 everywhere */
-#define DEREK_MIXED 1 /* complex64 FFT/H|psi>/density, BF16-in FP32-acc overlaps, FP64 small dense + energies */
+#define DEREK_MIXED 1 /* complex64 wavefunctions, FFTs, H|psi> and density; FP32 MFMA overlaps and
+                        rotations; FP64 subspace solves, reductions and energies; the last SCF
+                        steps refined in FP64 from the FP32-converged state */
 
 typedef struct derek_ctx derek_ctx;
 
@@ -55,6 +57,11 @@ typedef struct {
   int32_t history;       /* Pulay history, default 8 (max 16) */
   int32_t reserved;
   uint64_t seed;         /* initial-wavefunction seed */
+  double fp32_switch_tol; /* DEREK_MIXED: |E_i - E_{i-1}| below which the FP32 phase ends (0: 1e-5 Ha) */
+  int32_t refine_steps;   /* DEREK_MIXED: FP64 steps after the switch before convergence may be declared (0: 3) */
+  int32_t kbatch;         /* k-points per batched launch (0: all k-points in one launch) */
+  int32_t use_graph;      /* 1: capture the SCF step in a hipGraph once and replay it */
+  int32_t autotune;       /* 1: time the tunable launch shapes at scf_init and keep the fastest */
 } derek_scf_params;
 
 typedef struct {
@@ -69,8 +76,10 @@ typedef struct {
   double fermi;
   double drho;      /* || rho_out - rho_in ||_2 (bohr^-3/2) */
   double max_res;   /* largest Davidson residual norm */
-  double t_dense;   /* seconds spent in the FP64 subspace solves (Cholesky, eigensolves), incl. waits */
+  double t_dense;   /* GPU seconds of the FP64 subspace solves in this step (profiling enabled; else 0) */
   double t_step;    /* wall seconds of the step */
+  int32_t fp32;     /* 1 if the step ran in the FP32 phase of DEREK_MIXED */
+  int32_t refine_left; /* FP64 refinement steps still due before convergence may be declared */
 } derek_scf_report;
 
 /* ---- library ------------------------------------------------------------ */
@@ -127,7 +136,10 @@ int32_t derek_hip_fft3d(derek_ctx* ctx, double* grid, int32_t howmany, int32_t d
 int32_t derek_hip_apply_h(derek_ctx* ctx, int32_t k, int32_t nb, const double* psi, double* hpsi);
 /* S = X^H Y (m x n complex); X: npw x m, Y: npw x n (X == Y with m == n
  * computes only the upper tiles of the Hermitian result).  Replaces the
- * wave_dot_* stubs (wave.f90:929-1036) with the full band-band overlap. */
+ * wave_dot_* stubs (wave.f90:929-1036) with the full band-band overlap.
+ * In a DEREK_MIXED context X and Y are rounded to complex64 and multiplied
+ * on the f32 MFMA path (chunk sums in FP64), as in the SCF's FP32 phase;
+ * derek_hip_rotate likewise. */
 int32_t derek_hip_overlap(derek_ctx* ctx, int64_t npw, int32_t m, int32_t n, const double* X, const double* Y,
                           double* S);
 /* Z = X U; X: npw x m, U: m x n, Z: npw x n (complex). */
@@ -161,7 +173,8 @@ int32_t derek_hip_hegv(derek_ctx* ctx, int32_t m, int32_t nev, const double* S, 
 int32_t derek_hip_scf_init(derek_ctx* ctx, const derek_scf_params* params);
 /* One SCF step: V_in from rho_in, Davidson for every k, occupations,
  * rho_out, energies, Pulay mixing.  The caller tests convergence
- * (|E_i - E_{i-1}| < energy_tol, electronic.f90:156-165). */
+ * (|E_i - E_{i-1}| < energy_tol, electronic.f90:156-165), and in DEREK_MIXED
+ * mode also requires report->refine_left == 0. */
 int32_t derek_hip_scf_step(derek_ctx* ctx, derek_scf_report* report);
 /* Davidson on the last step's potential, up to max_iter steps per k-point or
  * until every residual norm is below res_tol (eigenvalues for reporting). */
@@ -177,16 +190,23 @@ int32_t derek_hip_put_psi(derek_ctx* ctx, int32_t k, const double* psi);
 /* H applied to the resident wavefunctions of k (for timing; result kept on device). */
 int32_t derek_hip_apply_h_resident(derek_ctx* ctx, int32_t k, int32_t nb);
 /* Per-kernel-class timers: while enabled, HIP events on the context's
- * stream bracket every launch; read returns launches, total milliseconds and
- * the algorithmic bytes / flops of those launches.  Classes: 0 zero-fill,
- * 1 plane-wave scatter, 2 FFT z pass, 3 FFT y pass, 4 FFT x pass fused with
- * the V(r) multiply, 5 gather + kinetic, 6 FFT x pass fused with the density
- * accumulation, 7 overlap GEMM, 8 rotation GEMM, 9 FFT x pass.
+ * stream bracket every launch (not inside a captured graph); read returns
+ * launches, total milliseconds and the algorithmic bytes / flops of those
+ * launches.  Classes: 0 z pass on sticks with the plane-wave scatter,
+ * 1 y pass on planes (inverse), 2 x pass fused with the V(r) multiply,
+ * 3 y pass on planes (forward), 4 z pass on sticks with the gather and the
+ * kinetic term, 5 x pass fused with the density accumulation, 6 overlap GEMM,
+ * 7 rotation GEMM, 8/9/10 full-grid FFT z/y/x passes, 11 FP64 subspace
+ * solves, 12 other SCF kernels (band statistics, TPA, potentials, mixing).
  * (The reference's equivalent is its trace/profile module, trace.f90:515-667.) */
 int32_t derek_hip_profile_enable(derek_ctx* ctx, int32_t on);
 int32_t derek_hip_profile_reset(derek_ctx* ctx);
 int32_t derek_hip_profile_read(derek_ctx* ctx, int32_t cls, int64_t* launches, double* total_ms, double* bytes,
                                double* flops);
+/* Launch shapes in use: out[0] bands per x-pass workgroup (1, 2, 4, 8),
+ * out[1] (k-point, band) items per pipeline launch, out[2] k-points per
+ * batched launch, out[3] 1 if SCF steps are replayed from a hipGraph. */
+int32_t derek_hip_get_tuning(derek_ctx* ctx, int32_t out[4]);
 /* Block until all work on the context's stream is done. */
 int32_t derek_hip_sync(derek_ctx* ctx);
 /* Host-visible stream handle (hipStream_t) for event timing. */

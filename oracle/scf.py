@@ -247,6 +247,6 @@ def scf(s, max_scf=40, energy_tol=1e-7, rho_tol=None, n_david_first=4, n_david=2
             def apply(P, ik=ik):
                 return _ks.apply_h(P, s.fft_index[ik], s.kin[ik], v_in, s.dims)
             HX = apply(X[ik])
-            X[ik], _, eps[ik], rn_all[ik] = davidson(X[ik], HX, s.kin[ik], apply, 50, res_tol=final_res_tol)
+            X[ik], _, eps[ik], rn_all[ik] = davidson(X[ik], HX, s.kin[ik], apply, 1000, res_tol=final_res_tol)
         out["eigs_final"] = eps.copy()
     return out

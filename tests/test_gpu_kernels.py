@@ -115,26 +115,42 @@ def test_apply_h_hermitian_full_size():
 
 
 @pytest.mark.parametrize("m,n,npw", [(10, 10, 1503), (37, 53, 777), (160, 160, 5003), (80, 80, 62669)])
-def test_overlap(m, n, npw):
+@pytest.mark.parametrize("precision", [lib.DEREK_FP64, lib.DEREK_MIXED])
+def test_overlap(m, n, npw, precision):
+    """FP64: f64 MFMA against numpy to 1e-13.  MIXED: operands rounded to
+    complex64, f32 MFMA accumulation; checked against the FP64 product of the
+    rounded operands (tolerance 1e-5 of the largest entry)."""
+    print("overlap", m, n, npw, precision, flush=True)
     rng = np.random.default_rng(m * 1000 + n)
-    ctx = lib.Context((8, 8, 8), 1, 1, 1.0, 1)
+    ctx = lib.Context((8, 8, 8), 1, 1, 1.0, 1, precision)
     X = rnd_c(rng, m, npw)
     Y = rnd_c(rng, n, npw)
-    assert_close(ctx.overlap(X, Y), np.conj(X) @ Y.T, 1e-13)
+    tol = 1e-13 if precision == lib.DEREK_FP64 else 1e-5
+    Xr = X if precision == lib.DEREK_FP64 else X.astype(np.complex64).astype(np.complex128)
+    Yr = Y if precision == lib.DEREK_FP64 else Y.astype(np.complex64).astype(np.complex128)
+    assert_close(ctx.overlap(X, Y), np.conj(Xr) @ Yr.T, tol)
     if m == n:
         S = ctx.overlap(X, X)
-        assert_close(S, np.conj(X) @ X.T, 1e-13)
+        assert_close(S, np.conj(Xr) @ Xr.T, tol)
         assert np.array_equal(S, S.conj().T)
     ctx.close()
 
 
 @pytest.mark.parametrize("m,n,npw", [(10, 10, 1503), (37, 53, 777), (160, 80, 4003)])
-def test_rotate(m, n, npw):
+@pytest.mark.parametrize("precision", [lib.DEREK_FP64, lib.DEREK_MIXED])
+def test_rotate(m, n, npw, precision):
+    """MIXED: X and U rounded to complex64, f32 MFMA accumulation."""
+    print("rotate", m, n, npw, precision, flush=True)
     rng = np.random.default_rng(m + n)
-    ctx = lib.Context((8, 8, 8), 1, 1, 1.0, 1)
+    ctx = lib.Context((8, 8, 8), 1, 1, 1.0, 1, precision)
     X = rnd_c(rng, m, npw)
     U = rnd_c(rng, m, n)
-    assert_close(ctx.rotate(X, U), U.T @ X, 1e-13)
+    if precision == lib.DEREK_FP64:
+        assert_close(ctx.rotate(X, U), U.T @ X, 1e-13)
+    else:
+        Xr = X.astype(np.complex64).astype(np.complex128)
+        Ur = U.astype(np.complex64).astype(np.complex128)
+        assert_close(ctx.rotate(X, U), Ur.T @ Xr, 1e-5)
     ctx.close()
 
 

@@ -1,0 +1,98 @@
+"""The SCF engine's execution modes on the GPU: k-point batching, hipGraph
+replay, autotuned launch shapes and the mixed-precision schedule.
+
+Batching, graphs and launch shapes change only the grouping of the same
+FP64 arithmetic, so their per-step energies must agree with the plain
+sequential run to rounding.  The mixed-precision run (complex64 bulk, FP64
+subspace solves and energies, FP64 refinement at the end) must land on the
+FP64 oracle's total energy within 1e-6 Ha (the target for the mixed form)
+and, after its FP64 refinement and polish, on the eigenvalues within 1e-8 Ha.
+"""
+import numpy as np
+import pytest
+
+from derek_amd import config, lib
+from derek_amd import scf as gscf
+from oracle import scf as oscf
+from tests.conftest import oracle_system
+
+pytestmark = pytest.mark.gpu
+
+
+def kspec():
+    # 4 k-points with different plane-wave counts, smeared occupations
+    return config.small_system(seed=17, mp=(2, 2, 1), kT=0.01, nelec=6, nbands=8)
+
+
+def energies(spec, steps, **kw):
+    su = gscf.Setup(spec)
+    ctx = su.context(kw.pop("precision", lib.DEREK_FP64))
+    res = gscf.run(spec, max_scf=steps, energy_tol=0.0, setup=su, ctx=ctx, log=None, **kw)
+    tuning = ctx.tuning()
+    ctx.close()
+    return np.array([h["etot"] for h in res["history"]]), res, tuning
+
+
+def test_kpoint_batching_matches_sequential():
+    print("batching", flush=True)
+    spec = kspec()
+    e1, _, t1 = energies(spec, 8, kbatch=1)
+    e2, _, t2 = energies(spec, 8, kbatch=2)
+    e4, _, t4 = energies(spec, 8, kbatch=0)
+    assert (t1["kbatch"], t2["kbatch"], t4["kbatch"]) == (1, 2, 4)
+    assert np.max(np.abs(e2 - e1)) < 1e-10
+    assert np.max(np.abs(e4 - e1)) < 1e-10
+
+
+def test_graph_replay_matches_eager():
+    print("graph", flush=True)
+    spec = kspec()
+    e0, _, _ = energies(spec, 7, kbatch=0, graph=False)
+    e1, _, t = energies(spec, 7, kbatch=0, graph=True)
+    assert t["graph"]
+    # steps 3.. are graph replays (step 2 is the eager first use of the shape)
+    assert np.max(np.abs(e1 - e0)) < 1e-11
+
+
+def test_autotune_keeps_results():
+    print("autotune", flush=True)
+    spec = kspec()
+    e0, _, _ = energies(spec, 5)
+    e1, _, t = energies(spec, 5, autotune=True)
+    assert t["bands_per_wg"] in (1, 2, 4, 8)
+    assert 1 <= t["items_per_launch"] <= 4 * spec.nbands
+    assert np.max(np.abs(e1 - e0)) < 1e-10
+
+
+@pytest.mark.parametrize("name", ["cfg1", "kpoints"])
+def test_mixed_precision_reaches_fp64_oracle(name):
+    print("mixed", name, flush=True)
+    spec = config.benchmark(1) if name == "cfg1" else kspec()
+    su = gscf.Setup(spec)
+    ctx = su.context(lib.DEREK_MIXED)
+    res = gscf.run(spec, max_scf=80, energy_tol=1e-11, rho_tol=1e-8, setup=su, ctx=ctx, log=print, kbatch=0,
+                   graph=True)
+    rmax = ctx.scf_polish(1000, 1e-10)
+    eg = ctx.eigenvalues()
+    ctx.close()
+    o = oracle_system(spec)
+    ro = oscf.scf(o, max_scf=80, energy_tol=1e-11, rho_tol=1e-8, final_res_tol=1e-10)
+    print(f"polished residual {rmax:.2e}")
+    assert res["converged"] and ro["converged"]
+    assert res["fp32_steps"] >= 2
+    # the last steps ran in FP64 (refinement)
+    assert res["history"][-1]["fp32"] == 0 and res["history"][-1]["refine_left"] == 0
+    de = abs(res["etot"] - ro["etot"])
+    print(f"mixed {name}: dE = {de:.3e} Ha, fp32 steps {res['fp32_steps']} of {res['iterations']}")
+    assert de < 1e-6
+    assert np.max(np.abs(eg - ro["eigs_final"])) < 1e-8
+
+
+def test_fp32_step_tracks_fp64_step():
+    """The first SCF step from the same seeded start: FP32 bulk vs FP64."""
+    print("fp32 step", flush=True)
+    spec = config.benchmark(1)
+    e64, _, _ = energies(spec, 2)
+    e32, r32, _ = energies(spec, 2, precision=lib.DEREK_MIXED)
+    assert r32["history"][0]["fp32"] == 1
+    assert np.max(np.abs(e32 - e64)) < 1e-4 * np.max(np.abs(e64))
