@@ -39,6 +39,21 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def copy_calibration(mib):
+    """One hipMemcpy device -> device of `mib` MiB (read mib, write mib)."""
+    import ctypes as C
+    hip = C.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"# copy calibration: {mib} MiB device to device")
+
+
 def hpsi_rate(ctx, nb, steps, warmup):
     """bands/s of H|psi> on the resident bands, with the per-class profile."""
     for _ in range(warmup):
@@ -80,6 +95,9 @@ def main():
     ap.add_argument("--scf", default="cfg2,cfg4")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (the known-size kernel that "
+                         "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -91,6 +109,8 @@ def main():
     from derek_amd import config, lib
     from derek_amd import scf as gscf
 
+    if args.copy_calib > 0:
+        copy_calibration(args.copy_calib)
     spec = config.benchmark(2)
     t0 = time.perf_counter()
     su = gscf.Setup(spec)

@@ -168,10 +168,10 @@ void derek_ctx::apply_h(int k0, int nkc, PwBlock<R> psi, PwBlock<R> hpsi, int nb
     PwBlock<R> pi{psi.p + kk * psi.ks + (int64_t)b0 * psi.ld, psi.ld, psi.ks};
     PwBlock<R> ho{hpsi.p + kk * hpsi.ks + (int64_t)b0 * hpsi.ld, hpsi.ld, hpsi.ks};
     timed(KC_STICKS_Z_INV, pw_b + 12.0 * npw + st_b, 0, [&] { sticks_z_inv<R>(B, pi, ax, dims[2], S, s); });
-    timed(KC_PLANES_Y_INV, st_b + pl_b, 0, [&] { planes_y<R>(B, S, P, ax, dims[1], dims[2], +1, s); });
+    timed(KC_PLANES_Y_INV, st_b + pl_b, 0, [&] { planes_y<R>(B, S, P, ax, dims[1], dims[2], +1, bpw, s); });
     timed(KC_PLANES_X_VMUL, 2 * pl_b + (double)N * sizeof(R), 0,
           [&] { planes_x_vmul<R>(B, P, ax, nyz, v, (R)(1.0 / (double)N), bpw, s); });
-    timed(KC_PLANES_Y_FWD, st_b + pl_b, 0, [&] { planes_y<R>(B, S, P, ax, dims[1], dims[2], -1, s); });
+    timed(KC_PLANES_Y_FWD, st_b + pl_b, 0, [&] { planes_y<R>(B, S, P, ax, dims[1], dims[2], -1, bpw, s); });
     timed(KC_STICKS_Z_FWD, st_b + 2 * pw_b + 20.0 * npw, 0,
           [&] { sticks_z_fwd<R>(B, S, ax, dims[2], pi, ho, s); });
   });
@@ -199,7 +199,7 @@ void derek_ctx::density(int k0, int nkc, PwBlock<R> psi, int nbands, const doubl
     first = false;
     PwBlock<R> pi{psi.p + kk * psi.ks + (int64_t)b0 * psi.ld, psi.ld, psi.ks};
     timed(KC_STICKS_Z_INV, pw_b + 12.0 * npw + st_b, 0, [&] { sticks_z_inv<R>(B, pi, ax, dims[2], S, s); });
-    timed(KC_PLANES_Y_INV, st_b + pl_b, 0, [&] { planes_y<R>(B, S, P, ax, dims[1], dims[2], +1, s); });
+    timed(KC_PLANES_Y_INV, st_b + pl_b, 0, [&] { planes_y<R>(B, S, P, ax, dims[1], dims[2], +1, bpw, s); });
     timed(KC_PLANES_X_DENSITY, pl_b + (1 + kp) * 8.0 * N, 0, [&] {
       planes_x_density<R>(B, P, ax, nyz, occw + (int64_t)kk * occ_ks + b0, occ_ks, rho, kp, s);
     });

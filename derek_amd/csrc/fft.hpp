@@ -174,11 +174,53 @@ template <int R, int S, typename C> __device__ __forceinline__ void dft_direct(C
 
 template <int R, int S, typename C> __device__ __forceinline__ void dftR(C* v, const C* tw, int N);
 
+// cos and sin of 2*pi*m/R for the composite radices 6, 9, 12, 16 (m is a
+// compile-time constant once the radix loops are unrolled, so these fold
+// into immediates instead of table loads held in registers).
+template <int R> __device__ __forceinline__ void root_cs(int m, double& c, double& s) {
+  constexpr double h3 = 0.86602540378443864676;  // sin(pi/3)
+  constexpr double h2 = 0.70710678118654752440;  // sin(pi/4)
+  constexpr double c8 = 0.92387953251128675613, s8 = 0.38268343236508977173;  // pi/8
+  constexpr double c40 = 0.76604444311897803520, s40 = 0.64278760968653932632;
+  constexpr double c80 = 0.17364817766693034885, s80 = 0.98480775301220805936;
+  constexpr double c20 = 0.93969262078590838405, s20 = 0.34202014332566873304;  // 20 degrees
+  const int q = ((m % R) + R) % R;
+  if constexpr (R == 6) {
+    constexpr double cc[6] = {1.0, 0.5, -0.5, -1.0, -0.5, 0.5};
+    constexpr double ss[6] = {0.0, h3, h3, 0.0, -h3, -h3};
+    c = cc[q];
+    s = ss[q];
+  } else if constexpr (R == 9) {
+    constexpr double cc[9] = {1.0, c40, c80, -0.5, -c20, -c20, -0.5, c80, c40};
+    constexpr double ss[9] = {0.0, s40, s80, h3, s20, -s20, -h3, -s80, -s40};
+    c = cc[q];
+    s = ss[q];
+  } else if constexpr (R == 12) {
+    constexpr double cc[12] = {1.0, h3, 0.5, 0.0, -0.5, -h3, -1.0, -h3, -0.5, 0.0, 0.5, h3};
+    constexpr double ss[12] = {0.0, 0.5, h3, 1.0, h3, 0.5, 0.0, -0.5, -h3, -1.0, -h3, -0.5};
+    c = cc[q];
+    s = ss[q];
+  } else {
+    static_assert(R == 16, "composite radix without a root table");
+    constexpr double cc[16] = {1.0, c8, h2, s8, 0.0, -s8, -h2, -c8, -1.0, -c8, -h2, -s8, 0.0, s8, h2, c8};
+    constexpr double ss[16] = {0.0, s8, h2, c8, 1.0, c8, h2, s8, 0.0, -s8, -h2, -c8, -1.0, -c8, -h2, -s8};
+    c = cc[q];
+    s = ss[q];
+  }
+}
+
+// exp(S*2*pi*i*m/R)
+template <int R, int S, typename C> __device__ __forceinline__ C root(int m) {
+  using Rl = decltype(C::x);
+  double c, s;
+  root_cs<R>(m, c, s);
+  return cmk<C>((Rl)c, (Rl)(S > 0 ? s : -s));
+}
+
 // Composite R = A*B in registers: n = B*a + b, k = ka + A*kb;
 // B DFTs of length A, twiddles exp(S*2*pi*i*b*ka/R), A DFTs of length B.
 template <int A, int B, int S, typename C> __device__ __forceinline__ void dft_comp(C* v, const C* tw, int N) {
   constexpr int R = A * B;
-  const int step = N / R;
   C y[B][A];
 #pragma unroll
   for (int b = 0; b < B; ++b) {
@@ -187,7 +229,7 @@ template <int A, int B, int S, typename C> __device__ __forceinline__ void dft_c
     for (int a = 0; a < A; ++a) t[a] = v[B * a + b];
     dftR<A, S>(t, tw, N);
 #pragma unroll
-    for (int ka = 0; ka < A; ++ka) y[b][ka] = (b * ka) % R ? cmul(t[ka], twid<S>(tw, ((b * ka) % R) * step)) : t[ka];
+    for (int ka = 0; ka < A; ++ka) y[b][ka] = (b * ka) % R ? cmul(t[ka], root<R, S, C>(b * ka)) : t[ka];
   }
 #pragma unroll
   for (int ka = 0; ka < A; ++ka) {

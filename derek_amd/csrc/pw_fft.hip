@@ -266,6 +266,350 @@ __global__ __launch_bounds__(PNT) void k_planes_x_density(PwBatch B, const cx_t<
   }
 }
 
+// ---------------------------------------------------------------- two-stage column passes
+// For the compile-time lengths N = R1*R2 (two-stage plans) the y and x
+// passes keep the transform in registers between the stages and use one LDS
+// buffer (16 columns x N) for the single transpose each transform needs:
+//   S1: thread (l, j1 < R2): v_r = in[j1 + R2 r] (r < R1), DFT_R1 -> Y[j1 R1 + k]
+//   S2: thread (l, j2 < R1): u_r = Y[j2 + R1 r] w^(j2 r), DFT_R2 -> X[j2 + R1 k]
+// S1 reads straight from global memory (through the sticks' or active
+// planes' index map) and S2 writes straight to it.  The x pass multiplies X
+// by V in registers and runs the forward transform by decimation in
+// frequency on the same threads:
+//   A: T[j2][k2] = w^(-j2 k2) DFT_R2(X[j2 + R1 r]) -> LDS
+//   B: thread (l, k2 < R2): F[k2 + R2 k1] = DFT_R1 over j2 of T[j2][k2]
+// V and the stage twiddles of each thread are loaded once; the inputs of
+// the next band are loaded while the current one is transformed.
+// Inputs of the next band are prefetched into registers only for complex64
+// (for complex128 the extra registers would halve the waves per SIMD).
+template <typename R> constexpr bool kPrefetch = sizeof(R) == 4;
+
+template <int N> struct Split {
+  static constexpr FftPlan p = make_plan(N);
+  static_assert(p.ns == 2, "two-stage plan expected");
+  static constexpr int R1 = p.r[0], R2 = p.r[1];
+  static constexpr int NT = ((PL * (R1 > R2 ? R1 : R2) + 63) / 64) * 64;
+  static constexpr size_t lds(size_t csize) { return csize * (size_t)N * (PL + 1) + sizeof(int) * N; }
+};
+
+template <typename C, int N>
+__device__ __forceinline__ void load_twiddles(C* twl, const C* __restrict__ tw) {
+  for (int t = threadIdx.x; t < N; t += Split<N>::NT) twl[t] = tw[t];
+}
+
+// x pass: inverse x transform of the active rows, * V * vscale, forward
+// transform, active rows written back; BPW bands of one k-point per block
+// (V is loaded once per block).
+template <typename R, int N, int BPW>
+__global__ __launch_bounds__(Split<N>::NT) void k_px_vmul2(PwBatch B, cx_t<R>* __restrict__ P, int64_t nyz,
+                                                          const cx_t<R>* __restrict__ tw,
+                                                          const R* __restrict__ vloc, R vscale) {
+  using C = cx_t<R>;
+  using SP = Split<N>;
+  constexpr int R1 = SP::R1, R2 = SP::R2, NT = SP::NT;
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* twl = reinterpret_cast<C*>(smem);
+  C* buf = twl + N;
+  int* inv = reinterpret_cast<int*>(buf + N * PL);
+  const PwMaps m = B.mt[B.k0 + blockIdx.z];
+  const int tid = threadIdx.x, l = tid % PL, jj = tid / PL;
+  const int64_t c0 = (int64_t)blockIdx.x * PL;
+  const bool lok = c0 + l < nyz;
+  load_twiddles<C, N>(twl, tw);
+  for (int x = tid; x < N; x += NT) inv[x] = -1;
+  __syncthreads();
+  for (int a = tid; a < m.nax; a += NT) inv[m.ax_x[a]] = a;
+  __syncthreads();
+  // V at x = j2 + R1 r, loaded once (the stage twiddles w^(j2 r) are read from LDS)
+  R vr[R2];
+  if (jj < R1) {
+#pragma unroll
+    for (int r = 0; r < R2; ++r) vr[r] = lok ? vloc[(int64_t)(jj + R1 * r) * nyz + c0 + l] * vscale : (R)0;
+  }
+  const int bb0 = blockIdx.y * BPW;
+  const int bb1 = min(B.nbb, bb0 + BPW);
+  // S1 input rows (active row index per r, -1 if inactive)
+  int arow[R1];
+#pragma unroll
+  for (int r = 0; r < R1; ++r) arow[r] = (jj < R2 && lok) ? inv[jj + R2 * r] : -1;
+  auto load_s1 = [&](int band, C* v) {
+    const C* base = P + ((int64_t)blockIdx.z * B.nbb + band) * B.nax_cap * nyz + c0 + l;
+#pragma unroll
+    for (int r = 0; r < R1; ++r) v[r] = arow[r] >= 0 ? base[(int64_t)arow[r] * nyz] : cmk<C>(0, 0);
+  };
+  C pre[kPrefetch<R> ? R1 : 1];
+  if (kPrefetch<R> && bb0 < bb1) load_s1(bb0, pre);
+  for (int band = bb0; band < bb1; ++band) {
+    C v[R1];
+    if constexpr (kPrefetch<R>) {
+#pragma unroll
+      for (int r = 0; r < R1; ++r) v[r] = pre[r];
+      if (band + 1 < bb1) load_s1(band + 1, pre);
+    } else {
+      load_s1(band, v);
+    }
+    if (jj < R2) {
+      dftR<R1, +1>(v, twl, N);
+#pragma unroll
+      for (int k = 0; k < R1; ++k) buf[(jj * R1 + k) * PL + l] = v[k];
+    }
+    __syncthreads();
+    C u[R2];
+    if (jj < R1) {
+#pragma unroll
+      for (int r = 0; r < R2; ++r) {
+        const C y = buf[(jj + R1 * r) * PL + l];
+        u[r] = r ? cmul(y, cconj(twl[(jj * r) % N])) : y;
+      }
+      dftR<R2, +1>(u, twl, N);
+#pragma unroll
+      for (int r = 0; r < R2; ++r) u[r] = cscale(u[r], vr[r]);
+      dftR<R2, -1>(u, twl, N);
+#pragma unroll
+      for (int k2 = 1; k2 < R2; ++k2) u[k2] = cmul(u[k2], twl[(jj * k2) % N]);
+    }
+    __syncthreads();
+    if (jj < R1) {
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) buf[(jj * R2 + k2) * PL + l] = u[k2];
+    }
+    __syncthreads();
+    if (jj < R2) {
+      C t[R1];
+#pragma unroll
+      for (int j2 = 0; j2 < R1; ++j2) t[j2] = buf[(j2 * R2 + jj) * PL + l];
+      dftR<R1, -1>(t, twl, N);
+      C* base = P + ((int64_t)blockIdx.z * B.nbb + band) * B.nax_cap * nyz + c0 + l;
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) {
+        const int a = lok ? inv[jj + R2 * k1] : -1;
+        if (a >= 0) base[(int64_t)a * nyz] = t[k1];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// x pass for the density: rho[x*nyz + c] = keep*rho + sum_items occ |IFFT_x|^2,
+// accumulated in registers (thread (l, j2) owns x = j2 + R1 r).
+template <typename R, int N>
+__global__ __launch_bounds__(Split<N>::NT) void k_px_density2(PwBatch B, const cx_t<R>* __restrict__ P, int64_t nyz,
+                                                             const cx_t<R>* __restrict__ tw,
+                                                             const double* __restrict__ occ, int occ_ks,
+                                                             double* __restrict__ rho, int keep) {
+  using C = cx_t<R>;
+  using SP = Split<N>;
+  constexpr int R1 = SP::R1, R2 = SP::R2, NT = SP::NT;
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* twl = reinterpret_cast<C*>(smem);
+  C* buf = twl + N;
+  int* inv = reinterpret_cast<int*>(buf + N * PL);
+  const int tid = threadIdx.x, l = tid % PL, jj = tid / PL;
+  const int64_t c0 = (int64_t)blockIdx.x * PL;
+  const bool lok = c0 + l < nyz;
+  load_twiddles<C, N>(twl, tw);
+  double acc[R2];
+#pragma unroll
+  for (int r = 0; r < R2; ++r) acc[r] = 0.0;
+  for (int kz = 0; kz < B.nkc; ++kz) {
+    const PwMaps m = B.mt[B.k0 + kz];
+    __syncthreads();
+    for (int x = tid; x < N; x += NT) inv[x] = -1;
+    __syncthreads();
+    for (int a = tid; a < m.nax; a += NT) inv[m.ax_x[a]] = a;
+    __syncthreads();
+    int arow[R1];
+#pragma unroll
+    for (int r = 0; r < R1; ++r) arow[r] = (jj < R2 && lok) ? inv[jj + R2 * r] : -1;
+    auto load_s1 = [&](int band, C* v) {
+      const C* base = P + ((int64_t)kz * B.nbb + band) * B.nax_cap * nyz + c0 + l;
+#pragma unroll
+      for (int r = 0; r < R1; ++r) v[r] = arow[r] >= 0 ? base[(int64_t)arow[r] * nyz] : cmk<C>(0, 0);
+    };
+    C pre[kPrefetch<R> ? R1 : 1];
+    if (kPrefetch<R>) load_s1(0, pre);
+    for (int band = 0; band < B.nbb; ++band) {
+      const double w = occ[(int64_t)kz * occ_ks + band];
+      C v[R1];
+      if constexpr (kPrefetch<R>) {
+#pragma unroll
+        for (int r = 0; r < R1; ++r) v[r] = pre[r];
+        if (band + 1 < B.nbb) load_s1(band + 1, pre);
+      } else if (w != 0.0) {
+        load_s1(band, v);
+      }
+      if (w == 0.0) continue;  // uniform over the block
+      if (jj < R2) {
+        dftR<R1, +1>(v, twl, N);
+#pragma unroll
+        for (int k = 0; k < R1; ++k) buf[(jj * R1 + k) * PL + l] = v[k];
+      }
+      __syncthreads();
+      if (jj < R1) {
+        C u[R2];
+#pragma unroll
+        for (int r = 0; r < R2; ++r) {
+          const C y = buf[(jj + R1 * r) * PL + l];
+          u[r] = r ? cmul(y, cconj(twl[(jj * r) % N])) : y;
+        }
+        dftR<R2, +1>(u, twl, N);
+#pragma unroll
+        for (int r = 0; r < R2; ++r) {
+          const double re = (double)u[r].x, im = (double)u[r].y;
+          acc[r] += w * (re * re + im * im);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (jj < R1 && lok) {
+#pragma unroll
+    for (int r = 0; r < R2; ++r) {
+      double* p = rho + (int64_t)(jj + R1 * r) * nyz + c0 + l;
+      *p = (keep ? *p : 0.0) + acc[r];
+    }
+  }
+}
+
+// y pass, inverse: sticks of plane a (zero rows elsewhere) -> full y rows of P.
+template <typename R, int N, int BPW>
+__global__ __launch_bounds__(Split<N>::NT) void k_py_inv2(PwBatch B, const cx_t<R>* __restrict__ S, int nz,
+                                                         int ztiles, const cx_t<R>* __restrict__ tw,
+                                                         cx_t<R>* __restrict__ P) {
+  using C = cx_t<R>;
+  using SP = Split<N>;
+  constexpr int R1 = SP::R1, R2 = SP::R2, NT = SP::NT;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const PwMaps m = B.mt[B.k0 + blockIdx.z];
+  const int a = blockIdx.x / ztiles;
+  if (a >= m.nax) return;
+  C* twl = reinterpret_cast<C*>(smem);
+  C* buf = twl + N;
+  int* inv = reinterpret_cast<int*>(buf + N * PL);
+  const int tid = threadIdx.x, l = tid % PL, jj = tid / PL;
+  const int z0 = (blockIdx.x - a * ztiles) * PL;
+  const bool lok = z0 + l < nz;
+  load_twiddles<C, N>(twl, tw);
+  for (int y = tid; y < N; y += NT) inv[y] = -1;
+  __syncthreads();
+  const int sa = m.pl_st[a], sb = m.pl_st[a + 1];
+  for (int j = sa + tid; j < sb; j += NT) inv[m.st_y[j]] = j;
+  __syncthreads();
+  int srow[R1];
+#pragma unroll
+  for (int r = 0; r < R1; ++r) srow[r] = (jj < R2 && lok) ? inv[jj + R2 * r] : -1;
+  const int bb0 = blockIdx.y * BPW;
+  const int bb1 = min(B.nbb, bb0 + BPW);
+  auto load_s1 = [&](int band, C* v) {
+    const C* base = S + ((int64_t)blockIdx.z * B.nbb + band) * B.nst_cap * nz + z0 + l;
+#pragma unroll
+    for (int r = 0; r < R1; ++r) v[r] = srow[r] >= 0 ? base[(int64_t)srow[r] * nz] : cmk<C>(0, 0);
+  };
+  C pre[kPrefetch<R> ? R1 : 1];
+  if (kPrefetch<R> && bb0 < bb1) load_s1(bb0, pre);
+  for (int band = bb0; band < bb1; ++band) {
+    C v[R1];
+    if constexpr (kPrefetch<R>) {
+#pragma unroll
+      for (int r = 0; r < R1; ++r) v[r] = pre[r];
+      if (band + 1 < bb1) load_s1(band + 1, pre);
+    } else {
+      load_s1(band, v);
+    }
+    if (jj < R2) {
+      dftR<R1, +1>(v, twl, N);
+#pragma unroll
+      for (int k = 0; k < R1; ++k) buf[(jj * R1 + k) * PL + l] = v[k];
+    }
+    __syncthreads();
+    if (jj < R1) {
+      C u[R2];
+#pragma unroll
+      for (int r = 0; r < R2; ++r) {
+        const C y = buf[(jj + R1 * r) * PL + l];
+        u[r] = r ? cmul(y, cconj(twl[(jj * r) % N])) : y;
+      }
+      dftR<R2, +1>(u, twl, N);
+      if (lok) {
+        C* dst = P + (((int64_t)blockIdx.z * B.nbb + band) * B.nax_cap + a) * N * nz + z0 + l;
+#pragma unroll
+        for (int k = 0; k < R2; ++k) dst[(int64_t)(jj + R1 * k) * nz] = u[k];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// y pass, forward: full y rows of P -> stick lines of plane a.
+template <typename R, int N, int BPW>
+__global__ __launch_bounds__(Split<N>::NT) void k_py_fwd2(PwBatch B, const cx_t<R>* __restrict__ P, int nz,
+                                                         int ztiles, const cx_t<R>* __restrict__ tw,
+                                                         cx_t<R>* __restrict__ S) {
+  using C = cx_t<R>;
+  using SP = Split<N>;
+  constexpr int R1 = SP::R1, R2 = SP::R2, NT = SP::NT;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const PwMaps m = B.mt[B.k0 + blockIdx.z];
+  const int a = blockIdx.x / ztiles;
+  if (a >= m.nax) return;
+  C* twl = reinterpret_cast<C*>(smem);
+  C* buf = twl + N;
+  int* inv = reinterpret_cast<int*>(buf + N * PL);
+  const int tid = threadIdx.x, l = tid % PL, jj = tid / PL;
+  const int z0 = (blockIdx.x - a * ztiles) * PL;
+  const bool lok = z0 + l < nz;
+  load_twiddles<C, N>(twl, tw);
+  for (int y = tid; y < N; y += NT) inv[y] = -1;
+  __syncthreads();
+  const int sa = m.pl_st[a], sb = m.pl_st[a + 1];
+  for (int j = sa + tid; j < sb; j += NT) inv[m.st_y[j]] = j;
+  __syncthreads();
+  int srow[R2];
+  if (jj < R1) {
+#pragma unroll
+    for (int r = 0; r < R2; ++r) srow[r] = lok ? inv[jj + R1 * r] : -1;
+  }
+  const int bb0 = blockIdx.y * BPW;
+  const int bb1 = min(B.nbb, bb0 + BPW);
+  auto load_s1 = [&](int band, C* v) {
+    const C* base = P + (((int64_t)blockIdx.z * B.nbb + band) * B.nax_cap + a) * N * nz + z0 + l;
+#pragma unroll
+    for (int r = 0; r < R1; ++r) v[r] = (jj < R2 && lok) ? base[(int64_t)(jj + R2 * r) * nz] : cmk<C>(0, 0);
+  };
+  C pre[kPrefetch<R> ? R1 : 1];
+  if (kPrefetch<R> && bb0 < bb1) load_s1(bb0, pre);
+  for (int band = bb0; band < bb1; ++band) {
+    C v[R1];
+    if constexpr (kPrefetch<R>) {
+#pragma unroll
+      for (int r = 0; r < R1; ++r) v[r] = pre[r];
+      if (band + 1 < bb1) load_s1(band + 1, pre);
+    } else {
+      load_s1(band, v);
+    }
+    if (jj < R2) {
+      dftR<R1, -1>(v, twl, N);
+#pragma unroll
+      for (int k = 0; k < R1; ++k) buf[(jj * R1 + k) * PL + l] = v[k];
+    }
+    __syncthreads();
+    if (jj < R1) {
+      C u[R2];
+#pragma unroll
+      for (int r = 0; r < R2; ++r) {
+        const C y = buf[(jj + R1 * r) * PL + l];
+        u[r] = r ? cmul(y, twl[(jj * r) % N]) : y;
+      }
+      dftR<R2, -1>(u, twl, N);
+      C* dst = S + ((int64_t)blockIdx.z * B.nbb + band) * B.nst_cap * nz + z0 + l;
+#pragma unroll
+      for (int k = 0; k < R2; ++k)
+        if (srow[k] >= 0) dst[(int64_t)srow[k] * nz] = u[k];
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------- launchers
 template <typename K> static void allow_lds_pw(K kern, size_t bytes) {
   if (bytes > 65536) allow_dynamic_lds(reinterpret_cast<const void*>(kern), bytes);
@@ -308,31 +652,58 @@ void sticks_z_inv(const PwBatch& B, PwBlock<R> psi, const Axis<R>* ax, int nz, c
   DEREK_HIP(hipGetLastError());
 }
 
-template <typename R>
-void planes_y(const PwBatch& B, cx_t<R>* S, cx_t<R>* P, const Axis<R>* ax, int ny, int nz, int sign, hipStream_t s) {
-  check_batch(B);
+template <typename R, int N, int BPW>
+static void planes_y2(const PwBatch& B, cx_t<R>* S, cx_t<R>* P, const Axis<R>* ax, int nz, int sign, hipStream_t s) {
   const int ztiles = (int)ceil_div(nz, PL);
-  const size_t lds = sizeof(cx_t<R>) * ((size_t)ny + 2 * (size_t)PL * ny);
-  check_lds_pw(lds, ny);
-  dim3 grid((unsigned)(B.nax_cap * ztiles), B.nbb, B.nkc);
+  const size_t lds = Split<N>::lds(sizeof(cx_t<R>));
+  dim3 grid((unsigned)(B.nax_cap * ztiles), (unsigned)ceil_div(B.nbb, BPW), B.nkc);
   if (sign > 0) {
-#define C_(NC)                                                                                 \
-  {                                                                                            \
-    auto kern = k_planes_y_inv<R, NC>;                                                         \
-    allow_lds_pw(kern, lds);                                                                   \
-    hipLaunchKernelGGL(kern, grid, dim3(PNT), lds, s, B, S, nz, ztiles, ax[1].plan, ax[1].tw, P); \
-  }
-    PW_DISPATCH(ny, C_)
-#undef C_
+    auto kern = k_py_inv2<R, N, BPW>;
+    allow_lds_pw(kern, lds);
+    hipLaunchKernelGGL(kern, grid, dim3(Split<N>::NT), lds, s, B, S, nz, ztiles, ax[1].tw, P);
   } else {
-#define C_(NC)                                                                                 \
-  {                                                                                            \
-    auto kern = k_planes_y_fwd<R, NC>;                                                         \
-    allow_lds_pw(kern, lds);                                                                   \
-    hipLaunchKernelGGL(kern, grid, dim3(PNT), lds, s, B, P, nz, ztiles, ax[1].plan, ax[1].tw, S); \
+    auto kern = k_py_fwd2<R, N, BPW>;
+    allow_lds_pw(kern, lds);
+    hipLaunchKernelGGL(kern, grid, dim3(Split<N>::NT), lds, s, B, P, nz, ztiles, ax[1].tw, S);
   }
-    PW_DISPATCH(ny, C_)
-#undef C_
+}
+
+template <typename R, int N>
+static void planes_y2_bpw(const PwBatch& B, cx_t<R>* S, cx_t<R>* P, const Axis<R>* ax, int nz, int sign, int bpw,
+                          hipStream_t s) {
+  switch (bpw) {
+    case 1: planes_y2<R, N, 1>(B, S, P, ax, nz, sign, s); break;
+    case 2: planes_y2<R, N, 2>(B, S, P, ax, nz, sign, s); break;
+    case 4: planes_y2<R, N, 4>(B, S, P, ax, nz, sign, s); break;
+    case 8: planes_y2<R, N, 8>(B, S, P, ax, nz, sign, s); break;
+    default: throw ArgError("bands per workgroup must be 1, 2, 4 or 8");
+  }
+}
+
+template <typename R>
+void planes_y(const PwBatch& B, cx_t<R>* S, cx_t<R>* P, const Axis<R>* ax, int ny, int nz, int sign, int bpw,
+              hipStream_t s) {
+  check_batch(B);
+  switch (ny) {
+    case 32: planes_y2_bpw<R, 32>(B, S, P, ax, nz, sign, bpw, s); break;
+    case 72: planes_y2_bpw<R, 72>(B, S, P, ax, nz, sign, bpw, s); break;
+    case 108: planes_y2_bpw<R, 108>(B, S, P, ax, nz, sign, bpw, s); break;
+    case 144: planes_y2_bpw<R, 144>(B, S, P, ax, nz, sign, bpw, s); break;
+    default: {
+      const int ztiles = (int)ceil_div(nz, PL);
+      const size_t lds = sizeof(cx_t<R>) * ((size_t)ny + 2 * (size_t)PL * ny);
+      check_lds_pw(lds, ny);
+      dim3 grid((unsigned)(B.nax_cap * ztiles), B.nbb, B.nkc);
+      if (sign > 0) {
+        auto kern = k_planes_y_inv<R, 0>;
+        allow_lds_pw(kern, lds);
+        hipLaunchKernelGGL(kern, grid, dim3(PNT), lds, s, B, S, nz, ztiles, ax[1].plan, ax[1].tw, P);
+      } else {
+        auto kern = k_planes_y_fwd<R, 0>;
+        allow_lds_pw(kern, lds);
+        hipLaunchKernelGGL(kern, grid, dim3(PNT), lds, s, B, P, nz, ztiles, ax[1].plan, ax[1].tw, S);
+      }
+    }
   }
   DEREK_HIP(hipGetLastError());
 }
@@ -360,17 +731,28 @@ template <typename R, int BPW>
 static void planes_x_vmul_bpw(const PwBatch& B, cx_t<R>* P, const Axis<R>* ax, int64_t nyz, const R* vloc,
                               R vscale, hipStream_t s) {
   const int nx = ax[0].n;
-  const size_t lds = sizeof(cx_t<R>) * ((size_t)nx + 2 * (size_t)PL * nx) + sizeof(R) * (size_t)PL * nx;
-  check_lds_pw(lds, nx);
   dim3 grid((unsigned)ceil_div(nyz, PL), (unsigned)ceil_div(B.nbb, BPW), B.nkc);
-#define C_(NC)                                                                                        \
-  {                                                                                                   \
-    auto kern = k_planes_x_vmul<R, NC, BPW>;                                                          \
-    allow_lds_pw(kern, lds);                                                                          \
-    hipLaunchKernelGGL(kern, grid, dim3(PNT), lds, s, B, P, nyz, ax[0].plan, ax[0].tw, vloc, vscale); \
+#define C2_(NC)                                                                                           \
+  {                                                                                                       \
+    auto kern = k_px_vmul2<R, NC, BPW>;                                                                   \
+    const size_t lds = Split<NC>::lds(sizeof(cx_t<R>));                                                   \
+    allow_lds_pw(kern, lds);                                                                              \
+    hipLaunchKernelGGL(kern, grid, dim3(Split<NC>::NT), lds, s, B, P, nyz, ax[0].tw, vloc, vscale);       \
   }
-  PW_DISPATCH(nx, C_)
-#undef C_
+  switch (nx) {
+    case 32: C2_(32); break;
+    case 72: C2_(72); break;
+    case 108: C2_(108); break;
+    case 144: C2_(144); break;
+    default: {
+      const size_t lds = sizeof(cx_t<R>) * ((size_t)nx + 2 * (size_t)PL * nx) + sizeof(R) * (size_t)PL * nx;
+      check_lds_pw(lds, nx);
+      auto kern = k_planes_x_vmul<R, 0, BPW>;
+      allow_lds_pw(kern, lds);
+      hipLaunchKernelGGL(kern, grid, dim3(PNT), lds, s, B, P, nyz, ax[0].plan, ax[0].tw, vloc, vscale);
+    }
+  }
+#undef C2_
 }
 
 template <typename R>
@@ -392,23 +774,34 @@ void planes_x_density(const PwBatch& B, const cx_t<R>* P, const Axis<R>* ax, int
                       int occ_ks, double* rho, int keep, hipStream_t s) {
   check_batch(B);
   const int nx = ax[0].n;
-  const size_t lds = sizeof(cx_t<R>) * ((size_t)nx + 2 * (size_t)PL * nx) + sizeof(double) * (size_t)PL * nx;
-  check_lds_pw(lds, nx);
   dim3 grid((unsigned)ceil_div(nyz, PL));
-#define C_(NC)                                                                                                  \
+#define C2_(NC)                                                                                                 \
   {                                                                                                             \
-    auto kern = k_planes_x_density<R, NC>;                                                                      \
+    auto kern = k_px_density2<R, NC>;                                                                           \
+    const size_t lds = Split<NC>::lds(sizeof(cx_t<R>));                                                         \
     allow_lds_pw(kern, lds);                                                                                    \
-    hipLaunchKernelGGL(kern, grid, dim3(PNT), lds, s, B, P, nyz, ax[0].plan, ax[0].tw, occ, occ_ks, rho, keep); \
+    hipLaunchKernelGGL(kern, grid, dim3(Split<NC>::NT), lds, s, B, P, nyz, ax[0].tw, occ, occ_ks, rho, keep);   \
   }
-  PW_DISPATCH(nx, C_)
-#undef C_
+  switch (nx) {
+    case 32: C2_(32); break;
+    case 72: C2_(72); break;
+    case 108: C2_(108); break;
+    case 144: C2_(144); break;
+    default: {
+      const size_t lds = sizeof(cx_t<R>) * ((size_t)nx + 2 * (size_t)PL * nx) + sizeof(double) * (size_t)PL * nx;
+      check_lds_pw(lds, nx);
+      auto kern = k_planes_x_density<R, 0>;
+      allow_lds_pw(kern, lds);
+      hipLaunchKernelGGL(kern, grid, dim3(PNT), lds, s, B, P, nyz, ax[0].plan, ax[0].tw, occ, occ_ks, rho, keep);
+    }
+  }
+#undef C2_
   DEREK_HIP(hipGetLastError());
 }
 
 #define INST(R)                                                                                                   \
   template void sticks_z_inv<R>(const PwBatch&, PwBlock<R>, const Axis<R>*, int, cx_t<R>*, hipStream_t);         \
-  template void planes_y<R>(const PwBatch&, cx_t<R>*, cx_t<R>*, const Axis<R>*, int, int, int, hipStream_t);      \
+  template void planes_y<R>(const PwBatch&, cx_t<R>*, cx_t<R>*, const Axis<R>*, int, int, int, int, hipStream_t); \
   template void sticks_z_fwd<R>(const PwBatch&, const cx_t<R>*, const Axis<R>*, int, PwBlock<R>, PwBlock<R>,      \
                                 hipStream_t);                                                                     \
   template void planes_x_vmul<R>(const PwBatch&, cx_t<R>*, const Axis<R>*, int64_t, const R*, R, int, hipStream_t); \

@@ -18,9 +18,11 @@ template <typename R> struct PwBlock {
 // z pass on sticks, inverse, with the plane-wave scatter: psi -> S.
 template <typename R>
 void sticks_z_inv(const PwBatch& B, PwBlock<R> psi, const Axis<R>* ax, int nz, cx_t<R>* S, hipStream_t s);
-// y pass on the active planes: sign +1 inverse S -> P, sign -1 forward P -> S.
+// y pass on the active planes: sign +1 inverse S -> P, sign -1 forward P -> S;
+// bands_per_wg (1, 2, 4, 8) bands of one k-point per workgroup.
 template <typename R>
-void planes_y(const PwBatch& B, cx_t<R>* S, cx_t<R>* P, const Axis<R>* ax, int ny, int nz, int sign, hipStream_t s);
+void planes_y(const PwBatch& B, cx_t<R>* S, cx_t<R>* P, const Axis<R>* ax, int ny, int nz, int sign,
+              int bands_per_wg, hipStream_t s);
 // z pass on sticks, forward, with the gather and the kinetic term:
 // hpsi = FFT(S)[plane waves] + kin * psi.
 template <typename R>
