@@ -72,11 +72,11 @@ def hpsi_rate(ctx, nb, steps, warmup):
     return steps * nb / dt, dt / steps * 1e3, prof
 
 
-def scf_run(gscf, lib, spec, su, precision, **kw):
+def scf_run(gscf, lib, spec, su, precision, verbose=False, **kw):
     """Time to |dE| < 1e-6 Ha from a fresh engine (init included)."""
     ctx = su.context(precision)
     t0 = time.perf_counter()
-    res = gscf.run(spec, max_scf=80, energy_tol=1e-6, setup=su, ctx=ctx, log=None, **kw)
+    res = gscf.run(spec, max_scf=80, energy_tol=1e-6, setup=su, ctx=ctx, log=log if verbose else None, **kw)
     wall = time.perf_counter() - t0
     out = {"time_to_scf_s": round(wall, 4), "init_s": round(res["wall_init"], 4), "steps": res["iterations"],
            "fp32_steps": res["fp32_steps"], "ms_per_step": round(res["wall"] / res["iterations"] * 1e3, 2),
@@ -95,6 +95,8 @@ def main():
     ap.add_argument("--scf", default="cfg2,cfg4")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="run the batched SCF modes eagerly (no hipGraph replay), e.g. under a kernel tracer")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -151,10 +153,11 @@ def main():
 
     scf_info = {}
     todo = [] if args.no_scf else [x.strip() for x in args.scf.split(",") if x.strip()]
+    graph = not args.no_graph
     if "cfg2" in todo:
         r64, e64 = scf_run(gscf, lib, spec, su, lib.DEREK_FP64, kbatch=1)
         log(f"# cfg2 FP64: {r64['steps']} steps, {r64['time_to_scf_s']:.3f} s, E = {r64['etot_ha']:.10f}")
-        rmx, emx = scf_run(gscf, lib, spec, su, lib.DEREK_MIXED, graph=True)
+        rmx, emx = scf_run(gscf, lib, spec, su, lib.DEREK_MIXED, graph=graph)
         log(f"# cfg2 mixed: {rmx['steps']} steps ({rmx['fp32_steps']} FP32), {rmx['time_to_scf_s']:.3f} s, "
             f"E = {rmx['etot_ha']:.10f}")
         scf_info["cfg2_fp64"] = r64
@@ -167,9 +170,9 @@ def main():
         log(f"# cfg4 grid {su4.dims} nk {su4.nk} npw {min(su4.npw)}..{max(su4.npw)}")
         p64, e_p = scf_run(gscf, lib, spec4, su4, lib.DEREK_FP64, kbatch=1)
         log(f"# cfg4 FP64 plain: {p64['steps']} steps, {p64['time_to_scf_s']:.3f} s, E = {p64['etot_ha']:.10f}")
-        b64, e_b = scf_run(gscf, lib, spec4, su4, lib.DEREK_FP64, kbatch=0, graph=True)
+        b64, e_b = scf_run(gscf, lib, spec4, su4, lib.DEREK_FP64, kbatch=0, graph=graph)
         log(f"# cfg4 FP64 batched: {b64['steps']} steps, {b64['time_to_scf_s']:.3f} s")
-        bmx, e_m = scf_run(gscf, lib, spec4, su4, lib.DEREK_MIXED, kbatch=0, graph=True, autotune=True)
+        bmx, e_m = scf_run(gscf, lib, spec4, su4, lib.DEREK_MIXED, kbatch=0, graph=graph, autotune=True)
         log(f"# cfg4 mixed batched: {bmx['steps']} steps ({bmx['fp32_steps']} FP32), {bmx['time_to_scf_s']:.3f} s")
         scf_info["cfg4_fp64_plain"] = p64
         scf_info["cfg4_fp64_batched"] = dict(b64, dE_vs_plain_ha=abs(b64["etot_ha"] - p64["etot_ha"]),
@@ -177,6 +180,20 @@ def main():
         scf_info["cfg4_mixed_batched"] = dict(bmx, dE_vs_plain_ha=abs(bmx["etot_ha"] - p64["etot_ha"]),
                                               max_deig_vs_plain_ha=float(np.max(np.abs(e_m - e_p))),
                                               speedup_vs_plain=round(p64["time_to_scf_s"] / bmx["time_to_scf_s"], 3))
+
+    if "cfg5" in todo:
+        spec5 = config.benchmark(5)
+        su5 = gscf.Setup(spec5)
+        log(f"# cfg5 grid {su5.dims} nk {su5.nk} npw {min(su5.npw)}..{max(su5.npw)} bands {spec5.nbands}")
+        f64, e_f = scf_run(gscf, lib, spec5, su5, lib.DEREK_FP64, verbose=True, kbatch=0, graph=graph)
+        log(f"# cfg5 FP64 batched: {f64['steps']} steps, {f64['time_to_scf_s']:.3f} s, E = {f64['etot_ha']:.10f}")
+        mx5, e_x = scf_run(gscf, lib, spec5, su5, lib.DEREK_MIXED, verbose=True, kbatch=0, graph=graph,
+                           autotune=True)
+        log(f"# cfg5 mixed batched: {mx5['steps']} steps ({mx5['fp32_steps']} FP32), {mx5['time_to_scf_s']:.3f} s")
+        scf_info["cfg5_fp64_batched"] = f64
+        scf_info["cfg5_mixed_batched"] = dict(mx5, dE_vs_fp64_ha=abs(mx5["etot_ha"] - f64["etot_ha"]),
+                                              max_deig_vs_fp64_ha=float(np.max(np.abs(e_x - e_f))),
+                                              speedup_vs_fp64=round(f64["time_to_scf_s"] / mx5["time_to_scf_s"], 3))
 
     cpu = None
     if not args.no_cpu:

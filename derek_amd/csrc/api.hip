@@ -558,10 +558,11 @@ int32_t derek_hip_overlap(derek_ctx* c, int64_t npw, int32_t m, int32_t n, const
     double2* ds = c->tC.get<double2>((size_t)m * n);
     DEREK_HIP(hipMemcpyAsync(dx, X, (size_t)m * npw * sizeof(double2), hipMemcpyHostToDevice, c->s));
     if (!herm) DEREK_HIP(hipMemcpyAsync(dy, Y, (size_t)n * npw * sizeof(double2), hipMemcpyHostToDevice, c->s));
-    double2* w = c->ovwork.get<double2>(overlap_work_elems(m, n, npw, 1));
+    const int64_t wcap = overlap_work_elems(m, n, npw, 1);
+    double2* w = c->ovwork.get<double2>(wcap);
     if (c->prec == DEREK_FP64) {
       c->timed(KC_OVERLAP, (double)(m + n) * npw * 16.0, 8.0 * npw * m * n, [&] {
-        overlap<double>(dx, npw, 0, dy, npw, 0, m, n, npw, nullptr, ds, m, 0, 1, w, herm, c->s);
+        overlap<double>(dx, npw, 0, dy, npw, 0, m, n, npw, nullptr, ds, m, 0, 1, w, wcap, herm, c->s);
       });
     } else {
       // complex64 operands on the f32 MFMA path, as in the SCF's FP32 phase
@@ -570,7 +571,7 @@ int32_t derek_hip_overlap(derek_ctx* c, int64_t npw, int32_t m, int32_t n, const
       convert<float, double>(fx, dx, (int64_t)m * npw, c->s);
       if (!herm) convert<float, double>(fy, dy, (int64_t)n * npw, c->s);
       c->timed(KC_OVERLAP, (double)(m + n) * npw * 8.0, 8.0 * npw * m * n, [&] {
-        overlap<float>(fx, npw, 0, fy, npw, 0, m, n, npw, nullptr, ds, m, 0, 1, w, herm, c->s);
+        overlap<float>(fx, npw, 0, fy, npw, 0, m, n, npw, nullptr, ds, m, 0, 1, w, wcap, herm, c->s);
       });
     }
     DEREK_HIP(hipMemcpyAsync(S, ds, (size_t)m * n * sizeof(double2), hipMemcpyDeviceToHost, c->s));

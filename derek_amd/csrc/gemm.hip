@@ -189,10 +189,11 @@ __global__ void k_overlap_reduce(const double2* __restrict__ part, OvGeom g, dou
 template <typename R>
 void overlap(const cx_t<R>* X, int64_t ldx, int64_t sx, const cx_t<R>* Y, int64_t ldy, int64_t sy, int m, int n,
              int64_t npw, const int* npwk, double2* C, int ldc, int64_t sc, int batch, double2* work,
-             bool hermitian, hipStream_t s) {
+             int64_t work_cap, bool hermitian, hipStream_t s) {
   DEREK_REQUIRE(m > 0 && n > 0 && batch > 0, "overlap: empty block");
   DEREK_REQUIRE(!hermitian || m == n, "overlap: hermitian needs a square result");
   OvGeom g = ov_geom(m, n, npw, hermitian, batch);
+  DEREK_REQUIRE((int64_t)batch * g.nchunk * m * n <= work_cap, "overlap: partial-sum buffer too small");
   hipLaunchKernelGGL(k_overlap<R>, dim3(g.ntiles, g.nchunk, batch), dim3(256), 0, s, X, ldx, sx, Y, ldy, sy, npwk, g,
                      work);
   DEREK_HIP(hipGetLastError());
@@ -285,9 +286,9 @@ void rotate(const cx_t<R>* X, int64_t ldx, int64_t sx, const double2* U, int ldu
 }
 
 template void overlap<double>(const double2*, int64_t, int64_t, const double2*, int64_t, int64_t, int, int, int64_t,
-                              const int*, double2*, int, int64_t, int, double2*, bool, hipStream_t);
+                              const int*, double2*, int, int64_t, int, double2*, int64_t, bool, hipStream_t);
 template void overlap<float>(const float2*, int64_t, int64_t, const float2*, int64_t, int64_t, int, int, int64_t,
-                             const int*, double2*, int, int64_t, int, double2*, bool, hipStream_t);
+                             const int*, double2*, int, int64_t, int, double2*, int64_t, bool, hipStream_t);
 template void rotate<double>(const double2*, int64_t, int64_t, const double2*, int, int64_t, int, int, int64_t,
                              const int*, double2*, int64_t, int64_t, int, hipStream_t);
 template void rotate<float>(const float2*, int64_t, int64_t, const double2*, int, int64_t, int, int, int64_t,

@@ -56,12 +56,12 @@ void convert_real_f(float* dst, const double* src, int64_t n, hipStream_t s);
 // matrix its own row count (<= npw), npw otherwise.
 // C (m x n, column-major, ldc) = X^H Y; X: npw x m (ldx), Y: npw x n (ldy),
 // accumulated on MFMA in R, chunk partials summed in FP64.
-// `work` must hold overlap_work_elems(m, n, npw, batch) double2.
+// `work` (work_cap double2, checked) must hold overlap_work_elems(m, n, npw, batch).
 int64_t overlap_work_elems(int m, int n, int64_t npw, int batch);
 template <typename R>
 void overlap(const cx_t<R>* X, int64_t ldx, int64_t sx, const cx_t<R>* Y, int64_t ldy, int64_t sy, int m, int n,
              int64_t npw, const int* npwk, double2* C, int ldc, int64_t sc, int batch, double2* work,
-             bool hermitian, hipStream_t s);
+             int64_t work_cap, bool hermitian, hipStream_t s);
 // Z (npw x n, ldz) = X (npw x m, ldx) * U (m x n, FP64, column-major, ldu),
 // U rounded to R.
 template <typename R>

@@ -33,10 +33,11 @@ struct ScfEngine {
   int64_t ldk = 0, kstride = 0;
   bool fp32 = false, graphs = false;
   int refine = 3, refine_left = 0;
-  double switch_tol = 1e-5, e_prev = 0.0;
+  double switch_tol = 1e-5, e_prev = 0.0, de_prev = 1e300;
   bool have_prev = false, have_rho_out = false;
   void *Y = nullptr, *HY = nullptr, *Y2 = nullptr, *HY2 = nullptr;
   double2 *S = nullptr, *H = nullptr, *U = nullptr, *ovw = nullptr;
+  int64_t ovw_cap = 0;  // double2 elements of ovw
   DenseWork dw;
   double *eps = nullptr, *ekin = nullptr, *rn = nullptr, *occ = nullptr, *occw = nullptr, *stats_partial = nullptr;
   double *rho_in = nullptr, *rho_out = nullptr, *vh = nullptr, *vxc = nullptr, *vhxc = nullptr, *tmp1 = nullptr,
@@ -102,7 +103,7 @@ void ov(derek_ctx* c, const cx_t<R>* X, const cx_t<R>* Yp, int m, int n, int k0,
   const double npw = npw_sum(c, k0, nkc);
   c->timed(KC_OVERLAP, (double)(herm ? m : m + n) * npw * sizeof(cx_t<R>), 8.0 * npw * m * n, [&] {
     overlap<R>(X, E.ldk, E.kstride, Yp, E.ldk, E.kstride, m, n, c->max_npw, c->d_npw + k0, C, m, (int64_t)m * n, nkc,
-               E.ovw, herm, c->s);
+               E.ovw, E.ovw_cap, herm, c->s);
   });
 }
 
@@ -143,10 +144,20 @@ template <typename R> int davidson_chunk(derek_ctx* c, int k0, int nkc, int nit,
                     s);
     });
   };
-  // Rayleigh-Ritz on X
+  // Rayleigh-Ritz on X.  Complex64 blocks drift from orthonormality with
+  // every rotation, so the FP32 phase solves X^H H X c = w X^H X c (which
+  // re-orthonormalises X) where FP64 uses the standard problem.
+  constexpr bool f32 = sizeof(R) == 4;
+  // dependent-column threshold of the Cholesky: FP32 overlaps carry ~1e-6 errors
+  const double drop = f32 ? 1e-5 : 1e-8;
   c->apply_h<R>(k0, nkc, pb(Y, 0), pb(HY, 0), nb);
   ov<R>(c, Y, HY, nb, nb, k0, nkc, H, false);
-  c->timed(KC_DENSE, 0, 0, [&] { rr_solve(E.dw, H, nb, nkc, eps, U, s); });
+  if (f32) {
+    ov<R>(c, Y, Y, nb, nb, k0, nkc, S, true);
+    c->timed(KC_DENSE, 0, 0, [&] { gen_solve(E.dw, S, H, nb, nb, nkc, drop, eps, U, s); });
+  } else {
+    c->timed(KC_DENSE, 0, 0, [&] { rr_solve(E.dw, H, nb, nkc, eps, U, s); });
+  }
   rot<R>(c, Y, U, nb, nb, (int64_t)nb * nb, k0, nkc, Y2);
   rot<R>(c, HY, U, nb, nb, (int64_t)nb * nb, k0, nkc, HY2);
   std::swap(Y, Y2);
@@ -171,7 +182,7 @@ template <typename R> int davidson_chunk(derek_ctx* c, int k0, int nkc, int nit,
     c->apply_h<R>(k0, nkc, pb(Y, nb), pb(HY, nb), nb);
     ov<R>(c, Y, Y, m2, m2, k0, nkc, S, true);
     ov<R>(c, Y, HY, m2, m2, k0, nkc, H, false);
-    c->timed(KC_DENSE, 0, 0, [&] { gen_solve(E.dw, S, H, m2, nb, nkc, 1e-8, eps, U, s); });
+    c->timed(KC_DENSE, 0, 0, [&] { gen_solve(E.dw, S, H, m2, nb, nkc, drop, eps, U, s); });
     rot<R>(c, Y, U, m2, nb, (int64_t)m2 * nb, k0, nkc, Y2);
     rot<R>(c, HY, U, m2, nb, (int64_t)m2 * nb, k0, nkc, HY2);
     std::swap(Y, Y2);
@@ -388,7 +399,12 @@ void scf_init(derek_ctx* c, const derek_scf_params& p) {
   E.H = E.alloc<double2>(mm * nk);
   E.U = E.alloc<double2>(mm * nk);
   E.dw.ensure(E.m2, E.m2, E.kc);
-  E.ovw = E.alloc<double2>(overlap_work_elems(E.m2, E.m2, c->max_npw, E.kc));
+  // partial sums of every overlap shape the engine launches: nb x nb and
+  // 2nb x 2nb, for full batches and the last, shorter one
+  for (int b : {E.kc, nk % E.kc})
+    if (b > 0)
+      for (int m : {nb, E.m2}) E.ovw_cap = std::max(E.ovw_cap, overlap_work_elems(m, m, c->max_npw, b));
+  E.ovw = E.alloc<double2>(E.ovw_cap);
   for (double** q : {&E.eps, &E.ekin, &E.rn, &E.occ, &E.occw}) *q = E.alloc<double>((size_t)nk * nb);
   E.stats_partial = E.alloc<double>((size_t)E.kc * nb * band_stats_chunks(c->max_npw) * 2);
   for (double** q : {&E.rho_in, &E.rho_out, &E.vh, &E.vxc, &E.vhxc, &E.tmp1, &E.tmp2}) *q = E.alloc<double>(c->N);
@@ -479,7 +495,16 @@ void scf_step(derek_ctx* c, derek_scf_report& rep) {
   rep.fp32 = fp32_step ? 1 : 0;
   if (fp32_step) {
     E.refine_left = E.refine;
-    if (E.have_prev && std::fabs(rep.etot - E.e_prev) < E.switch_tol) switch_to_fp64(c);
+    if (E.have_prev) {
+      // leave the FP32 phase at |dE| < switch_tol, at its noise floor
+      // (2e-7 |E|, a few complex64 roundings of the energy), or once |dE|
+      // grows again after the step has reached 1e-3 Ha
+      const double de = std::fabs(rep.etot - E.e_prev);
+      const bool floor = de < std::max(E.switch_tol, 2e-7 * std::fabs(rep.etot));
+      const bool stall = E.de_prev < 1e-3 && de > E.de_prev;
+      if (floor || stall) switch_to_fp64(c);
+      E.de_prev = de;
+    }
   } else if (E.refine_left > 0) {
     E.refine_left -= 1;
   }
