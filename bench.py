@@ -117,19 +117,21 @@ def main():
     t0 = time.perf_counter()
     su = gscf.Setup(spec)
     ctx = su.context()
-    ctx.scf_init(spec.nelec, kT=spec.kT, seed=spec.seed)
+    ctx.scf_init(spec.nelec, kT=spec.kT, seed=spec.seed, autotune=True)
     ctx.set_vloc(su.vext)
+    tune64 = ctx.tuning()
     nb = spec.nbands
     log(f"# setup {time.perf_counter() - t0:.2f}s grid {su.dims} npw {su.npw[0]} bands {nb}")
     bands_per_s, ms_per_step, prof = hpsi_rate(ctx, nb, args.steps, args.warmup)
-    log(f"# H|psi> FP64: {ms_per_step:.3f} ms per {nb} bands, {bands_per_s:.1f} bands/s")
+    log(f"# H|psi> FP64: {ms_per_step:.3f} ms per {nb} bands, {bands_per_s:.1f} bands/s, shapes {tune64}")
     ctx.close()
     # the same H|psi> in complex64 (the mixed form's bulk)
     ctx = su.context(lib.DEREK_MIXED)
-    ctx.scf_init(spec.nelec, kT=spec.kT, seed=spec.seed)
+    ctx.scf_init(spec.nelec, kT=spec.kT, seed=spec.seed, autotune=True)
     ctx.set_vloc(su.vext)
+    tune32 = ctx.tuning()
     bps32, ms32, prof32 = hpsi_rate(ctx, nb, args.steps, args.warmup)
-    log(f"# H|psi> FP32: {ms32:.3f} ms per {nb} bands, {bps32:.1f} bands/s")
+    log(f"# H|psi> FP32: {ms32:.3f} ms per {nb} bands, {bps32:.1f} bands/s, shapes {tune32}")
     ctx.close()
 
     # dominant kernel of the timed FP64 region
@@ -219,9 +221,13 @@ def main():
         "data": "synthetic (seeded wells and wavefunctions)",
         "config": {"workload": "plain FP64 H|psi>: cubic L=20 bohr, Ecut=30 Ha, 108^3 grid, 62669 plane waves, "
                                "80 bands (Gamma), 16 wells + 0.2 Ha cos", "bands": nb, "npw": su.npw[0],
-                   "grid": su.dims, "parallelism": "single GPU"},
+                   "grid": su.dims, "parallelism": "single GPU",
+                   "launch_shapes": {"bands_per_wg": tune64["bands_per_wg"],
+                                     "items_per_launch": tune64["items_per_launch"]}},
         "roofline": roofline, "kernels": kernels,
         "hpsi_fp32": {"value": round(bps32, 2), "unit": "bands/s", "ms_per_step": round(ms32, 4),
+                      "launch_shapes": {"bands_per_wg": tune32["bands_per_wg"],
+                                        "items_per_launch": tune32["items_per_launch"]},
                       "kernels": kernels32},
         "scf": scf_info, "cpu_baseline": cpu,
     }

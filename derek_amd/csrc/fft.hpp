@@ -7,7 +7,7 @@
 // (j/Ns)*Ns*R + k (Stockham autosort: natural order in, natural order out).
 //
 // Two LDS layouts:
-//   CONTIG  element (line, idx) at line*N + idx     - lines contiguous in
+//   CONTIG  element (line, idx) at line*(N+1) + idx - lines contiguous in
 //           global memory (the z axis of the flat z-fastest grid);
 //   TILE    element (line, idx) at idx*LINES + line - LINES neighbouring
 //           z columns of a strided axis (y or x), so global loads are rows
@@ -256,9 +256,13 @@ template <int R, int S, typename C> __device__ __forceinline__ void dftR(C* v, c
 }
 
 // ---------------------------------------------------------------- one stage
+// CONTIG lines are padded by one element so that the lines a stage touches
+// at once start in different LDS banks.
+__host__ __device__ constexpr int contig_stride(int N) { return N + 1; }
+
 template <bool TILE, int LINES>
 __device__ __forceinline__ int lds_addr(int line, int idx, int N) {
-  return TILE ? idx * LINES + line : line * N + idx;
+  return TILE ? idx * LINES + line : line * contig_stride(N) + idx;
 }
 
 template <int R, int S, bool TILE, int LINES, int NT, typename C>

@@ -17,20 +17,27 @@ __global__ __launch_bounds__(NT) void k_fft_contig(cx_t<Rl>* __restrict__ data, 
   using C = cx_t<Rl>;
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = NC > 0 ? NC : plan.n;
+  const int ls = contig_stride(N);
   C* twl = reinterpret_cast<C*>(smem);
   C* b0 = twl + N;
-  C* b1 = b0 + (size_t)lpb * N;
+  C* b1 = b0 + (size_t)lpb * ls;
   for (int t = threadIdx.x; t < N; t += NT) twl[t] = tw[t];
   const int64_t line0 = (int64_t)blockIdx.x * lpb;
   const int64_t left = nlines - line0;
   const int lines = left < lpb ? (int)left : lpb;
   const C* src = data + line0 * N;
   const int ne = lines * N;
-  for (int e = threadIdx.x; e < ne; e += NT) b0[e] = src[e];
+  for (int e = threadIdx.x; e < ne; e += NT) {
+    const int line = e / N;
+    b0[line * ls + (e - line * N)] = src[e];
+  }
   __syncthreads();
   C* res = fft_lds<S, false, 1, NT, NC>(b0, b1, twl, plan, lines);
   C* dst = data + line0 * N;
-  for (int e = threadIdx.x; e < ne; e += NT) dst[e] = res[e];
+  for (int e = threadIdx.x; e < ne; e += NT) {
+    const int line = e / N;
+    dst[e] = res[line * ls + (e - line * N)];
+  }
 }
 
 // Strided axis, tiles of TILE_LINES neighbouring inner indices.
@@ -85,9 +92,9 @@ static void check_lds(size_t bytes, int n) {
 template <typename Rl, int S, int NC>
 static void launch_contig(cx_t<Rl>* data, int64_t nlines, const Axis<Rl>& ax, hipStream_t s) {
   const int N = ax.n;
-  int lpb = 2048 / N;
+  int lpb = 1024 / N;
   if (lpb < 1) lpb = 1;
-  const size_t lds = sizeof(cx_t<Rl>) * ((size_t)N + 2 * (size_t)lpb * N);
+  const size_t lds = sizeof(cx_t<Rl>) * ((size_t)N + 2 * (size_t)lpb * contig_stride(N));
   check_lds(lds, N);
   auto kern = k_fft_contig<Rl, S, NC>;
   allow_lds(kern, lds);

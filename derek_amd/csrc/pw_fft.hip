@@ -30,26 +30,30 @@ __global__ __launch_bounds__(PNT) void k_sticks_z_inv(PwBatch B, PwBlock<R> psi,
   const int s0 = blockIdx.x * lpb;
   if (s0 >= m.nst) return;
   const int nz = NC > 0 ? NC : plan.n;
+  const int ls = contig_stride(nz);
   C* twl = reinterpret_cast<C*>(smem);
   C* b0 = twl + nz;
-  C* b1 = b0 + (size_t)lpb * nz;
+  C* b1 = b0 + (size_t)lpb * ls;
   for (int t = threadIdx.x; t < nz; t += PNT) twl[t] = tw[t];
   const int band = blockIdx.y;
   const int lines = min(lpb, m.nst - s0);
   const int ne = lines * nz;
-  for (int e = threadIdx.x; e < ne; e += PNT) b0[e] = cmk<C>(0, 0);
+  for (int e = threadIdx.x; e < lines * ls; e += PNT) b0[e] = cmk<C>(0, 0);
   __syncthreads();
   const int p0 = m.st_pw[s0], p1 = m.st_pw[s0 + lines];
   const C* pb = psi.p + blockIdx.z * psi.ks + (int64_t)band * psi.ld;
   for (int p = p0 + threadIdx.x; p < p1; p += PNT) {
     const int line = m.pw_st[p] - s0;
-    b0[line * nz + m.pw_z[p]] = pb[m.perm[p]];
+    b0[line * ls + m.pw_z[p]] = pb[m.perm[p]];
   }
   __syncthreads();
   C* res = fft_lds<+1, false, 1, PNT, NC>(b0, b1, twl, plan, lines);
   const int64_t item = (int64_t)blockIdx.z * B.nbb + band;
   C* dst = S + (item * B.nst_cap + s0) * nz;
-  for (int e = threadIdx.x; e < ne; e += PNT) dst[e] = res[e];
+  for (int e = threadIdx.x; e < ne; e += PNT) {
+    const int line = e / nz;
+    dst[e] = res[line * ls + (e - line * nz)];
+  }
 }
 
 // ---------------------------------------------------------------- z forward on sticks + gather
@@ -64,16 +68,20 @@ __global__ __launch_bounds__(PNT) void k_sticks_z_fwd(PwBatch B, const cx_t<R>* 
   const int s0 = blockIdx.x * lpb;
   if (s0 >= m.nst) return;
   const int nz = NC > 0 ? NC : plan.n;
+  const int ls = contig_stride(nz);
   C* twl = reinterpret_cast<C*>(smem);
   C* b0 = twl + nz;
-  C* b1 = b0 + (size_t)lpb * nz;
+  C* b1 = b0 + (size_t)lpb * ls;
   for (int t = threadIdx.x; t < nz; t += PNT) twl[t] = tw[t];
   const int band = blockIdx.y;
   const int lines = min(lpb, m.nst - s0);
   const int ne = lines * nz;
   const int64_t item = (int64_t)blockIdx.z * B.nbb + band;
   const C* src = S + (item * B.nst_cap + s0) * nz;
-  for (int e = threadIdx.x; e < ne; e += PNT) b0[e] = src[e];
+  for (int e = threadIdx.x; e < ne; e += PNT) {
+    const int line = e / nz;
+    b0[line * ls + (e - line * nz)] = src[e];
+  }
   __syncthreads();
   C* res = fft_lds<-1, false, 1, PNT, NC>(b0, b1, twl, plan, lines);
   const int p0 = m.st_pw[s0], p1 = m.st_pw[s0 + lines];
@@ -82,7 +90,7 @@ __global__ __launch_bounds__(PNT) void k_sticks_z_fwd(PwBatch B, const cx_t<R>* 
   for (int p = p0 + threadIdx.x; p < p1; p += PNT) {
     const int line = m.pw_st[p] - s0;
     const int g = m.perm[p];
-    const C v = res[line * nz + m.pw_z[p]];
+    const C v = res[line * ls + m.pw_z[p]];
     const C x = pb[g];
     const R t = (R)m.kin[g];
     hb[g] = cmk<C>(v.x + t * x.x, v.y + t * x.y);
@@ -300,8 +308,13 @@ __device__ __forceinline__ void load_twiddles(C* twl, const C* __restrict__ tw) 
 // x pass: inverse x transform of the active rows, * V * vscale, forward
 // transform, active rows written back; BPW bands of one k-point per block
 // (V is loaded once per block).
+// Waves per SIMD the x pass is compiled for: 3 (for complex128 a few
+// spilled registers, measured faster than 2 waves without; for complex64 4
+// waves measured slower).
+template <typename R> constexpr int kXWaves = 3;
+
 template <typename R, int N, int BPW>
-__global__ __launch_bounds__(Split<N>::NT) void k_px_vmul2(PwBatch B, cx_t<R>* __restrict__ P, int64_t nyz,
+__global__ __launch_bounds__(Split<N>::NT) __attribute__((amdgpu_waves_per_eu(kXWaves<R>))) void k_px_vmul2(PwBatch B, cx_t<R>* __restrict__ P, int64_t nyz,
                                                           const cx_t<R>* __restrict__ tw,
                                                           const R* __restrict__ vloc, R vscale) {
   using C = cx_t<R>;
@@ -328,14 +341,17 @@ __global__ __launch_bounds__(Split<N>::NT) void k_px_vmul2(PwBatch B, cx_t<R>* _
   }
   const int bb0 = blockIdx.y * BPW;
   const int bb1 = min(B.nbb, bb0 + BPW);
-  // S1 input rows (active row index per r, -1 if inactive)
-  int arow[R1];
-#pragma unroll
-  for (int r = 0; r < R1; ++r) arow[r] = (jj < R2 && lok) ? inv[jj + R2 * r] : -1;
+  // S1 input rows are looked up in LDS per band (no index registers held
+  // across the band loop)
   auto load_s1 = [&](int band, C* v) {
     const C* base = P + ((int64_t)blockIdx.z * B.nbb + band) * B.nax_cap * nyz + c0 + l;
+    int j = jj;
+    asm volatile("" : "+v"(j));
 #pragma unroll
-    for (int r = 0; r < R1; ++r) v[r] = arow[r] >= 0 ? base[(int64_t)arow[r] * nyz] : cmk<C>(0, 0);
+    for (int r = 0; r < R1; ++r) {
+      const int a = (j < R2 && lok) ? inv[j + R2 * r] : -1;
+      v[r] = a >= 0 ? base[(int64_t)a * nyz] : cmk<C>(0, 0);
+    }
   };
   C pre[kPrefetch<R> ? R1 : 1];
   if (kPrefetch<R> && bb0 < bb1) load_s1(bb0, pre);
@@ -619,7 +635,7 @@ static void check_lds_pw(size_t bytes, int n) {
   DEREK_REQUIRE(bytes <= 160 * 1024, "FFT length " + std::to_string(n) + " exceeds the LDS budget");
 }
 
-static int sticks_per_block(int nz) { return std::max(1, 2048 / nz); }
+static int sticks_per_block(int nz) { return std::max(1, 1024 / nz); }
 
 static void check_batch(const PwBatch& B) {
   DEREK_REQUIRE(B.mt && B.nkc >= 1 && B.nbb >= 1 && B.nkc <= 65535 && B.nbb <= 65535, "empty pipeline batch");
@@ -638,7 +654,7 @@ template <typename R>
 void sticks_z_inv(const PwBatch& B, PwBlock<R> psi, const Axis<R>* ax, int nz, cx_t<R>* S, hipStream_t s) {
   check_batch(B);
   const int lpb = sticks_per_block(nz);
-  const size_t lds = sizeof(cx_t<R>) * ((size_t)nz + 2 * (size_t)lpb * nz);
+  const size_t lds = sizeof(cx_t<R>) * ((size_t)nz + 2 * (size_t)lpb * contig_stride(nz));
   check_lds_pw(lds, nz);
   dim3 grid((unsigned)ceil_div(B.nst_cap, lpb), B.nbb, B.nkc);
 #define C_(NC)                                                                                  \
@@ -713,7 +729,7 @@ void sticks_z_fwd(const PwBatch& B, const cx_t<R>* S, const Axis<R>* ax, int nz,
                   hipStream_t s) {
   check_batch(B);
   const int lpb = sticks_per_block(nz);
-  const size_t lds = sizeof(cx_t<R>) * ((size_t)nz + 2 * (size_t)lpb * nz);
+  const size_t lds = sizeof(cx_t<R>) * ((size_t)nz + 2 * (size_t)lpb * contig_stride(nz));
   check_lds_pw(lds, nz);
   dim3 grid((unsigned)ceil_div(B.nst_cap, lpb), B.nbb, B.nkc);
 #define C_(NC)                                                                                       \
