@@ -140,20 +140,26 @@ __device__ void tri_solve(const double* d, const double* e, int n, double lambda
 struct HeevWork {
   double2* sub;  // [n]  T(k+1,k) before the phase change
   double2* ph;   // [n]  phases
-  int* skip;     // [n]  1 if column k needed no reflector
+  double2* vb;   // [2n] reflector vectors of the wide reduction, double-buffered by k&1
+  double2* uu;   // [n]  A v of the wide reduction
   double* d;     // [n]  diagonal
   double* e;     // [n]  |sub|
   double* lam;   // [nev]
+  double* scal;  // [8]  wide reduction: (skip, 1/|v|, v0) per buffer
+  double* xt;    // [n * nev] tridiagonal eigenvectors
+  int* skip;     // [n]  1 if column k needed no reflector
   int* cstart;   // [nev + 1]
   int* ncl;      // [1]
-  double* xt;    // [n * nev] tridiagonal eigenvectors
   __device__ __host__ HeevWork(double* w, int n, int nev) {
     sub = reinterpret_cast<double2*>(w);
     ph = sub + n;
-    d = reinterpret_cast<double*>(ph + n);
+    vb = ph + n;
+    uu = vb + 2 * n;
+    d = reinterpret_cast<double*>(uu + n);
     e = d + n;
     lam = e + n;
-    xt = lam + nev;
+    scal = lam + nev;
+    xt = scal + 8;
     skip = reinterpret_cast<int*>(xt + (size_t)n * nev);
     cstart = skip + n;
     ncl = cstart + nev + 1;
@@ -161,7 +167,9 @@ struct HeevWork {
 };
 
 int64_t heev_work_doubles(int n, int nev) {
-  return 4 * (int64_t)n + 2 * (int64_t)n + nev + (int64_t)n * nev + (n + nev + 2 + 1) / 2 + 8;
+  // rounded to 256 bytes so every matrix of a batch starts aligned
+  const int64_t w = 12 * (int64_t)n + nev + 8 + (int64_t)n * nev + (n + nev + 2 + 1) / 2 + 8;
+  return (w + 31) / 32 * 32;
 }
 
 // ---- 1. Householder tridiagonalisation + phases (one workgroup per matrix)
@@ -313,6 +321,174 @@ __global__ __launch_bounds__(DT) void k_tridiag(double2* __restrict__ Aall, int 
     }
     W.e[n - 1] = 0.0;
   }
+}
+
+// ---- 1'. wide tridiagonalisation (matrices that do not fit LDS): two
+// launches per column, the trailing matrix spread over many workgroups (one
+// wavefront per column).  Launch k computes u = A22 v; launch k' applies the
+// rank-2 update, and the wavefront that updates column k+1 forms the next
+// reflector from the values it just wrote.  Reflector k is kept in vb
+// (buffer k&1, scaled to unit norm) with its scalars in scal; A's column k
+// receives it for the back-transformation.
+constexpr int WT = 256;       // threads per workgroup
+constexpr int WC = WT / 64;   // columns per workgroup
+
+// Reflector of column k from sum |A(i,k)|^2 (i > k) and x0 = A(k+1,k), both
+// wave-uniform; every lane derives the scalars, lane 0 records them.
+__device__ __forceinline__ void wide_reflector_scalars(int lane, int k, double xn2, double2 x0, HeevWork& W,
+                                                       int& skip, double& iv, double2& v0) {
+  const double ax0 = sqrt(x0.x * x0.x + x0.y * x0.y);
+  const double rest = xn2 - ax0 * ax0;
+  skip = (xn2 == 0.0 || rest <= 1e-300) ? 1 : 0;
+  iv = 0.0;
+  v0 = make_double2(0.0, 0.0);
+  double2 alpha = x0;
+  if (!skip) {
+    const double xn = sqrt(xn2);
+    const double2 phs = ax0 > 0.0 ? make_double2(x0.x / ax0, x0.y / ax0) : make_double2(1.0, 0.0);
+    alpha = make_double2(-phs.x * xn, -phs.y * xn);
+    v0 = make_double2(x0.x - alpha.x, x0.y - alpha.y);
+    iv = 1.0 / sqrt(rest + v0.x * v0.x + v0.y * v0.y);
+  }
+  if (lane == 0) {
+    W.skip[k] = skip;
+    W.sub[k] = alpha;
+    double* sc = W.scal + 4 * (k & 1);
+    sc[0] = skip;
+    sc[1] = iv;
+    sc[2] = v0.x;
+    sc[3] = v0.y;
+  }
+}
+
+// column k's reflector from its current values (one wavefront)
+__device__ __forceinline__ void wide_reflector(const double2* __restrict__ A, int lda, int k, int n, HeevWork& W,
+                                               int lane) {
+  const double2* col = A + (int64_t)k * lda;
+  double part = 0.0;
+  for (int i = k + 1 + lane; i < n; i += 64) part += col[i].x * col[i].x + col[i].y * col[i].y;
+  part = wave_sum(part);
+  const double2 x0 = col[k + 1];
+  int skip;
+  double iv;
+  double2 v0;
+  wide_reflector_scalars(lane, k, part, x0, W, skip, iv, v0);
+  double2* v = W.vb + (size_t)(k & 1) * n;
+  for (int i = k + 1 + lane; i < n; i += 64) {
+    const double2 a = i == k + 1 ? v0 : col[i];
+    v[i] = make_double2(a.x * iv, a.y * iv);
+  }
+}
+
+__global__ __launch_bounds__(64) void k_wide_first(double2* __restrict__ Aall, int lda, int64_t strideA, int n,
+                                                   int nev, double* __restrict__ work, int64_t strideWork) {
+  const int mat = blockIdx.x;
+  HeevWork W(work + mat * strideWork, n, nev);
+  wide_reflector(Aall + mat * strideA, lda, 0, n, W, threadIdx.x);
+}
+
+// u_i = sum_j conj(A(j,i)) v_j for the trailing columns i > k; the first
+// workgroup also stores the reflector into column k.
+__global__ __launch_bounds__(WT) void k_wide_gemv(double2* __restrict__ Aall, int lda, int64_t strideA, int n,
+                                                  int nev, int k, double* __restrict__ work, int64_t strideWork) {
+  const int mat = blockIdx.y;
+  HeevWork W(work + mat * strideWork, n, nev);
+  if (W.scal[4 * (k & 1)] != 0.0) return;  // no reflector for this column
+  double2* A = Aall + mat * strideA;
+  const double2* v = W.vb + (size_t)(k & 1) * n;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (blockIdx.x == 0 && wid == 0)
+    for (int i = k + 1 + lane; i < n; i += 64) A[i + (int64_t)k * lda] = v[i];
+  const int i = k + 1 + blockIdx.x * WC + wid;
+  if (i >= n) return;
+  const double2* col = A + (int64_t)i * lda;
+  double sr = 0.0, si = 0.0;
+  for (int j = k + 1 + lane; j < n; j += 64) {
+    const double2 t = zconjmul(col[j], v[j]);
+    sr += t.x;
+    si += t.y;
+  }
+  sr = wave_sum(sr);
+  si = wave_sum(si);
+  if (lane == 0) W.uu[i] = make_double2(sr, si);
+}
+
+// A22 -= v w^H + w v^H with w = 2u - 2 (v^H u) v; the wavefront of column
+// k+1 then forms reflector k+1 from the updated column.
+__global__ __launch_bounds__(WT) void k_wide_update(double2* __restrict__ Aall, int lda, int64_t strideA, int n,
+                                                    int nev, int k, double* __restrict__ work, int64_t strideWork) {
+  const int mat = blockIdx.y;
+  HeevWork W(work + mat * strideWork, n, nev);
+  double2* A = Aall + mat * strideA;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int j = k + 1 + blockIdx.x * WC + wid;
+  const bool next = (j == k + 1) && (k + 3 < n);  // this wavefront forms reflector k+1
+  if (W.scal[4 * (k & 1)] != 0.0) {               // column k needed no reflector
+    if (next) wide_reflector(A, lda, k + 1, n, W, lane);
+    return;
+  }
+  if (j >= n) return;
+  const double2* v = W.vb + (size_t)(k & 1) * n;
+  const double2* u = W.uu;
+  double gp = 0.0;
+  for (int i = k + 1 + lane; i < n; i += 64) gp += zconjmul(v[i], u[i]).x;
+  const double g2 = 2.0 * wave_sum(gp);
+  const double2 vj = v[j], uj = u[j];
+  const double2 wj = make_double2(2.0 * uj.x - g2 * vj.x, 2.0 * uj.y - g2 * vj.y);
+  double2* col = A + (int64_t)j * lda;
+  double part = 0.0;
+  double2 x0 = make_double2(0.0, 0.0);
+  for (int i = k + 1 + lane; i < n; i += 64) {
+    const double2 vi = v[i], ui = u[i];
+    const double2 wi = make_double2(2.0 * ui.x - g2 * vi.x, 2.0 * ui.y - g2 * vi.y);
+    const double2 t1 = zmul(vi, make_double2(wj.x, -wj.y));
+    const double2 t2 = zmul(wi, make_double2(vj.x, -vj.y));
+    const double2 a = col[i];
+    const double2 r = make_double2(a.x - t1.x - t2.x, a.y - t1.y - t2.y);
+    col[i] = r;
+    if (i > j) part += r.x * r.x + r.y * r.y;
+    if (i == j + 1) x0 = r;
+  }
+  if (next) {
+    part = wave_sum(part);
+    x0 = make_double2(wave_sum(x0.x), wave_sum(x0.y));  // one lane holds it
+    int skip;
+    double iv;
+    double2 v0;
+    wide_reflector_scalars(lane, k + 1, part, x0, W, skip, iv, v0);
+    double2* vn = W.vb + (size_t)((k + 1) & 1) * n;
+    // same row-to-lane mapping as the update loop: each lane reads back
+    // only the rows it wrote
+    for (int i = k + 1 + lane; i < n; i += 64) {
+      if (i <= j) continue;
+      const double2 a = i == j + 1 ? v0 : col[i];
+      vn[i] = make_double2(a.x * iv, a.y * iv);
+    }
+  }
+}
+
+// tail of the reduction: last sub-diagonal, diagonal, phases
+__global__ __launch_bounds__(64) void k_wide_last(const double2* __restrict__ Aall, int lda, int64_t strideA, int n,
+                                                  int nev, double* __restrict__ work, int64_t strideWork) {
+  const int mat = blockIdx.x;
+  const double2* A = Aall + mat * strideA;
+  HeevWork W(work + mat * strideWork, n, nev);
+  for (int i = threadIdx.x; i < n; i += 64) W.d[i] = A[i + (int64_t)i * lda].x;
+  if (threadIdx.x != 0) return;
+  if (n >= 2) {
+    W.sub[n - 2] = A[(n - 1) + (int64_t)(n - 2) * lda];
+    W.skip[n - 2] = 1;
+  }
+  W.skip[n - 1] = 1;
+  double2 p = make_double2(1.0, 0.0);
+  W.ph[0] = p;
+  for (int k = 0; k + 1 < n; ++k) {
+    const double a = sqrt(W.sub[k].x * W.sub[k].x + W.sub[k].y * W.sub[k].y);
+    W.e[k] = a;
+    if (a > 0.0) p = zmul(p, make_double2(W.sub[k].x / a, W.sub[k].y / a));
+    W.ph[k + 1] = p;
+  }
+  W.e[n - 1] = 0.0;
 }
 
 // ---- 2. eigenvalues by multisection: one wavefront per eigenvalue; the 64
@@ -499,8 +675,14 @@ void heev_gpu(double2* A, int lda, int64_t strideA, int n, int nev, double* Wv, 
       allow_dynamic_lds(reinterpret_cast<const void*>(k_tridiag<true>), kTridiagLds);
       hipLaunchKernelGGL(k_tridiag<true>, dim3(batch), dim3(DT), lds_l, s, A, lda, strideA, n, nev, work, strideWork);
     } else {
-      hipLaunchKernelGGL(k_tridiag<false>, dim3(batch), dim3(DT), lds_g, s, A, lda, strideA, n, nev, work,
-                         strideWork);
+      // two launches per column over all of the trailing matrix
+      hipLaunchKernelGGL(k_wide_first, dim3(batch), dim3(64), 0, s, A, lda, strideA, n, nev, work, strideWork);
+      for (int k = 0; k + 2 < n; ++k) {
+        dim3 grid((unsigned)ceil_div(n - k - 1, WC), batch);
+        hipLaunchKernelGGL(k_wide_gemv, grid, dim3(WT), 0, s, A, lda, strideA, n, nev, k, work, strideWork);
+        hipLaunchKernelGGL(k_wide_update, grid, dim3(WT), 0, s, A, lda, strideA, n, nev, k, work, strideWork);
+      }
+      hipLaunchKernelGGL(k_wide_last, dim3(batch), dim3(64), 0, s, A, lda, strideA, n, nev, work, strideWork);
     }
     DEREK_HIP(hipGetLastError());
   }

@@ -19,7 +19,8 @@ def ctx():
     c.close()
 
 
-@pytest.mark.parametrize("n,nev", [(1, 1), (2, 2), (3, 2), (17, 17), (80, 80), (160, 80), (300, 150)])
+@pytest.mark.parametrize("n,nev", [(1, 1), (2, 2), (3, 2), (17, 17), (80, 80), (98, 98), (99, 99), (160, 80), (300, 150),
+                                   (600, 300)])
 def test_heev_random(ctx, n, nev):
     print("heev", n, nev, flush=True)
     rng = np.random.default_rng(n)
@@ -48,6 +49,31 @@ def test_heev_degenerate_and_diagonal(ctx):
     w, Z = ctx.heev(D, 5)
     assert np.allclose(w, np.sort(np.diag(D).real)[:5], atol=1e-14)
     assert np.max(np.abs(D @ Z - Z * w[None, :])) < 1e-12
+
+
+@pytest.mark.parametrize("n", [60, 150])
+def test_heev_columns_without_reflector(ctx, n):
+    """Columns already zero below the subdiagonal take no reflector, both in
+    the in-LDS reduction (n <= 98) and in the multi-workgroup one."""
+    print("heev skip", n, flush=True)
+    rng = np.random.default_rng(11 + n)
+    A = herm(rng, n)
+    # a complex tridiagonal block followed by a dense block: columns 0..n/2-2
+    # have nothing below their subdiagonal
+    h = n // 2
+    A[:h, :h] = 0
+    A[:h, h:] = 0
+    A[h:, :h] = 0
+    idx = np.arange(h)
+    A[idx, idx] = rng.normal(size=h)
+    off = rng.normal(size=h - 1) + 1j * rng.normal(size=h - 1)
+    A[idx[1:], idx[:-1]] = off
+    A[idx[:-1], idx[1:]] = off.conj()
+    w, Z = ctx.heev(A, n)
+    ref = np.linalg.eigvalsh(A)
+    assert np.max(np.abs(w - ref)) < 1e-12 * n * max(1.0, np.abs(ref).max())
+    assert np.max(np.abs(A @ Z - Z * w[None, :])) < 1e-11 * np.sqrt(n) * max(1.0, np.abs(ref).max())
+    assert np.max(np.abs(Z.conj().T @ Z - np.eye(n))) < 1e-11 * np.sqrt(n)
 
 
 @pytest.mark.parametrize("m,nev", [(20, 10), (160, 80)])
