@@ -308,9 +308,8 @@ __device__ __forceinline__ void load_twiddles(C* twl, const C* __restrict__ tw) 
 // x pass: inverse x transform of the active rows, * V * vscale, forward
 // transform, active rows written back; BPW bands of one k-point per block
 // (V is loaded once per block).
-// Waves per SIMD the x pass is compiled for: 3 (for complex128 a few
-// spilled registers, measured faster than 2 waves without; for complex64 4
-// waves measured slower).
+// Waves per SIMD the x pass is compiled for: 3 (complex64 at 4 waves
+// measured slower).
 template <typename R> constexpr int kXWaves = 3;
 
 template <typename R, int N, int BPW>
@@ -333,9 +332,11 @@ __global__ __launch_bounds__(Split<N>::NT) __attribute__((amdgpu_waves_per_eu(kX
   __syncthreads();
   for (int a = tid; a < m.nax; a += NT) inv[m.ax_x[a]] = a;
   __syncthreads();
-  // V at x = j2 + R1 r, loaded once (the stage twiddles w^(j2 r) are read from LDS)
-  R vr[R2];
-  if (jj < R1) {
+  // V at x = j2 + R1 r: held in registers across the band loop, or, where
+  // that would spill (complex128 at N = 144), re-read per band (an L2 hit)
+  constexpr bool kVregs = !(sizeof(R) == 8 && N >= 144);
+  R vr[kVregs ? R2 : 1];
+  if (kVregs && jj < R1) {
 #pragma unroll
     for (int r = 0; r < R2; ++r) vr[r] = lok ? vloc[(int64_t)(jj + R1 * r) * nyz + c0 + l] * vscale : (R)0;
   }
@@ -371,18 +372,23 @@ __global__ __launch_bounds__(Split<N>::NT) __attribute__((amdgpu_waves_per_eu(kX
     }
     __syncthreads();
     C u[R2];
-    if (jj < R1) {
+    // an opaque copy of jj: the twiddle, V and row-index loads below are
+    // redone per band instead of being hoisted into registers held across
+    // the band loop
+    int jx = jj;
+    asm volatile("" : "+v"(jx));
+    if (jx < R1) {
 #pragma unroll
       for (int r = 0; r < R2; ++r) {
-        const C y = buf[(jj + R1 * r) * PL + l];
-        u[r] = r ? cmul(y, cconj(twl[(jj * r) % N])) : y;
+        const C y = buf[(jx + R1 * r) * PL + l];
+        u[r] = r ? cmul(y, cconj(twl[(jx * r) % N])) : y;
       }
       dftR<R2, +1>(u, twl, N);
 #pragma unroll
-      for (int r = 0; r < R2; ++r) u[r] = cscale(u[r], vr[r]);
+      for (int r = 0; r < R2; ++r) u[r] = cscale(u[r], kVregs ? vr[kVregs ? r : 0] : (lok ? vloc[(int64_t)(jx + R1 * r) * nyz + c0 + l] * vscale : (R)0));
       dftR<R2, -1>(u, twl, N);
 #pragma unroll
-      for (int k2 = 1; k2 < R2; ++k2) u[k2] = cmul(u[k2], twl[(jj * k2) % N]);
+      for (int k2 = 1; k2 < R2; ++k2) u[k2] = cmul(u[k2], twl[(jx * k2) % N]);
     }
     __syncthreads();
     if (jj < R1) {
@@ -398,7 +404,7 @@ __global__ __launch_bounds__(Split<N>::NT) __attribute__((amdgpu_waves_per_eu(kX
       C* base = P + ((int64_t)blockIdx.z * B.nbb + band) * B.nax_cap * nyz + c0 + l;
 #pragma unroll
       for (int k1 = 0; k1 < R1; ++k1) {
-        const int a = lok ? inv[jj + R2 * k1] : -1;
+        const int a = lok ? inv[jx + R2 * k1] : -1;
         if (a >= 0) base[(int64_t)a * nyz] = t[k1];
       }
     }
