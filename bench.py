@@ -55,18 +55,22 @@ def copy_calibration(mib):
 
 
 def hpsi_rate(ctx, nb, steps, warmup):
-    """bands/s of H|psi> on the resident bands, with the per-class profile."""
+    """bands/s of H|psi> on the resident bands (timed without events), then
+    the same steps again with a HIP event pair around every launch for the
+    per-kernel-class durations."""
     for _ in range(warmup):
         ctx.apply_h_resident(0, nb)
-    ctx.sync()
-    ctx.profile(True)
-    ctx.profile_reset()
     ctx.sync()
     t1 = time.perf_counter()
     for _ in range(steps):
         ctx.apply_h_resident(0, nb)
     ctx.sync()
     dt = time.perf_counter() - t1
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(steps):
+        ctx.apply_h_resident(0, nb)
+    ctx.sync()
     prof = ctx.profile_read()
     ctx.profile(False)
     return steps * nb / dt, dt / steps * 1e3, prof
@@ -145,10 +149,16 @@ def main():
                    "share": v[1] / total} for k, v in busy.items()}
     busy32 = {k: v for k, v in prof32.items() if v[0] > 0}
     kernels32 = {k: {"avg_us": v[1] / v[0] * 1e3, "GB_s": v[2] / (v[1] * 1e-3) / 1e9} for k, v in busy32.items()}
+    # HBM bytes per launch from the committed counter summary: FETCH_SIZE and
+    # WRITE_SIZE per (band) item at each bands-per-workgroup setting, times
+    # the items of one launch
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_cfg2.json")
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_108cubed.json")
     if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get("bytes_per_launch")
+        per = json.load(open(tfile)).get("per_item", {}).get(str(tune64["bands_per_wg"]))
+        if per:
+            items = min(tune64["items_per_launch"], nb)
+            traffic = round(items * (per["fetch"] + per["write"]))
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                 "bytes_per_launch": by / n_l, "avg_launch_us": avg_s * 1e6}

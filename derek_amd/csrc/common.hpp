@@ -48,6 +48,15 @@ __device__ __forceinline__ C cmk(decltype(C::x) a, decltype(C::x) b) {
   r.y = b;
   return r;
 }
+// Element at a 32-bit byte offset from a wave-uniform base pointer: the
+// access compiles to a scalar base plus a 32-bit vector offset, with no
+// 64-bit address arithmetic per lane.
+template <typename T> __device__ __forceinline__ T ld_at(const T* base, uint32_t boff) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
+}
+template <typename T> __device__ __forceinline__ void st_at(T* base, uint32_t boff, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff) = v;
+}
 template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return cmk<C>(a.x + b.x, a.y + b.y); }
 template <typename C> __device__ __forceinline__ C csub(C a, C b) { return cmk<C>(a.x - b.x, a.y - b.y); }
 template <typename C> __device__ __forceinline__ C cmul(C a, C b) {

@@ -3,12 +3,14 @@
 // Overlap  C = X^H Y  (X: npw x m, Y: npw x n, npw >> m, n): split-K over
 // the plane-wave index; each workgroup owns a 64x64 tile of C and a chunk of
 // rows of one k-point (blockIdx.z), stages 16-row slabs of X and Y through
-// LDS and accumulates the four real products of the complex product with
-// 16x16x4 MFMAs (f64, or f32 for complex64 data); a second kernel sums the
-// chunk partials in a fixed order in FP64 (deterministic).
+// LDS and accumulates three real products per complex product (Gauss's
+// method: 25% fewer matrix-core operations than the four-product form) with
+// 16x16x4 MFMAs (f64, or f32 for complex64 data); the next slab is loaded
+// into registers while the current one is multiplied.  A second kernel sums
+// the chunk partials in a fixed order in FP64 (deterministic).
 // Rotation Z = X U: each workgroup owns 64 rows x 64 columns of Z; U slabs
 // (FP64, rounded to R) are staged through LDS, X is read straight into the
-// MFMA A operand.
+// MFMA A operand; three real products per complex product as well.
 //
 // 16x16x4 MFMA operand layout (lane l of a 64-wide wave):
 //   A (16x4): A[l%16][l/16]      B (4x16): B[l/16][l%16]
@@ -47,7 +49,11 @@ struct OvGeom {
   bool herm;
 };
 
-static OvGeom ov_geom(int m, int n, int64_t npw, bool herm, int batch) {
+// Row chunks per tile: at least two full rounds of workgroups over the
+// chip's resident slots (`slots` = resident workgroups per CU x CUs), then,
+// among the next few counts, the one that leaves the smallest idle tail in
+// the last round; every chunk keeps at least 256 rows.
+static OvGeom ov_geom(int m, int n, int64_t npw, bool herm, int batch, int slots) {
   OvGeom g{};
   g.m = m;
   g.n = n;
@@ -56,29 +62,40 @@ static OvGeom ov_geom(int m, int n, int64_t npw, bool herm, int batch) {
   g.tm = (int)ceil_div(m, OV_T);
   g.tn = (int)ceil_div(n, OV_T);
   g.ntiles = herm ? g.tm * (g.tm + 1) / 2 : g.tm * g.tn;
-  // about 2048 workgroups per launch
-  int64_t target = std::max<int64_t>(1, 2048 / ((int64_t)g.ntiles * std::max(1, batch)));
-  int64_t kch = ceil_div(std::max<int64_t>(npw, 1), target);
-  kch = std::max<int64_t>(256, ceil_div(kch, OV_KS) * OV_KS);
+  const int64_t units = (int64_t)g.ntiles * std::max(1, batch);
+  const int64_t cmax = std::max<int64_t>(1, npw / 256);
+  const int64_t c0 = std::min(cmax, std::max<int64_t>(1, ceil_div(2 * (int64_t)slots, units)));
+  int64_t best = c0;
+  double best_eff = -1.0;
+  for (int64_t c = c0; c <= std::min(cmax, c0 + 16); ++c) {
+    const int64_t w = units * c;
+    const double eff = (double)w / (double)(ceil_div(w, (int64_t)slots) * slots);
+    if (eff > best_eff + 1e-9) {
+      best_eff = eff;
+      best = c;
+    }
+  }
+  int64_t kch = ceil_div(std::max<int64_t>(npw, 1), best);
+  kch = std::max<int64_t>(OV_KS, ceil_div(kch, OV_KS) * OV_KS);
   g.kch = kch;
   g.nchunk = (int)std::max<int64_t>(1, ceil_div(npw, kch));
   return g;
 }
 
-int64_t overlap_work_elems(int m, int n, int64_t npw, int batch) {
-  OvGeom g = ov_geom(m, n, npw, false, batch);
-  OvGeom h = ov_geom(m, n, npw, m == n, batch);
-  return (int64_t)batch * std::max<int64_t>((int64_t)g.nchunk * m * n, (int64_t)h.nchunk * m * n);
-}
-
+// at least two waves per SIMD (register budget 256 per lane incl. accumulators)
 template <typename R>
-__global__ __launch_bounds__(256) void k_overlap(const cx_t<R>* __restrict__ X, int64_t ldx, int64_t sx,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_overlap(const cx_t<R>* __restrict__ X, int64_t ldx, int64_t sx,
                                                  const cx_t<R>* __restrict__ Y, int64_t ldy, int64_t sy,
                                                  const int* __restrict__ npwk, OvGeom g,
                                                  double2* __restrict__ part) {
   using M = Mfma<R>;
   using A4 = typename M::acc;
-  __shared__ R xr[OV_KS][OV_PAD], xi[OV_KS][OV_PAD], yr[OV_KS][OV_PAD], yi[OV_KS][OV_PAD];
+  using C = cx_t<R>;
+  // slab operands for the three real products of the complex product
+  // conj(x) y (Gauss):  P1 = xr yr,  P2 = xi yi,  P3 = (xr - xi)(yr + yi);
+  // Re = P1 + P2,  Im = P3 - P1 + P2
+  __shared__ R xr[OV_KS][OV_PAD], xi[OV_KS][OV_PAD], xs[OV_KS][OV_PAD];
+  __shared__ R yr[OV_KS][OV_PAD], yi[OV_KS][OV_PAD], ys[OV_KS][OV_PAD];
   const int kz = blockIdx.z;
   X += kz * sx;
   Y += kz * sy;
@@ -99,52 +116,64 @@ __global__ __launch_bounds__(256) void k_overlap(const cx_t<R>* __restrict__ X, 
   const int64_t r1 = std::min<int64_t>(npw, r0 + g.kch);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wi = wave & 1, wj = wave >> 1;
-  A4 cr[2][2], ci[2][2];
+  A4 p1[2][2], p2[2][2], p3[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
-      cr[a][b] = A4{0, 0, 0, 0};
-      ci[a][b] = A4{0, 0, 0, 0};
+      p1[a][b] = A4{0, 0, 0, 0};
+      p2[a][b] = A4{0, 0, 0, 0};
+      p3[a][b] = A4{0, 0, 0, 0};
     }
   const int lrow = threadIdx.x & 15, lcol = threadIdx.x >> 4;  // 16 rows x 16 column groups
-  for (int64_t k0 = r0; k0 < r1; k0 += OV_KS) {
+  // the next slab is loaded into registers while the current one is multiplied
+  C px[4], py[4];
+  auto gload = [&](int64_t k0) {
     const int64_t gr = k0 + lrow;
     const bool rok = gr < r1;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int c = lcol + 16 * q;
-      cx_t<R> vx = cmk<cx_t<R>>(0, 0), vy = cmk<cx_t<R>>(0, 0);
-      if (rok && i0 + c < g.m) vx = X[(int64_t)(i0 + c) * ldx + gr];
-      if (rok && j0 + c < g.n) vy = Y[(int64_t)(j0 + c) * ldy + gr];
-      xr[lrow][c] = vx.x;
-      xi[lrow][c] = vx.y;
-      yr[lrow][c] = vy.x;
-      yi[lrow][c] = vy.y;
+      px[q] = (rok && i0 + c < g.m) ? X[(int64_t)(i0 + c) * ldx + gr] : cmk<C>(0, 0);
+      py[q] = (rok && j0 + c < g.n) ? Y[(int64_t)(j0 + c) * ldy + gr] : cmk<C>(0, 0);
+    }
+  };
+  if (r0 < r1) gload(r0);
+  for (int64_t k0 = r0; k0 < r1; k0 += OV_KS) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = lcol + 16 * q;
+      xr[lrow][c] = px[q].x;
+      xi[lrow][c] = px[q].y;
+      xs[lrow][c] = px[q].x - px[q].y;
+      yr[lrow][c] = py[q].x;
+      yi[lrow][c] = py[q].y;
+      ys[lrow][c] = py[q].x + py[q].y;
     }
     __syncthreads();
+    if (k0 + OV_KS < r1) gload(k0 + OV_KS);
 #pragma unroll
     for (int kk = 0; kk < OV_KS / 4; ++kk) {
       const int k = kk * 4 + (lane >> 4);
-      R ar[2], ai[2], br[2], bi[2];
+      R ar[2], ai[2], as[2], br[2], bi[2], bs[2];
 #pragma unroll
       for (int a = 0; a < 2; ++a) {
         const int ii = wi * 32 + a * 16 + (lane & 15);
         ar[a] = xr[k][ii];
         ai[a] = xi[k][ii];
+        as[a] = xs[k][ii];
         const int jj = wj * 32 + a * 16 + (lane & 15);
         br[a] = yr[k][jj];
         bi[a] = yi[k][jj];
+        bs[a] = ys[k][jj];
       }
 #pragma unroll
       for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-          // conj(x) * y = (xr yr + xi yi) + i (xr yi - xi yr)
-          cr[a][b] = M::op(ar[a], br[b], cr[a][b]);
-          cr[a][b] = M::op(ai[a], bi[b], cr[a][b]);
-          ci[a][b] = M::op(ar[a], bi[b], ci[a][b]);
-          ci[a][b] = M::op(-ai[a], br[b], ci[a][b]);
+          p1[a][b] = M::op(ar[a], br[b], p1[a][b]);
+          p2[a][b] = M::op(ai[a], bi[b], p2[a][b]);
+          p3[a][b] = M::op(as[a], bs[b], p3[a][b]);
         }
     }
     __syncthreads();
@@ -158,8 +187,30 @@ __global__ __launch_bounds__(256) void k_overlap(const cx_t<R>* __restrict__ X, 
       for (int r = 0; r < 4; ++r) {
         const int i = i0 + wi * 32 + a * 16 + M::row(lane, r);
         const int j = j0 + wj * 32 + b * 16 + (lane & 15);
-        if (i < g.m && j < g.n) P[(int64_t)j * g.m + i] = make_double2((double)cr[a][b][r], (double)ci[a][b][r]);
+        const double q1 = (double)p1[a][b][r], q2 = (double)p2[a][b][r], q3 = (double)p3[a][b][r];
+        if (i < g.m && j < g.n) P[(int64_t)j * g.m + i] = make_double2(q1 + q2, q3 - q1 + q2);
       }
+}
+
+// resident overlap workgroups on the current device (CUs x per-CU occupancy)
+template <typename R> static int ov_slots() {
+  static const int slots = [] {
+    int dev = 0;
+    DEREK_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    DEREK_HIP(hipGetDeviceProperties(&p, dev));
+    int per = 0;
+    DEREK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(k_overlap<R>), 256, 0));
+    return std::max(1, per) * std::max(1, p.multiProcessorCount);
+  }();
+  return slots;
+}
+
+int64_t overlap_work_elems(int m, int n, int64_t npw, int batch) {
+  int64_t chunks = 1;
+  for (int slots : {ov_slots<double>(), ov_slots<float>()})
+    for (bool h : {false, m == n}) chunks = std::max<int64_t>(chunks, ov_geom(m, n, npw, h, batch, slots).nchunk);
+  return (int64_t)batch * chunks * m * n;
 }
 
 // C[kz] = sum over chunks, columns spread over the workgroups of blockIdx.x
@@ -192,7 +243,7 @@ void overlap(const cx_t<R>* X, int64_t ldx, int64_t sx, const cx_t<R>* Y, int64_
              int64_t work_cap, bool hermitian, hipStream_t s) {
   DEREK_REQUIRE(m > 0 && n > 0 && batch > 0, "overlap: empty block");
   DEREK_REQUIRE(!hermitian || m == n, "overlap: hermitian needs a square result");
-  OvGeom g = ov_geom(m, n, npw, hermitian, batch);
+  OvGeom g = ov_geom(m, n, npw, hermitian, batch, ov_slots<R>());
   DEREK_REQUIRE((int64_t)batch * g.nchunk * m * n <= work_cap, "overlap: partial-sum buffer too small");
   hipLaunchKernelGGL(k_overlap<R>, dim3(g.ntiles, g.nchunk, batch), dim3(256), 0, s, X, ldx, sx, Y, ldy, sy, npwk, g,
                      work);
@@ -208,13 +259,16 @@ constexpr int RT_COLS = 64;
 constexpr int RT_KS = 16;
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_rotate(const cx_t<R>* __restrict__ X, int64_t ldx, int64_t sx,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_rotate(const cx_t<R>* __restrict__ X, int64_t ldx, int64_t sx,
                                                 const double2* __restrict__ U, int ldu, int64_t su, int m, int n,
                                                 int64_t npw, const int* __restrict__ npwk, cx_t<R>* __restrict__ Z,
                                                 int64_t ldz, int64_t sz) {
   using M = Mfma<R>;
   using A4 = typename M::acc;
-  __shared__ R ur[RT_KS][RT_COLS + 2], ui[RT_KS][RT_COLS + 2];
+  using C = cx_t<R>;
+  // three real products of x u (Gauss): P1 = xr ur, P2 = xi ui,
+  // P3 = (xr + xi)(ur + ui);  Re = P1 - P2,  Im = P3 - P1 - P2
+  __shared__ R ur[RT_KS][RT_COLS + 2], ui[RT_KS][RT_COLS + 2], us[RT_KS][RT_COLS + 2];
   const int kz = blockIdx.z;
   const int64_t nrow = npwk ? (int64_t)npwk[kz] : npw;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -226,40 +280,55 @@ __global__ __launch_bounds__(256) void k_rotate(const cx_t<R>* __restrict__ X, i
   const int j0 = blockIdx.y * RT_COLS;
   const int64_t grow = g0 + (lane & 15);
   const bool rok = grow < nrow;
-  A4 cr[4], ci[4];
+  A4 p1[4], p2[4], p3[4];
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
-    cr[b] = A4{0, 0, 0, 0};
-    ci[b] = A4{0, 0, 0, 0};
+    p1[b] = A4{0, 0, 0, 0};
+    p2[b] = A4{0, 0, 0, 0};
+    p3[b] = A4{0, 0, 0, 0};
   }
-  for (int k0 = 0; k0 < m; k0 += RT_KS) {
-    // stage U[k0:k0+16, j0:j0+64]: thread -> (k = t%16, column group t/16)
-    {
-      const int kk = threadIdx.x & 15, cg = threadIdx.x >> 4;
+  // next slab's U (staging thread -> (k = t%16, column group t/16)) and X
+  // (A operand: row grow, k = 4 kk + lane/16) held in registers
+  const int sk = threadIdx.x & 15, cg = threadIdx.x >> 4;
+  double2 pu[4];
+  C pxv[RT_KS / 4];
+  auto gload = [&](int k0) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int c = cg + 16 * q;
-        double2 v = make_double2(0.0, 0.0);
-        if (k0 + kk < m && j0 + c < n) v = U[(int64_t)(j0 + c) * ldu + k0 + kk];
-        ur[kk][c] = (R)v.x;
-        ui[kk][c] = (R)v.y;
-      }
+    for (int q = 0; q < 4; ++q) {
+      const int c = cg + 16 * q;
+      pu[q] = (k0 + sk < m && j0 + c < n) ? U[(int64_t)(j0 + c) * ldu + k0 + sk] : make_double2(0.0, 0.0);
     }
+#pragma unroll
+    for (int kk = 0; kk < RT_KS / 4; ++kk) {
+      const int k = k0 + kk * 4 + (lane >> 4);
+      pxv[kk] = (rok && k < m) ? X[(int64_t)k * ldx + grow] : cmk<C>(0, 0);
+    }
+  };
+  gload(0);
+  for (int k0 = 0; k0 < m; k0 += RT_KS) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = cg + 16 * q;
+      const R vr = (R)pu[q].x, vi = (R)pu[q].y;
+      ur[sk][c] = vr;
+      ui[sk][c] = vi;
+      us[sk][c] = vr + vi;
+    }
+    C xv[RT_KS / 4];
+#pragma unroll
+    for (int kk = 0; kk < RT_KS / 4; ++kk) xv[kk] = pxv[kk];
     __syncthreads();
+    if (k0 + RT_KS < m) gload(k0 + RT_KS);
 #pragma unroll
     for (int kk = 0; kk < RT_KS / 4; ++kk) {
       const int k = kk * 4 + (lane >> 4);
-      cx_t<R> xv = cmk<cx_t<R>>(0, 0);
-      if (rok && k0 + k < m) xv = X[(int64_t)(k0 + k) * ldx + grow];
+      const R xs = xv[kk].x + xv[kk].y;
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
-        const R vr = ur[k][b * 16 + (lane & 15)];
-        const R vi = ui[k][b * 16 + (lane & 15)];
-        // x * u = (xr ur - xi ui) + i (xr ui + xi ur)
-        cr[b] = M::op(xv.x, vr, cr[b]);
-        cr[b] = M::op(-xv.y, vi, cr[b]);
-        ci[b] = M::op(xv.x, vi, ci[b]);
-        ci[b] = M::op(xv.y, vr, ci[b]);
+        const int c = b * 16 + (lane & 15);
+        p1[b] = M::op(xv[kk].x, ur[k][c], p1[b]);
+        p2[b] = M::op(xv[kk].y, ui[k][c], p2[b]);
+        p3[b] = M::op(xs, us[k][c], p3[b]);
       }
     }
     __syncthreads();
@@ -271,7 +340,7 @@ __global__ __launch_bounds__(256) void k_rotate(const cx_t<R>* __restrict__ X, i
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int64_t gg = (int64_t)blockIdx.x * RT_ROWS + wave * 16 + M::row(lane, r);
-      if (gg < nrow) Z[(int64_t)j * ldz + gg] = cmk<cx_t<R>>(cr[b][r], ci[b][r]);
+      if (gg < nrow) Z[(int64_t)j * ldz + gg] = cmk<C>(p1[b][r] - p2[b][r], p3[b][r] - p1[b][r] - p2[b][r]);
     }
   }
 }

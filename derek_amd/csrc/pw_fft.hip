@@ -322,7 +322,7 @@ __global__ __launch_bounds__(Split<N>::NT) __attribute__((amdgpu_waves_per_eu(kX
   extern __shared__ __align__(16) unsigned char smem[];
   C* twl = reinterpret_cast<C*>(smem);
   C* buf = twl + N;
-  int* inv = reinterpret_cast<int*>(buf + N * PL);
+  int* inv = reinterpret_cast<int*>(buf + N * PL);  // byte offset of row x in a band, or -1
   const PwMaps m = B.mt[B.k0 + blockIdx.z];
   const int tid = threadIdx.x, l = tid % PL, jj = tid / PL;
   const int64_t c0 = (int64_t)blockIdx.x * PL;
@@ -330,7 +330,7 @@ __global__ __launch_bounds__(Split<N>::NT) __attribute__((amdgpu_waves_per_eu(kX
   load_twiddles<C, N>(twl, tw);
   for (int x = tid; x < N; x += NT) inv[x] = -1;
   __syncthreads();
-  for (int a = tid; a < m.nax; a += NT) inv[m.ax_x[a]] = a;
+  for (int a = tid; a < m.nax; a += NT) inv[m.ax_x[a]] = a * (int)nyz * (int)sizeof(C);
   __syncthreads();
   // V at x = j2 + R1 r: held in registers across the band loop, or, where
   // that would spill (complex128 at N = 144), re-read per band (an L2 hit)
@@ -344,14 +344,16 @@ __global__ __launch_bounds__(Split<N>::NT) __attribute__((amdgpu_waves_per_eu(kX
   const int bb1 = min(B.nbb, bb0 + BPW);
   // S1 input rows are looked up in LDS per band (no index registers held
   // across the band loop)
+  const uint32_t lo = (uint32_t)l * sizeof(C);
+  auto band_base = [&](int band) { return P + ((int64_t)blockIdx.z * B.nbb + band) * B.nax_cap * nyz + c0; };
   auto load_s1 = [&](int band, C* v) {
-    const C* base = P + ((int64_t)blockIdx.z * B.nbb + band) * B.nax_cap * nyz + c0 + l;
+    const C* base = band_base(band);
     int j = jj;
     asm volatile("" : "+v"(j));
 #pragma unroll
     for (int r = 0; r < R1; ++r) {
-      const int a = (j < R2 && lok) ? inv[j + R2 * r] : -1;
-      v[r] = a >= 0 ? base[(int64_t)a * nyz] : cmk<C>(0, 0);
+      const int o = (j < R2 && lok) ? inv[j + R2 * r] : -1;
+      v[r] = o >= 0 ? ld_at(base, (uint32_t)o + lo) : cmk<C>(0, 0);
     }
   };
   C pre[kPrefetch<R> ? R1 : 1];
@@ -401,11 +403,11 @@ __global__ __launch_bounds__(Split<N>::NT) __attribute__((amdgpu_waves_per_eu(kX
 #pragma unroll
       for (int j2 = 0; j2 < R1; ++j2) t[j2] = buf[(j2 * R2 + jj) * PL + l];
       dftR<R1, -1>(t, twl, N);
-      C* base = P + ((int64_t)blockIdx.z * B.nbb + band) * B.nax_cap * nyz + c0 + l;
+      C* base = band_base(band);
 #pragma unroll
       for (int k1 = 0; k1 < R1; ++k1) {
-        const int a = lok ? inv[jx + R2 * k1] : -1;
-        if (a >= 0) base[(int64_t)a * nyz] = t[k1];
+        const int o = lok ? inv[jx + R2 * k1] : -1;
+        if (o >= 0) st_at(base, (uint32_t)o + lo, t[k1]);
       }
     }
     __syncthreads();
@@ -438,15 +440,17 @@ __global__ __launch_bounds__(Split<N>::NT) void k_px_density2(PwBatch B, const c
     __syncthreads();
     for (int x = tid; x < N; x += NT) inv[x] = -1;
     __syncthreads();
-    for (int a = tid; a < m.nax; a += NT) inv[m.ax_x[a]] = a;
+    for (int a = tid; a < m.nax; a += NT) inv[m.ax_x[a]] = a * (int)nyz * (int)sizeof(C);
     __syncthreads();
+    // byte offsets of this thread's rows in a band (-1: inactive row)
     int arow[R1];
 #pragma unroll
     for (int r = 0; r < R1; ++r) arow[r] = (jj < R2 && lok) ? inv[jj + R2 * r] : -1;
+    const uint32_t lo = (uint32_t)l * sizeof(C);
     auto load_s1 = [&](int band, C* v) {
-      const C* base = P + ((int64_t)kz * B.nbb + band) * B.nax_cap * nyz + c0 + l;
+      const C* base = P + ((int64_t)kz * B.nbb + band) * B.nax_cap * nyz + c0;
 #pragma unroll
-      for (int r = 0; r < R1; ++r) v[r] = arow[r] >= 0 ? base[(int64_t)arow[r] * nyz] : cmk<C>(0, 0);
+      for (int r = 0; r < R1; ++r) v[r] = arow[r] >= 0 ? ld_at(base, (uint32_t)arow[r] + lo) : cmk<C>(0, 0);
     };
     C pre[kPrefetch<R> ? R1 : 1];
     if (kPrefetch<R>) load_s1(0, pre);
@@ -515,17 +519,19 @@ __global__ __launch_bounds__(Split<N>::NT) void k_py_inv2(PwBatch B, const cx_t<
   for (int y = tid; y < N; y += NT) inv[y] = -1;
   __syncthreads();
   const int sa = m.pl_st[a], sb = m.pl_st[a + 1];
-  for (int j = sa + tid; j < sb; j += NT) inv[m.st_y[j]] = j;
+  for (int j = sa + tid; j < sb; j += NT) inv[m.st_y[j]] = j * nz * (int)sizeof(C);
   __syncthreads();
+  // byte offsets of this thread's stick lines in a band (-1: no stick)
   int srow[R1];
 #pragma unroll
   for (int r = 0; r < R1; ++r) srow[r] = (jj < R2 && lok) ? inv[jj + R2 * r] : -1;
   const int bb0 = blockIdx.y * BPW;
   const int bb1 = min(B.nbb, bb0 + BPW);
+  const uint32_t lo = (uint32_t)l * sizeof(C);
   auto load_s1 = [&](int band, C* v) {
-    const C* base = S + ((int64_t)blockIdx.z * B.nbb + band) * B.nst_cap * nz + z0 + l;
+    const C* base = S + ((int64_t)blockIdx.z * B.nbb + band) * B.nst_cap * nz + z0;
 #pragma unroll
-    for (int r = 0; r < R1; ++r) v[r] = srow[r] >= 0 ? base[(int64_t)srow[r] * nz] : cmk<C>(0, 0);
+    for (int r = 0; r < R1; ++r) v[r] = srow[r] >= 0 ? ld_at(base, (uint32_t)srow[r] + lo) : cmk<C>(0, 0);
   };
   C pre[kPrefetch<R> ? R1 : 1];
   if (kPrefetch<R> && bb0 < bb1) load_s1(bb0, pre);
@@ -553,9 +559,9 @@ __global__ __launch_bounds__(Split<N>::NT) void k_py_inv2(PwBatch B, const cx_t<
       }
       dftR<R2, +1>(u, twl, N);
       if (lok) {
-        C* dst = P + (((int64_t)blockIdx.z * B.nbb + band) * B.nax_cap + a) * N * nz + z0 + l;
+        C* dst = P + (((int64_t)blockIdx.z * B.nbb + band) * B.nax_cap + a) * N * nz + z0;
 #pragma unroll
-        for (int k = 0; k < R2; ++k) dst[(int64_t)(jj + R1 * k) * nz] = u[k];
+        for (int k = 0; k < R2; ++k) st_at(dst, (uint32_t)((jj + R1 * k) * nz) * (uint32_t)sizeof(C) + lo, u[k]);
       }
     }
     __syncthreads();
@@ -584,8 +590,9 @@ __global__ __launch_bounds__(Split<N>::NT) void k_py_fwd2(PwBatch B, const cx_t<
   for (int y = tid; y < N; y += NT) inv[y] = -1;
   __syncthreads();
   const int sa = m.pl_st[a], sb = m.pl_st[a + 1];
-  for (int j = sa + tid; j < sb; j += NT) inv[m.st_y[j]] = j;
+  for (int j = sa + tid; j < sb; j += NT) inv[m.st_y[j]] = j * nz * (int)sizeof(C);
   __syncthreads();
+  // byte offsets of this thread's output stick lines (-1: no stick)
   int srow[R2];
   if (jj < R1) {
 #pragma unroll
@@ -593,10 +600,12 @@ __global__ __launch_bounds__(Split<N>::NT) void k_py_fwd2(PwBatch B, const cx_t<
   }
   const int bb0 = blockIdx.y * BPW;
   const int bb1 = min(B.nbb, bb0 + BPW);
+  const uint32_t lo = (uint32_t)l * sizeof(C);
   auto load_s1 = [&](int band, C* v) {
-    const C* base = P + (((int64_t)blockIdx.z * B.nbb + band) * B.nax_cap + a) * N * nz + z0 + l;
+    const C* base = P + (((int64_t)blockIdx.z * B.nbb + band) * B.nax_cap + a) * N * nz + z0;
 #pragma unroll
-    for (int r = 0; r < R1; ++r) v[r] = (jj < R2 && lok) ? base[(int64_t)(jj + R2 * r) * nz] : cmk<C>(0, 0);
+    for (int r = 0; r < R1; ++r)
+      v[r] = (jj < R2 && lok) ? ld_at(base, (uint32_t)((jj + R2 * r) * nz) * (uint32_t)sizeof(C) + lo) : cmk<C>(0, 0);
   };
   C pre[kPrefetch<R> ? R1 : 1];
   if (kPrefetch<R> && bb0 < bb1) load_s1(bb0, pre);
@@ -623,10 +632,10 @@ __global__ __launch_bounds__(Split<N>::NT) void k_py_fwd2(PwBatch B, const cx_t<
         u[r] = r ? cmul(y, twl[(jj * r) % N]) : y;
       }
       dftR<R2, -1>(u, twl, N);
-      C* dst = S + ((int64_t)blockIdx.z * B.nbb + band) * B.nst_cap * nz + z0 + l;
+      C* dst = S + ((int64_t)blockIdx.z * B.nbb + band) * B.nst_cap * nz + z0;
 #pragma unroll
       for (int k = 0; k < R2; ++k)
-        if (srow[k] >= 0) dst[(int64_t)srow[k] * nz] = u[k];
+        if (srow[k] >= 0) st_at(dst, (uint32_t)srow[k] + lo, u[k]);
     }
     __syncthreads();
   }
@@ -645,6 +654,13 @@ static int sticks_per_block(int nz) { return std::max(1, 1024 / nz); }
 
 static void check_batch(const PwBatch& B) {
   DEREK_REQUIRE(B.mt && B.nkc >= 1 && B.nbb >= 1 && B.nkc <= 65535 && B.nbb <= 65535, "empty pipeline batch");
+}
+
+// the column passes address one item's sticks / planes by 32-bit byte offsets
+template <typename R> static void check_offsets(const PwBatch& B, int64_t nyz, int nz) {
+  const int64_t cs = (int64_t)sizeof(cx_t<R>);
+  DEREK_REQUIRE((int64_t)B.nst_cap * nz * cs < INT32_MAX && (int64_t)B.nax_cap * nyz * cs < INT32_MAX,
+                "one item's sticks or planes exceed 2 GiB");
 }
 
 #define PW_DISPATCH(n, CALL) \
@@ -706,6 +722,7 @@ template <typename R>
 void planes_y(const PwBatch& B, cx_t<R>* S, cx_t<R>* P, const Axis<R>* ax, int ny, int nz, int sign, int bpw,
               hipStream_t s) {
   check_batch(B);
+  check_offsets<R>(B, (int64_t)ny * nz, nz);
   switch (ny) {
     case 32: planes_y2_bpw<R, 32>(B, S, P, ax, nz, sign, bpw, s); break;
     case 72: planes_y2_bpw<R, 72>(B, S, P, ax, nz, sign, bpw, s); break;
@@ -781,6 +798,7 @@ template <typename R>
 void planes_x_vmul(const PwBatch& B, cx_t<R>* P, const Axis<R>* ax, int64_t nyz, const R* vloc, R vscale,
                    int bands_per_wg, hipStream_t s) {
   check_batch(B);
+  check_offsets<R>(B, nyz, 1);
   switch (bands_per_wg) {
     case 1: planes_x_vmul_bpw<R, 1>(B, P, ax, nyz, vloc, vscale, s); break;
     case 2: planes_x_vmul_bpw<R, 2>(B, P, ax, nyz, vloc, vscale, s); break;
@@ -795,6 +813,7 @@ template <typename R>
 void planes_x_density(const PwBatch& B, const cx_t<R>* P, const Axis<R>* ax, int64_t nyz, const double* occ,
                       int occ_ks, double* rho, int keep, hipStream_t s) {
   check_batch(B);
+  check_offsets<R>(B, nyz, 1);
   const int nx = ax[0].n;
   dim3 grid((unsigned)ceil_div(nyz, PL));
 #define C2_(NC)                                                                                                 \
