@@ -57,6 +57,34 @@ template <typename T> __device__ __forceinline__ T ld_at(const T* base, uint32_t
 template <typename T> __device__ __forceinline__ void st_at(T* base, uint32_t boff, T v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff) = v;
 }
+// Raw buffer access (CDNA buffer resources): a wave-uniform base and a byte
+// size; a load at an offset outside [0, size) returns zero and a store there
+// is dropped by the hardware, so masked-out rows take the offset kOob
+// instead of a branch.  Sizes stay below kOob.
+constexpr uint32_t kOob = 0x7FFF0000u;
+#ifdef __HIPCC__
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+template <typename C> __device__ __forceinline__ C buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  if constexpr (sizeof(C) == 16) {
+    return __builtin_bit_cast(C, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
+  } else {
+    static_assert(sizeof(C) == 8, "8- or 16-byte elements");
+    return __builtin_bit_cast(C, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+  }
+}
+template <typename C> __device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t off, C v) {
+  if constexpr (sizeof(C) == 16) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, v), r, (int)off, 0, 0);
+  } else {
+    static_assert(sizeof(C) == 8, "8- or 16-byte elements");
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, v), r, (int)off, 0, 0);
+  }
+}
+#endif
 template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return cmk<C>(a.x + b.x, a.y + b.y); }
 template <typename C> __device__ __forceinline__ C csub(C a, C b) { return cmk<C>(a.x - b.x, a.y - b.y); }
 template <typename C> __device__ __forceinline__ C cmul(C a, C b) {

@@ -322,15 +322,15 @@ __global__ __launch_bounds__(Split<N>::NT) __attribute__((amdgpu_waves_per_eu(kX
   extern __shared__ __align__(16) unsigned char smem[];
   C* twl = reinterpret_cast<C*>(smem);
   C* buf = twl + N;
-  int* inv = reinterpret_cast<int*>(buf + N * PL);  // byte offset of row x in a band, or -1
+  uint32_t* inv = reinterpret_cast<uint32_t*>(buf + N * PL);  // byte offset of row x in a band, or kOob
   const PwMaps m = B.mt[B.k0 + blockIdx.z];
   const int tid = threadIdx.x, l = tid % PL, jj = tid / PL;
   const int64_t c0 = (int64_t)blockIdx.x * PL;
   const bool lok = c0 + l < nyz;
   load_twiddles<C, N>(twl, tw);
-  for (int x = tid; x < N; x += NT) inv[x] = -1;
+  for (int x = tid; x < N; x += NT) inv[x] = kOob;
   __syncthreads();
-  for (int a = tid; a < m.nax; a += NT) inv[m.ax_x[a]] = a * (int)nyz * (int)sizeof(C);
+  for (int a = tid; a < m.nax; a += NT) inv[m.ax_x[a]] = (uint32_t)a * (uint32_t)nyz * (uint32_t)sizeof(C);
   __syncthreads();
   // V at x = j2 + R1 r: held in registers across the band loop, or, where
   // that would spill (complex128 at N = 144), re-read per band (an L2 hit)
@@ -344,17 +344,21 @@ __global__ __launch_bounds__(Split<N>::NT) __attribute__((amdgpu_waves_per_eu(kX
   const int bb1 = min(B.nbb, bb0 + BPW);
   // S1 input rows are looked up in LDS per band (no index registers held
   // across the band loop)
-  const uint32_t lo = (uint32_t)l * sizeof(C);
-  auto band_base = [&](int band) { return P + ((int64_t)blockIdx.z * B.nbb + band) * B.nax_cap * nyz + c0; };
+  // lane offset within a row; kOob makes every access of a lane past the
+  // last column (and of the S1 rows j >= R2) fall outside the buffer
+  const uint32_t lo = lok ? (uint32_t)l * (uint32_t)sizeof(C) : kOob;
+  const uint32_t band_bytes = (uint32_t)((B.nax_cap * nyz - c0) * (int64_t)sizeof(C));
+  auto band_rsrc = [&](int band) {
+    return buf_rsrc(P + ((int64_t)blockIdx.z * B.nbb + band) * B.nax_cap * nyz + c0, band_bytes);
+  };
   auto load_s1 = [&](int band, C* v) {
-    const C* base = band_base(band);
+    const auto rs = band_rsrc(band);
     int j = jj;
     asm volatile("" : "+v"(j));
+    const int jr = j < R2 ? j : 0;
+    const uint32_t lj = j < R2 ? lo : kOob;
 #pragma unroll
-    for (int r = 0; r < R1; ++r) {
-      const int o = (j < R2 && lok) ? inv[j + R2 * r] : -1;
-      v[r] = o >= 0 ? ld_at(base, (uint32_t)o + lo) : cmk<C>(0, 0);
-    }
+    for (int r = 0; r < R1; ++r) v[r] = buf_ld<C>(rs, inv[jr + R2 * r] + lj);
   };
   C pre[kPrefetch<R> ? R1 : 1];
   if (kPrefetch<R> && bb0 < bb1) load_s1(bb0, pre);
@@ -403,12 +407,9 @@ __global__ __launch_bounds__(Split<N>::NT) __attribute__((amdgpu_waves_per_eu(kX
 #pragma unroll
       for (int j2 = 0; j2 < R1; ++j2) t[j2] = buf[(j2 * R2 + jj) * PL + l];
       dftR<R1, -1>(t, twl, N);
-      C* base = band_base(band);
+      const auto rs = band_rsrc(band);
 #pragma unroll
-      for (int k1 = 0; k1 < R1; ++k1) {
-        const int o = lok ? inv[jx + R2 * k1] : -1;
-        if (o >= 0) st_at(base, (uint32_t)o + lo, t[k1]);
-      }
+      for (int k1 = 0; k1 < R1; ++k1) buf_st<C>(rs, inv[jx + R2 * k1] + lo, t[k1]);
     }
     __syncthreads();
   }
@@ -438,19 +439,19 @@ __global__ __launch_bounds__(Split<N>::NT) void k_px_density2(PwBatch B, const c
   for (int kz = 0; kz < B.nkc; ++kz) {
     const PwMaps m = B.mt[B.k0 + kz];
     __syncthreads();
-    for (int x = tid; x < N; x += NT) inv[x] = -1;
+    for (int x = tid; x < N; x += NT) inv[x] = (int)kOob;
     __syncthreads();
     for (int a = tid; a < m.nax; a += NT) inv[m.ax_x[a]] = a * (int)nyz * (int)sizeof(C);
     __syncthreads();
-    // byte offsets of this thread's rows in a band (-1: inactive row)
-    int arow[R1];
+    // byte offsets of this thread's rows in a band (kOob: inactive row or lane)
+    uint32_t arow[R1];
 #pragma unroll
-    for (int r = 0; r < R1; ++r) arow[r] = (jj < R2 && lok) ? inv[jj + R2 * r] : -1;
-    const uint32_t lo = (uint32_t)l * sizeof(C);
+    for (int r = 0; r < R1; ++r) arow[r] = (jj < R2 && lok) ? (uint32_t)inv[jj + R2 * r] + (uint32_t)l * sizeof(C) : kOob;
+    const uint32_t band_bytes = (uint32_t)((B.nax_cap * nyz - c0) * (int64_t)sizeof(C));
     auto load_s1 = [&](int band, C* v) {
-      const C* base = P + ((int64_t)kz * B.nbb + band) * B.nax_cap * nyz + c0;
+      const auto rs = buf_rsrc(P + ((int64_t)kz * B.nbb + band) * B.nax_cap * nyz + c0, band_bytes);
 #pragma unroll
-      for (int r = 0; r < R1; ++r) v[r] = arow[r] >= 0 ? ld_at(base, (uint32_t)arow[r] + lo) : cmk<C>(0, 0);
+      for (int r = 0; r < R1; ++r) v[r] = buf_ld<C>(rs, arow[r]);
     };
     C pre[kPrefetch<R> ? R1 : 1];
     if (kPrefetch<R>) load_s1(0, pre);
@@ -516,22 +517,23 @@ __global__ __launch_bounds__(Split<N>::NT) void k_py_inv2(PwBatch B, const cx_t<
   const int z0 = (blockIdx.x - a * ztiles) * PL;
   const bool lok = z0 + l < nz;
   load_twiddles<C, N>(twl, tw);
-  for (int y = tid; y < N; y += NT) inv[y] = -1;
+  for (int y = tid; y < N; y += NT) inv[y] = (int)kOob;
   __syncthreads();
   const int sa = m.pl_st[a], sb = m.pl_st[a + 1];
   for (int j = sa + tid; j < sb; j += NT) inv[m.st_y[j]] = j * nz * (int)sizeof(C);
   __syncthreads();
-  // byte offsets of this thread's stick lines in a band (-1: no stick)
-  int srow[R1];
+  // byte offsets of this thread's stick lines in a band (kOob: no stick)
+  const uint32_t lo = lok ? (uint32_t)l * (uint32_t)sizeof(C) : kOob;
+  uint32_t srow[R1];
 #pragma unroll
-  for (int r = 0; r < R1; ++r) srow[r] = (jj < R2 && lok) ? inv[jj + R2 * r] : -1;
+  for (int r = 0; r < R1; ++r) srow[r] = jj < R2 ? (uint32_t)inv[jj + R2 * r] + lo : kOob;
   const int bb0 = blockIdx.y * BPW;
   const int bb1 = min(B.nbb, bb0 + BPW);
-  const uint32_t lo = (uint32_t)l * sizeof(C);
+  const uint32_t st_bytes = (uint32_t)(((int64_t)B.nst_cap * nz - z0) * (int64_t)sizeof(C));
   auto load_s1 = [&](int band, C* v) {
-    const C* base = S + ((int64_t)blockIdx.z * B.nbb + band) * B.nst_cap * nz + z0;
+    const auto rs = buf_rsrc(S + ((int64_t)blockIdx.z * B.nbb + band) * B.nst_cap * nz + z0, st_bytes);
 #pragma unroll
-    for (int r = 0; r < R1; ++r) v[r] = srow[r] >= 0 ? ld_at(base, (uint32_t)srow[r] + lo) : cmk<C>(0, 0);
+    for (int r = 0; r < R1; ++r) v[r] = buf_ld<C>(rs, srow[r]);
   };
   C pre[kPrefetch<R> ? R1 : 1];
   if (kPrefetch<R> && bb0 < bb1) load_s1(bb0, pre);
@@ -558,11 +560,10 @@ __global__ __launch_bounds__(Split<N>::NT) void k_py_inv2(PwBatch B, const cx_t<
         u[r] = r ? cmul(y, cconj(twl[(jj * r) % N])) : y;
       }
       dftR<R2, +1>(u, twl, N);
-      if (lok) {
-        C* dst = P + (((int64_t)blockIdx.z * B.nbb + band) * B.nax_cap + a) * N * nz + z0;
+      const auto rd = buf_rsrc(P + (((int64_t)blockIdx.z * B.nbb + band) * B.nax_cap + a) * N * nz + z0,
+                               (uint32_t)(((int64_t)N * nz - z0) * (int64_t)sizeof(C)));
 #pragma unroll
-        for (int k = 0; k < R2; ++k) st_at(dst, (uint32_t)((jj + R1 * k) * nz) * (uint32_t)sizeof(C) + lo, u[k]);
-      }
+      for (int k = 0; k < R2; ++k) buf_st<C>(rd, (uint32_t)((jj + R1 * k) * nz) * (uint32_t)sizeof(C) + lo, u[k]);
     }
     __syncthreads();
   }
@@ -587,25 +588,26 @@ __global__ __launch_bounds__(Split<N>::NT) void k_py_fwd2(PwBatch B, const cx_t<
   const int z0 = (blockIdx.x - a * ztiles) * PL;
   const bool lok = z0 + l < nz;
   load_twiddles<C, N>(twl, tw);
-  for (int y = tid; y < N; y += NT) inv[y] = -1;
+  for (int y = tid; y < N; y += NT) inv[y] = (int)kOob;
   __syncthreads();
   const int sa = m.pl_st[a], sb = m.pl_st[a + 1];
   for (int j = sa + tid; j < sb; j += NT) inv[m.st_y[j]] = j * nz * (int)sizeof(C);
   __syncthreads();
-  // byte offsets of this thread's output stick lines (-1: no stick)
-  int srow[R2];
+  // byte offsets of this thread's output stick lines (kOob: no stick)
+  const uint32_t lo = lok ? (uint32_t)l * (uint32_t)sizeof(C) : kOob;
+  uint32_t srow[R2];
   if (jj < R1) {
 #pragma unroll
-    for (int r = 0; r < R2; ++r) srow[r] = lok ? inv[jj + R1 * r] : -1;
+    for (int r = 0; r < R2; ++r) srow[r] = (uint32_t)inv[jj + R1 * r] + lo;
   }
   const int bb0 = blockIdx.y * BPW;
   const int bb1 = min(B.nbb, bb0 + BPW);
-  const uint32_t lo = (uint32_t)l * sizeof(C);
+  const uint32_t pl_bytes = (uint32_t)(((int64_t)N * nz - z0) * (int64_t)sizeof(C));
+  const uint32_t ls1 = jj < R2 ? lo : kOob;
   auto load_s1 = [&](int band, C* v) {
-    const C* base = P + (((int64_t)blockIdx.z * B.nbb + band) * B.nax_cap + a) * N * nz + z0;
+    const auto rs = buf_rsrc(P + (((int64_t)blockIdx.z * B.nbb + band) * B.nax_cap + a) * N * nz + z0, pl_bytes);
 #pragma unroll
-    for (int r = 0; r < R1; ++r)
-      v[r] = (jj < R2 && lok) ? ld_at(base, (uint32_t)((jj + R2 * r) * nz) * (uint32_t)sizeof(C) + lo) : cmk<C>(0, 0);
+    for (int r = 0; r < R1; ++r) v[r] = buf_ld<C>(rs, (uint32_t)((jj + R2 * r) * nz) * (uint32_t)sizeof(C) + ls1);
   };
   C pre[kPrefetch<R> ? R1 : 1];
   if (kPrefetch<R> && bb0 < bb1) load_s1(bb0, pre);
@@ -632,10 +634,10 @@ __global__ __launch_bounds__(Split<N>::NT) void k_py_fwd2(PwBatch B, const cx_t<
         u[r] = r ? cmul(y, twl[(jj * r) % N]) : y;
       }
       dftR<R2, -1>(u, twl, N);
-      C* dst = S + ((int64_t)blockIdx.z * B.nbb + band) * B.nst_cap * nz + z0;
+      const auto rd = buf_rsrc(S + ((int64_t)blockIdx.z * B.nbb + band) * B.nst_cap * nz + z0,
+                               (uint32_t)(((int64_t)B.nst_cap * nz - z0) * (int64_t)sizeof(C)));
 #pragma unroll
-      for (int k = 0; k < R2; ++k)
-        if (srow[k] >= 0) st_at(dst, (uint32_t)srow[k] + lo, u[k]);
+      for (int k = 0; k < R2; ++k) buf_st<C>(rd, srow[k], u[k]);
     }
     __syncthreads();
   }
@@ -659,7 +661,7 @@ static void check_batch(const PwBatch& B) {
 // the column passes address one item's sticks / planes by 32-bit byte offsets
 template <typename R> static void check_offsets(const PwBatch& B, int64_t nyz, int nz) {
   const int64_t cs = (int64_t)sizeof(cx_t<R>);
-  DEREK_REQUIRE((int64_t)B.nst_cap * nz * cs < INT32_MAX && (int64_t)B.nax_cap * nyz * cs < INT32_MAX,
+  DEREK_REQUIRE((int64_t)B.nst_cap * nz * cs < (int64_t)kOob && (int64_t)B.nax_cap * nyz * cs < (int64_t)kOob,
                 "one item's sticks or planes exceed 2 GiB");
 }
 

@@ -151,7 +151,9 @@ template <typename R> int davidson_chunk(derek_ctx* c, int k0, int nkc, int nit,
   // dependent-column threshold of the Cholesky: FP32 overlaps carry ~1e-6 errors
   const double drop = f32 ? 1e-5 : 1e-8;
   c->apply_h<R>(k0, nkc, pb(Y, 0), pb(HY, 0), nb);
-  ov<R>(c, Y, HY, nb, nb, k0, nkc, H, false);
+  // Y^H (H Y) is Hermitian: only its upper tiles are multiplied, the lower
+  // triangle is their conjugate mirror
+  ov<R>(c, Y, HY, nb, nb, k0, nkc, H, true);
   if (f32) {
     ov<R>(c, Y, Y, nb, nb, k0, nkc, S, true);
     c->timed(KC_DENSE, 0, 0, [&] { gen_solve(E.dw, S, H, nb, nb, nkc, drop, eps, U, s); });
@@ -181,7 +183,7 @@ template <typename R> int davidson_chunk(derek_ctx* c, int k0, int nkc, int nit,
     });
     c->apply_h<R>(k0, nkc, pb(Y, nb), pb(HY, nb), nb);
     ov<R>(c, Y, Y, m2, m2, k0, nkc, S, true);
-    ov<R>(c, Y, HY, m2, m2, k0, nkc, H, false);
+    ov<R>(c, Y, HY, m2, m2, k0, nkc, H, true);
     c->timed(KC_DENSE, 0, 0, [&] { gen_solve(E.dw, S, H, m2, nb, nkc, drop, eps, U, s); });
     rot<R>(c, Y, U, m2, nb, (int64_t)m2 * nb, k0, nkc, Y2);
     rot<R>(c, HY, U, m2, nb, (int64_t)m2 * nb, k0, nkc, HY2);
@@ -291,11 +293,15 @@ void switch_to_fp64(derek_ctx* c) {
   c->P32.release();
 }
 
-// Times H|psi> on the first batch for the tunable launch shapes and keeps
-// the fastest: bands per x-pass workgroup, then items per launch.
+// Times H|psi> for the tunable launch shapes and keeps the fastest: bands
+// per x-pass workgroup, then items per launch.  The timed sample is at most
+// about 256 items (the first k-points' leading bands): enough to resolve
+// launch shapes up to that size at a small fraction of a step's cost.
 template <typename R> void autotune(derek_ctx* c) {
   ScfEngine& E = *c->scf;
-  const int nkc = std::min(E.kc, c->nk);
+  constexpr int kTuneItems = 256;
+  const int nbt = std::min(c->nb, kTuneItems);
+  const int nkc = std::min({E.kc, c->nk, (int)ceil_div(kTuneItems, nbt)});
   cx_t<R>* Y = reinterpret_cast<cx_t<R>*>(E.Y);
   cx_t<R>* HY = reinterpret_cast<cx_t<R>*>(E.HY);
   const PwBlock<R> x{Y, E.ldk, E.kstride}, hx{HY, E.ldk, E.kstride};
@@ -306,10 +312,10 @@ template <typename R> void autotune(derek_ctx* c) {
   DEREK_HIP(hipEventCreate(&b));
   auto time_it = [&] {
     float best = 1e30f;
-    c->apply_h<R>(0, nkc, x, hx, c->nb);
+    c->apply_h<R>(0, nkc, x, hx, nbt);
     for (int r = 0; r < 3; ++r) {
       DEREK_HIP(hipEventRecord(a, c->s));
-      c->apply_h<R>(0, nkc, x, hx, c->nb);
+      c->apply_h<R>(0, nkc, x, hx, nbt);
       DEREK_HIP(hipEventRecord(b, c->s));
       DEREK_HIP(hipEventSynchronize(b));
       float t = 0.f;
@@ -321,7 +327,7 @@ template <typename R> void autotune(derek_ctx* c) {
   // bytes of stick + plane buffers per item; the sweep stays within 8 GiB
   const double item_bytes =
       (double)sizeof(cx_t<R>) * ((double)c->nst_max * c->dims[2] + (double)c->nax_max * c->dims[1] * c->dims[2]);
-  const int all = nkc * c->nb;
+  const int all = nkc * nbt;
   const int cap_mem = (int)std::max(1.0, std::floor(8.0 * 1073741824.0 / item_bytes));
   std::vector<int> caps;
   for (int q : {8, 16, 32, 64, 128, 256, 512, all})
@@ -331,7 +337,7 @@ template <typename R> void autotune(derek_ctx* c) {
   caps.erase(std::unique(caps.begin(), caps.end()), caps.end());
   c->item_cap = caps[caps.size() / 2];
   // bring the clocks up before the first timed shape
-  for (int r = 0; r < 4; ++r) c->apply_h<R>(0, nkc, x, hx, c->nb);
+  for (int r = 0; r < 4; ++r) c->apply_h<R>(0, nkc, x, hx, nbt);
   float best = 1e30f;
   int bpw = c->bpw;
   for (int q : {1, 2, 4, 8}) {
