@@ -643,6 +643,123 @@ __global__ __launch_bounds__(Split<N>::NT) void k_py_fwd2(PwBatch B, const cx_t<
   }
 }
 
+// z passes on sticks for the two-stage lengths: SL sticks per workgroup,
+// TPL = max(R1, R2) threads per stick; the sticks are padded lines of one
+// LDS buffer (the plane-wave scatter / gather needs them there anyway), the
+// two DFT stages run in registers with one transpose through that buffer.
+template <int N> struct StickSplit {
+  static constexpr int R1 = Split<N>::R1, R2 = Split<N>::R2;
+  static constexpr int TPL = R1 > R2 ? R1 : R2;
+  static constexpr int NT = 256;
+  static constexpr int SL = NT / TPL;  // sticks per workgroup
+  static constexpr int LS = N + 1;     // padded line pitch
+  static constexpr size_t lds(size_t csize) { return csize * ((size_t)N + (size_t)SL * LS); }
+};
+
+// S1 + S2 of one transform on a padded LDS line, in place; thread (line, j)
+// with j < TPL, `act` false for lines past the block's last stick
+template <int SGN, typename C, int N>
+__device__ __forceinline__ void stick_two_stage(C* line, const C* twl, int j, bool act) {
+  using SS = StickSplit<N>;
+  constexpr int R1 = SS::R1, R2 = SS::R2;
+  C v[R1];
+  if (act && j < R2) {
+#pragma unroll
+    for (int r = 0; r < R1; ++r) v[r] = line[j + R2 * r];
+    dftR<R1, SGN>(v, twl, N);
+  }
+  __syncthreads();
+  if (act && j < R2) {
+#pragma unroll
+    for (int k = 0; k < R1; ++k) line[j * R1 + k] = v[k];
+  }
+  __syncthreads();
+  C u[R2];
+  if (act && j < R1) {
+#pragma unroll
+    for (int r = 0; r < R2; ++r) {
+      const C y = line[j + R1 * r];
+      u[r] = r ? cmul(y, SGN > 0 ? cconj(twl[(j * r) % N]) : twl[(j * r) % N]) : y;
+    }
+    dftR<R2, SGN>(u, twl, N);
+  }
+  __syncthreads();
+  if (act && j < R1) {
+#pragma unroll
+    for (int k = 0; k < R2; ++k) line[j + R1 * k] = u[k];
+  }
+  __syncthreads();
+}
+
+// psi[b][perm[p]] -> S[item][s][:] for the block's sticks
+template <typename R, int N>
+__global__ __launch_bounds__(StickSplit<N>::NT) void k_sticks_z_inv2(PwBatch B, PwBlock<R> psi,
+                                                                     const cx_t<R>* __restrict__ tw,
+                                                                     cx_t<R>* __restrict__ S) {
+  using C = cx_t<R>;
+  using SS = StickSplit<N>;
+  constexpr int NT = SS::NT, SL = SS::SL, LS = SS::LS, TPL = SS::TPL;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const PwMaps m = B.mt[B.k0 + blockIdx.z];
+  const int s0 = blockIdx.x * SL;
+  if (s0 >= m.nst) return;
+  C* twl = reinterpret_cast<C*>(smem);
+  C* buf = twl + N;
+  const int lines = min(SL, m.nst - s0);
+  const int band = blockIdx.y;
+  for (int t = threadIdx.x; t < N; t += NT) twl[t] = tw[t];
+  for (int e = threadIdx.x; e < lines * LS; e += NT) buf[e] = cmk<C>(0, 0);
+  __syncthreads();
+  const int p0 = m.st_pw[s0], p1 = m.st_pw[s0 + lines];
+  const C* pb = psi.p + blockIdx.z * psi.ks + (int64_t)band * psi.ld;
+  for (int p = p0 + threadIdx.x; p < p1; p += NT) buf[(m.pw_st[p] - s0) * LS + m.pw_z[p]] = pb[m.perm[p]];
+  __syncthreads();
+  const int line = threadIdx.x / TPL, j = threadIdx.x % TPL;
+  stick_two_stage<+1, C, N>(buf + line * LS, twl, j, line < lines);
+  C* dst = S + (((int64_t)blockIdx.z * B.nbb + band) * B.nst_cap + s0) * N;
+  for (int e = threadIdx.x; e < lines * N; e += NT) {
+    const int q = e / N;
+    dst[e] = buf[q * LS + (e - q * N)];
+  }
+}
+
+// hpsi[b][perm[p]] = (z FFT of the stick)[pw_z[p]] + kin[perm[p]] psi[b][perm[p]]
+template <typename R, int N>
+__global__ __launch_bounds__(StickSplit<N>::NT) void k_sticks_z_fwd2(PwBatch B, const cx_t<R>* __restrict__ S,
+                                                                     const cx_t<R>* __restrict__ tw, PwBlock<R> psi,
+                                                                     PwBlock<R> hpsi) {
+  using C = cx_t<R>;
+  using SS = StickSplit<N>;
+  constexpr int NT = SS::NT, SL = SS::SL, LS = SS::LS, TPL = SS::TPL;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const PwMaps m = B.mt[B.k0 + blockIdx.z];
+  const int s0 = blockIdx.x * SL;
+  if (s0 >= m.nst) return;
+  C* twl = reinterpret_cast<C*>(smem);
+  C* buf = twl + N;
+  const int lines = min(SL, m.nst - s0);
+  const int band = blockIdx.y;
+  for (int t = threadIdx.x; t < N; t += NT) twl[t] = tw[t];
+  const C* src = S + (((int64_t)blockIdx.z * B.nbb + band) * B.nst_cap + s0) * N;
+  for (int e = threadIdx.x; e < lines * N; e += NT) {
+    const int q = e / N;
+    buf[q * LS + (e - q * N)] = src[e];
+  }
+  __syncthreads();
+  const int line = threadIdx.x / TPL, j = threadIdx.x % TPL;
+  stick_two_stage<-1, C, N>(buf + line * LS, twl, j, line < lines);
+  const int p0 = m.st_pw[s0], p1 = m.st_pw[s0 + lines];
+  const C* pb = psi.p + blockIdx.z * psi.ks + (int64_t)band * psi.ld;
+  C* hb = hpsi.p + blockIdx.z * hpsi.ks + (int64_t)band * hpsi.ld;
+  for (int p = p0 + threadIdx.x; p < p1; p += NT) {
+    const int g = m.perm[p];
+    const C v = buf[(m.pw_st[p] - s0) * LS + m.pw_z[p]];
+    const C x = pb[g];
+    const R t = (R)m.kin[g];
+    hb[g] = cmk<C>(v.x + t * x.x, v.y + t * x.y);
+  }
+}
+
 // ---------------------------------------------------------------- launchers
 template <typename K> static void allow_lds_pw(K kern, size_t bytes) {
   if (bytes > 65536) allow_dynamic_lds(reinterpret_cast<const void*>(kern), bytes);
@@ -665,18 +782,37 @@ template <typename R> static void check_offsets(const PwBatch& B, int64_t nyz, i
                 "one item's sticks or planes exceed 2 GiB");
 }
 
-#define PW_DISPATCH(n, CALL) \
-  switch (n) {               \
-    case 32: CALL(32); break; \
-    case 72: CALL(72); break; \
-    case 108: CALL(108); break; \
-    case 144: CALL(144); break; \
-    default: CALL(0); break; \
-  }
+template <typename R, int N>
+static void sticks_z_inv2(const PwBatch& B, PwBlock<R> psi, const Axis<R>* ax, cx_t<R>* S, hipStream_t s) {
+  using SS = StickSplit<N>;
+  const size_t lds = SS::lds(sizeof(cx_t<R>));
+  auto kern = k_sticks_z_inv2<R, N>;
+  allow_lds_pw(kern, lds);
+  dim3 grid((unsigned)ceil_div(B.nst_cap, SS::SL), B.nbb, B.nkc);
+  hipLaunchKernelGGL(kern, grid, dim3(SS::NT), lds, s, B, psi, ax[2].tw, S);
+}
+
+template <typename R, int N>
+static void sticks_z_fwd2(const PwBatch& B, const cx_t<R>* S, const Axis<R>* ax, PwBlock<R> psi, PwBlock<R> hpsi,
+                          hipStream_t s) {
+  using SS = StickSplit<N>;
+  const size_t lds = SS::lds(sizeof(cx_t<R>));
+  auto kern = k_sticks_z_fwd2<R, N>;
+  allow_lds_pw(kern, lds);
+  dim3 grid((unsigned)ceil_div(B.nst_cap, SS::SL), B.nbb, B.nkc);
+  hipLaunchKernelGGL(kern, grid, dim3(SS::NT), lds, s, B, S, ax[2].tw, psi, hpsi);
+}
 
 template <typename R>
 void sticks_z_inv(const PwBatch& B, PwBlock<R> psi, const Axis<R>* ax, int nz, cx_t<R>* S, hipStream_t s) {
   check_batch(B);
+  switch (nz) {
+    case 32: sticks_z_inv2<R, 32>(B, psi, ax, S, s); DEREK_HIP(hipGetLastError()); return;
+    case 72: sticks_z_inv2<R, 72>(B, psi, ax, S, s); DEREK_HIP(hipGetLastError()); return;
+    case 108: sticks_z_inv2<R, 108>(B, psi, ax, S, s); DEREK_HIP(hipGetLastError()); return;
+    case 144: sticks_z_inv2<R, 144>(B, psi, ax, S, s); DEREK_HIP(hipGetLastError()); return;
+    default: break;
+  }
   const int lpb = sticks_per_block(nz);
   const size_t lds = sizeof(cx_t<R>) * ((size_t)nz + 2 * (size_t)lpb * contig_stride(nz));
   check_lds_pw(lds, nz);
@@ -687,7 +823,7 @@ void sticks_z_inv(const PwBatch& B, PwBlock<R> psi, const Axis<R>* ax, int nz, c
     allow_lds_pw(kern, lds);                                                                    \
     hipLaunchKernelGGL(kern, grid, dim3(PNT), lds, s, B, psi, lpb, ax[2].plan, ax[2].tw, S);    \
   }
-  PW_DISPATCH(nz, C_)
+  C_(0)
 #undef C_
   DEREK_HIP(hipGetLastError());
 }
@@ -753,6 +889,13 @@ template <typename R>
 void sticks_z_fwd(const PwBatch& B, const cx_t<R>* S, const Axis<R>* ax, int nz, PwBlock<R> psi, PwBlock<R> hpsi,
                   hipStream_t s) {
   check_batch(B);
+  switch (nz) {
+    case 32: sticks_z_fwd2<R, 32>(B, S, ax, psi, hpsi, s); DEREK_HIP(hipGetLastError()); return;
+    case 72: sticks_z_fwd2<R, 72>(B, S, ax, psi, hpsi, s); DEREK_HIP(hipGetLastError()); return;
+    case 108: sticks_z_fwd2<R, 108>(B, S, ax, psi, hpsi, s); DEREK_HIP(hipGetLastError()); return;
+    case 144: sticks_z_fwd2<R, 144>(B, S, ax, psi, hpsi, s); DEREK_HIP(hipGetLastError()); return;
+    default: break;
+  }
   const int lpb = sticks_per_block(nz);
   const size_t lds = sizeof(cx_t<R>) * ((size_t)nz + 2 * (size_t)lpb * contig_stride(nz));
   check_lds_pw(lds, nz);
@@ -763,7 +906,7 @@ void sticks_z_fwd(const PwBatch& B, const cx_t<R>* S, const Axis<R>* ax, int nz,
     allow_lds_pw(kern, lds);                                                                         \
     hipLaunchKernelGGL(kern, grid, dim3(PNT), lds, s, B, S, lpb, ax[2].plan, ax[2].tw, psi, hpsi);   \
   }
-  PW_DISPATCH(nz, C_)
+  C_(0)
 #undef C_
   DEREK_HIP(hipGetLastError());
 }
