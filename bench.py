@@ -101,6 +101,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
                     help="run the batched SCF modes eagerly (no hipGraph replay), e.g. under a kernel tracer")
+    ap.add_argument("--fixed-shapes", action="store_true",
+                    help="time H|psi> at the default launch shapes (4 bands per x-pass workgroup, 16 items per "
+                         "launch) instead of autotuned ones, so that every launch of a kernel has one size")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -121,7 +124,7 @@ def main():
     t0 = time.perf_counter()
     su = gscf.Setup(spec)
     ctx = su.context()
-    ctx.scf_init(spec.nelec, kT=spec.kT, seed=spec.seed, autotune=True)
+    ctx.scf_init(spec.nelec, kT=spec.kT, seed=spec.seed, autotune=not args.fixed_shapes)
     ctx.set_vloc(su.vext)
     tune64 = ctx.tuning()
     nb = spec.nbands
@@ -131,7 +134,7 @@ def main():
     ctx.close()
     # the same H|psi> in complex64 (the mixed form's bulk)
     ctx = su.context(lib.DEREK_MIXED)
-    ctx.scf_init(spec.nelec, kT=spec.kT, seed=spec.seed, autotune=True)
+    ctx.scf_init(spec.nelec, kT=spec.kT, seed=spec.seed, autotune=not args.fixed_shapes)
     ctx.set_vloc(su.vext)
     tune32 = ctx.tuning()
     bps32, ms32, prof32 = hpsi_rate(ctx, nb, args.steps, args.warmup)

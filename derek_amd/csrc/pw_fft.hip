@@ -308,12 +308,12 @@ __device__ __forceinline__ void load_twiddles(C* twl, const C* __restrict__ tw) 
 // x pass: inverse x transform of the active rows, * V * vscale, forward
 // transform, active rows written back; BPW bands of one k-point per block
 // (V is loaded once per block).
-// Waves per SIMD the x pass is compiled for: 3 (complex64 at 4 waves
-// measured slower).
-template <typename R> constexpr int kXWaves = 3;
+// Waves per SIMD the x pass is compiled for (register budget per lane
+// 512 / waves): the most that each length fits without spilling.
+template <typename R, int N> constexpr int kXWaves = (sizeof(R) == 8 ? 4 : 5) - (N >= 144 ? 1 : 0);
 
 template <typename R, int N, int BPW>
-__global__ __launch_bounds__(Split<N>::NT) __attribute__((amdgpu_waves_per_eu(kXWaves<R>))) void k_px_vmul2(PwBatch B, cx_t<R>* __restrict__ P, int64_t nyz,
+__global__ __launch_bounds__(Split<N>::NT) __attribute__((amdgpu_waves_per_eu(kXWaves<R, N>))) void k_px_vmul2(PwBatch B, cx_t<R>* __restrict__ P, int64_t nyz,
                                                           const cx_t<R>* __restrict__ tw,
                                                           const R* __restrict__ vloc, R vscale) {
   using C = cx_t<R>;
