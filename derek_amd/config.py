@@ -82,7 +82,9 @@ class SystemSpec:
     def lengths(self):
         return [self.length] * 3
 
-    def to_info(self, energy_tol=1e-7, max_scf=40, precision="fp64"):
+    def to_info(self, energy_tol=1e-7, max_scf=40, precision="fp64", extra=()):
+        """The system as a reference-style .info file; ``extra`` adds
+        ``key : value`` lines (e.g. ("write_density", "true"))."""
         lines = [
             "! %s: generated benchmark system" % self.name,
             "unit_energy : Ha",
@@ -99,6 +101,7 @@ class SystemSpec:
             "cos_pot_amp : %r" % self.cos_amp,
             "wfn_seed : %d" % self.seed,
             "precision : %s" % precision,
+        ] + ["%s : %s" % kv for kv in extra] + [
             "$DATA lattice",
             " %r 0.0 0.0" % self.length,
             " 0.0 %r 0.0" % self.length,

@@ -7,6 +7,7 @@
 
 #include "basis.hpp"
 #include "context.hpp"
+#include "interchange.hpp"
 #include "scf.hpp"
 #include "subspace.hpp"
 
@@ -29,6 +30,9 @@ template <typename F> int32_t guard(F&& f) {
   } catch (const StateError& e) {
     g_last_error = e.what();
     return DEREK_ERR_STATE;
+  } catch (const IoError& e) {
+    g_last_error = e.what();
+    return DEREK_ERR_IO;
   } catch (const std::exception& e) {
     g_last_error = e.what();
     return DEREK_ERR_HIP;
@@ -348,6 +352,72 @@ int32_t derek_basis_gsq(const double inv[9], const int32_t dims[3], double* gsq)
     check_dims(dims);
     const int d[3] = {dims[0], dims[1], dims[2]};
     basis::gsq(inv, d, gsq);
+  });
+}
+
+int32_t derek_basis_std_index(const int32_t fine_dims[3], const int32_t std_dims[3], int64_t npw,
+                              const int32_t* fft_index, int64_t* std_index) {
+  return guard([&] {
+    DEREK_REQUIRE(npw >= 0 && (npw == 0 || (fft_index && std_index)), "null argument");
+    check_dims(fine_dims);
+    check_dims(std_dims);
+    const int f[3] = {fine_dims[0], fine_dims[1], fine_dims[2]};
+    const int d[3] = {std_dims[0], std_dims[1], std_dims[2]};
+    const int64_t ng = (int64_t)f[0] * f[1] * f[2];
+    for (int64_t p = 0; p < npw; ++p) DEREK_REQUIRE(fft_index[p] >= 0 && fft_index[p] < ng, "FFT bin outside the grid");
+    basis::std_index(f, d, npw, fft_index, std_index);
+  });
+}
+
+// ---- interchange files (host only)
+int32_t derek_io_pot_write(const char* path, const int32_t std_dims[3], int64_t nfine, const double* nc_pot,
+                           int32_t allocated) {
+  return guard([&] {
+    DEREK_REQUIRE(path, "null path");
+    pot_write(path, std_dims, nfine, nc_pot, allocated != 0);
+  });
+}
+
+int32_t derek_io_pot_read(const char* path, const int32_t std_dims[3], int64_t nfine, double* nc_pot,
+                          int32_t* allocated) {
+  return guard([&] {
+    DEREK_REQUIRE(path, "null path");
+    const bool a = pot_read(path, std_dims, nfine, nc_pot);
+    if (allocated) *allocated = a ? 1 : 0;
+  });
+}
+
+int32_t derek_io_den_write(const char* path, const int32_t std_dims[3], int64_t nfine, const double* charge,
+                           const double* spin, int32_t allocated) {
+  return guard([&] {
+    DEREK_REQUIRE(path, "null path");
+    den_write(path, std_dims, nfine, charge, spin, allocated != 0);
+  });
+}
+
+int32_t derek_io_den_read(const char* path, const int32_t std_dims[3], int64_t nfine, double* charge, double* spin,
+                          int32_t* allocated) {
+  return guard([&] {
+    DEREK_REQUIRE(path, "null path");
+    const bool a = den_read(path, std_dims, nfine, charge, spin);
+    if (allocated) *allocated = a ? 1 : 0;
+  });
+}
+
+int32_t derek_io_wvfn_write(const char* path, const int32_t std_dims[3], int32_t nk, int32_t nb,
+                            const int64_t* npw, const int64_t* std_index, const double* psi, int32_t allocated) {
+  return guard([&] {
+    DEREK_REQUIRE(path, "null path");
+    wvfn_write(path, std_dims, nk, nb, npw, std_index, psi, allocated != 0);
+  });
+}
+
+int32_t derek_io_wvfn_read(const char* path, const int32_t std_dims[3], int32_t nk, int32_t nb, const int64_t* npw,
+                           const int64_t* std_index, double* psi, int32_t* allocated) {
+  return guard([&] {
+    DEREK_REQUIRE(path, "null path");
+    const bool a = wvfn_read(path, std_dims, nk, nb, npw, std_index, psi);
+    if (allocated) *allocated = a ? 1 : 0;
   });
 }
 
@@ -776,6 +846,13 @@ int32_t derek_hip_get_density(derek_ctx* c, double* rho) {
   return guard([&] {
     DEREK_REQUIRE(c && rho, "null argument");
     scf_get(c, 2, rho);
+  });
+}
+
+int32_t derek_hip_get_potential(derek_ctx* c, double* v) {
+  return guard([&] {
+    DEREK_REQUIRE(c && v, "null argument");
+    scf_get(c, 3, v);
   });
 }
 

@@ -217,5 +217,23 @@ void gaussian_pot(const double cell[9], const int dims[3], int nwells, const dou
     for (int64_t p = 0; p < N; ++p) v[p] += cos_amp * std::cos(2.0 * kPi * r[p] / L[0]);
 }
 
+void std_index(const int fine[3], const int std_dims[3], int64_t n, const int32_t* fft_index, int64_t* out) {
+  const int64_t nyz = (int64_t)fine[1] * fine[2];
+  for (int64_t p = 0; p < n; ++p) {
+    const int64_t b = fft_index[p];
+    const int64_t bin[3] = {b / nyz, (b / fine[2]) % fine[1], b % fine[2]};
+    int64_t pos = 0;
+    bool inside = true;
+    for (int a = 0; a < 3; ++a) {
+      // FFT bin -> Miller index in [-n/2, n - n/2 - 1] of the fine grid
+      const int64_t m = bin[a] < fine[a] - fine[a] / 2 ? bin[a] : bin[a] - fine[a];
+      const int64_t lo = -(std_dims[a] / 2), hi = std_dims[a] - std_dims[a] / 2 - 1;
+      if (m < lo || m > hi) inside = false;
+      pos = pos * std_dims[a] + (m - lo);
+    }
+    out[p] = inside ? pos : -1;
+  }
+}
+
 }  // namespace basis
 }  // namespace derek

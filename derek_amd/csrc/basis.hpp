@@ -24,5 +24,9 @@ void gsq(const double inv[9], const int dims[3], double* out);
 void gaussian_pot(const double cell[9], const int dims[3], int nwells, const double* centres,
                   const double* heights, const double* widths, double cos_amp, double* v);
 
+// Position on the standard grid (basis order: x outer, z inner, m from
+// -n/2 to n-n/2-1, basis.f90:124-135) of each fine-grid FFT bin; -1 where
+// the Miller index lies outside the standard grid.
+void std_index(const int fine[3], const int std_dims[3], int64_t n, const int32_t* fft_index, int64_t* out);
 }  // namespace basis
 }  // namespace derek

@@ -547,6 +547,9 @@ void scf_get(derek_ctx* c, int what, double* out) {
   if (what == 2) {
     if (!E.have_rho_out) throw StateError("no density computed yet");
     DEREK_HIP(hipMemcpyAsync(out, E.rho_out, c->N * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  } else if (what == 3) {
+    if (!c->vloc_set) throw StateError("no potential set yet");
+    DEREK_HIP(hipMemcpyAsync(out, c->vloc, c->N * sizeof(double), hipMemcpyDeviceToHost, c->s));
   } else {
     const double* src = what == 0 ? E.eps : E.occ;
     DEREK_HIP(hipMemcpyAsync(out, src, (size_t)c->nk * c->nb * sizeof(double), hipMemcpyDeviceToHost, c->s));

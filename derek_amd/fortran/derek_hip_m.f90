@@ -108,6 +108,77 @@ module derek_hip_m
        integer(c_int64_t) :: npw
      end function derek_basis_sphere
 
+     integer(c_int32_t) function derek_basis_std_index(fine_dims, std_dims, npw, fft_index, std_index) &
+          bind(C, name='derek_basis_std_index')
+       import :: c_int32_t, c_int64_t
+       integer(c_int32_t), dimension(*) :: fine_dims, std_dims, fft_index
+       integer(c_int64_t), value :: npw
+       integer(c_int64_t), dimension(*) :: std_index
+     end function derek_basis_std_index
+
+     ! interchange files (pot.f90:194-224,303-340; density.f90:133-163;
+     ! wave.f90:896-926); path is a NUL-terminated name
+     integer(c_int32_t) function derek_io_pot_write(path, std_dims, nfine, nc_pot, allocated) &
+          bind(C, name='derek_io_pot_write')
+       import :: c_int32_t, c_int64_t, c_char, c_double_complex
+       character(kind=c_char), dimension(*) :: path
+       integer(c_int32_t), dimension(*) :: std_dims
+       integer(c_int64_t), value :: nfine
+       complex(c_double_complex), dimension(*) :: nc_pot
+       integer(c_int32_t), value :: allocated
+     end function derek_io_pot_write
+
+     integer(c_int32_t) function derek_io_pot_read(path, std_dims, nfine, nc_pot, allocated) &
+          bind(C, name='derek_io_pot_read')
+       import :: c_int32_t, c_int64_t, c_char, c_double_complex
+       character(kind=c_char), dimension(*) :: path
+       integer(c_int32_t), dimension(*) :: std_dims
+       integer(c_int64_t), value :: nfine
+       complex(c_double_complex), dimension(*) :: nc_pot
+       integer(c_int32_t) :: allocated
+     end function derek_io_pot_read
+
+     integer(c_int32_t) function derek_io_den_write(path, std_dims, nfine, charge, spin, allocated) &
+          bind(C, name='derek_io_den_write')
+       import :: c_int32_t, c_int64_t, c_char, c_double_complex
+       character(kind=c_char), dimension(*) :: path
+       integer(c_int32_t), dimension(*) :: std_dims
+       integer(c_int64_t), value :: nfine
+       complex(c_double_complex), dimension(*) :: charge, spin
+       integer(c_int32_t), value :: allocated
+     end function derek_io_den_write
+
+     integer(c_int32_t) function derek_io_den_read(path, std_dims, nfine, charge, spin, allocated) &
+          bind(C, name='derek_io_den_read')
+       import :: c_int32_t, c_int64_t, c_char, c_double_complex
+       character(kind=c_char), dimension(*) :: path
+       integer(c_int32_t), dimension(*) :: std_dims
+       integer(c_int64_t), value :: nfine
+       complex(c_double_complex), dimension(*) :: charge, spin
+       integer(c_int32_t) :: allocated
+     end function derek_io_den_read
+
+     integer(c_int32_t) function derek_io_wvfn_write(path, std_dims, nk, nb, npw, std_index, psi, allocated) &
+          bind(C, name='derek_io_wvfn_write')
+       import :: c_int32_t, c_int64_t, c_char, c_double_complex
+       character(kind=c_char), dimension(*) :: path
+       integer(c_int32_t), dimension(*) :: std_dims
+       integer(c_int32_t), value :: nk, nb, allocated
+       integer(c_int64_t), dimension(*) :: npw, std_index
+       complex(c_double_complex), dimension(*) :: psi
+     end function derek_io_wvfn_write
+
+     integer(c_int32_t) function derek_io_wvfn_read(path, std_dims, nk, nb, npw, std_index, psi, allocated) &
+          bind(C, name='derek_io_wvfn_read')
+       import :: c_int32_t, c_int64_t, c_char, c_double_complex
+       character(kind=c_char), dimension(*) :: path
+       integer(c_int32_t), dimension(*) :: std_dims
+       integer(c_int32_t), value :: nk, nb
+       integer(c_int64_t), dimension(*) :: npw, std_index
+       complex(c_double_complex), dimension(*) :: psi
+       integer(c_int32_t) :: allocated
+     end function derek_io_wvfn_read
+
      integer(c_int32_t) function derek_basis_gsq(inv_cell, dims, gsq) bind(C, name='derek_basis_gsq')
        import :: c_int32_t, c_double
        real(c_double), dimension(*) :: inv_cell, gsq
@@ -262,6 +333,12 @@ module derek_hip_m
        real(c_double), dimension(*) :: rho
      end function derek_hip_get_density
 
+     integer(c_int32_t) function derek_hip_get_potential(ctx, v) bind(C, name='derek_hip_get_potential')
+       import :: c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       real(c_double), dimension(*) :: v
+     end function derek_hip_get_potential
+
      integer(c_int32_t) function derek_hip_get_psi(ctx, k, psi) bind(C, name='derek_hip_get_psi')
        import :: c_int32_t, c_double_complex, c_ptr
        type(c_ptr), value :: ctx
@@ -288,7 +365,9 @@ module derek_hip_m
        derek_hip_set_vloc, derek_hip_set_vext, derek_hip_fft3d, derek_hip_apply_h, derek_hip_overlap, &
        derek_hip_rotate, derek_hip_density, derek_hip_lda, derek_hip_hartree, derek_hip_scf_init, &
        derek_hip_scf_step, derek_hip_scf_polish, derek_hip_get_eigenvalues, derek_hip_get_occupations, &
-       derek_hip_get_density, derek_hip_get_psi, derek_hip_put_psi, derek_hip_sync
+       derek_hip_get_density, derek_hip_get_psi, derek_hip_put_psi, derek_hip_sync, derek_hip_get_potential, &
+       derek_basis_std_index, derek_io_pot_write, derek_io_pot_read, derek_io_den_write, derek_io_den_read, &
+       derek_io_wvfn_write, derek_io_wvfn_read
 
 contains
 

@@ -10,7 +10,10 @@
 ! for the keys the SCF needs: cut_off_energy, g_fine_scale, kpt_mp_grid,
 ! n_electrons, electronic_temp (Kelvin, io.f90:340), energy_tol, max_scf,
 ! unit_energy / unit_length (Ha and bohr only: their factors are exactly 1),
-! plus nbands, cos_pot_amp, wfn_seed and precision.
+! plus nbands, cos_pot_amp, wfn_seed and precision; external_pot (a preset
+! name, or a <file>.potex whose potential replaces the wells, pot.f90:167-184)
+! and write_potential / write_density / write_wvfn, which write <seed>.pot,
+! <seed>.den and <seed>.wvfn after the SCF (state.f90:350-507).
 ! The loop, its table and the stop test |E_i - E_{i-1}| < energy_tol follow
 ! electronic_scf (electronic.f90:86-175).
 !-------------------------------------------------------------------------------
@@ -31,6 +34,14 @@ program derek_scf
   integer(c_int32_t) :: std_dims(3), fine_dims(3), nk, k, nwells = 0, iscf
   integer(c_int64_t) :: npw, max_npw, ngrid
   logical :: check_only = .false., have_cell = .false.
+  logical :: write_pot = .false., write_den = .false., write_wfn = .false., have_file
+  character(len=256) :: ext_pot = 'gaussian_pot'
+  complex(c_double_complex), allocatable :: ncpot(:, :, :), charge(:), psi_all(:)
+  real(c_double), allocatable :: rho(:)
+  integer(c_int64_t), allocatable :: npw_k(:), std_idx(:)
+  integer(c_int32_t), allocatable :: fidx(:)
+  integer(c_int32_t) :: alloc_flag
+  integer(c_int64_t) :: off
   integer :: u, ios, i, n
   type(c_ptr) :: ctx
   type(derek_dims) :: dims
@@ -81,6 +92,10 @@ program derek_scf
      case ('max_scf'); read(val, *) max_scf
      case ('cos_pot_amp'); read(val, *) cos_amp
      case ('wfn_seed'); read(val, *) wfn_seed
+     case ('external_pot'); ext_pot = val
+     case ('write_potential'); write_pot = truth(val)
+     case ('write_density'); write_den = truth(val)
+     case ('write_wvfn'); write_wfn = truth(val)
      case ('precision')
         if (trim(lower(val)) == 'mixed') prec = DEREK_MIXED
      case ('unit_energy')
@@ -93,6 +108,12 @@ program derek_scf
   end do
   close(u)
   if (.not. have_cell) call fail(' No lattice provided')
+  ! external_pot other than a preset names a .potex file (pot.f90:167-176)
+  if (trim(lower(ext_pot)) /= 'gaussian_pot') then
+     if (index(trim(ext_pot), '.potex') == 0) call fail('potential naming convension not correct')
+     inquire(file=trim(ext_pot), exist=have_file)
+     if (.not. have_file) call fail('file '//trim(ext_pot)//' does not exist')
+  end if
   if (nbands <= 0) nbands = int(nelec / 2.0d0) + 4
 
   call derek_check(derek_basis_grid_dims(cell, ecut, g_fine, std_dims, fine_dims), 'basis_grid_dims')
@@ -131,8 +152,18 @@ program derek_scf
   call derek_check(derek_basis_gsq(inv_cell, fine_dims, gsq), 'basis_gsq')
   call derek_check(derek_hip_set_gsq(ctx, gsq), 'hip_set_gsq')
   n = size(heights)
-  call derek_check(derek_gaussian_pot(cell, fine_dims, n, centres, heights, widths, cos_amp, vext), &
-       'gaussian_pot')
+  select case (trim(lower(ext_pot)))
+  case ('gaussian_pot')
+     call derek_check(derek_gaussian_pot(cell, fine_dims, n, centres, heights, widths, cos_amp, vext), &
+          'gaussian_pot')
+  case default
+     ! a potential file replaces the preset (pot.f90:167-184)
+     allocate(ncpot(ngrid, 2, 2))
+     call derek_check(derek_io_pot_read(trim(ext_pot)//c_null_char, std_dims, ngrid, ncpot, alloc_flag), &
+          'io_pot_read '//trim(ext_pot))
+     vext = 0.5d0 * real(ncpot(:, 1, 1) + ncpot(:, 2, 2), c_double)
+     deallocate(ncpot)
+  end select
   call derek_check(derek_hip_set_vext(ctx, vext), 'hip_set_vext')
 
   par%nelec = nelec
@@ -177,6 +208,49 @@ program derek_scf
   write(*, '(a,f20.12)') ' Fermi level (Ha)       :', rep%fermi
   write(*, '(a,l2)') ' Converged              :', converged
   write(*, '(a,4f14.8)') ' lowest eigenvalues (Ha):', eig(1:min(4, nbands), 1)
+  if (write_pot) then
+     allocate(rho(ngrid), ncpot(ngrid, 2, 2))
+     call derek_check(derek_hip_get_potential(ctx, rho), 'hip_get_potential')
+     ncpot = (0.0d0, 0.0d0)
+     ncpot(:, 1, 1) = cmplx(rho, 0.0d0, c_double_complex)
+     ncpot(:, 2, 2) = ncpot(:, 1, 1)
+     call derek_check(derek_io_pot_write(trim(seed)//'.pot'//c_null_char, std_dims, ngrid, ncpot, 1), 'io_pot_write')
+     write(*, '(a)') ' Total potential to     : '//trim(seed)//'.pot'
+     deallocate(rho, ncpot)
+  end if
+  if (write_den) then
+     allocate(rho(ngrid), charge(ngrid))
+     call derek_check(derek_hip_get_density(ctx, rho), 'hip_get_density')
+     charge = cmplx(rho, 0.0d0, c_double_complex)
+     allocate(psi_all(3 * ngrid))
+     psi_all = (0.0d0, 0.0d0)
+     call derek_check(derek_io_den_write(trim(seed)//'.den'//c_null_char, std_dims, ngrid, charge, psi_all, 1), &
+          'io_den_write')
+     write(*, '(a)') ' Density to             : '//trim(seed)//'.den'
+     deallocate(rho, charge, psi_all)
+  end if
+  if (write_wfn) then
+     allocate(npw_k(nk))
+     do k = 1, nk
+        call derek_check(derek_basis_sphere(inv_cell, fine_dims, kfrac(k, :), ecut, 0_c_int64_t, npw_k(k), &
+             fft_dummy(), kin_dummy()), 'basis_sphere')
+     end do
+     allocate(std_idx(sum(npw_k)), psi_all(sum(npw_k) * nbands), fidx(max_npw))
+     off = 0
+     do k = 1, nk
+        call derek_check(derek_basis_sphere(inv_cell, fine_dims, kfrac(k, :), ecut, max_npw, npw, fidx, kin), &
+             'basis_sphere')
+        call derek_check(derek_basis_std_index(fine_dims, std_dims, npw, fidx, std_idx(off + 1:off + npw)), &
+             'basis_std_index')
+        if (any(std_idx(off + 1:off + npw) < 0)) call fail('plane wave outside the standard grid')
+        call derek_check(derek_hip_get_psi(ctx, k - 1, psi_all(off * nbands + 1:(off + npw) * nbands)), 'hip_get_psi')
+        off = off + npw
+     end do
+     call derek_check(derek_io_wvfn_write(trim(seed)//'.wvfn'//c_null_char, std_dims, nk, nbands, npw_k, std_idx, &
+          psi_all, 1), 'io_wvfn_write')
+     write(*, '(a)') ' Wavefunction to        : '//trim(seed)//'.wvfn'
+     deallocate(npw_k, std_idx, psi_all, fidx)
+  end if
   call derek_check(derek_hip_destroy(ctx), 'hip_destroy')
 11 format(1x,'|',T6,I05,T11,"|",T15,ES16.8E3,T31,"|",T51,'|',T53,f8.2,T61,'| <-- SCF')
 12 format(1x,'|',T6,I05,T11,"|",T15,ES16.8E3,T31,"|",T35,ES16.8E3,T51,'|',T53,f8.2,T61,'| <-- SCF')
@@ -192,6 +266,16 @@ contains
        if (t(j:j) >= 'A' .and. t(j:j) <= 'Z') t(j:j) = achar(iachar(t(j:j)) + 32)
     end do
   end function lower
+
+  logical function truth(v)
+    character(*), intent(in) :: v
+    select case (trim(lower(adjustl(v))))
+    case ('true', 't', '.true.', 'yes', 'on')
+       truth = .true.
+    case default
+       truth = .false.
+    end select
+  end function truth
 
   subroutine fail(msg)
     character(*), intent(in) :: msg

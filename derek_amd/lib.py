@@ -12,20 +12,22 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libderek_hip.so")
 
-DEREK_OK, DEREK_ERR_ARG, DEREK_ERR_HIP, DEREK_ERR_NUMERIC, DEREK_ERR_STATE = 0, 1, 2, 3, 4
+DEREK_OK, DEREK_ERR_ARG, DEREK_ERR_HIP, DEREK_ERR_NUMERIC, DEREK_ERR_STATE, DEREK_ERR_IO = 0, 1, 2, 3, 4, 5
 DEREK_FP64, DEREK_MIXED = 0, 1
 
 # every symbol declared in include/derek_hip.h
 EXPORTS = (
     "derek_hip_version", "derek_hip_last_error",
     "derek_basis_grid_dims", "derek_basis_inv_cell", "derek_basis_g_map", "derek_basis_real_map",
-    "derek_basis_kpoints", "derek_basis_sphere", "derek_basis_gsq", "derek_gaussian_pot",
+    "derek_basis_kpoints", "derek_basis_sphere", "derek_basis_gsq", "derek_basis_std_index", "derek_gaussian_pot",
+    "derek_io_pot_write", "derek_io_pot_read", "derek_io_den_write", "derek_io_den_read",
+    "derek_io_wvfn_write", "derek_io_wvfn_read",
     "derek_hip_init", "derek_hip_destroy", "derek_hip_set_basis", "derek_hip_set_gsq",
     "derek_hip_set_vloc", "derek_hip_set_vext",
     "derek_hip_fft3d", "derek_hip_apply_h", "derek_hip_overlap", "derek_hip_rotate",
     "derek_hip_density", "derek_hip_lda", "derek_hip_hartree", "derek_hip_heev", "derek_hip_hegv",
     "derek_hip_scf_init", "derek_hip_scf_step", "derek_hip_scf_polish", "derek_hip_get_eigenvalues",
-    "derek_hip_get_occupations", "derek_hip_get_density", "derek_hip_get_psi", "derek_hip_put_psi",
+    "derek_hip_get_occupations", "derek_hip_get_density", "derek_hip_get_potential", "derek_hip_get_psi", "derek_hip_put_psi",
     "derek_hip_apply_h_resident", "derek_hip_sync", "derek_hip_stream",
     "derek_hip_profile_enable", "derek_hip_profile_reset", "derek_hip_profile_read", "derek_hip_get_tuning",
 )
@@ -157,6 +159,89 @@ def sphere(inv, dims, kfrac, ecut):
     check(load().derek_basis_sphere(ptr(inv9), d, ptr(kf), C.c_double(ecut), C.c_int64(n.value), C.byref(n),
                                     idx.ctypes.data_as(C.POINTER(C.c_int32)), ptr(kin)))
     return idx, kin
+
+
+def std_index(fine_dims, std_dims, fft_index):
+    """Standard-grid positions (basis order, -1 outside) of fine-grid FFT bins."""
+    idx = np.ascontiguousarray(fft_index, dtype=np.int32)
+    out = np.zeros(len(idx), dtype=np.int64)
+    check(load().derek_basis_std_index((C.c_int32 * 3)(*fine_dims), (C.c_int32 * 3)(*std_dims),
+                                       C.c_int64(len(idx)), idx.ctypes.data_as(C.POINTER(C.c_int32)),
+                                       out.ctypes.data_as(C.POINTER(C.c_int64))))
+    return out
+
+
+# ---------------------------------------------------------------- interchange files (host)
+def _path(path):
+    return os.fsencode(path)
+
+
+def pot_write(path, std_dims, nc_pot, allocated=True):
+    """Potential record: nc_pot is complex (nfine, 2, 2)."""
+    a = np.asarray(nc_pot, dtype=np.complex128)
+    flat = np.ascontiguousarray(a.ravel(order="F"))
+    check(load().derek_io_pot_write(_path(path), (C.c_int32 * 3)(*std_dims), C.c_int64(a.shape[0]), ptr(flat),
+                                    C.c_int32(1 if allocated else 0)))
+
+
+def pot_read(path, std_dims, nfine):
+    """-> (nc_pot complex (nfine, 2, 2), allocated)."""
+    flat = np.zeros(4 * nfine, dtype=np.complex128)
+    alloc = C.c_int32()
+    check(load().derek_io_pot_read(_path(path), (C.c_int32 * 3)(*std_dims), C.c_int64(nfine), ptr(flat),
+                                   C.byref(alloc)))
+    return flat.reshape((nfine, 2, 2), order="F"), bool(alloc.value)
+
+
+def den_write(path, std_dims, charge, spin=None, allocated=True):
+    """Density record: charge complex (nfine,), spin complex (nfine, 3) or None (zeros)."""
+    ch = np.ascontiguousarray(charge, dtype=np.complex128)
+    sp = None if spin is None else np.ascontiguousarray(np.asarray(spin, dtype=np.complex128).ravel(order="F"))
+    check(load().derek_io_den_write(_path(path), (C.c_int32 * 3)(*std_dims), C.c_int64(len(ch)), ptr(ch), ptr(sp),
+                                    C.c_int32(1 if allocated else 0)))
+
+
+def den_read(path, std_dims, nfine):
+    """-> (charge (nfine,), spin (nfine, 3), allocated)."""
+    ch = np.zeros(nfine, dtype=np.complex128)
+    sp = np.zeros(3 * nfine, dtype=np.complex128)
+    alloc = C.c_int32()
+    check(load().derek_io_den_read(_path(path), (C.c_int32 * 3)(*std_dims), C.c_int64(nfine), ptr(ch), ptr(sp),
+                                   C.byref(alloc)))
+    return ch, sp.reshape((nfine, 3), order="F"), bool(alloc.value)
+
+
+def _sphere_args(psis, std_idx):
+    npw = np.array([len(i) for i in std_idx], dtype=np.int64)
+    idx = np.ascontiguousarray(np.concatenate([np.asarray(i, dtype=np.int64) for i in std_idx]))
+    return npw, idx
+
+
+def wvfn_write(path, std_dims, psis, std_idx, allocated=True):
+    """Wavefunction record from per-k sphere blocks psis[k] (npw_k x nb)."""
+    nb = psis[0].shape[1]
+    npw, idx = _sphere_args(psis, std_idx)
+    flat = np.ascontiguousarray(np.concatenate([np.asarray(p, dtype=np.complex128).ravel(order="F")
+                                                for p in psis]))
+    check(load().derek_io_wvfn_write(_path(path), (C.c_int32 * 3)(*std_dims), C.c_int32(len(psis)), C.c_int32(nb),
+                                     npw.ctypes.data_as(C.POINTER(C.c_int64)),
+                                     idx.ctypes.data_as(C.POINTER(C.c_int64)), ptr(flat),
+                                     C.c_int32(1 if allocated else 0)))
+
+
+def wvfn_read(path, std_dims, std_idx, nb):
+    """-> (list of per-k sphere blocks npw_k x nb, allocated)."""
+    npw, idx = _sphere_args(None, std_idx)
+    flat = np.zeros(int(npw.sum()) * nb, dtype=np.complex128)
+    alloc = C.c_int32()
+    check(load().derek_io_wvfn_read(_path(path), (C.c_int32 * 3)(*std_dims), C.c_int32(len(std_idx)), C.c_int32(nb),
+                                    npw.ctypes.data_as(C.POINTER(C.c_int64)),
+                                    idx.ctypes.data_as(C.POINTER(C.c_int64)), ptr(flat), C.byref(alloc)))
+    out, o = [], 0
+    for n in npw:
+        out.append(flat[o:o + n * nb].reshape((n, nb), order="F"))
+        o += n * nb
+    return out, bool(alloc.value)
 
 
 def gsq(inv, dims):
@@ -357,6 +442,11 @@ class Context:
         r = np.zeros(self.N)
         check(self.lib.derek_hip_get_density(self.h, ptr(r)))
         return r
+
+    def get_potential(self):
+        out = np.zeros(self.N)
+        check(self.lib.derek_hip_get_potential(self.h, ptr(out)))
+        return out
 
     def get_psi(self, k):
         out = np.zeros((self.nbands, self.npw[k]), dtype=np.complex128)

@@ -30,6 +30,7 @@ extern "C" {
 #define DEREK_ERR_HIP 2     /* HIP runtime / device failure */
 #define DEREK_ERR_NUMERIC 3 /* e.g. overlap matrix not positive definite */
 #define DEREK_ERR_STATE 4   /* call out of order (e.g. basis not set) */
+#define DEREK_ERR_IO 5      /* file cannot be opened / read / written, or malformed record */
 
 #define DEREK_FP64 0  /* complex128 wavefunctions, FP64 everywhere */
 #define DEREK_MIXED 1 /* complex64 wavefunctions, FFTs, H|psi> and density; FP32 MFMA overlaps and
@@ -107,6 +108,12 @@ int32_t derek_basis_sphere(const double inv_cell[9], const int32_t dims[3], cons
                            int64_t cap, int64_t* npw, int32_t* fft_index, double* kin);
 /* |G|^2 of every FFT bin of the grid (flat layout). */
 int32_t derek_basis_gsq(const double inv_cell[9], const int32_t dims[3], double* gsq);
+/* Standard-grid position (0-based, basis order of basis.f90:124-135: x
+ * outer, z inner, m from -n/2 to n-n/2-1) of each plane wave given by its
+ * fine-grid FFT bin; -1 where it lies outside the standard grid.  Places
+ * sphere coefficients in the reference's coeff(gvec, ...) layout. */
+int32_t derek_basis_std_index(const int32_t fine_dims[3], const int32_t std_dims[3], int64_t npw,
+                              const int32_t* fft_index, int64_t* std_index);
 /* External potential of $DATA gaussian_pot wells (io.f90:1888-1922; the
  * reference's pot_external_pot sets it to zero, pot.f90:165-166) at the
  * real_fine_grid_points: V = sum_i h_i sum_R exp(-|r-c_i-R|^2/(2 w_i^2))
@@ -116,6 +123,45 @@ int32_t derek_gaussian_pot(const double cell[9], const int32_t dims[3], int32_t 
 
 /* ---- context -------------------------------------------------------------- */
 /* Allocates the device state (replaces fft_init's plans, fft.f90:139-219). */
+/* ---- interchange files (host only; no context needed)
+ * Fortran unformatted sequential files as the reference's gfortran build
+ * writes them (-fconvert=big-endian, Makefile:86): one record per file (the
+ * reference writes each type through a defined unformatted I/O procedure,
+ * whose child statements continue the parent record), 4-byte big-endian
+ * record markers, records over 2147483639 bytes split into subrecords the
+ * way gfortran does.  The header holds the STANDARD grid dimensions (as
+ * pot_write / pot_read do, pot.f90:216-218,326-332) and the arrays are
+ * fine-grid sized.  Readers fail with DEREK_ERR_ARG "fine grid mismatch"
+ * when the header differs from std_dims, as pot_read does.
+ * Complex arrays are interleaved (re, im) doubles, Fortran order. */
+/* potential record (write(unit) pot -> pot_write, pot.f90:194-224): header,
+ * nc_pot(nfine, 2, 2), allocated.  A scalar potential V is nc_pot(:,1,1) =
+ * nc_pot(:,2,2) = V, off-diagonal blocks 0 (xsf.f90:43-45 reads back V as
+ * the mean of the diagonal).  Used for .pot and .potex (external_pot, read
+ * with pot_read at pot.f90:167-184). */
+int32_t derek_io_pot_write(const char* path, const int32_t std_dims[3], int64_t nfine, const double* nc_pot,
+                           int32_t allocated);
+int32_t derek_io_pot_read(const char* path, const int32_t std_dims[3], int64_t nfine, double* nc_pot,
+                          int32_t* allocated);
+/* density record (density_write, density.f90:133-163): header,
+ * charge(nfine), spin(nfine, 3) (spin may be NULL: written as zeros /
+ * skipped on reading), allocated. */
+int32_t derek_io_den_write(const char* path, const int32_t std_dims[3], int64_t nfine, const double* charge,
+                           const double* spin, int32_t allocated);
+int32_t derek_io_den_read(const char* path, const int32_t std_dims[3], int64_t nfine, double* charge, double* spin,
+                          int32_t* allocated);
+/* wavefunction record (wave_write, wave.f90:896-926): header,
+ * coeff(nstd, nk, nb, 2), nbands, kpts, allocated.  The caller passes the
+ * sphere coefficients (per k-point npw[k] x nb column-major, k-points one
+ * after the other) and their standard-grid positions (derek_basis_std_index,
+ * per k-point, concatenated); the writer scatters them into spinor 1 of
+ * the standard-grid array (zeros elsewhere and in spinor 2), the reader
+ * gathers them back and checks nbands and kpts. */
+int32_t derek_io_wvfn_write(const char* path, const int32_t std_dims[3], int32_t nk, int32_t nb,
+                            const int64_t* npw, const int64_t* std_index, const double* psi, int32_t allocated);
+int32_t derek_io_wvfn_read(const char* path, const int32_t std_dims[3], int32_t nk, int32_t nb, const int64_t* npw,
+                           const int64_t* std_index, double* psi, int32_t* allocated);
+
 int32_t derek_hip_init(derek_ctx** ctx, const derek_dims* dims);
 int32_t derek_hip_destroy(derek_ctx* ctx);
 /* Plane-wave set of k-point k (0-based) and its weight. */
@@ -185,6 +231,10 @@ int32_t derek_hip_get_eigenvalues(derek_ctx* ctx, double* eig);
 int32_t derek_hip_get_occupations(derek_ctx* ctx, double* occ);
 /* rho_out of the last step (N_g doubles) */
 int32_t derek_hip_get_density(derek_ctx* ctx, double* rho);
+/* The total local potential V_ext + V_H + V_xc (fine grid, Ha) that the next
+ * H|psi> uses: the reference's tot_pot (state.f90:350-367 writes it as
+ * <seed>.pot). */
+int32_t derek_hip_get_potential(derek_ctx* ctx, double* v);
 /* resident wavefunctions of k (npw_k x nbands complex, FP64 on the host) */
 int32_t derek_hip_get_psi(derek_ctx* ctx, int32_t k, double* psi);
 int32_t derek_hip_put_psi(derek_ctx* ctx, int32_t k, const double* psi);

@@ -19,8 +19,8 @@ def driver():
     return EXE
 
 
-def write_info(tmp_path, spec, **kw):
-    seed = str(tmp_path / spec.name)
+def write_info(tmp_path, spec, name=None, **kw):
+    seed = str(tmp_path / (name or spec.name))
     open(seed + ".info", "w").write(spec.to_info(**kw))
     return seed
 
@@ -43,6 +43,45 @@ def test_missing_lattice_is_an_error(tmp_path):
     open(seed + ".info", "w").write("cut_off_energy : 5\nunit_energy : Ha\nunit_length : bohr\n")
     r = subprocess.run([driver(), seed, "-c"], capture_output=True, text=True)
     assert r.returncode != 0 and "No lattice provided" in r.stdout
+
+
+@pytest.mark.parametrize("val,msg", [("wells.dat", "potential naming convension not correct"),
+                                     ("missing.potex", "does not exist")])
+def test_external_potential_file_errors(tmp_path, val, msg):
+    spec = config.benchmark(1)
+    seed = write_info(tmp_path, spec, extra=[("external_pot", str(tmp_path / val))])
+    r = subprocess.run([driver(), seed, "-c"], capture_output=True, text=True)
+    assert r.returncode != 0 and msg in r.stdout
+
+
+@pytest.mark.gpu
+def test_fortran_writes_and_reads_interchange_files(tmp_path):
+    """write_potential / write_density / write_wvfn after the SCF, and a
+    second run whose external potential comes from a .potex file."""
+    import numpy as np
+    from derek_amd import interchange, lib
+    spec = config.benchmark(1)
+    su = Setup(spec)
+    flags = [("write_potential", "true"), ("write_density", "true"), ("write_wvfn", "true")]
+    seed = write_info(tmp_path, spec, energy_tol=1e-10, max_scf=60, extra=flags)
+    out = subprocess.run([driver(), seed], capture_output=True, text=True, check=True, timeout=300).stdout
+    print(out)
+    e1 = float(re.search(r"Final energy \(Ha\)\s*:\s*(\S+)", out).group(1))
+    rho = interchange.read_density(seed + ".den", su)
+    assert abs(rho.sum() * su.volume / su.N - spec.nelec) < 1e-8
+    v = interchange.read_potential(seed + ".pot", su)
+    assert np.isfinite(v).all() and v.std() > 0
+    psis = interchange.read_wvfn(seed + ".wvfn", su, spec.nbands)
+    for p in psis:  # the converged bands are orthonormal
+        assert np.abs(p.conj().T @ p - np.eye(spec.nbands)).max() < 1e-10
+    # the wells as a .potex file: the same SCF
+    potex = str(tmp_path / "wells.potex")
+    interchange.write_potential(potex, su, su.vext)
+    seed2 = write_info(tmp_path, spec, name="from_potex", energy_tol=1e-10, max_scf=60,
+                       extra=[("external_pot", potex)])
+    out2 = subprocess.run([driver(), seed2], capture_output=True, text=True, check=True, timeout=300).stdout
+    e2 = float(re.search(r"Final energy \(Ha\)\s*:\s*(\S+)", out2).group(1))
+    assert abs(e1 - e2) < 1e-10
 
 
 @pytest.mark.gpu
