@@ -819,6 +819,15 @@ int32_t derek_hip_scf_step(derek_ctx* c, derek_scf_report* rep) {
   });
 }
 
+int32_t derek_hip_nscf(derek_ctx* c, int32_t max_iter, double res_tol, double* max_res) {
+  return guard([&] {
+    DEREK_REQUIRE(c, "null context");
+    DEREK_REQUIRE(max_iter >= 1 && res_tol >= 0.0, "max_iter must be >= 1 and res_tol >= 0");
+    const double r = scf_nscf(c, max_iter, res_tol);
+    if (max_res) *max_res = r;
+  });
+}
+
 int32_t derek_hip_scf_polish(derek_ctx* c, int32_t max_iter, double res_tol, double* max_res) {
   return guard([&] {
     DEREK_REQUIRE(c, "null context");

@@ -542,6 +542,27 @@ double scf_polish(derek_ctx* c, int max_iter, double res_tol) {
   return r;
 }
 
+// Eigenpairs at a fixed potential: Davidson on every k-point at the total
+// potential set with set_vloc (no density or potential update), until every
+// residual is below res_tol or max_iter iterations; the non-self-consistent
+// part of the spectral tasks.
+double scf_nscf(derek_ctx* c, int max_iter, double res_tol) {
+  ScfEngine& E = engine(c);
+  if (!c->vloc_set) throw StateError("non-self-consistent run needs the potential (set_vloc)");
+  if (E.fp32) convert_real_f(c->vloc32, c->vloc, c->N, c->s);
+  for (int k0 = 0; k0 < c->nk; k0 += E.kc) {
+    const int nkc = std::min(E.kc, c->nk - k0);
+    if (E.fp32) davidson_chunk<float>(c, k0, nkc, max_iter, res_tol);
+    else davidson_chunk<double>(c, k0, nkc, max_iter, res_tol);
+  }
+  std::vector<double> h((size_t)c->nk * c->nb);
+  DEREK_HIP(hipMemcpyAsync(h.data(), E.rn, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  c->sync();
+  double r = 0.0;
+  for (double x : h) r = std::max(r, x);
+  return r;
+}
+
 void scf_get(derek_ctx* c, int what, double* out) {
   ScfEngine& E = engine(c);
   if (what == 2) {

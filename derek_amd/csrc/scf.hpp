@@ -7,6 +7,7 @@ namespace derek {
 void scf_init(derek_ctx* c, const derek_scf_params& p);
 void scf_step(derek_ctx* c, derek_scf_report& rep);
 double scf_polish(derek_ctx* c, int max_iter, double res_tol);
+double scf_nscf(derek_ctx* c, int max_iter, double res_tol);
 // what: 0 eigenvalues (nb x nk), 1 occupations, 2 rho_out (N)
 void scf_get(derek_ctx* c, int what, double* out);
 // resident wavefunctions of k as complex128 npw_k x nb; put = upload

@@ -315,6 +315,15 @@ module derek_hip_m
        real(c_double) :: max_res
      end function derek_hip_scf_polish
 
+     ! non-self-consistent eigenpairs at the potential of derek_hip_set_vloc
+     integer(c_int32_t) function derek_hip_nscf(ctx, max_iter, res_tol, max_res) bind(C, name='derek_hip_nscf')
+       import :: c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: max_iter
+       real(c_double), value :: res_tol
+       real(c_double) :: max_res
+     end function derek_hip_nscf
+
      integer(c_int32_t) function derek_hip_get_eigenvalues(ctx, eig) bind(C, name='derek_hip_get_eigenvalues')
        import :: c_int32_t, c_double, c_ptr
        type(c_ptr), value :: ctx
@@ -367,7 +376,7 @@ module derek_hip_m
        derek_hip_scf_step, derek_hip_scf_polish, derek_hip_get_eigenvalues, derek_hip_get_occupations, &
        derek_hip_get_density, derek_hip_get_psi, derek_hip_put_psi, derek_hip_sync, derek_hip_get_potential, &
        derek_basis_std_index, derek_io_pot_write, derek_io_pot_read, derek_io_den_write, derek_io_den_read, &
-       derek_io_wvfn_write, derek_io_wvfn_read
+       derek_io_wvfn_write, derek_io_wvfn_read, derek_hip_nscf
 
 contains
 

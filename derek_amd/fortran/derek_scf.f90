@@ -13,7 +13,13 @@
 ! plus nbands, cos_pot_amp, wfn_seed and precision; external_pot (a preset
 ! name, or a <file>.potex whose potential replaces the wells, pot.f90:167-184)
 ! and write_potential / write_density / write_wvfn, which write <seed>.pot,
-! <seed>.den and <seed>.wvfn after the SCF (state.f90:350-507).
+! <seed>.den and <seed>.wvfn after the SCF (state.f90:350-507); task
+! (singlepoint, dos, bandstructure; io.f90:518-527) with dos_kpt_mp_grid:
+! after the SCF the converged total potential is held fixed and the bands are
+! found without self-consistency on the DOS mesh (<seed>.spec eigenvalues,
+! <seed>.dos Gaussian-broadened density of states) or along the simple-cubic
+! path G-X-M-G-R (<seed>.bands); the reference's spec_dos / spec_bands are
+! empty (spectral.f90:33-44).
 ! The loop, its table and the stop test |E_i - E_{i-1}| < energy_tol follow
 ! electronic_scf (electronic.f90:86-175).
 !-------------------------------------------------------------------------------
@@ -35,7 +41,8 @@ program derek_scf
   integer(c_int64_t) :: npw, max_npw, ngrid
   logical :: check_only = .false., have_cell = .false.
   logical :: write_pot = .false., write_den = .false., write_wfn = .false., have_file
-  character(len=256) :: ext_pot = 'gaussian_pot'
+  character(len=256) :: ext_pot = 'gaussian_pot', task = 'singlepoint'
+  integer(c_int32_t) :: dos_mp(3) = (/11, 11, 11/)
   complex(c_double_complex), allocatable :: ncpot(:, :, :), charge(:), psi_all(:)
   real(c_double), allocatable :: rho(:)
   integer(c_int64_t), allocatable :: npw_k(:), std_idx(:)
@@ -93,6 +100,15 @@ program derek_scf
      case ('cos_pot_amp'); read(val, *) cos_amp
      case ('wfn_seed'); read(val, *) wfn_seed
      case ('external_pot'); ext_pot = val
+     case ('task')
+        task = lower(val)
+        select case (trim(task))
+        case ('singlepoint', 'bandstructure', 'dos')
+           continue
+        case default
+           call fail('Unknown task: '//trim(val))
+        end select
+     case ('dos_kpt_mp_grid'); read(val, *) dos_mp
      case ('write_potential'); write_pot = truth(val)
      case ('write_density'); write_den = truth(val)
      case ('write_wvfn'); write_wfn = truth(val)
@@ -251,6 +267,7 @@ program derek_scf
      write(*, '(a)') ' Wavefunction to        : '//trim(seed)//'.wvfn'
      deallocate(npw_k, std_idx, psi_all, fidx)
   end if
+  if (trim(task) == 'dos' .or. trim(task) == 'bandstructure') call spectral_task()
   call derek_check(derek_hip_destroy(ctx), 'hip_destroy')
 11 format(1x,'|',T6,I05,T11,"|",T15,ES16.8E3,T31,"|",T51,'|',T53,f8.2,T61,'| <-- SCF')
 12 format(1x,'|',T6,I05,T11,"|",T15,ES16.8E3,T31,"|",T35,ES16.8E3,T51,'|',T53,f8.2,T61,'| <-- SCF')
@@ -266,6 +283,108 @@ contains
        if (t(j:j) >= 'A' .and. t(j:j) <= 'Z') t(j:j) = achar(iachar(t(j:j)) + 32)
     end do
   end function lower
+
+  ! Bands at the converged total potential on the DOS mesh or the band path
+  subroutine spectral_task()
+    real(c_double), allocatable :: kf2(:, :), vtot(:), eig2(:, :), path_len(:), egrid(:), g(:)
+    integer(c_int32_t) :: nk2, k2, b, ie
+    integer(c_int64_t) :: npw2, maxnpw2
+    type(c_ptr) :: ctx2
+    type(derek_dims) :: dims2
+    real(c_double) :: res, emin, emax, sigma, x
+    real(c_double), parameter :: verts(3, 5) = reshape((/0.0d0, 0.0d0, 0.0d0, 0.5d0, 0.0d0, 0.0d0, &
+         0.5d0, 0.5d0, 0.0d0, 0.0d0, 0.0d0, 0.0d0, 0.5d0, 0.5d0, 0.5d0/), (/3, 5/))
+    integer, parameter :: nseg = 10
+    integer :: sv, su, j
+    allocate(vtot(ngrid))
+    call derek_check(derek_hip_get_potential(ctx, vtot), 'hip_get_potential')
+    if (trim(task) == 'dos') then
+       nk2 = dos_mp(1) * dos_mp(2) * dos_mp(3)
+       if (nk2 <= 0) call fail(' invalid dos_kpt_mp_grid')
+       allocate(kf2(nk2, 3))
+       call derek_check(derek_basis_kpoints(dos_mp, kf2, nk2), 'basis_kpoints')
+    else
+       nk2 = 4 * nseg + 1
+       allocate(kf2(nk2, 3), path_len(nk2))
+       kf2(1, :) = verts(:, 1)
+       path_len(1) = 0.0d0
+       k2 = 1
+       do sv = 1, 4
+          do j = 1, nseg
+             k2 = k2 + 1
+             kf2(k2, :) = verts(:, sv) + (verts(:, sv + 1) - verts(:, sv)) * (real(j, c_double) / nseg)
+             path_len(k2) = path_len(k2 - 1) + norm2(kf2(k2, :) - kf2(k2 - 1, :))
+          end do
+       end do
+    end if
+    maxnpw2 = 0
+    do k2 = 1, nk2
+       call derek_check(derek_basis_sphere(inv_cell, fine_dims, kf2(k2, :), ecut, 0_c_int64_t, npw2, fft_dummy(), &
+            kin_dummy()), 'basis_sphere')
+       maxnpw2 = max(maxnpw2, npw2)
+    end do
+    dims2 = dims
+    dims2%nk = nk2
+    dims2%max_npw = maxnpw2
+    call derek_check(derek_hip_init(ctx2, dims2), 'hip_init')
+    deallocate(fft_index, kin)
+    allocate(fft_index(maxnpw2), kin(maxnpw2), eig2(nbands, nk2))
+    do k2 = 1, nk2
+       call derek_check(derek_basis_sphere(inv_cell, fine_dims, kf2(k2, :), ecut, maxnpw2, npw2, fft_index, kin), &
+            'basis_sphere')
+       call derek_check(derek_hip_set_basis(ctx2, k2 - 1, npw2, fft_index, kin, 1.0d0 / nk2), 'hip_set_basis')
+    end do
+    call derek_check(derek_hip_set_gsq(ctx2, gsq), 'hip_set_gsq')
+    call derek_check(derek_hip_set_vext(ctx2, vext), 'hip_set_vext')
+    call derek_check(derek_hip_scf_init(ctx2, par), 'hip_scf_init')
+    call derek_check(derek_hip_set_vloc(ctx2, vtot), 'hip_set_vloc')
+    call derek_check(derek_hip_nscf(ctx2, 300, 1.0d-9, res), 'hip_nscf')
+    call derek_check(derek_hip_get_eigenvalues(ctx2, eig2), 'hip_get_eigenvalues')
+    call derek_check(derek_hip_destroy(ctx2), 'hip_destroy')
+    write(*, '(a,i8,a,es10.2)') ' Non-SCF k-points       :', nk2, '   max residual', res
+    if (trim(task) == 'dos') then
+       open(newunit=su, file=trim(seed)//'.spec', status='replace', action='write')
+       write(su, '(a)') '# non-self-consistent eigenvalues (Ha) at the converged potential'
+       write(su, '(a,3i5,a,i6,a,i5)') '# dos_kpt_mp_grid', dos_mp, '   k-points', nk2, '   bands', nbands
+       write(su, '(a,f20.12)') '# Fermi level (Ha)', rep%fermi
+       do k2 = 1, nk2
+          write(su, '(3f12.8,*(f16.10))') kf2(k2, :), eig2(:, k2)
+       end do
+       close(su)
+       ! Gaussian broadening, states per Ha per cell, both spin channels
+       sigma = 0.01d0
+       emin = minval(eig2) - 5.0d0 * sigma
+       emax = maxval(eig2) + 5.0d0 * sigma
+       allocate(egrid(2001), g(2001))
+       do ie = 1, 2001
+          egrid(ie) = emin + (emax - emin) * (ie - 1) / 2000.0d0
+          g(ie) = 0.0d0
+          do k2 = 1, nk2
+             do b = 1, nbands
+                x = (egrid(ie) - eig2(b, k2)) / sigma
+                g(ie) = g(ie) + 2.0d0 / nk2 * exp(-0.5d0 * x * x) / (sigma * sqrt(2.0d0 * acos(-1.0d0)))
+             end do
+          end do
+       end do
+       open(newunit=su, file=trim(seed)//'.dos', status='replace', action='write')
+       write(su, '(a,f8.4,a)') '# energy (Ha)   DOS (states/Ha/cell, both spins; Gaussian sigma', sigma, ' Ha)'
+       write(su, '(a,f20.12)') '# Fermi level (Ha)', rep%fermi
+       do ie = 1, 2001
+          write(su, '(f16.10,es20.10)') egrid(ie), g(ie)
+       end do
+       close(su)
+       write(*, '(a)') ' Spectral files         : '//trim(seed)//'.spec, '//trim(seed)//'.dos'
+    else
+       open(newunit=su, file=trim(seed)//'.bands', status='replace', action='write')
+       write(su, '(a)') '# band structure along G-X-M-G-R at the converged potential (Ha)'
+       write(su, '(a,f20.12)') '# Fermi level (Ha)', rep%fermi
+       do k2 = 1, nk2
+          write(su, '(f12.8,3f12.8,*(f16.10))') path_len(k2), kf2(k2, :), eig2(:, k2)
+       end do
+       close(su)
+       write(*, '(a)') ' Band structure         : '//trim(seed)//'.bands'
+    end if
+  end subroutine spectral_task
 
   logical function truth(v)
     character(*), intent(in) :: v

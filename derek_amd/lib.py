@@ -26,7 +26,7 @@ EXPORTS = (
     "derek_hip_set_vloc", "derek_hip_set_vext",
     "derek_hip_fft3d", "derek_hip_apply_h", "derek_hip_overlap", "derek_hip_rotate",
     "derek_hip_density", "derek_hip_lda", "derek_hip_hartree", "derek_hip_heev", "derek_hip_hegv",
-    "derek_hip_scf_init", "derek_hip_scf_step", "derek_hip_scf_polish", "derek_hip_get_eigenvalues",
+    "derek_hip_scf_init", "derek_hip_scf_step", "derek_hip_scf_polish", "derek_hip_nscf", "derek_hip_get_eigenvalues",
     "derek_hip_get_occupations", "derek_hip_get_density", "derek_hip_get_potential", "derek_hip_get_psi", "derek_hip_put_psi",
     "derek_hip_apply_h_resident", "derek_hip_sync", "derek_hip_stream",
     "derek_hip_profile_enable", "derek_hip_profile_reset", "derek_hip_profile_read", "derek_hip_get_tuning",
@@ -426,6 +426,12 @@ class Context:
     def scf_polish(self, max_iter=100, res_tol=1e-9):
         r = C.c_double()
         check(self.lib.derek_hip_scf_polish(self.h, C.c_int32(max_iter), C.c_double(res_tol), C.byref(r)))
+        return r.value
+
+    def nscf(self, max_iter=200, res_tol=1e-9):
+        """Eigenpairs at the potential given with set_vloc; returns the largest residual."""
+        r = C.c_double()
+        check(self.lib.derek_hip_nscf(self.h, C.c_int32(max_iter), C.c_double(res_tol), C.byref(r)))
         return r.value
 
     def eigenvalues(self):

@@ -19,14 +19,16 @@ from . import lib
 class Setup:
     """Discretisation of one system, computed by the library (host side)."""
 
-    def __init__(self, spec):
+    def __init__(self, spec, kfrac=None):
+        """kfrac: fractional k-points (nk x 3) instead of the system's
+        Monkhorst-Pack mesh (e.g. a band-structure path), equal weights."""
         self.spec = spec
         self.cell = np.diag([float(x) for x in spec.lengths])
         self.std_dims, self.dims = lib.grid_dims(self.cell, spec.ecut, spec.g_fine_scale)
         self.N = int(np.prod(self.dims))
         self.inv = lib.inv_cell(self.cell)
         self.volume = float(np.prod(spec.lengths))
-        self.kfrac = lib.kpoints(spec.mp)
+        self.kfrac = lib.kpoints(spec.mp) if kfrac is None else np.atleast_2d(np.asarray(kfrac, dtype=np.float64))
         self.nk = self.kfrac.shape[0]
         self.weights = np.full(self.nk, 1.0 / self.nk)
         self.fft_index, self.kin = [], []

@@ -227,6 +227,14 @@ int32_t derek_hip_scf_step(derek_ctx* ctx, derek_scf_report* report);
  * until every residual norm is below res_tol (eigenvalues for reporting). */
 int32_t derek_hip_scf_polish(derek_ctx* ctx, int32_t max_iter, double res_tol, double* max_res);
 /* eigenvalues (nbands, nk) column-major, Ha */
+/* Non-self-consistent eigenpairs (task : bandstructure / dos, io.f90:522-527;
+ * the reference's spec_bands / spec_dos are empty, spectral.f90:33-44): after
+ * scf_init, Davidson on every k-point of the context at the total potential
+ * given with derek_hip_set_vloc (e.g. the converged potential of an SCF run
+ * on another k-point set), without density or potential updates, until all
+ * residuals are below res_tol or max_iter iterations.  *max_res receives the
+ * largest residual norm. */
+int32_t derek_hip_nscf(derek_ctx* ctx, int32_t max_iter, double res_tol, double* max_res);
 int32_t derek_hip_get_eigenvalues(derek_ctx* ctx, double* eig);
 int32_t derek_hip_get_occupations(derek_ctx* ctx, double* occ);
 /* rho_out of the last step (N_g doubles) */

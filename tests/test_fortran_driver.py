@@ -84,6 +84,41 @@ def test_fortran_writes_and_reads_interchange_files(tmp_path):
     assert abs(e1 - e2) < 1e-10
 
 
+def test_unknown_task_is_an_error(tmp_path):
+    spec = config.benchmark(1)
+    seed = write_info(tmp_path, spec, extra=[("task", "phonons")])
+    r = subprocess.run([driver(), seed, "-c"], capture_output=True, text=True)
+    assert r.returncode != 0 and "Unknown task: phonons" in r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("task", ["dos", "bandstructure"])
+def test_fortran_spectral_tasks(tmp_path, task):
+    """task : dos / bandstructure: the bands written by the driver equal the
+    library's non-self-consistent bands at the potential the driver wrote."""
+    import numpy as np
+    from derek_amd import interchange, spectral
+    spec = config.benchmark(1)
+    su = Setup(spec)
+    extra = [("task", task), ("write_potential", "true"), ("dos_kpt_mp_grid", "2 2 2")]
+    seed = write_info(tmp_path, spec, energy_tol=1e-10, max_scf=60, extra=extra)
+    out = subprocess.run([driver(), seed], capture_output=True, text=True, check=True, timeout=300).stdout
+    print(out)
+    v = interchange.read_potential(seed + ".pot", su)
+    if task == "dos":
+        rows = np.loadtxt(seed + ".spec")
+        kf, eig = rows[:, :3], rows[:, 3:]
+        assert kf.shape[0] == 8
+        g = np.loadtxt(seed + ".dos")
+        assert abs(np.trapezoid(g[:, 1], g[:, 0]) - 2 * spec.nbands) < 1e-3
+    else:
+        rows = np.loadtxt(seed + ".bands")
+        kf, eig = rows[:, 1:4], rows[:, 4:]
+        assert kf.shape[0] == 41 and np.all(np.diff(rows[:, 0]) > 0)
+    e, res = spectral.nscf_bands(spec, v, kf, res_tol=1e-10, max_iter=1000)
+    assert np.max(np.abs(e - eig)) < 1e-7
+
+
 @pytest.mark.gpu
 def test_fortran_scf_matches_oracle(tmp_path):
     from oracle import scf as oscf
