@@ -102,16 +102,18 @@ class SystemSpec:
             "wfn_seed : %d" % self.seed,
             "precision : %s" % precision,
         ] + ["%s : %s" % kv for kv in extra] + [
+            # block rows start in column 1: the reference treats a row that
+            # starts with a blank as a comment (io.f90:1818-1824)
             "$DATA lattice",
-            " %r 0.0 0.0" % self.length,
-            " 0.0 %r 0.0" % self.length,
-            " 0.0 0.0 %r" % self.length,
+            "%r 0.0 0.0" % self.length,
+            "0.0 %r 0.0" % self.length,
+            "0.0 0.0 %r" % self.length,
             "$ENDDATA lattice",
         ]
         if self.nwells:
             lines.append("$DATA gaussian_pot")
             for c, h, w in zip(self.centres, self.heights, self.widths):
-                lines.append(" %r %r %r %r %r" % (c[0], c[1], c[2], h, w))
+                lines.append("%r %r %r %r %r" % (c[0], c[1], c[2], h, w))
             lines.append("$ENDDATA gaussian_pot")
         return "\n".join(lines) + "\n"
 

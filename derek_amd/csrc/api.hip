@@ -7,6 +7,7 @@
 
 #include "basis.hpp"
 #include "context.hpp"
+#include "info.hpp"
 #include "interchange.hpp"
 #include "scf.hpp"
 #include "subspace.hpp"
@@ -366,6 +367,109 @@ int32_t derek_basis_std_index(const int32_t fine_dims[3], const int32_t std_dims
     const int64_t ng = (int64_t)f[0] * f[1] * f[2];
     for (int64_t p = 0; p < npw; ++p) DEREK_REQUIRE(fft_index[p] >= 0 && fft_index[p] < ng, "FFT bin outside the grid");
     basis::std_index(f, d, npw, fft_index, std_index);
+  });
+}
+
+// ---- .info input (host only)
+struct derek_info {
+  info::Params p;
+};
+
+static void copy_text(const std::string& t, char* buf, int64_t len) {
+  DEREK_REQUIRE(buf && len > 0, "null buffer");
+  const size_t n = std::min<size_t>((size_t)len - 1, t.size());
+  std::memcpy(buf, t.data(), n);
+  buf[n] = '\0';
+}
+
+int32_t derek_info_read(const char* path, const char* safety, derek_info** out) {
+  return guard([&] {
+    DEREK_REQUIRE(path && out, "null argument");
+    *out = nullptr;
+    auto h = std::make_unique<derek_info>();
+    h->p = info::read(path, safety && *safety ? safety : "Normal");
+    *out = h.release();
+  });
+}
+
+int32_t derek_info_free(derek_info* h) {
+  delete h;
+  return DEREK_OK;
+}
+
+int32_t derek_info_num(const derek_info* h, const char* key, double* v) {
+  return guard([&] {
+    DEREK_REQUIRE(h && key && v, "null argument");
+    auto it = h->p.num.find(key);
+    if (it == h->p.num.end()) throw ArgError(std::string("no numeric key ") + key);
+    for (size_t i = 0; i < it->second.size(); ++i) v[i] = it->second[i];
+  });
+}
+
+int32_t derek_info_str(const derek_info* h, const char* key, char* buf, int64_t len) {
+  return guard([&] {
+    DEREK_REQUIRE(h && key, "null argument");
+    auto it = h->p.str.find(key);
+    if (it == h->p.str.end()) throw ArgError(std::string("no text key ") + key);
+    copy_text(it->second, buf, len);
+  });
+}
+
+int32_t derek_info_present(const derek_info* h, const char* key, int32_t* present) {
+  return guard([&] {
+    DEREK_REQUIRE(h && key && present, "null argument");
+    *present = std::find(h->p.present.begin(), h->p.present.end(), std::string(key)) != h->p.present.end();
+  });
+}
+
+int32_t derek_info_cell(const derek_info* h, double cell[9]) {
+  return guard([&] {
+    DEREK_REQUIRE(h && cell, "null argument");
+    if (!h->p.cell_declared) throw ArgError(" No lattice provided");
+    for (int i = 0; i < 9; ++i) cell[i] = h->p.cell[i];
+  });
+}
+
+int32_t derek_info_wells(const derek_info* h, int32_t* n, double* centres, double* heights, double* widths) {
+  return guard([&] {
+    DEREK_REQUIRE(h && n, "null argument");
+    const int32_t m = (int32_t)h->p.wells.size();
+    if (*n > 0 && centres && heights && widths) {
+      DEREK_REQUIRE(*n >= m, "well arrays too small");
+      for (int32_t i = 0; i < m; ++i) {
+        for (int a = 0; a < 3; ++a) centres[3 * i + a] = h->p.wells[i].c[a];
+        heights[i] = h->p.wells[i].height;
+        widths[i] = h->p.wells[i].width;
+      }
+    }
+    *n = m;
+  });
+}
+
+int32_t derek_info_log(const derek_info* h, char* buf, int64_t len) {
+  return guard([&] {
+    DEREK_REQUIRE(h, "null argument");
+    copy_text(h->p.log, buf, len);
+  });
+}
+
+int32_t derek_info_preset_pot(const derek_info* h, const int32_t fine_dims[3], double* v) {
+  return guard([&] {
+    DEREK_REQUIRE(h && v, "null argument");
+    check_dims(fine_dims);
+    const int f[3] = {fine_dims[0], fine_dims[1], fine_dims[2]};
+    info::preset_potential(h->p, f, v);
+  });
+}
+
+int32_t derek_info_help(const char* mode, const char* term, char* buf, int64_t len) {
+  return guard([&] {
+    DEREK_REQUIRE(mode, "null argument");
+    const std::string m = mode, t = term ? term : "";
+    if (m == "help") copy_text(info::help_key(t), buf, len);
+    else if (m == "search") copy_text(info::search(t), buf, len);
+    else if (m == "list") copy_text(info::list(), buf, len);
+    else throw ArgError("help mode must be help, search or list");
   });
 }
 

@@ -116,6 +116,72 @@ module derek_hip_m
        integer(c_int64_t), dimension(*) :: std_index
      end function derek_basis_std_index
 
+     ! .info input (io.f90:360-831) and command-line help texts
+     integer(c_int32_t) function derek_info_read(path, safety, info) bind(C, name='derek_info_read')
+       import :: c_int32_t, c_char, c_ptr
+       character(kind=c_char), dimension(*) :: path, safety
+       type(c_ptr) :: info
+     end function derek_info_read
+
+     integer(c_int32_t) function derek_info_free(info) bind(C, name='derek_info_free')
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: info
+     end function derek_info_free
+
+     integer(c_int32_t) function derek_info_num(info, key, v) bind(C, name='derek_info_num')
+       import :: c_int32_t, c_char, c_ptr, c_double
+       type(c_ptr), value :: info
+       character(kind=c_char), dimension(*) :: key
+       real(c_double), dimension(*) :: v
+     end function derek_info_num
+
+     integer(c_int32_t) function derek_info_str(info, key, buf, len) bind(C, name='derek_info_str')
+       import :: c_int32_t, c_int64_t, c_char, c_ptr
+       type(c_ptr), value :: info
+       character(kind=c_char), dimension(*) :: key, buf
+       integer(c_int64_t), value :: len
+     end function derek_info_str
+
+     integer(c_int32_t) function derek_info_present(info, key, present) bind(C, name='derek_info_present')
+       import :: c_int32_t, c_char, c_ptr
+       type(c_ptr), value :: info
+       character(kind=c_char), dimension(*) :: key
+       integer(c_int32_t) :: present
+     end function derek_info_present
+
+     integer(c_int32_t) function derek_info_cell(info, cell) bind(C, name='derek_info_cell')
+       import :: c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: info
+       real(c_double), dimension(*) :: cell
+     end function derek_info_cell
+
+     integer(c_int32_t) function derek_info_wells(info, n, centres, heights, widths) bind(C, name='derek_info_wells')
+       import :: c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: info
+       integer(c_int32_t) :: n
+       real(c_double), dimension(*) :: centres, heights, widths
+     end function derek_info_wells
+
+     integer(c_int32_t) function derek_info_log(info, buf, len) bind(C, name='derek_info_log')
+       import :: c_int32_t, c_int64_t, c_char, c_ptr
+       type(c_ptr), value :: info
+       character(kind=c_char), dimension(*) :: buf
+       integer(c_int64_t), value :: len
+     end function derek_info_log
+
+     integer(c_int32_t) function derek_info_preset_pot(info, fine_dims, v) bind(C, name='derek_info_preset_pot')
+       import :: c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: info
+       integer(c_int32_t), dimension(*) :: fine_dims
+       real(c_double), dimension(*) :: v
+     end function derek_info_preset_pot
+
+     integer(c_int32_t) function derek_info_help(mode, term, buf, len) bind(C, name='derek_info_help')
+       import :: c_int32_t, c_int64_t, c_char
+       character(kind=c_char), dimension(*) :: mode, term, buf
+       integer(c_int64_t), value :: len
+     end function derek_info_help
+
      ! interchange files (pot.f90:194-224,303-340; density.f90:133-163;
      ! wave.f90:896-926); path is a NUL-terminated name
      integer(c_int32_t) function derek_io_pot_write(path, std_dims, nfine, nc_pot, allocated) &
@@ -376,7 +442,9 @@ module derek_hip_m
        derek_hip_scf_step, derek_hip_scf_polish, derek_hip_get_eigenvalues, derek_hip_get_occupations, &
        derek_hip_get_density, derek_hip_get_psi, derek_hip_put_psi, derek_hip_sync, derek_hip_get_potential, &
        derek_basis_std_index, derek_io_pot_write, derek_io_pot_read, derek_io_den_write, derek_io_den_read, &
-       derek_io_wvfn_write, derek_io_wvfn_read, derek_hip_nscf
+       derek_io_wvfn_write, derek_io_wvfn_read, derek_hip_nscf, derek_info_read, derek_info_free, derek_info_num, &
+       derek_info_str, derek_info_present, derek_info_cell, derek_info_wells, derek_info_log, derek_info_preset_pot, &
+       derek_info_help
 
 contains
 

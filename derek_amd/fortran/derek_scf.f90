@@ -1,34 +1,36 @@
 !-------------------------------------------------------------------------------
 ! derek_scf: Fortran SCF driver on the gfx950 library.
 !
-! Usage: derek_scf <seed> [-c]     reads <seed>.info
-!   -c   check mode (like the reference's `derek -c`, io.f90:1074-1080):
-!        parse the input, build the basis, report the grids and stop (no GPU).
+! Usage (io_cl_parser, io.f90:994-1138):
+!   derek_scf <seed>             run <seed>.info
+!   derek_scf -c|--check <seed>  parse, build the basis, report the grids, stop
+!   derek_scf -h|--help [key]    usage, or the description of one key
+!   derek_scf -s|--search <text> keys whose name or description contains text
+!   derek_scf -l|--list          every key with its default
+!   derek_scf -v                 version
 !
-! Input: the reference's `key : value` lines (io.f90:360-831; `!`/`#`
-! comments) and the $DATA lattice / gaussian_pot blocks (io.f90:1765-1935),
-! for the keys the SCF needs: cut_off_energy, g_fine_scale, kpt_mp_grid,
-! n_electrons, electronic_temp (Kelvin, io.f90:340), energy_tol, max_scf,
-! unit_energy / unit_length (Ha and bohr only: their factors are exactly 1),
-! plus nbands, cos_pot_amp, wfn_seed and precision; external_pot (a preset
-! name, or a <file>.potex whose potential replaces the wells, pot.f90:167-184)
-! and write_potential / write_density / write_wvfn, which write <seed>.pot,
-! <seed>.den and <seed>.wvfn after the SCF (state.f90:350-507); task
-! (singlepoint, dos, bandstructure; io.f90:518-527) with dos_kpt_mp_grid:
-! after the SCF the converged total potential is held fixed and the bands are
-! found without self-consistency on the DOS mesh (<seed>.spec eigenvalues,
-! <seed>.dos Gaussian-broadened density of states) or along the simple-cubic
-! path G-X-M-G-R (<seed>.bands); the reference's spec_dos / spec_bands are
-! empty (spectral.f90:33-44).
+! <seed>.info is read by the library's restatement of the reference's input
+! language (derek_info_*: all 45 keys, spell check, $DATA lattice and
+! gaussian_pot blocks, output_level presets, units), plus this build's keys
+! nbands, cos_pot_amp, wfn_seed and precision.  The external potential is
+! the gaussian_pot block (wells), a preset (jelly, finite_barrier,
+! periodic_pot; pot.f90:130-163) or a <file>.potex (pot.f90:167-184).
 ! The loop, its table and the stop test |E_i - E_{i-1}| < energy_tol follow
-! electronic_scf (electronic.f90:86-175).
+! electronic_scf (electronic.f90:86-175); write_potential / write_density /
+! write_wvfn write <seed>.pot, <seed>.den and <seed>.wvfn after the SCF
+! (state.f90:350-507); task (singlepoint, dos, bandstructure; io.f90:518-527)
+! with dos_kpt_mp_grid: after the SCF the converged total potential is held
+! fixed and the bands are found without self-consistency on the DOS mesh
+! (<seed>.spec eigenvalues, <seed>.dos Gaussian-broadened density of states)
+! or along the simple-cubic path G-X-M-G-R (<seed>.bands); the reference's
+! spec_dos / spec_bands are empty (spectral.f90:33-44).
 !-------------------------------------------------------------------------------
 program derek_scf
   use iso_c_binding
   use derek_hip_m
   implicit none
 
-  real(c_double), parameter :: kelvin_ha = 3.1668115634556076d-6  ! k_B / E_h (CODATA 2018)
+  real(c_double) :: dummy3(3)
   character(len=256) :: seed, arg, line, key, val
   real(c_double) :: cell(3, 3), inv_cell(3, 3), ecut = 200.0d0, g_fine = 2.0d0, temp_k = 0.0d0
   real(c_double) :: energy_tol = 1.0d-7, cos_amp = 0.0d0, nelec = 1.0d0
@@ -39,8 +41,9 @@ program derek_scf
   integer(c_int32_t), allocatable :: fft_index(:)
   integer(c_int32_t) :: std_dims(3), fine_dims(3), nk, k, nwells = 0, iscf
   integer(c_int64_t) :: npw, max_npw, ngrid
-  logical :: check_only = .false., have_cell = .false.
-  logical :: write_pot = .false., write_den = .false., write_wfn = .false., have_file
+  logical :: check_only = .false.
+  type(c_ptr) :: inf
+  logical :: write_pot = .false., write_den = .false., write_wfn = .false.
   character(len=256) :: ext_pot = 'gaussian_pot', task = 'singlepoint'
   integer(c_int32_t) :: dos_mp(3) = (/11, 11, 11/)
   complex(c_double_complex), allocatable :: ncpot(:, :, :), charge(:), psi_all(:)
@@ -57,80 +60,37 @@ program derek_scf
   real(c_double) :: e_old, t0, t1
   logical :: converged
 
-  call get_command_argument(1, seed)
-  if (len_trim(seed) == 0) then
-     write(*, *) 'usage: derek_scf <seed> [-c]'
-     error stop 2
-  end if
-  do i = 2, command_argument_count()
-     call get_command_argument(i, arg)
-     if (trim(arg) == '-c') check_only = .true.
-  end do
-  cell = 0.0d0
-  allocate(centres(3, 0), heights(0), widths(0))
-
-  open(newunit=u, file=trim(seed)//'.info', status='old', action='read', iostat=ios)
-  if (ios /= 0) then
-     write(*, *) "Error: file '"//trim(seed)//".info' does not exist."
-     error stop 1
-  end if
-  do
-     read(u, '(a)', iostat=ios) line
-     if (ios /= 0) exit
-     line = adjustl(line)
-     if (len_trim(line) == 0) cycle
-     if (line(1:1) == '!' .or. line(1:1) == '#') cycle
-     if (index(lower(line), '$data') == 1) then
-        call read_block(u, lower(line))
-        cycle
-     end if
-     i = scan(line, ':=')
-     if (i == 0) cycle
-     key = lower(adjustl(line(1:i-1)))
-     val = adjustl(line(i+1:))
-     select case (trim(key))
-     case ('cut_off_energy'); read(val, *) ecut
-     case ('g_fine_scale'); read(val, *) g_fine
-     case ('kpt_mp_grid'); read(val, *) mp
-     case ('n_electrons'); read(val, *) nelec
-     case ('nbands'); read(val, *) nbands
-     case ('electronic_temp'); read(val, *) temp_k
-     case ('energy_tol'); read(val, *) energy_tol
-     case ('max_scf'); read(val, *) max_scf
-     case ('cos_pot_amp'); read(val, *) cos_amp
-     case ('wfn_seed'); read(val, *) wfn_seed
-     case ('external_pot'); ext_pot = val
-     case ('task')
-        task = lower(val)
-        select case (trim(task))
-        case ('singlepoint', 'bandstructure', 'dos')
-           continue
-        case default
-           call fail('Unknown task: '//trim(val))
-        end select
-     case ('dos_kpt_mp_grid'); read(val, *) dos_mp
-     case ('write_potential'); write_pot = truth(val)
-     case ('write_density'); write_den = truth(val)
-     case ('write_wvfn'); write_wfn = truth(val)
-     case ('precision')
-        if (trim(lower(val)) == 'mixed') prec = DEREK_MIXED
-     case ('unit_energy')
-        if (trim(val) /= 'Ha') call fail('only unit_energy : Ha is supported by this driver')
-     case ('unit_length')
-        if (trim(val) /= 'bohr') call fail('only unit_length : bohr is supported by this driver')
-     case default
-        continue
-     end select
-  end do
-  close(u)
-  if (.not. have_cell) call fail(' No lattice provided')
-  ! external_pot other than a preset names a .potex file (pot.f90:167-176)
-  if (trim(lower(ext_pot)) /= 'gaussian_pot') then
-     if (index(trim(ext_pot), '.potex') == 0) call fail('potential naming convension not correct')
-     inquire(file=trim(ext_pot), exist=have_file)
-     if (.not. have_file) call fail('file '//trim(ext_pot)//' does not exist')
-  end if
-  if (nbands <= 0) nbands = int(nelec / 2.0d0) + 4
+  call parse_command_line()
+  call derek_check(derek_info_read(trim(seed)//'.info'//c_null_char, 'Normal'//c_null_char, inf), 'info')
+  call print_text_log()
+  call derek_check(derek_info_cell(inf, cell), 'info')
+  ecut = getr('cut_off_energy')
+  g_fine = getr('g_fine_scale')
+  mp = get3('kpt_mp_grid')
+  dos_mp = get3('dos_kpt_mp_grid')
+  nelec = getr('n_electrons')
+  temp_k = getr('electronic_temp')        ! already in Ha (converted from Kelvin)
+  energy_tol = getr('energy_tol')
+  max_scf = nint(getr('max_scf'))
+  cos_amp = getr('cos_pot_amp')
+  wfn_seed = nint(getr('wfn_seed'), c_int64_t)
+  if (trim(gets('precision')) == 'mixed') prec = DEREK_MIXED
+  nbands = nint(getr('nbands'))
+  ! scalar bands hold two electrons; the reference counts spinor bands,
+  ! n_electrons + conduction_bands (io.f90:316)
+  if (nbands <= 0) nbands = ceiling(nelec / 2.0d0) + nint(getr('conduction_bands'))
+  ext_pot = gets('external_pot')
+  task = gets('task')
+  write_pot = getr('write_potential') /= 0.0d0
+  write_den = getr('write_density') /= 0.0d0
+  write_wfn = getr('write_wvfn') /= 0.0d0
+  if (getr('check') /= 0.0d0) check_only = .true.
+  n = 0
+  call derek_check(derek_info_wells(inf, n, dummy3, dummy3, dummy3), 'info')
+  deallocate(centres, heights, widths)
+  allocate(centres(3, max(n, 1)), heights(max(n, 1)), widths(max(n, 1)))
+  call derek_check(derek_info_wells(inf, n, centres, heights, widths), 'info')
+  nwells = n
 
   call derek_check(derek_basis_grid_dims(cell, ecut, g_fine, std_dims, fine_dims), 'basis_grid_dims')
   call derek_check(derek_basis_inv_cell(cell, inv_cell), 'basis_inv_cell')
@@ -167,11 +127,14 @@ program derek_scf
   end do
   call derek_check(derek_basis_gsq(inv_cell, fine_dims, gsq), 'basis_gsq')
   call derek_check(derek_hip_set_gsq(ctx, gsq), 'hip_set_gsq')
-  n = size(heights)
-  select case (trim(lower(ext_pot)))
+  select case (trim(ext_pot))
   case ('gaussian_pot')
-     call derek_check(derek_gaussian_pot(cell, fine_dims, n, centres, heights, widths, cos_amp, vext), &
+     call derek_check(derek_gaussian_pot(cell, fine_dims, nwells, centres, heights, widths, cos_amp, vext), &
           'gaussian_pot')
+  case ('jelly', 'finite_barrier', 'periodic_pot')
+     call derek_check(derek_info_preset_pot(inf, fine_dims, vext), 'preset potential')
+  case ('lorentzian_pot')
+     call fail('lorentzian_pot is not supported by this build')
   case default
      ! a potential file replaces the preset (pot.f90:167-184)
      allocate(ncpot(ngrid, 2, 2))
@@ -183,7 +146,7 @@ program derek_scf
   call derek_check(derek_hip_set_vext(ctx, vext), 'hip_set_vext')
 
   par%nelec = nelec
-  par%kT = temp_k * kelvin_ha
+  par%kT = temp_k
   par%mix_beta = 0.5d0
   par%mix_q0 = 1.0d0
   par%n_david_first = 4
@@ -386,15 +349,137 @@ contains
     end if
   end subroutine spectral_task
 
-  logical function truth(v)
-    character(*), intent(in) :: v
-    select case (trim(lower(adjustl(v))))
-    case ('true', 't', '.true.', 'yes', 'on')
-       truth = .true.
-    case default
-       truth = .false.
-    end select
-  end function truth
+  real(c_double) function getr(key)
+    character(*), intent(in) :: key
+    real(c_double) :: v(3)
+    call derek_check(derek_info_num(inf, trim(key)//c_null_char, v), 'info '//trim(key))
+    getr = v(1)
+  end function getr
+
+  function get3(key) result(m)
+    character(*), intent(in) :: key
+    integer(c_int32_t) :: m(3)
+    real(c_double) :: v(3)
+    call derek_check(derek_info_num(inf, trim(key)//c_null_char, v), 'info '//trim(key))
+    m = nint(v)
+  end function get3
+
+  function gets(key) result(t)
+    character(*), intent(in) :: key
+    character(len=256) :: t
+    character(kind=c_char) :: buf(256)
+    call derek_check(derek_info_str(inf, trim(key)//c_null_char, buf, int(size(buf), c_int64_t)), 'info '//trim(key))
+    t = c_text(buf)
+  end function gets
+
+  function c_text(buf) result(t)
+    character(kind=c_char), intent(in) :: buf(:)
+    character(len=size(buf)) :: t
+    integer :: j
+    t = ''
+    do j = 1, size(buf)
+       if (buf(j) == c_null_char) exit
+       t(j:j) = buf(j)
+    end do
+  end function c_text
+
+  ! spell-check warnings of the reader, printed as the reference prints its
+  ! autocorrect table (io.f90:454-470)
+  subroutine print_text_log()
+    character(kind=c_char) :: buf(8192)
+    character(len=8192) :: t
+    integer :: p0, p1
+    call derek_check(derek_info_log(inf, buf, int(size(buf), c_int64_t)), 'info log')
+    t = c_text(buf)
+    if (len_trim(t) == 0) return
+    write(*, '(a)') ' ********************************************************************'
+    write(*, '(a)') ' *                         *** WARNING ***                          *'
+    p0 = 1
+    do
+       p1 = index(t(p0:), achar(10))
+       if (p1 == 0) exit
+       write(*, '(a)') ' '//t(p0:p0 + p1 - 2)
+       p0 = p0 + p1
+    end do
+    write(*, '(a)') ' ********************************************************************'
+  end subroutine print_text_log
+
+  subroutine print_help(mode, term)
+    character(*), intent(in) :: mode, term
+    character(kind=c_char) :: buf(65536)
+    character(len=65536) :: t
+    integer :: p0, p1
+    call derek_check(derek_info_help(trim(mode)//c_null_char, trim(term)//c_null_char, buf, &
+         int(size(buf), c_int64_t)), 'help')
+    t = c_text(buf)
+    p0 = 1
+    do
+       p1 = index(t(p0:), achar(10))
+       if (p1 == 0) exit
+       write(*, '(a)') t(p0:p0 + p1 - 2)
+       p0 = p0 + p1
+    end do
+  end subroutine print_help
+
+  subroutine usage()
+    write(*, '(a)') ' Usage:'
+    write(*, '(a)') '   derek_scf    <seed>                 : Run a calculation from <seed>.info'
+    write(*, '(a)') '       "        -v                     : Print version information.'
+    write(*, '(a)') '       "        -h,--help    <keyword> : Get help and commandline options. Optional keyword.'
+    write(*, '(a)') '       "        -s,--search  <keyword> : Search list of available parameters.'
+    write(*, '(a)') '       "        -l,--list              : Print list of parameters.'
+    write(*, '(a)') '       "        -c,--check   <seed>    : Run calculation to check input files.'
+  end subroutine usage
+
+  ! command line of io_cl_parser (io.f90:994-1138)
+  subroutine parse_command_line()
+    integer :: na, ia
+    character(len=256) :: a1, a2
+    na = command_argument_count()
+    seed = ''
+    if (na == 0) then
+       call usage()
+       stop
+    end if
+    ia = 1
+    do while (ia <= na)
+       call get_command_argument(ia, a1)
+       a2 = ''
+       if (ia < na) call get_command_argument(ia + 1, a2)
+       select case (trim(a1))
+       case ('-h', '--help')
+          if (len_trim(a2) > 0) then
+             call print_help('help', a2)
+          else
+             call usage()
+          end if
+          stop
+       case ('-s', '--search')
+          call print_help('search', a2)
+          stop
+       case ('-l', '--list')
+          call print_help('list', '')
+          stop
+       case ('-v')
+          write(*, '(a,i0)') ' derek_scf on libderek_hip, ABI version ', derek_hip_version()
+          stop
+       case ('-c', '--check')
+          check_only = .true.
+       case default
+          if (a1(1:1) == '-') then
+             write(*, '(a)') ' Unknown argument: '//trim(a1)
+             call usage()
+             error stop 2
+          end if
+          seed = a1
+       end select
+       ia = ia + 1
+    end do
+    if (len_trim(seed) == 0) then
+       call usage()
+       error stop 2
+    end if
+  end subroutine parse_command_line
 
   subroutine fail(msg)
     character(*), intent(in) :: msg
@@ -412,48 +497,6 @@ contains
     p = 0.0d0
   end function kin_dummy
 
-  ! $DATA <type> ... $ENDDATA <type>; lattice rows are cell(row, :)
-  ! (io.f90:1859-1878); gaussian_pot rows are cx cy cz height width.
-  subroutine read_block(unit, head)
-    integer, intent(in) :: unit
-    character(*), intent(in) :: head
-    character(len=256) :: l, kind
-    integer :: r, st
-    real(c_double) :: row(5)
-    real(c_double), allocatable :: tc(:, :), th(:), tw(:)
-    read(head(6:), *) kind
-    r = 0
-    do
-       read(unit, '(a)', iostat=st) l
-       if (st /= 0) call fail('Block '//trim(kind)//' not closed')
-       l = adjustl(l)
-       if (index(lower(l), '$enddata') == 1) exit
-       if (len_trim(l) == 0 .or. l(1:1) == '!' .or. l(1:1) == '#') cycle
-       r = r + 1
-       select case (trim(kind))
-       case ('lattice')
-          if (r > 3) call fail('LATTICE must have 3 lines')
-          read(l, *) cell(r, 1), cell(r, 2), cell(r, 3)
-          have_cell = .true.
-       case ('gaussian_pot')
-          read(l, *) row
-          allocate(tc(3, r), th(r), tw(r))
-          if (r > 1) then
-             tc(:, 1:r-1) = centres
-             th(1:r-1) = heights
-             tw(1:r-1) = widths
-          end if
-          tc(:, r) = row(1:3)
-          th(r) = row(4)
-          tw(r) = row(5)
-          call move_alloc(tc, centres)
-          call move_alloc(th, heights)
-          call move_alloc(tw, widths)
-       case default
-          call fail(' Unknown block type '//trim(kind))
-       end select
-    end do
-    if (trim(kind) == 'lattice' .and. r /= 3) call fail('LATTICE must have 3 lines')
-  end subroutine read_block
+
 
 end program derek_scf

@@ -22,6 +22,8 @@ EXPORTS = (
     "derek_basis_kpoints", "derek_basis_sphere", "derek_basis_gsq", "derek_basis_std_index", "derek_gaussian_pot",
     "derek_io_pot_write", "derek_io_pot_read", "derek_io_den_write", "derek_io_den_read",
     "derek_io_wvfn_write", "derek_io_wvfn_read",
+    "derek_info_read", "derek_info_free", "derek_info_num", "derek_info_str", "derek_info_present",
+    "derek_info_cell", "derek_info_wells", "derek_info_log", "derek_info_preset_pot", "derek_info_help",
     "derek_hip_init", "derek_hip_destroy", "derek_hip_set_basis", "derek_hip_set_gsq",
     "derek_hip_set_vloc", "derek_hip_set_vext",
     "derek_hip_fft3d", "derek_hip_apply_h", "derek_hip_overlap", "derek_hip_rotate",
@@ -169,6 +171,70 @@ def std_index(fine_dims, std_dims, fft_index):
                                        C.c_int64(len(idx)), idx.ctypes.data_as(C.POINTER(C.c_int32)),
                                        out.ctypes.data_as(C.POINTER(C.c_int64))))
     return out
+
+
+# ---------------------------------------------------------------- .info input (host)
+class Info:
+    """A parsed .info file (the reference's input language)."""
+
+    def __init__(self, path, safety="Normal"):
+        self.h = C.c_void_p()
+        check(load().derek_info_read(_path(path), safety.encode(), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            load().derek_info_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def num(self, key, n=1):
+        v = np.zeros(3)
+        check(load().derek_info_num(self.h, key.encode(), ptr(v)))
+        return v[0] if n == 1 else v[:n].copy()
+
+    def str(self, key):
+        buf = C.create_string_buffer(512)
+        check(load().derek_info_str(self.h, key.encode(), buf, C.c_int64(len(buf))))
+        return buf.value.decode()
+
+    def present(self, key):
+        p = C.c_int32()
+        check(load().derek_info_present(self.h, key.encode(), C.byref(p)))
+        return bool(p.value)
+
+    def cell(self):
+        c = np.zeros(9)
+        check(load().derek_info_cell(self.h, ptr(c)))
+        return c.reshape(3, 3, order="F")
+
+    def wells(self):
+        n = C.c_int32(0)
+        check(load().derek_info_wells(self.h, C.byref(n), None, None, None))
+        c, hh, w = np.zeros(3 * n.value), np.zeros(n.value), np.zeros(n.value)
+        if n.value:
+            check(load().derek_info_wells(self.h, C.byref(n), ptr(c), ptr(hh), ptr(w)))
+        return c.reshape(-1, 3), hh, w
+
+    def log(self):
+        buf = C.create_string_buffer(1 << 16)
+        check(load().derek_info_log(self.h, buf, C.c_int64(len(buf))))
+        return buf.value.decode()
+
+    def preset_potential(self, fine_dims):
+        v = np.zeros(int(np.prod(fine_dims)))
+        check(load().derek_info_preset_pot(self.h, (C.c_int32 * 3)(*fine_dims), ptr(v)))
+        return v
+
+
+def info_help(mode, term=""):
+    buf = C.create_string_buffer(1 << 16)
+    check(load().derek_info_help(mode.encode(), term.encode(), buf, C.c_int64(len(buf))))
+    return buf.value.decode()
 
 
 # ---------------------------------------------------------------- interchange files (host)

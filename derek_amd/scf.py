@@ -7,7 +7,7 @@ The discretisation comes from the library's restatement of
 Source/basis.f90 (grid sizes, fine G map, plane-wave sphere, |G|^2) and the
 external potential from its Gaussian-well evaluator, so a run needs nothing
 but a system description (``derek_amd.config.SystemSpec``) or a ``.info``
-file (``derek_amd.info``).
+file (``lib.Info``).
 """
 import time
 

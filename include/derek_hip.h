@@ -123,6 +123,36 @@ int32_t derek_gaussian_pot(const double cell[9], const int32_t dims[3], int32_t 
 
 /* ---- context -------------------------------------------------------------- */
 /* Allocates the device state (replaces fft_init's plans, fft.f90:139-219). */
+/* ---- .info input (host only): the reference's input language
+ * io_read_param (io.f90:360-831): lines with '!' anywhere are skipped, '#' in
+ * the first column is a comment, `key : value` or `key = value`, keys are
+ * case-insensitive and values lower-cased except for unit keys; a key within
+ * Levenshtein distance < 3 ("Normal"; "Risky" 5, "Safe" 0) of a known key is
+ * autocorrected with a warning (derek_info_log), otherwise "Unknown parameter
+ * - <key>"; duplicates, invalid values and units fail with the reference's
+ * messages; $DATA lattice / gaussian_pot blocks (io.f90:1765-1935);
+ * output_level presets (io.f90:778-812); values converted to atomic units
+ * as io_convert_all does (io.f90:324-358).  All 45 reference keys plus this
+ * build's nbands, cos_pot_amp, wfn_seed, precision.  Errors -> DEREK_ERR_ARG. */
+typedef struct derek_info derek_info;
+int32_t derek_info_read(const char* path, const char* safety, derek_info** info);
+int32_t derek_info_free(derek_info* info);
+/* numeric key (logicals as 0/1, triplets as 3 values) */
+int32_t derek_info_num(const derek_info* info, const char* key, double* v);
+int32_t derek_info_str(const derek_info* info, const char* key, char* buf, int64_t len);
+int32_t derek_info_present(const derek_info* info, const char* key, int32_t* present);
+/* cell(row, col) column-major in bohr; " No lattice provided" without a block */
+int32_t derek_info_cell(const derek_info* info, double cell[9]);
+/* gaussian_pot rows: *n in = capacity (0: count only), out = count */
+int32_t derek_info_wells(const derek_info* info, int32_t* n, double* centres, double* heights, double* widths);
+int32_t derek_info_log(const derek_info* info, char* buf, int64_t len);
+/* jelly / finite_barrier / periodic_pot potentials of pot_external_pot
+ * (pot.f90:130-163) on the fine grid */
+int32_t derek_info_preset_pot(const derek_info* info, const int32_t fine_dims[3], double* v);
+/* command-line help texts (io_help / io_search / io_list_params,
+ * io.f90:1140-1673): mode "help" (one key), "search" (substring), "list" */
+int32_t derek_info_help(const char* mode, const char* term, char* buf, int64_t len);
+
 /* ---- interchange files (host only; no context needed)
  * Fortran unformatted sequential files as the reference's gfortran build
  * writes them (-fconvert=big-endian, Makefile:86): one record per file (the

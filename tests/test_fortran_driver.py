@@ -45,13 +45,39 @@ def test_missing_lattice_is_an_error(tmp_path):
     assert r.returncode != 0 and "No lattice provided" in r.stdout
 
 
-@pytest.mark.parametrize("val,msg", [("wells.dat", "potential naming convension not correct"),
-                                     ("missing.potex", "does not exist")])
+@pytest.mark.parametrize("val,msg", [("wells.dat", "Unsupported external potential file"),
+                                     ("missing.potex", "Potential file does not exist")])
 def test_external_potential_file_errors(tmp_path, val, msg):
-    spec = config.benchmark(1)
+    (tmp_path / "wells.dat").write_text("x")
+    spec = config.small_system()
+    spec.nwells = 0
     seed = write_info(tmp_path, spec, extra=[("external_pot", str(tmp_path / val))])
     r = subprocess.run([driver(), seed, "-c"], capture_output=True, text=True)
     assert r.returncode != 0 and msg in r.stdout
+
+
+def test_command_line_help(tmp_path):
+    out = subprocess.run([driver(), "-h", "cut_off_energy"], capture_output=True, text=True, check=True).stdout
+    assert "CUT_OFF_ENERGY" in out and "Default" in out
+    out = subprocess.run([driver(), "-s", "unit"], capture_output=True, text=True, check=True).stdout
+    assert "UNIT_ENERGY" in out
+    out = subprocess.run([driver(), "-l"], capture_output=True, text=True, check=True).stdout
+    assert "dos_kpt_mp_grid" in out and "write_xsf" in out
+    assert "Usage" in subprocess.run([driver()], capture_output=True, text=True).stdout
+    r = subprocess.run([driver(), "--bogus"], capture_output=True, text=True)
+    assert r.returncode != 0 and "Unknown argument: --bogus" in r.stdout
+
+
+def test_spell_check_autocorrects(tmp_path):
+    spec = config.benchmark(1)
+    text = spec.to_info().replace("cut_off_energy", "cut_of_energy")
+    seed = str(tmp_path / "typo")
+    open(seed + ".info", "w").write(text)
+    out = subprocess.run([driver(), seed, "-c"], capture_output=True, text=True, check=True).stdout
+    assert "cut_of_energy" in out and "cut_off_energy" in out
+    su = Setup(spec)
+    fine = [int(x) for x in re.search(r"Fine FFT grid\s*:\s*(\d+)\s+(\d+)\s+(\d+)", out).groups()]
+    assert fine == list(su.dims)
 
 
 @pytest.mark.gpu
@@ -77,7 +103,10 @@ def test_fortran_writes_and_reads_interchange_files(tmp_path):
     # the wells as a .potex file: the same SCF
     potex = str(tmp_path / "wells.potex")
     interchange.write_potential(potex, su, su.vext)
-    seed2 = write_info(tmp_path, spec, name="from_potex", energy_tol=1e-10, max_scf=60,
+    # the wells block and an external_pot key together are a duplicate (io.f90:428-433)
+    import dataclasses
+    bare = dataclasses.replace(spec, nwells=0, centres=[], heights=[], widths=[])
+    seed2 = write_info(tmp_path, bare, name="from_potex", energy_tol=1e-10, max_scf=60,
                        extra=[("external_pot", potex)])
     out2 = subprocess.run([driver(), seed2], capture_output=True, text=True, check=True, timeout=300).stdout
     e2 = float(re.search(r"Final energy \(Ha\)\s*:\s*(\S+)", out2).group(1))

@@ -1,0 +1,147 @@
+"""The .info input language (io.f90:360-831, 1765-1935, 324-358) through
+derek_info_*: values, unit conversion against CODATA 2018 conversion
+factors, spell check, duplicates, comments, blocks, presets and the
+reference's error messages."""
+import numpy as np
+import pytest
+
+from derek_amd import config, lib
+
+HA_EV = 27.211386245988          # CODATA 2018: hartree energy in eV
+BOHR_A = 0.529177210903          # CODATA 2018: bohr radius in angstrom
+KB_HA = 3.166811563e-6           # CODATA 2018: Boltzmann constant in Ha/K (10 digits)
+
+
+def write(tmp_path, text, name="t"):
+    p = tmp_path / (name + ".info")
+    p.write_text(text)
+    return str(p)
+
+
+LATTICE = "$DATA lattice\n10.0 0.0 0.0\n0.0 10.0 0.0\n0.0 0.0 10.0\n$ENDDATA lattice\n"
+
+
+def err(tmp_path, text, safety="Normal"):
+    with pytest.raises(lib.DerekError) as e:
+        lib.Info(write(tmp_path, text), safety)
+    assert e.value.code == lib.DEREK_ERR_ARG
+    return str(e.value)
+
+
+def test_benchmark_file_round_trip(tmp_path):
+    spec = config.benchmark(4)
+    inf = lib.Info(write(tmp_path, spec.to_info(energy_tol=1e-8, max_scf=33)))
+    assert inf.num("cut_off_energy") == spec.ecut and inf.num("g_fine_scale") == spec.g_fine_scale
+    assert list(inf.num("kpt_mp_grid", 3)) == list(spec.mp)
+    assert inf.num("n_electrons") == spec.nelec and inf.num("nbands") == spec.nbands
+    assert inf.num("max_scf") == 33 and inf.num("energy_tol") == 1e-8
+    assert np.array_equal(inf.cell(), np.diag([spec.length] * 3))
+    c, h, w = inf.wells()
+    assert np.array_equal(c, np.array(spec.centres)) and np.array_equal(h, spec.heights)
+    assert np.array_equal(w, spec.widths)
+    assert abs(inf.num("electronic_temp") - spec.kT) < 1e-15
+    assert inf.str("external_pot") == "gaussian_pot" and inf.present("external_pot")
+
+
+def test_defaults_and_units(tmp_path):
+    """The reference derives its units from e, h, m_e, c and mu_0 = 4 pi 1e-7
+    (constants.f90:82-106, units.f90:36-128); with the 2019 SI, mu_0 differs
+    from 4 pi 1e-7 by 5.5e-10, so the factors sit within ~1.1e-9 of the CODATA
+    conversion values, not on them."""
+    inf = lib.Info(write(tmp_path, "cut_off_energy : %r\nelectronic_temp : 1000\n" % HA_EV + LATTICE))
+    # reference defaults: eV and angstrom (io.f90:103-104)
+    assert abs(inf.num("cut_off_energy") - 1.0) < 2e-9
+    assert abs(inf.num("energy_tol") * HA_EV / 1e-7 - 1.0) < 2e-9
+    assert np.allclose(inf.cell(), np.diag([10.0 / BOHR_A] * 3), rtol=1e-9, atol=0)
+    assert abs(inf.num("electronic_temp") / 1000 / KB_HA - 1.0) < 2e-9
+    assert inf.str("task") == "singlepoint" and inf.str("external_pot") == "jelly"
+    assert inf.num("conduction_bands") == 20 and list(inf.num("dos_kpt_mp_grid", 3)) == [11, 11, 11]
+    assert inf.num("write_state") == 1 and inf.num("write_wvfn") == 0
+
+
+def test_comment_rules(tmp_path):
+    text = ("max_scf : 12 ! a comment anywhere drops the whole line\n"
+            "# comment\n"
+            "MAX_SCF = 17\n" + LATTICE)
+    inf = lib.Info(write(tmp_path, text))
+    assert inf.num("max_scf") == 17
+    assert "Error parsing line" in err(tmp_path, "max_scf 17\n" + LATTICE)
+
+
+def test_spell_check(tmp_path):
+    inf = lib.Info(write(tmp_path, "cut_of_energy : 5\nunit_energy : Ha\n" + LATTICE))
+    assert inf.num("cut_off_energy") == 5.0
+    assert "cut_of_energy" in inf.log() and "cut_off_energy" in inf.log()
+    # cutoff_enrg is 3 edits from cut_off_energy: rejected by "Normal" (< 3)
+    assert "Unknown parameter - cutoff_enrg" in err(tmp_path, "cutoff_enrg : 5\n" + LATTICE)
+    # "Risky" accepts distances 3 and 4
+    inf = lib.Info(write(tmp_path, "cutoff_enrg : 5\n" + LATTICE), "Risky")
+    assert inf.present("cut_off_energy")
+    # "Safe" (maximum distance 0) rejects even exact keys, as the reference's test does
+    assert "Unknown parameter - max_scf" in err(tmp_path, "max_scf : 5\n" + LATTICE, "Safe")
+
+
+def test_errors_match_reference_messages(tmp_path):
+    assert "Duplicate parameter found: max_scf" in err(tmp_path, "max_scf : 5\nmax_scf : 6\n" + LATTICE)
+    assert "Unknown task: phonons" in err(tmp_path, "task : phonons\n" + LATTICE)
+    assert "print_level  must be between 1 and 3" in err(tmp_path, "print_level : 4\n" + LATTICE)
+    assert "Invalid output: loud for key: output_level" in err(tmp_path, "output_level : loud\n" + LATTICE)
+    assert "Invalid energy unit" in err(tmp_path, "unit_energy : kcal\n" + LATTICE)
+    assert "Invalid length unit" in err(tmp_path, "unit_length : au\n" + LATTICE)
+    assert "Error parsing value: abc" in err(tmp_path, "max_scf : abc\n" + LATTICE)
+    assert "Error parsing keyword: lattice" in err(tmp_path, "lattice : 1\n" + LATTICE)
+    assert "should only be set in a data block" in err(tmp_path, "external_pot : gaussian_pot\n" + LATTICE)
+    assert "Potential file does not exist" in err(tmp_path, "external_pot : nothere.potex\n" + LATTICE)
+    other = tmp_path / "v.dat"
+    other.write_text("x")
+    assert "Unsupported external potential file" in err(tmp_path, "external_pot : %s\n" % other + LATTICE)
+    assert "Duplicate parameter found: external_pot" in err(
+        tmp_path, "external_pot : jelly\n$DATA gaussian_pot\n0.1 0.2 0.3 -1.0 1.0\n$ENDDATA gaussian_pot\n" + LATTICE)
+
+
+def test_blocks(tmp_path):
+    assert "LATTICE must have 3 lines" in err(tmp_path, "$DATA lattice\n1 0 0\n0 1 0\n$ENDDATA lattice\n")
+    assert "Block lattice not closed" in err(tmp_path, "$DATA lattice\n1 0 0\n0 1 0\n0 0 1\n")
+    assert "Unknown block type" in err(tmp_path, "$DATA species\nH 1\n$ENDDATA species\n" + LATTICE)
+    # rows starting with a blank are comments inside blocks (io.f90:1818-1824)
+    assert "No data in block" in err(tmp_path, "$DATA lattice\n 1 0 0\n 0 1 0\n 0 0 1\n$ENDDATA lattice\n")
+    with pytest.raises(lib.DerekError) as e:
+        lib.Info(write(tmp_path, "max_scf : 5\n")).cell()
+    assert "No lattice provided" in str(e.value)
+
+
+def test_output_level_presets(tmp_path):
+    inf = lib.Info(write(tmp_path, "output_level : all\nwrite_wvfn : f\n" + LATTICE))
+    for k in ("write_wvfn", "write_spec", "write_density", "write_potential", "write_formatted_density"):
+        assert inf.num(k) == 1
+    inf = lib.Info(write(tmp_path, "output_level : none\n" + LATTICE))
+    for k in ("write_state", "write_density", "write_potex"):
+        assert inf.num(k) == 0
+    inf = lib.Info(write(tmp_path, "task : dos\n" + LATTICE))
+    assert inf.num("write_spec") == 1
+
+
+def test_preset_potentials(tmp_path):
+    dims = (6, 6, 6)
+    inf = lib.Info(write(tmp_path, "n_electrons : 8\n" + LATTICE))
+    v = inf.preset_potential(dims)
+    assert np.all(v == 0.5 * 8 / 216)
+    inf = lib.Info(write(tmp_path, "external_pot : periodic_pot\nperiodic_pot_amp : 2.0\nunit_energy : Ha\n"
+                               "periodic_pot_grid : 1 2 1\n" + LATTICE))
+    v = inf.preset_potential(dims).reshape(dims)
+    fx = np.arange(1, 7) / 6.0
+    ref = 0.5 * 2.0 * np.sin(2 * np.pi * fx) * np.sin(2 * np.pi * 2 * fx) * np.sin(2 * np.pi * fx)
+    assert np.allclose(v[:, 0, 0], ref, atol=1e-15) and np.all(v == v[:, :1, :1])
+    inf = lib.Info(write(tmp_path, "external_pot : finite_barrier\nfinite_barrier_height : 3.0\nunit_energy : Ha\n"
+                               "unit_length : bohr\nfinite_barrier_width : 0.4 0.4 0.4\n" + LATTICE))
+    v = inf.preset_potential(dims).reshape(dims)
+    inside = np.abs(fx - 0.5) < 0.2
+    assert np.array_equal(v, 1.5 * (inside[:, None, None] & inside[None, :, None] & inside[None, None, :]))
+
+
+def test_help_texts():
+    assert "CUT_OFF_ENERGY" in lib.info_help("help", "cut_off_energy")
+    assert "NO MATCHING PARAMETERS" in lib.info_help("help", "nonsense")
+    s = lib.info_help("search", "unit")
+    assert "UNIT_ENERGY" in s and "UNIT_LENGTH_OUT" in s
+    assert lib.info_help("list").count("\n") == 49
