@@ -32,6 +32,9 @@ struct ScfEngine {
   int iscf = 0, m2 = 0, kc = 1;
   int64_t ldk = 0, kstride = 0;
   bool fp32 = false, graphs = false;
+  // the wavefunctions may be off orthonormal (just promoted from complex64):
+  // the next FP64 step forms the full subspace overlaps, which restores it
+  bool full_overlaps = false;
   int refine = 3, refine_left = 0;
   double switch_tol = 1e-5, e_prev = 0.0, de_prev = 1e300;
   bool have_prev = false, have_rho_out = false;
@@ -48,7 +51,7 @@ struct ScfEngine {
   double* rep_h = nullptr;  // pinned
   struct Graph {
     int nit;
-    bool fp32;
+    bool fp32, full;
     hipGraphExec_t exec;
   };
   std::vector<Graph> cache;
@@ -104,6 +107,30 @@ void ov(derek_ctx* c, const cx_t<R>* X, const cx_t<R>* Yp, int m, int n, int k0,
   c->timed(KC_OVERLAP, (double)(herm ? m : m + n) * npw * sizeof(cx_t<R>), 8.0 * npw * m * n, [&] {
     overlap<R>(X, E.ldk, E.kstride, Yp, E.ldk, E.kstride, m, n, c->max_npw, c->d_npw + k0, C, m, (int64_t)m * n, nkc,
                E.ovw, E.ovw_cap, herm, c->s);
+  });
+}
+
+// The X^H R / R^H R block form pays when it multiplies at least 20 % fewer
+// 64 x 64 tiles than the Hermitian 2nb x 2nb overlap (it needs one more
+// launch per matrix): from about 128 bands on.
+static bool block_overlaps(int nb) {
+  const int64_t t = ceil_div(nb, 64), T = ceil_div(2 * nb, 64);
+  return 5 * (t * t + t * (t + 1) / 2) < 4 * (T * (T + 1) / 2);
+}
+
+// Block (rows of column block ca, columns of block cb) of the 2nb x 2nb
+// subspace matrix C[kz] (ld 2nb): the nb x nb product X[:, ca:ca+nb]^H
+// Yp[:, cb:cb+nb] for the batch k0..k0+nkc-1.
+template <typename R>
+void ov_block(derek_ctx* c, const cx_t<R>* X, int ca, const cx_t<R>* Yp, int cb, int k0, int nkc, double2* C,
+              bool herm) {
+  ScfEngine& E = *c->scf;
+  const int nb = c->nb, m2 = E.m2;
+  const double npw = npw_sum(c, k0, nkc);
+  c->timed(KC_OVERLAP, (double)(herm ? nb : 2 * nb) * npw * sizeof(cx_t<R>), 8.0 * npw * nb * nb, [&] {
+    overlap<R>(X + (int64_t)ca * E.ldk, E.ldk, E.kstride, Yp + (int64_t)cb * E.ldk, E.ldk, E.kstride, nb, nb,
+               c->max_npw, c->d_npw + k0, C + ca + (int64_t)cb * m2, m2, (int64_t)m2 * m2, nkc, E.ovw, E.ovw_cap,
+               herm, c->s);
   });
 }
 
@@ -182,8 +209,18 @@ template <typename R> int davidson_chunk(derek_ctx* c, int k0, int nkc, int nit,
                    s);
     });
     c->apply_h<R>(k0, nkc, pb(Y, nb), pb(HY, nb), nb);
-    ov<R>(c, Y, Y, m2, m2, k0, nkc, S, true);
-    ov<R>(c, Y, HY, m2, m2, k0, nkc, H, true);
+    if (f32 || E.full_overlaps || !block_overlaps(nb)) {
+      ov<R>(c, Y, Y, m2, m2, k0, nkc, S, true);
+      ov<R>(c, Y, HY, m2, m2, k0, nkc, H, true);
+    } else {
+      // X holds orthonormal Ritz vectors: S(X,X) = I, H(X,X) = diag(eps);
+      // only X^H R, R^H R, X^H HR and R^H HR are multiplied
+      ov_block<R>(c, Y, 0, Y, nb, k0, nkc, S, false);
+      ov_block<R>(c, Y, nb, Y, nb, k0, nkc, S, true);
+      ov_block<R>(c, Y, 0, HY, nb, k0, nkc, H, false);
+      ov_block<R>(c, Y, nb, HY, nb, k0, nkc, H, true);
+      c->timed(KC_OTHER, 0, 0, [&] { subspace_blocks(S, H, eps, m2, nb, nkc, s); });
+    }
     c->timed(KC_DENSE, 0, 0, [&] { gen_solve(E.dw, S, H, m2, nb, nkc, drop, eps, U, s); });
     rot<R>(c, Y, U, m2, nb, (int64_t)m2 * nb, k0, nkc, Y2);
     rot<R>(c, HY, U, m2, nb, (int64_t)m2 * nb, k0, nkc, HY2);
@@ -218,6 +255,7 @@ template <typename R> void enqueue_step(derek_ctx* c, int nit) {
   hipStream_t s = c->s;
   potential(c);
   for (int k0 = 0; k0 < nk; k0 += E.kc) davidson_chunk<R>(c, k0, std::min(E.kc, nk - k0), nit, 0.0);
+  E.full_overlaps = false;
   occupations(E.eps, c->d_wk, nk, nb, E.P.nelec, E.P.kT, c->volume, E.occ, E.occw, E.rep, s);
   c->density<R>(0, nk, PwBlock<R>{reinterpret_cast<cx_t<R>*>(E.Y), E.ldk, E.kstride}, nb, E.occw, nb, E.rho_out,
                 false);
@@ -247,11 +285,12 @@ void run_step(derek_ctx* c, int nit) {
     return;
   }
   ScfEngine::Graph* g = nullptr;
+  const bool full = E.full_overlaps;
   for (auto& e : E.cache)
-    if (e.nit == nit && e.fp32 == E.fp32) g = &e;
+    if (e.nit == nit && e.fp32 == E.fp32 && e.full == full) g = &e;
   if (!g) {
     enqueue();
-    E.cache.push_back({nit, E.fp32, nullptr});
+    E.cache.push_back({nit, E.fp32, full, nullptr});
     return;
   }
   if (!g->exec) {
@@ -289,6 +328,7 @@ void switch_to_fp64(derek_ctx* c) {
     zero<double>(reinterpret_cast<double2*>(*q), (int64_t)n, c->s);
   }
   E.fp32 = false;
+  E.full_overlaps = true;
   c->S32.release();
   c->P32.release();
 }

@@ -475,4 +475,33 @@ INST(double)
 INST(float)
 #undef INST
 
+// ------------------------------------------------------------------ Davidson subspace blocks
+// For Y = [X, R] with orthonormal Ritz vectors X: S(X,X) = I and
+// H(X,X) = diag(eps) (per matrix kz of the batch, m2 x m2, ld m2), and the
+// lower-left blocks S(R,X), H(R,X) mirrored from the computed upper-right
+// ones.
+__global__ void k_subspace_blocks(double2* __restrict__ S, double2* __restrict__ H, const double* __restrict__ eps,
+                                  int m2, int nb) {
+  const int kz = blockIdx.y;
+  double2* s = S + (int64_t)kz * m2 * m2;
+  double2* h = H + (int64_t)kz * m2 * m2;
+  const double* e = eps + (int64_t)kz * nb;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nb * nb; q += gridDim.x * blockDim.x) {
+    const int i = q % nb, j = q / nb;
+    s[i + (int64_t)j * m2] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+    h[i + (int64_t)j * m2] = make_double2(i == j ? e[i] : 0.0, 0.0);
+    // (nb + i, j) = conj((j, nb + i))
+    const double2 a = s[j + (int64_t)(nb + i) * m2], b = h[j + (int64_t)(nb + i) * m2];
+    s[nb + i + (int64_t)j * m2] = make_double2(a.x, -a.y);
+    h[nb + i + (int64_t)j * m2] = make_double2(b.x, -b.y);
+  }
+}
+
+void subspace_blocks(double2* S, double2* H, const double* eps, int m2, int nb, int nkc, hipStream_t s) {
+  DEREK_REQUIRE(m2 == 2 * nb, "subspace blocks: m2 must be 2 nb");
+  const int nblk = (int)std::min<int64_t>(64, ceil_div((int64_t)nb * nb, NT));
+  hipLaunchKernelGGL(k_subspace_blocks, dim3(nblk, nkc), dim3(NT), 0, s, S, H, eps, m2, nb);
+  DEREK_HIP(hipGetLastError());
+}
+
 }  // namespace derek

@@ -52,6 +52,9 @@ void finish_max(const double* x, int n, double* rep, int slot, hipStream_t s);
 // Pulay step on device: push (rho_in, rho_out - rho_in) into the ring, update
 // the residual overlaps A (H x H), solve for the coefficients and form
 // r_opt = sum c_i rho_in,i and f_opt = sum c_i F_i.  st[0] counts pushes.
+// S(X,X) = I, H(X,X) = diag(eps), lower-left blocks mirrored (Y = [X, R]).
+void subspace_blocks(double2* S, double2* H, const double* eps, int m2, int nb, int nkc, hipStream_t s);
+
 // partial: H * pulay_blocks(n) doubles.
 int pulay_blocks(int64_t n);
 void pulay_update(int* st, int H, const double* rin, const double* rout, double* hin, double* hres, double* A,
