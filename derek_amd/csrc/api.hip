@@ -100,7 +100,7 @@ derek_ctx::~derek_ctx() {
     dfree(k.mapbuf);
   }
   for (void* p : {(void*)d_maps, (void*)d_npw, (void*)d_wk, (void*)gsq, (void*)vext, (void*)vloc, (void*)vloc32,
-                  (void*)cgrid})
+                  (void*)vt64, (void*)vt32, (void*)cgrid})
     dfree(p);
   S64.release();
   P64.release();
@@ -119,6 +119,18 @@ template <> const Axis<double>* derek_ctx::axes<double>() const { return ax64; }
 template <> const Axis<float>* derek_ctx::axes<float>() const { return ax32; }
 template <> const double* derek_ctx::vlocal<double>() const { return vloc; }
 template <> const float* derek_ctx::vlocal<float>() const { return vloc32; }
+template <> const double* derek_ctx::vtrans<double>() const { return vt64; }
+template <> const float* derek_ctx::vtrans<float>() const { return vt32; }
+template <typename R> bool derek_ctx::use_slab() const {
+  return fused && slab_supported<R>(dims[0], dims[1], dims[2], nax_max, nst_max);
+}
+template bool derek_ctx::use_slab<double>() const;
+template bool derek_ctx::use_slab<float>() const;
+
+void derek_ctx::vloc_changed(bool f32) {
+  if (f32) convert_real_f(vloc32, vloc, N, s);
+  transpose_xz(vloc, vt64, f32 ? vt32 : nullptr, dims[0], dims[1], dims[2], 1.0 / (double)N, s);
+}
 template <> double2* derek_ctx::sticks<double>(int items) {
   return S64.get<double2>((size_t)items * nst_max * dims[2]);
 }
@@ -159,10 +171,10 @@ void derek_ctx::apply_h(int k0, int nkc, PwBlock<R> psi, PwBlock<R> hpsi, int nb
   const R* v = vlocal<R>();
   const double sc = sizeof(cx_t<R>);
   const int64_t nyz = (int64_t)dims[1] * dims[2];
+  const bool fz = use_slab<R>();
   for_item_launches(this, k0, nkc, nbands, [&](int ka, int kk, int kn, int b0, int nbb) {
     PwBatch B{d_maps, ka, kn, nbb, nst_max, nax_max};
     cx_t<R>* S = sticks<R>(kn * nbb);
-    cx_t<R>* P = planes<R>(kn * nbb);
     double npw = 0, nst = 0, nax = 0;
     for (int k = ka; k < ka + kn; ++k) {
       npw += (double)ks[k].npw;
@@ -172,13 +184,20 @@ void derek_ctx::apply_h(int k0, int nkc, PwBlock<R> psi, PwBlock<R> hpsi, int nb
     const double pw_b = nbb * npw * sc, st_b = nbb * nst * dims[2] * sc, pl_b = nbb * nax * nyz * sc;
     PwBlock<R> pi{psi.p + kk * psi.ks + (int64_t)b0 * psi.ld, psi.ld, psi.ks};
     PwBlock<R> ho{hpsi.p + kk * hpsi.ks + (int64_t)b0 * hpsi.ld, hpsi.ld, hpsi.ks};
-    timed(KC_STICKS_Z_INV, pw_b + 12.0 * npw + st_b, 0, [&] { sticks_z_inv<R>(B, pi, ax, dims[2], S, s); });
-    timed(KC_PLANES_Y_INV, st_b + pl_b, 0, [&] { planes_y<R>(B, S, P, ax, dims[1], dims[2], +1, bpw, s); });
-    timed(KC_PLANES_X_VMUL, 2 * pl_b + (double)N * sizeof(R), 0,
-          [&] { planes_x_vmul<R>(B, P, ax, nyz, v, (R)(1.0 / (double)N), bpw, s); });
-    timed(KC_PLANES_Y_FWD, st_b + pl_b, 0, [&] { planes_y<R>(B, S, P, ax, dims[1], dims[2], -1, bpw, s); });
+    timed(KC_STICKS_Z_INV, pw_b + 12.0 * npw + st_b, 0, [&] { sticks_z_inv<R>(B, pi, ax, dims[2], S, fz, s); });
+    if (fz) {
+      // sticks in and out once, V once per launch
+      timed(KC_SLAB_VMUL, 2 * st_b + (double)N * sizeof(R), 0,
+            [&] { slab_vmul<R>(B, S, ax, dims[2], vtrans<R>(), bpw, s); });
+    } else {
+      cx_t<R>* P = planes<R>(kn * nbb);
+      timed(KC_PLANES_Y_INV, st_b + pl_b, 0, [&] { planes_y<R>(B, S, P, ax, dims[1], dims[2], +1, bpw, s); });
+      timed(KC_PLANES_X_VMUL, 2 * pl_b + (double)N * sizeof(R), 0,
+            [&] { planes_x_vmul<R>(B, P, ax, nyz, v, (R)(1.0 / (double)N), bpw, s); });
+      timed(KC_PLANES_Y_FWD, st_b + pl_b, 0, [&] { planes_y<R>(B, S, P, ax, dims[1], dims[2], -1, bpw, s); });
+    }
     timed(KC_STICKS_Z_FWD, st_b + 2 * pw_b + 20.0 * npw, 0,
-          [&] { sticks_z_fwd<R>(B, S, ax, dims[2], pi, ho, s); });
+          [&] { sticks_z_fwd<R>(B, S, ax, dims[2], pi, ho, fz, s); });
   });
 }
 
@@ -203,7 +222,7 @@ void derek_ctx::density(int k0, int nkc, PwBlock<R> psi, int nbands, const doubl
     const int kp = (keep || !first) ? 1 : 0;
     first = false;
     PwBlock<R> pi{psi.p + kk * psi.ks + (int64_t)b0 * psi.ld, psi.ld, psi.ks};
-    timed(KC_STICKS_Z_INV, pw_b + 12.0 * npw + st_b, 0, [&] { sticks_z_inv<R>(B, pi, ax, dims[2], S, s); });
+    timed(KC_STICKS_Z_INV, pw_b + 12.0 * npw + st_b, 0, [&] { sticks_z_inv<R>(B, pi, ax, dims[2], S, false, s); });
     timed(KC_PLANES_Y_INV, st_b + pl_b, 0, [&] { planes_y<R>(B, S, P, ax, dims[1], dims[2], +1, bpw, s); });
     timed(KC_PLANES_X_DENSITY, pl_b + (1 + kp) * 8.0 * N, 0, [&] {
       planes_x_density<R>(B, P, ax, nyz, occw + (int64_t)kk * occ_ks + b0, occ_ks, rho, kp, s);
@@ -575,6 +594,8 @@ int32_t derek_hip_init(derek_ctx** out, const derek_dims* d) {
     c->vext = dmalloc<double>(c->N);
     c->vloc = dmalloc<double>(c->N);
     c->vloc32 = dmalloc<float>(c->N);
+    c->vt64 = dmalloc<double>(c->N);
+    c->vt32 = dmalloc<float>(c->N);
     c->cgrid = dmalloc<double2>(c->N);
     // (k-point, band) items per pipeline launch; the SCF autotuner may change it
     c->item_cap = 16;
@@ -610,7 +631,7 @@ int32_t derek_hip_set_basis(derek_ctx* c, int32_t k, int64_t npw, const int32_t*
     const PwMapsHost h = build_pw_maps(idx, npw, c->dims[0], c->dims[1], c->dims[2]);
     dfree(K.mapbuf);
     std::vector<int32_t> flat;
-    for (const std::vector<int32_t>* v : {&h.perm, &h.pw_z, &h.pw_st, &h.st_pw, &h.st_y, &h.pl_st, &h.ax_x})
+    for (const std::vector<int32_t>* v : {&h.perm, &h.pw_z, &h.pw_st, &h.st_pw, &h.st_y, &h.pl_st, &h.ax_x, &h.pl_y})
       flat.insert(flat.end(), v->begin(), v->end());
     K.mapbuf = dmalloc<int32_t>(flat.size());
     DEREK_HIP(hipMemcpy(K.mapbuf, flat.data(), flat.size() * sizeof(int32_t), hipMemcpyHostToDevice));
@@ -631,6 +652,8 @@ int32_t derek_hip_set_basis(derek_ctx* c, int32_t k, int64_t npw, const int32_t*
     K.maps.pl_st = q;
     q += h.nax + 1;
     K.maps.ax_x = q;
+    q += h.nax;
+    K.maps.pl_y = q;
     K.maps.kin = K.kin;
     K.weight = weight;
     K.set = true;
@@ -660,7 +683,7 @@ int32_t derek_hip_set_vloc(derek_ctx* c, const double* v) {
   return guard([&] {
     DEREK_REQUIRE(c && v, "null argument");
     DEREK_HIP(hipMemcpy(c->vloc, v, c->N * sizeof(double), hipMemcpyHostToDevice));
-    convert_real_f(c->vloc32, c->vloc, c->N, c->s);
+    c->vloc_changed(true);
     c->sync();
     c->vloc_set = true;
   });
@@ -995,12 +1018,26 @@ int32_t derek_hip_apply_h_resident(derek_ctx* c, int32_t k, int32_t nb) {
   });
 }
 
-int32_t derek_hip_get_tuning(derek_ctx* c, int32_t out[4]) {
+int32_t derek_hip_get_tuning(derek_ctx* c, int32_t out[5]) {
   return guard([&] {
     DEREK_REQUIRE(c && out, "null argument");
     out[0] = c->bpw;
     out[1] = c->item_cap;
     scf_tuning(c, out + 2);
+    out[4] = c->fused;
+  });
+}
+
+int32_t derek_hip_set_launch(derek_ctx* c, int32_t bands_per_wg, int32_t items_per_launch, int32_t fused) {
+  return guard([&] {
+    DEREK_REQUIRE(c, "null context");
+    DEREK_REQUIRE(bands_per_wg == 1 || bands_per_wg == 2 || bands_per_wg == 4 || bands_per_wg == 8,
+                  "bands per workgroup must be 1, 2, 4 or 8");
+    DEREK_REQUIRE(items_per_launch >= 1, "items per launch must be positive");
+    DEREK_REQUIRE(fused == 0 || fused == 1, "fused must be 0 or 1");
+    c->bpw = bands_per_wg;
+    c->item_cap = items_per_launch;
+    c->fused = fused;
   });
 }
 

@@ -68,6 +68,7 @@ enum KernelClass {
   KC_FFT_X,
   KC_DENSE,
   KC_OTHER,
+  KC_SLAB_VMUL,
   KC_COUNT
 };
 
@@ -127,10 +128,14 @@ struct derek_ctx {
   double* vext = nullptr;
   double* vloc = nullptr;
   float* vloc32 = nullptr;
+  // vloc / N with x and z swapped, [z][y][x], FP64 and FP32, for the fused y/x pass
+  double* vt64 = nullptr;
+  float* vt32 = nullptr;
   bool gsq_set = false, vloc_set = false, vext_set = false;
   // launch shapes (autotuned by the SCF engine when asked)
   int bpw = 4;         // bands per x-pass workgroup
   int item_cap = 64;   // (k-point, band) items per pipeline launch
+  int fused = 0;       // 1: fused y/x pass on z-major sticks where the grid allows it
   // pruned-FFT work buffers for item_cap items
   derek::DevBuf S64, P64, S32, P32;
   template <typename R> derek::cx_t<R>* sticks(int items);
@@ -156,6 +161,10 @@ struct derek_ctx {
   ~derek_ctx();
   template <typename R> const derek::Axis<R>* axes() const;
   template <typename R> const R* vlocal() const;
+  template <typename R> const R* vtrans() const;
+  template <typename R> bool use_slab() const;
+  // after vloc changed: its FP32 copy (when f32) and the transposed copies
+  void vloc_changed(bool f32);
   void require_basis(int k) const;
   void require_all_bases() const;
   void sync() { DEREK_HIP(hipStreamSynchronize(s)); }

@@ -217,6 +217,18 @@ template <int R, int S, typename C> __device__ __forceinline__ C root(int m) {
   return cmk<C>((Rl)c, (Rl)(S > 0 ? s : -s));
 }
 
+// t * exp(S*2*pi*i*m/R) for a compile-time m; quarter turns are sign flips
+// and swaps, which fold into the neighbouring adds.
+template <int R, int S, typename C> __device__ __forceinline__ C mul_root(C t, int m) {
+  const int q = m % R;
+  if (q == 0) return t;
+  if ((4 * q) % R == 0) {
+    const int h = 4 * q / R;
+    return h == 1 ? cmul_si<S>(t) : (h == 2 ? cmk<C>(-t.x, -t.y) : cmul_si<-S>(t));
+  }
+  return cmul(t, root<R, S, C>(q));
+}
+
 // Composite R = A*B in registers: n = B*a + b, k = ka + A*kb;
 // B DFTs of length A, twiddles exp(S*2*pi*i*b*ka/R), A DFTs of length B.
 template <int A, int B, int S, typename C> __device__ __forceinline__ void dft_comp(C* v, const C* tw, int N) {
@@ -229,7 +241,7 @@ template <int A, int B, int S, typename C> __device__ __forceinline__ void dft_c
     for (int a = 0; a < A; ++a) t[a] = v[B * a + b];
     dftR<A, S>(t, tw, N);
 #pragma unroll
-    for (int ka = 0; ka < A; ++ka) y[b][ka] = (b * ka) % R ? cmul(t[ka], root<R, S, C>(b * ka)) : t[ka];
+    for (int ka = 0; ka < A; ++ka) y[b][ka] = mul_root<R, S>(t[ka], b * ka);
   }
 #pragma unroll
   for (int ka = 0; ka < A; ++ka) {

@@ -349,6 +349,34 @@ __global__ void k_convert_real_f(float* dst, const double* src, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     dst[i] = (float)src[i];
 }
+// t[(z*ny + y)*nx + x] = scale * v[(x*ny + y)*nz + z] (and its FP32 copy when
+// t32 is given): 32 x 32 (x, z) tiles of one y through LDS.
+__global__ __launch_bounds__(256) void k_transpose_xz(const double* __restrict__ v, double* __restrict__ t64,
+                                                      float* __restrict__ t32, int nx, int ny, int nz, double scale) {
+  __shared__ double tile[32][33];
+  const int z0 = blockIdx.x * 32, x0 = blockIdx.y * 32, y = blockIdx.z;
+  const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
+  for (int i = ty; i < 32; i += 8) {
+    const int x = x0 + i, z = z0 + tx;
+    tile[i][tx] = (x < nx && z < nz) ? v[((int64_t)x * ny + y) * nz + z] * scale : 0.0;
+  }
+  __syncthreads();
+  for (int i = ty; i < 32; i += 8) {
+    const int z = z0 + i, x = x0 + tx;
+    if (z < nz && x < nx) {
+      const int64_t o = ((int64_t)z * ny + y) * nx + x;
+      const double w = tile[tx][i];
+      t64[o] = w;
+      if (t32) t32[o] = (float)w;
+    }
+  }
+}
+
+void transpose_xz(const double* v, double* t64, float* t32, int nx, int ny, int nz, double scale, hipStream_t s) {
+  dim3 grid((unsigned)ceil_div(nz, 32), (unsigned)ceil_div(nx, 32), (unsigned)ny);
+  hipLaunchKernelGGL(k_transpose_xz, grid, dim3(256), 0, s, v, t64, t32, nx, ny, nz, scale);
+}
+
 void convert_real_f(float* dst, const double* src, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_convert_real_f, dim3(grid1d(n)), dim3(256), 0, s, dst, src, n);
   DEREK_HIP(hipGetLastError());

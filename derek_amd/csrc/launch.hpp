@@ -50,6 +50,8 @@ void sub(double* out, const double* a, const double* b, int64_t n, hipStream_t s
 template <typename Rd, typename Rs>
 void convert(cx_t<Rd>* dst, const cx_t<Rs>* src, int64_t n, hipStream_t s);
 void convert_real_f(float* dst, const double* src, int64_t n, hipStream_t s);
+// t = scale * v with x and z swapped: t[(z*ny + y)*nx + x]; t32 (nullable) its FP32 copy.
+void transpose_xz(const double* v, double* t64, float* t32, int nx, int ny, int nz, double scale, hipStream_t s);
 
 // ---- tall-skinny complex GEMMs (MFMA), batched: matrix kz of a batch is at
 // X + kz*sx, Y + kz*sy, C + kz*sc ...; npwk (device, nullable) gives each

@@ -692,7 +692,7 @@ __device__ __forceinline__ void stick_two_stage(C* line, const C* twl, int j, bo
 }
 
 // psi[b][perm[p]] -> S[item][s][:] for the block's sticks
-template <typename R, int N>
+template <typename R, int N, bool ZM>
 __global__ __launch_bounds__(StickSplit<N>::NT) void k_sticks_z_inv2(PwBatch B, PwBlock<R> psi,
                                                                      const cx_t<R>* __restrict__ tw,
                                                                      cx_t<R>* __restrict__ S) {
@@ -716,15 +716,24 @@ __global__ __launch_bounds__(StickSplit<N>::NT) void k_sticks_z_inv2(PwBatch B, 
   __syncthreads();
   const int line = threadIdx.x / TPL, j = threadIdx.x % TPL;
   stick_two_stage<+1, C, N>(buf + line * LS, twl, j, line < lines);
-  C* dst = S + (((int64_t)blockIdx.z * B.nbb + band) * B.nst_cap + s0) * N;
-  for (int e = threadIdx.x; e < lines * N; e += NT) {
-    const int q = e / N;
-    dst[e] = buf[q * LS + (e - q * N)];
+  if constexpr (ZM) {
+    // z-major sticks: row z of the item holds every stick at that z
+    C* dst = S + ((int64_t)blockIdx.z * B.nbb + band) * B.nst_cap * N + s0;
+    for (int e = threadIdx.x; e < lines * N; e += NT) {
+      const int z = e / lines, q = e - z * lines;
+      dst[(int64_t)z * B.nst_cap + q] = buf[q * LS + z];
+    }
+  } else {
+    C* dst = S + (((int64_t)blockIdx.z * B.nbb + band) * B.nst_cap + s0) * N;
+    for (int e = threadIdx.x; e < lines * N; e += NT) {
+      const int q = e / N;
+      dst[e] = buf[q * LS + (e - q * N)];
+    }
   }
 }
 
 // hpsi[b][perm[p]] = (z FFT of the stick)[pw_z[p]] + kin[perm[p]] psi[b][perm[p]]
-template <typename R, int N>
+template <typename R, int N, bool ZM>
 __global__ __launch_bounds__(StickSplit<N>::NT) void k_sticks_z_fwd2(PwBatch B, const cx_t<R>* __restrict__ S,
                                                                      const cx_t<R>* __restrict__ tw, PwBlock<R> psi,
                                                                      PwBlock<R> hpsi) {
@@ -740,10 +749,18 @@ __global__ __launch_bounds__(StickSplit<N>::NT) void k_sticks_z_fwd2(PwBatch B, 
   const int lines = min(SL, m.nst - s0);
   const int band = blockIdx.y;
   for (int t = threadIdx.x; t < N; t += NT) twl[t] = tw[t];
-  const C* src = S + (((int64_t)blockIdx.z * B.nbb + band) * B.nst_cap + s0) * N;
-  for (int e = threadIdx.x; e < lines * N; e += NT) {
-    const int q = e / N;
-    buf[q * LS + (e - q * N)] = src[e];
+  if constexpr (ZM) {
+    const C* src = S + ((int64_t)blockIdx.z * B.nbb + band) * B.nst_cap * N + s0;
+    for (int e = threadIdx.x; e < lines * N; e += NT) {
+      const int z = e / lines, q = e - z * lines;
+      buf[q * LS + z] = src[(int64_t)z * B.nst_cap + q];
+    }
+  } else {
+    const C* src = S + (((int64_t)blockIdx.z * B.nbb + band) * B.nst_cap + s0) * N;
+    for (int e = threadIdx.x; e < lines * N; e += NT) {
+      const int q = e / N;
+      buf[q * LS + (e - q * N)] = src[e];
+    }
   }
   __syncthreads();
   const int line = threadIdx.x / TPL, j = threadIdx.x % TPL;
@@ -783,10 +800,11 @@ template <typename R> static void check_offsets(const PwBatch& B, int64_t nyz, i
 }
 
 template <typename R, int N>
-static void sticks_z_inv2(const PwBatch& B, PwBlock<R> psi, const Axis<R>* ax, cx_t<R>* S, hipStream_t s) {
+static void sticks_z_inv2(const PwBatch& B, PwBlock<R> psi, const Axis<R>* ax, cx_t<R>* S, bool zmajor,
+                          hipStream_t s) {
   using SS = StickSplit<N>;
   const size_t lds = SS::lds(sizeof(cx_t<R>));
-  auto kern = k_sticks_z_inv2<R, N>;
+  auto kern = zmajor ? k_sticks_z_inv2<R, N, true> : k_sticks_z_inv2<R, N, false>;
   allow_lds_pw(kern, lds);
   dim3 grid((unsigned)ceil_div(B.nst_cap, SS::SL), B.nbb, B.nkc);
   hipLaunchKernelGGL(kern, grid, dim3(SS::NT), lds, s, B, psi, ax[2].tw, S);
@@ -794,25 +812,27 @@ static void sticks_z_inv2(const PwBatch& B, PwBlock<R> psi, const Axis<R>* ax, c
 
 template <typename R, int N>
 static void sticks_z_fwd2(const PwBatch& B, const cx_t<R>* S, const Axis<R>* ax, PwBlock<R> psi, PwBlock<R> hpsi,
-                          hipStream_t s) {
+                          bool zmajor, hipStream_t s) {
   using SS = StickSplit<N>;
   const size_t lds = SS::lds(sizeof(cx_t<R>));
-  auto kern = k_sticks_z_fwd2<R, N>;
+  auto kern = zmajor ? k_sticks_z_fwd2<R, N, true> : k_sticks_z_fwd2<R, N, false>;
   allow_lds_pw(kern, lds);
   dim3 grid((unsigned)ceil_div(B.nst_cap, SS::SL), B.nbb, B.nkc);
   hipLaunchKernelGGL(kern, grid, dim3(SS::NT), lds, s, B, S, ax[2].tw, psi, hpsi);
 }
 
 template <typename R>
-void sticks_z_inv(const PwBatch& B, PwBlock<R> psi, const Axis<R>* ax, int nz, cx_t<R>* S, hipStream_t s) {
+void sticks_z_inv(const PwBatch& B, PwBlock<R> psi, const Axis<R>* ax, int nz, cx_t<R>* S, bool zmajor,
+                  hipStream_t s) {
   check_batch(B);
   switch (nz) {
-    case 32: sticks_z_inv2<R, 32>(B, psi, ax, S, s); DEREK_HIP(hipGetLastError()); return;
-    case 72: sticks_z_inv2<R, 72>(B, psi, ax, S, s); DEREK_HIP(hipGetLastError()); return;
-    case 108: sticks_z_inv2<R, 108>(B, psi, ax, S, s); DEREK_HIP(hipGetLastError()); return;
-    case 144: sticks_z_inv2<R, 144>(B, psi, ax, S, s); DEREK_HIP(hipGetLastError()); return;
+    case 32: sticks_z_inv2<R, 32>(B, psi, ax, S, zmajor, s); DEREK_HIP(hipGetLastError()); return;
+    case 72: sticks_z_inv2<R, 72>(B, psi, ax, S, zmajor, s); DEREK_HIP(hipGetLastError()); return;
+    case 108: sticks_z_inv2<R, 108>(B, psi, ax, S, zmajor, s); DEREK_HIP(hipGetLastError()); return;
+    case 144: sticks_z_inv2<R, 144>(B, psi, ax, S, zmajor, s); DEREK_HIP(hipGetLastError()); return;
     default: break;
   }
+  DEREK_REQUIRE(!zmajor, "z-major sticks need a z length with a compiled kernel");
   const int lpb = sticks_per_block(nz);
   const size_t lds = sizeof(cx_t<R>) * ((size_t)nz + 2 * (size_t)lpb * contig_stride(nz));
   check_lds_pw(lds, nz);
@@ -887,15 +907,16 @@ void planes_y(const PwBatch& B, cx_t<R>* S, cx_t<R>* P, const Axis<R>* ax, int n
 
 template <typename R>
 void sticks_z_fwd(const PwBatch& B, const cx_t<R>* S, const Axis<R>* ax, int nz, PwBlock<R> psi, PwBlock<R> hpsi,
-                  hipStream_t s) {
+                  bool zmajor, hipStream_t s) {
   check_batch(B);
   switch (nz) {
-    case 32: sticks_z_fwd2<R, 32>(B, S, ax, psi, hpsi, s); DEREK_HIP(hipGetLastError()); return;
-    case 72: sticks_z_fwd2<R, 72>(B, S, ax, psi, hpsi, s); DEREK_HIP(hipGetLastError()); return;
-    case 108: sticks_z_fwd2<R, 108>(B, S, ax, psi, hpsi, s); DEREK_HIP(hipGetLastError()); return;
-    case 144: sticks_z_fwd2<R, 144>(B, S, ax, psi, hpsi, s); DEREK_HIP(hipGetLastError()); return;
+    case 32: sticks_z_fwd2<R, 32>(B, S, ax, psi, hpsi, zmajor, s); DEREK_HIP(hipGetLastError()); return;
+    case 72: sticks_z_fwd2<R, 72>(B, S, ax, psi, hpsi, zmajor, s); DEREK_HIP(hipGetLastError()); return;
+    case 108: sticks_z_fwd2<R, 108>(B, S, ax, psi, hpsi, zmajor, s); DEREK_HIP(hipGetLastError()); return;
+    case 144: sticks_z_fwd2<R, 144>(B, S, ax, psi, hpsi, zmajor, s); DEREK_HIP(hipGetLastError()); return;
     default: break;
   }
+  DEREK_REQUIRE(!zmajor, "z-major sticks need a z length with a compiled kernel");
   const int lpb = sticks_per_block(nz);
   const size_t lds = sizeof(cx_t<R>) * ((size_t)nz + 2 * (size_t)lpb * contig_stride(nz));
   check_lds_pw(lds, nz);
@@ -985,11 +1006,233 @@ void planes_x_density(const PwBatch& B, const cx_t<R>* P, const Axis<R>* ax, int
   DEREK_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------- fused y / x pass per z plane
+// One workgroup per (z plane, group of bands of one k-point) for nx = ny = N.
+// The sticks are z-major (row z of an item holds every stick at that z) and
+// are transformed in place:
+//   A  y inverse of each active x plane into A[a][.] (LDS);
+//   B  x lines in rounds of LR: inverse transform of the active rows,
+//      * V(x, y, z) / (nx ny nz), forward transform, active rows back into A;
+//   C  y forward of each active plane, stick rows written back.
+// Each length-N transform is two register stages, N = R1 R2: R2 threads
+// (digit j2) take R1-point DFTs over the inputs j2 + R2 r, R1 threads (digit
+// k1) finish with R2-point DFTs; the result with index k1 + R1 k2 sits in
+// slot k1 R2P + k2 of its line (R2P = R2 rounded up to odd, so that lanes
+// striding by R2P hit different LDS banks).  Every stage writes the slots it
+// read, so one barrier per stage and no second buffer.  The workgroup size is
+// a multiple of R1 and R2: each thread keeps its digits, its twiddles and its
+// x rows in registers for the whole launch.  Compared with the separate y and
+// x passes the active planes stay on chip: HBM sees the stick rows once each
+// way and V once per band group.
+template <typename R, int N> struct Slab {
+  static constexpr int R1 = Split<N>::R1, R2 = Split<N>::R2;
+  static constexpr int R2P = R2 % 2 ? R2 : R2 + 1;
+  static constexpr int L = R1 * R2P;              // slots of a line
+  static constexpr int LP = L % 2 ? L : L + 1;    // line pitch
+  static constexpr int NT = N == 32 ? 256 : (N == 72 ? 288 : 504);
+  // x lines per round: the complex64 108 case is sized for two workgroups per CU
+  static constexpr int LR = N == 32 ? 32 : (N == 72 ? 24 : (N == 108 && sizeof(R) == 4 ? 27 : 36));
+  static constexpr int NR = (N + LR - 1) / LR;
+  static constexpr int LJ = NT / R2, LK = NT / R1;  // lines per pass of the R2- / R1-thread stages
+  static_assert(NT % R1 == 0 && NT % R2 == 0, "workgroup size must keep each thread's digits");
+  static_assert(LR <= LJ && LR <= LK, "one pass per stage of a round");
+  static constexpr size_t lds(int nax_cap) {
+    return sizeof(cx_t<R>) * (size_t)(nax_cap + LR) * LP + sizeof(int) * N +
+           sizeof(int16_t) * (((size_t)nax_cap * N + 1) / 2 * 2);
+  }
+};
+
+template <typename R, int N>
+__global__ __launch_bounds__((Slab<R, N>::NT)) void k_slab_vmul(PwBatch B, cx_t<R>* __restrict__ S, int nz, int bpw,
+                                                              const cx_t<R>* __restrict__ tw,
+                                                              const R* __restrict__ vt) {
+  using C = cx_t<R>;
+  using SL = Slab<R, N>;
+  constexpr int R1 = SL::R1, R2 = SL::R2, R2P = SL::R2P, NT = SL::NT, LR = SL::LR, NR = SL::NR, LP = SL::LP;
+  constexpr int LJ = SL::LJ, LK = SL::LK;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const PwMaps m = B.mt[B.k0 + blockIdx.z];
+  const int z = blockIdx.x, nax = m.nax, tid = threadIdx.x;
+  C* A = reinterpret_cast<C*>(smem);                           // [nax_cap][LP]
+  C* X = A + (size_t)B.nax_cap * LP;                           // [LR][LP]
+  int* xa = reinterpret_cast<int*>(X + LR * LP);               // active plane of x bin, -1 if none
+  int16_t* sty = reinterpret_cast<int16_t*>(xa + N);           // stick of (plane, y bin), -1 if none
+  const int j2 = tid % R2, lj = tid / R2, k1 = tid % R1, lk = tid / R1;
+  for (int x = tid; x < N; x += NT) xa[x] = -1;
+  for (int i = tid; i < nax * N; i += NT) sty[i] = (int16_t)m.pl_y[i];
+  C tk[R2];  // exp(-2 pi i k1 r / N), r >= 1
+#pragma unroll
+  for (int r = 1; r < R2; ++r) tk[r] = tw[(k1 * r) % N];
+  __syncthreads();
+  for (int a = tid; a < nax; a += NT) xa[m.ax_x[a]] = a;
+  __syncthreads();
+  int xr[R1];  // LDS row of the active plane of x = j2 + R2 r, -1 if none
+#pragma unroll
+  for (int r = 0; r < R1; ++r) {
+    const int a = xa[j2 + R2 * r];
+    xr[r] = a >= 0 ? a * LP : -1;
+  }
+  auto pos = [](int y) { return (y % R1) * R2P + y / R1; };  // slot of index y
+  auto stick_off = [](int st) { return st >= 0 ? (uint32_t)st * (uint32_t)sizeof(C) : kOob; };
+  const R* vz = vt + (int64_t)z * N * N;  // V at (x, y, z) in vz[y*N + x]
+  const int bb0 = blockIdx.y * bpw, bb1 = min(B.nbb, bb0 + bpw);
+  const uint32_t row_bytes = (uint32_t)B.nst_cap * (uint32_t)sizeof(C);
+  for (int band = bb0; band < bb1; ++band) {
+    const auto rs = buf_rsrc(S + (((int64_t)blockIdx.z * B.nbb + band) * nz + z) * B.nst_cap, row_bytes);
+    // A1: y = j2 + R2 r from the sticks, output k in slot k R2P + j2
+    for (int a = lj; a < nax; a += LJ) {
+      const int16_t* ty = sty + a * N + j2;
+      C v[R1];
+#pragma unroll
+      for (int r = 0; r < R1; ++r) v[r] = buf_ld<C>(rs, stick_off(ty[R2 * r]));
+      dftR<R1, +1>(v, tw, N);
+      C* row = A + a * LP + j2;
+#pragma unroll
+      for (int k = 0; k < R1; ++k) row[k * R2P] = v[k];
+    }
+    __syncthreads();
+    // A2: y = k1 + R1 k2 into slot k1 R2P + k2
+    for (int a = lk; a < nax; a += LK) {
+      C* row = A + a * LP + k1 * R2P;
+      C u[R2];
+#pragma unroll
+      for (int r = 0; r < R2; ++r) u[r] = r ? cmul(row[r], cconj(tk[r])) : row[r];
+      dftR<R2, +1>(u, tw, N);
+#pragma unroll
+      for (int k = 0; k < R2; ++k) row[k] = u[k];
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int q = 0; q < NR; ++q) {
+      const int y0 = q * LR, ln = min(LR, N - y0);
+      const bool b2 = lk < ln;
+      // V of this thread's B2 line, loaded before B1 and used after its barrier
+      R vq[R2];
+      {
+        const R* vy = vz + (y0 + (b2 ? lk : 0)) * N + k1;
+#pragma unroll
+        for (int k = 0; k < R2; ++k) vq[k] = vy[R1 * k];
+      }
+      // B1: line y = y0 + lj, active rows x = j2 + R2 r
+      if (lj < ln) {
+        const int py = pos(y0 + lj);
+        C v[R1];
+#pragma unroll
+        for (int r = 0; r < R1; ++r) v[r] = xr[r] >= 0 ? A[xr[r] + py] : cmk<C>(0, 0);
+        dftR<R1, +1>(v, tw, N);
+        C* row = X + lj * LP + j2;
+#pragma unroll
+        for (int k = 0; k < R1; ++k) row[k * R2P] = v[k];
+      }
+      __syncthreads();
+      // B2: finish the inverse (x = k1 + R1 k2), * V, start the forward transform
+      if (b2) {
+        C* row = X + lk * LP + k1 * R2P;
+        C u[R2];
+#pragma unroll
+        for (int r = 0; r < R2; ++r) u[r] = r ? cmul(row[r], cconj(tk[r])) : row[r];
+        dftR<R2, +1>(u, tw, N);
+#pragma unroll
+        for (int k = 0; k < R2; ++k) u[k] = cscale(u[k], vq[k]);
+        dftR<R2, -1>(u, tw, N);
+#pragma unroll
+        for (int k = 0; k < R2; ++k) row[k] = k ? cmul(u[k], tk[k]) : u[k];
+      }
+      __syncthreads();
+      // B3: finish the forward transform, x = j2 + R2 m1, active rows back into A
+      if (lj < ln) {
+        const int py = pos(y0 + lj);
+        const C* row = X + lj * LP + j2;
+        C t[R1];
+#pragma unroll
+        for (int j = 0; j < R1; ++j) t[j] = row[j * R2P];
+        dftR<R1, -1>(t, tw, N);
+#pragma unroll
+        for (int k = 0; k < R1; ++k)
+          if (xr[k] >= 0) A[xr[k] + py] = t[k];
+      }
+      __syncthreads();
+    }
+    // C1: y = j2 + R2 r, output k1 back into the slot of y = j2 + R2 k1
+    for (int a = lj; a < nax; a += LJ) {
+      C* row = A + a * LP;
+      C v[R1];
+#pragma unroll
+      for (int r = 0; r < R1; ++r) v[r] = row[pos(j2 + R2 * r)];
+      dftR<R1, -1>(v, tw, N);
+#pragma unroll
+      for (int k = 0; k < R1; ++k) row[pos(j2 + R2 * k)] = v[k];
+    }
+    __syncthreads();
+    // C2: y = k1 + R1 k2 to the sticks
+    for (int a = lk; a < nax; a += LK) {
+      const C* row = A + a * LP;
+      const int16_t* ty = sty + a * N + k1;
+      C u[R2];
+#pragma unroll
+      for (int j = 0; j < R2; ++j) {
+        const C w = row[pos(j + R2 * k1)];
+        u[j] = j ? cmul(w, tk[j]) : w;
+      }
+      dftR<R2, -1>(u, tw, N);
+#pragma unroll
+      for (int k = 0; k < R2; ++k) buf_st<C>(rs, stick_off(ty[R1 * k]), u[k]);
+    }
+    __syncthreads();
+  }
+}
+
+template <typename R, int N> static bool slab_fits(int nax_cap) {
+  return Slab<R, N>::lds(nax_cap) <= 160 * 1024;
+}
+
+template <typename R> bool slab_supported(int nx, int ny, int nz, int nax_cap, int nst_cap) {
+  if (nx != ny || nax_cap < 1 || nax_cap > nx || nst_cap > 32767) return false;
+  if (nz != 32 && nz != 72 && nz != 108 && nz != 144) return false;  // z-major stick kernels
+  switch (nx) {
+    case 32: return slab_fits<R, 32>(nax_cap);
+    case 72: return slab_fits<R, 72>(nax_cap);
+    case 108: return slab_fits<R, 108>(nax_cap);
+    case 144: return slab_fits<R, 144>(nax_cap);
+    default: return false;
+  }
+}
+
+template <typename R, int N>
+static void slab_vmul_n(const PwBatch& B, cx_t<R>* S, const Axis<R>* ax, int nz, const R* vt, int bpw, hipStream_t s) {
+  using SL = Slab<R, N>;
+  const size_t lds = SL::lds(B.nax_cap);
+  check_lds_pw(lds, N);
+  auto kern = k_slab_vmul<R, N>;
+  allow_lds_pw(kern, lds);
+  dim3 grid((unsigned)nz, (unsigned)ceil_div(B.nbb, bpw), B.nkc);
+  hipLaunchKernelGGL(kern, grid, dim3(SL::NT), lds, s, B, S, nz, bpw, ax[0].tw, vt);
+}
+
+template <typename R>
+void slab_vmul(const PwBatch& B, cx_t<R>* S, const Axis<R>* ax, int nz, const R* vt, int bpw, hipStream_t s) {
+  check_batch(B);
+  DEREK_REQUIRE(slab_supported<R>(ax[0].n, ax[1].n, nz, B.nax_cap, B.nst_cap),
+                "grid not supported by the fused y/x pass");
+  DEREK_REQUIRE(bpw >= 1, "bands per workgroup must be positive");
+  DEREK_REQUIRE((int64_t)B.nst_cap * (int64_t)sizeof(cx_t<R>) < (int64_t)kOob, "stick row exceeds 2 GiB");
+  switch (ax[0].n) {
+    case 32: slab_vmul_n<R, 32>(B, S, ax, nz, vt, bpw, s); break;
+    case 72: slab_vmul_n<R, 72>(B, S, ax, nz, vt, bpw, s); break;
+    case 108: slab_vmul_n<R, 108>(B, S, ax, nz, vt, bpw, s); break;
+    case 144: slab_vmul_n<R, 144>(B, S, ax, nz, vt, bpw, s); break;
+    default: throw ArgError("grid not supported by the fused y/x pass");
+  }
+  DEREK_HIP(hipGetLastError());
+}
+
 #define INST(R)                                                                                                   \
-  template void sticks_z_inv<R>(const PwBatch&, PwBlock<R>, const Axis<R>*, int, cx_t<R>*, hipStream_t);         \
+  template void sticks_z_inv<R>(const PwBatch&, PwBlock<R>, const Axis<R>*, int, cx_t<R>*, bool, hipStream_t);   \
   template void planes_y<R>(const PwBatch&, cx_t<R>*, cx_t<R>*, const Axis<R>*, int, int, int, int, hipStream_t); \
   template void sticks_z_fwd<R>(const PwBatch&, const cx_t<R>*, const Axis<R>*, int, PwBlock<R>, PwBlock<R>,      \
-                                hipStream_t);                                                                     \
+                                bool, hipStream_t);                                                               \
+  template bool slab_supported<R>(int, int, int, int, int);                                                       \
+  template void slab_vmul<R>(const PwBatch&, cx_t<R>*, const Axis<R>*, int, const R*, int, hipStream_t);          \
   template void planes_x_vmul<R>(const PwBatch&, cx_t<R>*, const Axis<R>*, int64_t, const R*, R, int, hipStream_t); \
   template void planes_x_density<R>(const PwBatch&, const cx_t<R>*, const Axis<R>*, int64_t, const double*, int,  \
                                     double*, int, hipStream_t);

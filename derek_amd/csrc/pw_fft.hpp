@@ -16,8 +16,10 @@ template <typename R> struct PwBlock {
 };
 
 // z pass on sticks, inverse, with the plane-wave scatter: psi -> S.
+// zmajor: S[item][z][stick] (for the fused y/x pass) instead of S[item][stick][z].
 template <typename R>
-void sticks_z_inv(const PwBatch& B, PwBlock<R> psi, const Axis<R>* ax, int nz, cx_t<R>* S, hipStream_t s);
+void sticks_z_inv(const PwBatch& B, PwBlock<R> psi, const Axis<R>* ax, int nz, cx_t<R>* S, bool zmajor,
+                  hipStream_t s);
 // y pass on the active planes: sign +1 inverse S -> P, sign -1 forward P -> S;
 // bands_per_wg (1, 2, 4, 8) bands of one k-point per workgroup.
 template <typename R>
@@ -27,7 +29,13 @@ void planes_y(const PwBatch& B, cx_t<R>* S, cx_t<R>* P, const Axis<R>* ax, int n
 // hpsi = FFT(S)[plane waves] + kin * psi.
 template <typename R>
 void sticks_z_fwd(const PwBatch& B, const cx_t<R>* S, const Axis<R>* ax, int nz, PwBlock<R> psi, PwBlock<R> hpsi,
-                  hipStream_t s);
+                  bool zmajor, hipStream_t s);
+// Fused y and x passes on z-major sticks, in place: y inverse, x inverse,
+// * vt, x forward, y forward per z plane.  vt[(z*ny + y)*nx + x] is V / (nx ny nz);
+// bands_per_wg bands of one k-point per workgroup.
+template <typename R> bool slab_supported(int nx, int ny, int nz, int nax_cap, int nst_cap);
+template <typename R>
+void slab_vmul(const PwBatch& B, cx_t<R>* S, const Axis<R>* ax, int nz, const R* vt, int bands_per_wg, hipStream_t s);
 // x pass: inverse, * vloc * vscale, forward, in place on the active rows of P.
 template <typename R>
 void planes_x_vmul(const PwBatch& B, cx_t<R>* P, const Axis<R>* ax, int64_t nyz, const R* vloc, R vscale,

@@ -45,6 +45,9 @@ PwMapsHost build_pw_maps(const int32_t* idx, int64_t npw, int nx, int ny, int nz
   }
   h.st_pw.push_back((int32_t)npw);
   h.pl_st.push_back(h.nst);
+  h.pl_y.assign((size_t)h.nax * ny, -1);
+  for (int a = 0; a < h.nax; ++a)
+    for (int st = h.pl_st[a]; st < h.pl_st[a + 1]; ++st) h.pl_y[(size_t)a * ny + h.st_y[st]] = st;
   return h;
 }
 

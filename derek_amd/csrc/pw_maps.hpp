@@ -19,6 +19,7 @@ struct PwMaps {
   const int32_t* st_y = nullptr;  // [nst] y bin of each stick
   const int32_t* pl_st = nullptr; // [nax+1] first stick of each active plane
   const int32_t* ax_x = nullptr;  // [nax] x bin of each active plane
+  const int32_t* pl_y = nullptr;  // [nax*ny] stick at (active plane a, y bin), -1 if none
   const double* kin = nullptr;    // [npw] kinetic energy 1/2 |G+k|^2, caller order
 };
 
@@ -35,7 +36,7 @@ struct PwBatch {
 // Host construction of the maps from flat FFT-bin indices (z fastest).
 struct PwMapsHost {
   int npw = 0, nst = 0, nax = 0;
-  std::vector<int32_t> perm, pw_z, pw_st, st_pw, st_y, pl_st, ax_x;
+  std::vector<int32_t> perm, pw_z, pw_st, st_pw, st_y, pl_st, ax_x, pl_y;
 };
 PwMapsHost build_pw_maps(const int32_t* fft_index, int64_t npw, int nx, int ny, int nz);
 

@@ -243,7 +243,7 @@ void potential(derek_ctx* c) {
   c->lda(E.rho_in, E.vxc, nullptr, 0);
   add3(E.vh, E.vxc, nullptr, E.vhxc, c->N, c->s);
   add3(c->vext, E.vhxc, nullptr, c->vloc, c->N, c->s);
-  if (E.fp32) convert_real_f(c->vloc32, c->vloc, c->N, c->s);
+  c->vloc_changed(E.fp32);
   c->vloc_set = true;
 }
 
@@ -333,8 +333,9 @@ void switch_to_fp64(derek_ctx* c) {
   c->P32.release();
 }
 
-// Times H|psi> for the tunable launch shapes and keeps the fastest: bands
-// per x-pass workgroup, then items per launch.  The timed sample is at most
+// Times H|psi> for the tunable launch shapes and keeps the fastest: the
+// separate or the fused y/x passes with their bands per workgroup, then
+// items per launch.  The timed sample is at most
 // about 256 items (the first k-points' leading bands): enough to resolve
 // launch shapes up to that size at a small fraction of a step's cost.
 template <typename R> void autotune(derek_ctx* c) {
@@ -379,16 +380,22 @@ template <typename R> void autotune(derek_ctx* c) {
   // bring the clocks up before the first timed shape
   for (int r = 0; r < 4; ++r) c->apply_h<R>(0, nkc, x, hx, nbt);
   float best = 1e30f;
-  int bpw = c->bpw;
-  for (int q : {1, 2, 4, 8}) {
-    c->bpw = q;
-    const float t = time_it();
-    if (t < best) {
-      best = t;
-      bpw = q;
+  int bpw = c->bpw, fz = c->fused;
+  for (int f : {0, 1}) {
+    c->fused = f;
+    if (f && !c->use_slab<R>()) continue;
+    for (int q : {1, 2, 4, 8}) {
+      c->bpw = q;
+      const float t = time_it();
+      if (t < best) {
+        best = t;
+        bpw = q;
+        fz = f;
+      }
     }
   }
   c->bpw = bpw;
+  c->fused = fz;
   best = 1e30f;
   int cap = c->item_cap;
   for (int q : caps) {
@@ -505,7 +512,7 @@ void scf_init(derek_ctx* c, const derek_scf_params& p) {
   DEREK_HIP(hipMemcpy(E.rho_in, r.data(), c->N * sizeof(double), hipMemcpyHostToDevice));
   if (p.autotune) {
     add3(c->vext, nullptr, nullptr, c->vloc, c->N, c->s);
-    convert_real_f(c->vloc32, c->vloc, c->N, c->s);
+    c->vloc_changed(true);
     if (E.fp32) autotune<float>(c);
     else autotune<double>(c);
   }
@@ -589,7 +596,7 @@ double scf_polish(derek_ctx* c, int max_iter, double res_tol) {
 double scf_nscf(derek_ctx* c, int max_iter, double res_tol) {
   ScfEngine& E = engine(c);
   if (!c->vloc_set) throw StateError("non-self-consistent run needs the potential (set_vloc)");
-  if (E.fp32) convert_real_f(c->vloc32, c->vloc, c->N, c->s);
+  c->vloc_changed(E.fp32);
   for (int k0 = 0; k0 < c->nk; k0 += E.kc) {
     const int nkc = std::min(E.kc, c->nk - k0);
     if (E.fp32) davidson_chunk<float>(c, k0, nkc, max_iter, res_tol);

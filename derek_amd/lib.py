@@ -32,10 +32,11 @@ EXPORTS = (
     "derek_hip_get_occupations", "derek_hip_get_density", "derek_hip_get_potential", "derek_hip_get_psi", "derek_hip_put_psi",
     "derek_hip_apply_h_resident", "derek_hip_sync", "derek_hip_stream",
     "derek_hip_profile_enable", "derek_hip_profile_reset", "derek_hip_profile_read", "derek_hip_get_tuning",
+    "derek_hip_set_launch",
 )
 
 KERNEL_CLASSES = ("sticks_z_inv", "planes_y_inv", "planes_x_vmul", "planes_y_fwd", "sticks_z_fwd", "planes_x_density",
-                  "overlap", "rotate", "fft_z", "fft_y", "fft_x", "dense", "other")
+                  "overlap", "rotate", "fft_z", "fft_y", "fft_x", "dense", "other", "slab_yx_vmul")
 
 
 class DerekError(RuntimeError):
@@ -479,10 +480,15 @@ class Context:
         check(self.lib.derek_hip_scf_init(self.h, C.byref(p)))
 
     def tuning(self):
-        """{bands_per_wg, items_per_launch, kbatch, graph}"""
-        out = (C.c_int32 * 4)()
+        """{bands_per_wg, items_per_launch, kbatch, graph, fused}"""
+        out = (C.c_int32 * 5)()
         check(self.lib.derek_hip_get_tuning(self.h, out))
-        return {"bands_per_wg": out[0], "items_per_launch": out[1], "kbatch": out[2], "graph": bool(out[3])}
+        return {"bands_per_wg": out[0], "items_per_launch": out[1], "kbatch": out[2], "graph": bool(out[3]),
+                "fused": bool(out[4])}
+
+    def set_launch(self, bands_per_wg=4, items_per_launch=16, fused=False):
+        """H|psi> launch shapes; fused: the fused y/x pass where the grid allows it."""
+        check(self.lib.derek_hip_set_launch(self.h, bands_per_wg, items_per_launch, 1 if fused else 0))
 
     def scf_step(self):
         r = ScfReport()

@@ -286,7 +286,8 @@ int32_t derek_hip_apply_h_resident(derek_ctx* ctx, int32_t k, int32_t nb);
  * 3 y pass on planes (forward), 4 z pass on sticks with the gather and the
  * kinetic term, 5 x pass fused with the density accumulation, 6 overlap GEMM,
  * 7 rotation GEMM, 8/9/10 full-grid FFT z/y/x passes, 11 FP64 subspace
- * solves, 12 other SCF kernels (band statistics, TPA, potentials, mixing).
+ * solves, 12 other SCF kernels (band statistics, TPA, potentials, mixing),
+ * 13 fused y and x passes with the V(r) multiply (in place of 1, 2 and 3).
  * (The reference's equivalent is its trace/profile module, trace.f90:515-667.) */
 int32_t derek_hip_profile_enable(derek_ctx* ctx, int32_t on);
 int32_t derek_hip_profile_reset(derek_ctx* ctx);
@@ -294,8 +295,16 @@ int32_t derek_hip_profile_read(derek_ctx* ctx, int32_t cls, int64_t* launches, d
                                double* flops);
 /* Launch shapes in use: out[0] bands per x-pass workgroup (1, 2, 4, 8),
  * out[1] (k-point, band) items per pipeline launch, out[2] k-points per
- * batched launch, out[3] 1 if SCF steps are replayed from a hipGraph. */
-int32_t derek_hip_get_tuning(derek_ctx* ctx, int32_t out[4]);
+ * batched launch, out[3] 1 if SCF steps are replayed from a hipGraph,
+ * out[4] 1 if H|psi> uses the fused y/x pass where the grid allows it. */
+int32_t derek_hip_get_tuning(derek_ctx* ctx, int32_t out[5]);
+/* Set the H|psi> launch shapes (what scf_init's autotuner would choose):
+ * bands per workgroup (1, 2, 4, 8), items per launch (>= 1), fused 0/1.
+ * The fused y/x pass is used for nx = ny in {32, 72, 108, 144} and z lengths
+ * in the same set when its LDS working set fits; otherwise the separate
+ * passes run.  No counterpart in the reference (launch-shape control of the
+ * GPU pipeline). */
+int32_t derek_hip_set_launch(derek_ctx* ctx, int32_t bands_per_wg, int32_t items_per_launch, int32_t fused);
 /* Block until all work on the context's stream is done. */
 int32_t derek_hip_sync(derek_ctx* ctx);
 /* Host-visible stream handle (hipStream_t) for event timing. */
