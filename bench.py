@@ -22,6 +22,7 @@ headline system, timed on this host on one core and on every core of the
 host's share, before the GPU is touched).
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-scf] [--scf cfg2,cfg4] [--cpu-seconds S] [--no-cpu]
+                [--fixed-shapes | --shapes BPW,ITEMS,FUSED]
 """
 import argparse
 import json
@@ -34,6 +35,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = ("time-to-SCF (|ΔE|<1e-6 Ha) mixed/batched vs FP64 GPU; H|ψ⟩ bands/s; HBM GB/s")
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6  # FP64 vector (and dense MFMA) peak
 
 
 def log(msg):
@@ -143,8 +145,11 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="run the batched SCF modes eagerly (no hipGraph replay), e.g. under a kernel tracer")
     ap.add_argument("--fixed-shapes", action="store_true",
-                    help="time H|psi> at the default launch shapes (4 bands per x-pass workgroup, 16 items per "
-                         "launch) instead of autotuned ones, so that every launch of a kernel has one size")
+                    help="time H|psi> at the default launch shapes (fused y/x pass where the grid allows it, 2 bands "
+                         "per workgroup, 256 items per launch) instead of autotuned ones")
+    ap.add_argument("--shapes", default=None,
+                    help="BPW,ITEMS,FUSED: time H|psi> at these launch shapes (bands per workgroup, items per "
+                         "launch, 1 for the fused y/x pass) instead of autotuned ones")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -169,9 +174,13 @@ def main():
             f"{cpu_all['cores']} cores")
     if args.copy_calib > 0:
         copy_calibration(args.copy_calib)
+    shapes = [int(x) for x in args.shapes.split(",")] if args.shapes else None
+    tuned = not (args.fixed_shapes or shapes)
     ctx = su.context()
-    ctx.scf_init(spec.nelec, kT=spec.kT, seed=spec.seed, autotune=not args.fixed_shapes)
+    ctx.scf_init(spec.nelec, kT=spec.kT, seed=spec.seed, autotune=tuned)
     ctx.set_vloc(su.vext)
+    if shapes:
+        ctx.set_launch(shapes[0], shapes[1], bool(shapes[2]))
     tune64 = ctx.tuning()
     nb = spec.nbands
     log(f"# setup {time.perf_counter() - t0:.2f}s grid {su.dims} npw {su.npw[0]} bands {nb}")
@@ -180,8 +189,10 @@ def main():
     ctx.close()
     # the same H|psi> in complex64 (the mixed form's bulk)
     ctx = su.context(lib.DEREK_MIXED)
-    ctx.scf_init(spec.nelec, kT=spec.kT, seed=spec.seed, autotune=not args.fixed_shapes)
+    ctx.scf_init(spec.nelec, kT=spec.kT, seed=spec.seed, autotune=tuned)
     ctx.set_vloc(su.vext)
+    if shapes:
+        ctx.set_launch(shapes[0], shapes[1], bool(shapes[2]))
     tune32 = ctx.tuning()
     bps32, ms32, prof32 = hpsi_rate(ctx, nb, args.steps, args.warmup)
     log(f"# H|psi> FP32: {ms32:.3f} ms per {nb} bands, {bps32:.1f} bands/s, shapes {tune32}")
@@ -190,12 +201,15 @@ def main():
     # dominant kernel of the timed FP64 region
     busy = {k: v for k, v in prof.items() if v[0] > 0}
     dom = max(busy, key=lambda k: busy[k][1])
-    n_l, tot_ms, by, _ = busy[dom]
+    n_l, tot_ms, by, fl = busy[dom]
     avg_s = tot_ms / n_l * 1e-3
     achieved = by / n_l / avg_s / 1e9
     total = sum(x[1] for x in busy.values())
     kernels = {k: {"launches": v[0], "avg_us": v[1] / v[0] * 1e3, "GB_s": v[2] / (v[1] * 1e-3) / 1e9,
                    "share": v[1] / total} for k, v in busy.items()}
+    for k, v in busy.items():
+        if v[3] > 0:
+            kernels[k]["TFLOP_s"] = v[3] / (v[1] * 1e-3) / 1e12
     busy32 = {k: v for k, v in prof32.items() if v[0] > 0}
     kernels32 = {k: {"avg_us": v[1] / v[0] * 1e3, "GB_s": v[2] / (v[1] * 1e-3) / 1e9} for k, v in busy32.items()}
     # HBM bytes per launch from the committed counter summary: FETCH_SIZE and
@@ -208,9 +222,19 @@ def main():
         if per:
             items = min(tune64["items_per_launch"], nb)
             traffic = round(items * (per["fetch"] + per["write"]))
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
-                "bytes_per_launch": by / n_l, "avg_launch_us": avg_s * 1e6}
+    if dom == "slab_yx_vmul":
+        # the fused y/x pass keeps the planes in LDS: its HBM bytes are the
+        # stick rows only, and the bound is the FP64 vector pipe (VALU + LDS);
+        # achieved = FFT flops (5 n log2 n per line) per launch / duration
+        tf = fl / n_l / avg_s / 1e12
+        roofline = {"bound": "valu", "achieved": round(tf, 2), "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
+                    "frac": round(tf / FP64_PEAK_TFS, 4), "traffic": traffic, "kernel": dom,
+                    "flops_per_launch": fl / n_l, "bytes_per_launch": by / n_l, "avg_launch_us": avg_s * 1e6,
+                    "hbm_GB_s": round(achieved, 1), "hbm_frac": round(achieved / HBM_PEAK_GBS, 4)}
+    else:
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
+                    "bytes_per_launch": by / n_l, "avg_launch_us": avg_s * 1e6}
 
     scf_info = {}
     todo = [] if args.no_scf else [x.strip() for x in args.scf.split(",") if x.strip()]
@@ -265,11 +289,13 @@ def main():
                                "80 bands (Gamma), 16 wells + 0.2 Ha cos", "bands": nb, "npw": su.npw[0],
                    "grid": su.dims, "parallelism": "single GPU",
                    "launch_shapes": {"bands_per_wg": tune64["bands_per_wg"],
-                                     "items_per_launch": tune64["items_per_launch"]}},
+                                     "items_per_launch": tune64["items_per_launch"],
+                                     "fused_yx": tune64["fused"]}},
         "roofline": roofline, "kernels": kernels,
         "hpsi_fp32": {"value": round(bps32, 2), "unit": "bands/s", "ms_per_step": round(ms32, 4),
                       "launch_shapes": {"bands_per_wg": tune32["bands_per_wg"],
-                                        "items_per_launch": tune32["items_per_launch"]},
+                                        "items_per_launch": tune32["items_per_launch"],
+                                        "fused_yx": tune32["fused"]},
                       "kernels": kernels32},
         "scf": scf_info, "cpu_baseline": cpu, "cpu_baseline_all_cores": cpu_all,
     }

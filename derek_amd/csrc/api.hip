@@ -186,8 +186,12 @@ void derek_ctx::apply_h(int k0, int nkc, PwBlock<R> psi, PwBlock<R> hpsi, int nb
     PwBlock<R> ho{hpsi.p + kk * hpsi.ks + (int64_t)b0 * hpsi.ld, hpsi.ld, hpsi.ks};
     timed(KC_STICKS_Z_INV, pw_b + 12.0 * npw + st_b, 0, [&] { sticks_z_inv<R>(B, pi, ax, dims[2], S, fz, s); });
     if (fz) {
-      // sticks in and out once, V once per launch
-      timed(KC_SLAB_VMUL, 2 * st_b + (double)N * sizeof(R), 0,
+      // bytes: sticks in and out once, V once per launch; flops: 5 n log2 n
+      // per line (y lines on the active planes, x lines on every y) and the
+      // V multiply, per item
+      const double fy = 5.0 * dims[1] * std::log2((double)dims[1]), fx = 5.0 * dims[0] * std::log2((double)dims[0]);
+      const double fl = nbb * (2.0 * nax * dims[2] * fy + 2.0 * kn * (double)dims[1] * dims[2] * fx) + 2.0 * nbb * kn * N;
+      timed(KC_SLAB_VMUL, 2 * st_b + (double)N * sizeof(R), fl,
             [&] { slab_vmul<R>(B, S, ax, dims[2], vtrans<R>(), bpw, s); });
     } else {
       cx_t<R>* P = planes<R>(kn * nbb);
@@ -598,7 +602,6 @@ int32_t derek_hip_init(derek_ctx** out, const derek_dims* d) {
     c->vt32 = dmalloc<float>(c->N);
     c->cgrid = dmalloc<double2>(c->N);
     // (k-point, band) items per pipeline launch; the SCF autotuner may change it
-    c->item_cap = 16;
     if (const char* b = std::getenv("DEREK_BAND_BATCH")) c->item_cap = std::max(1, std::atoi(b));
     *out = c.release();
   });

@@ -133,9 +133,9 @@ struct derek_ctx {
   float* vt32 = nullptr;
   bool gsq_set = false, vloc_set = false, vext_set = false;
   // launch shapes (autotuned by the SCF engine when asked)
-  int bpw = 4;         // bands per x-pass workgroup
-  int item_cap = 64;   // (k-point, band) items per pipeline launch
-  int fused = 0;       // 1: fused y/x pass on z-major sticks where the grid allows it
+  int bpw = 2;         // bands per y/x-pass workgroup
+  int item_cap = 256;  // (k-point, band) items per pipeline launch
+  int fused = 1;       // 1: fused y/x pass on z-major sticks where the grid allows it
   // pruned-FFT work buffers for item_cap items
   derek::DevBuf S64, P64, S32, P32;
   template <typename R> derek::cx_t<R>* sticks(int items);

@@ -1077,14 +1077,30 @@ __global__ __launch_bounds__((Slab<R, N>::NT)) void k_slab_vmul(PwBatch B, cx_t<
   const R* vz = vt + (int64_t)z * N * N;  // V at (x, y, z) in vz[y*N + x]
   const int bb0 = blockIdx.y * bpw, bb1 = min(B.nbb, bb0 + bpw);
   const uint32_t row_bytes = (uint32_t)B.nst_cap * (uint32_t)sizeof(C);
+  auto row_rsrc = [&](int band) {
+    return buf_rsrc(S + (((int64_t)blockIdx.z * B.nbb + band) * nz + z) * B.nst_cap, row_bytes);
+  };
+  // the first A1 item's sticks of the next band are loaded during phase C
+  // (complex128 at one workgroup per CU, where the registers are free)
+  constexpr bool kPre = sizeof(R) == 8 && N >= 100;
+  C pre[kPre ? R1 : 1];
+  auto load_a1 = [&](const decltype(row_rsrc(0))& rs, int a, C* v) {
+    const int16_t* ty = sty + a * N + j2;
+#pragma unroll
+    for (int r = 0; r < R1; ++r) v[r] = buf_ld<C>(rs, stick_off(ty[R2 * r]));
+  };
+  if (kPre && lj < nax) load_a1(row_rsrc(bb0), lj, pre);
   for (int band = bb0; band < bb1; ++band) {
-    const auto rs = buf_rsrc(S + (((int64_t)blockIdx.z * B.nbb + band) * nz + z) * B.nst_cap, row_bytes);
+    const auto rs = row_rsrc(band);
     // A1: y = j2 + R2 r from the sticks, output k in slot k R2P + j2
     for (int a = lj; a < nax; a += LJ) {
-      const int16_t* ty = sty + a * N + j2;
       C v[R1];
+      if (kPre && a == lj) {
 #pragma unroll
-      for (int r = 0; r < R1; ++r) v[r] = buf_ld<C>(rs, stick_off(ty[R2 * r]));
+        for (int r = 0; r < R1; ++r) v[r] = pre[r];
+      } else {
+        load_a1(rs, a, v);
+      }
       dftR<R1, +1>(v, tw, N);
       C* row = A + a * LP + j2;
 #pragma unroll
@@ -1153,6 +1169,7 @@ __global__ __launch_bounds__((Slab<R, N>::NT)) void k_slab_vmul(PwBatch B, cx_t<
       }
       __syncthreads();
     }
+    if (kPre && band + 1 < bb1 && lj < nax) load_a1(row_rsrc(band + 1), lj, pre);
     // C1: y = j2 + R2 r, output k1 back into the slot of y = j2 + R2 k1
     for (int a = lj; a < nax; a += LJ) {
       C* row = A + a * LP;

@@ -333,14 +333,15 @@ void switch_to_fp64(derek_ctx* c) {
   c->P32.release();
 }
 
-// Times H|psi> for the tunable launch shapes and keeps the fastest: the
-// separate or the fused y/x passes with their bands per workgroup, then
-// items per launch.  The timed sample is at most
-// about 256 items (the first k-points' leading bands): enough to resolve
-// launch shapes up to that size at a small fraction of a step's cost.
+// Times H|psi> for the tunable launch shapes, starting from the current
+// ones: the separate or the fused y/x passes with their bands per
+// workgroup, then items per launch.  The timed sample is at most about 256
+// items (the first k-points' leading bands), so timings are noisy: another
+// shape replaces the current one only when it is more than 3 % faster.
 template <typename R> void autotune(derek_ctx* c) {
   ScfEngine& E = *c->scf;
   constexpr int kTuneItems = 256;
+  constexpr float kGain = 0.97f;
   const int nbt = std::min(c->nb, kTuneItems);
   const int nkc = std::min({E.kc, c->nk, (int)ceil_div(kTuneItems, nbt)});
   cx_t<R>* Y = reinterpret_cast<cx_t<R>*>(E.Y);
@@ -372,22 +373,31 @@ template <typename R> void autotune(derek_ctx* c) {
   const int cap_mem = (int)std::max(1.0, std::floor(8.0 * 1073741824.0 / item_bytes));
   std::vector<int> caps;
   for (int q : {8, 16, 32, 64, 128, 256, 512, all})
-    if (q <= all && q <= cap_mem && (caps.empty() || q != caps.back())) caps.push_back(q);
+    if (q <= all && q <= cap_mem) caps.push_back(q);
   if (caps.empty()) caps.push_back(std::min(all, cap_mem));
   std::sort(caps.begin(), caps.end());
   caps.erase(std::unique(caps.begin(), caps.end()), caps.end());
-  c->item_cap = caps[caps.size() / 2];
-  // bring the clocks up before the first timed shape
-  for (int r = 0; r < 4; ++r) c->apply_h<R>(0, nkc, x, hx, nbt);
-  float best = 1e30f;
-  int bpw = c->bpw, fz = c->fused;
+  c->item_cap = std::min(c->item_cap, cap_mem);
+  // bring the clocks up before the first timed shape: at least 50 ms of work
+  {
+    const auto w0 = std::chrono::steady_clock::now();
+    for (int r = 0; r < 200; ++r) {
+      c->apply_h<R>(0, nkc, x, hx, nbt);
+      c->sync();
+      if (std::chrono::steady_clock::now() - w0 > std::chrono::milliseconds(50)) break;
+    }
+  }
+  float best = time_it();
+  const int bpw0 = c->bpw, fz0 = c->fused;
+  int bpw = bpw0, fz = fz0;
   for (int f : {0, 1}) {
     c->fused = f;
     if (f && !c->use_slab<R>()) continue;
     for (int q : {1, 2, 4, 8}) {
+      if (q == bpw0 && f == fz0) continue;
       c->bpw = q;
       const float t = time_it();
-      if (t < best) {
+      if (t < kGain * best) {
         best = t;
         bpw = q;
         fz = f;
@@ -396,12 +406,13 @@ template <typename R> void autotune(derek_ctx* c) {
   }
   c->bpw = bpw;
   c->fused = fz;
-  best = 1e30f;
-  int cap = c->item_cap;
+  const int cap0 = c->item_cap;
+  int cap = cap0;
   for (int q : caps) {
+    if (q == cap0) continue;
     c->item_cap = q;
     const float t = time_it();
-    if (t < best) {
+    if (t < kGain * best) {
       best = t;
       cap = q;
     }

@@ -486,7 +486,7 @@ class Context:
         return {"bands_per_wg": out[0], "items_per_launch": out[1], "kbatch": out[2], "graph": bool(out[3]),
                 "fused": bool(out[4])}
 
-    def set_launch(self, bands_per_wg=4, items_per_launch=16, fused=False):
+    def set_launch(self, bands_per_wg=2, items_per_launch=256, fused=True):
         """H|psi> launch shapes; fused: the fused y/x pass where the grid allows it."""
         check(self.lib.derek_hip_set_launch(self.h, bands_per_wg, items_per_launch, 1 if fused else 0))
 

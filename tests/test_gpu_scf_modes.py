@@ -60,7 +60,7 @@ def test_autotune_keeps_results():
     e0, _, _ = energies(spec, 5)
     e1, _, t = energies(spec, 5, autotune=True)
     assert t["bands_per_wg"] in (1, 2, 4, 8)
-    assert 1 <= t["items_per_launch"] <= 4 * spec.nbands
+    assert t["items_per_launch"] >= 1
     assert np.max(np.abs(e1 - e0)) < 1e-10
 
 
