@@ -432,6 +432,20 @@ module derek_hip_m
        import :: c_int32_t, c_ptr
        type(c_ptr), value :: ctx
      end function derek_hip_sync
+
+     ! out(1:5): bands per workgroup, items per launch, k-points per batch, graph, fused y/x pass
+     integer(c_int32_t) function derek_hip_get_tuning(ctx, out) bind(C, name='derek_hip_get_tuning')
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), dimension(5) :: out
+     end function derek_hip_get_tuning
+
+     integer(c_int32_t) function derek_hip_set_launch(ctx, bands_per_wg, items_per_launch, fused) &
+          bind(C, name='derek_hip_set_launch')
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: bands_per_wg, items_per_launch, fused
+     end function derek_hip_set_launch
   end interface
 
   public :: derek_hip_version, derek_hip_last_error, derek_basis_grid_dims, derek_basis_inv_cell, &
@@ -444,7 +458,7 @@ module derek_hip_m
        derek_basis_std_index, derek_io_pot_write, derek_io_pot_read, derek_io_den_write, derek_io_den_read, &
        derek_io_wvfn_write, derek_io_wvfn_read, derek_hip_nscf, derek_info_read, derek_info_free, derek_info_num, &
        derek_info_str, derek_info_present, derek_info_cell, derek_info_wells, derek_info_log, derek_info_preset_pot, &
-       derek_info_help
+       derek_info_help, derek_hip_get_tuning, derek_hip_set_launch
 
 contains
 
