@@ -149,7 +149,7 @@ def main():
                          "per workgroup, 256 items per launch) instead of autotuned ones")
     ap.add_argument("--shapes", default=None,
                     help="BPW,ITEMS,FUSED: time H|psi> at these launch shapes (bands per workgroup, items per "
-                         "launch, 1 for the fused y/x pass) instead of autotuned ones")
+                         "launch, fused passes: bit 0 H|psi>, bit 1 density) instead of autotuned ones")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -180,7 +180,7 @@ def main():
     ctx.scf_init(spec.nelec, kT=spec.kT, seed=spec.seed, autotune=tuned)
     ctx.set_vloc(su.vext)
     if shapes:
-        ctx.set_launch(shapes[0], shapes[1], bool(shapes[2]))
+        ctx.set_launch(shapes[0], shapes[1], bool(shapes[2] & 1), bool(shapes[2] & 2))
     tune64 = ctx.tuning()
     nb = spec.nbands
     log(f"# setup {time.perf_counter() - t0:.2f}s grid {su.dims} npw {su.npw[0]} bands {nb}")
@@ -192,7 +192,7 @@ def main():
     ctx.scf_init(spec.nelec, kT=spec.kT, seed=spec.seed, autotune=tuned)
     ctx.set_vloc(su.vext)
     if shapes:
-        ctx.set_launch(shapes[0], shapes[1], bool(shapes[2]))
+        ctx.set_launch(shapes[0], shapes[1], bool(shapes[2] & 1), bool(shapes[2] & 2))
     tune32 = ctx.tuning()
     bps32, ms32, prof32 = hpsi_rate(ctx, nb, args.steps, args.warmup)
     log(f"# H|psi> FP32: {ms32:.3f} ms per {nb} bands, {bps32:.1f} bands/s, shapes {tune32}")
@@ -290,12 +290,12 @@ def main():
                    "grid": su.dims, "parallelism": "single GPU",
                    "launch_shapes": {"bands_per_wg": tune64["bands_per_wg"],
                                      "items_per_launch": tune64["items_per_launch"],
-                                     "fused_yx": tune64["fused"]}},
+                                     "fused_yx": tune64["fused"], "fused_density": tune64["fused_density"]}},
         "roofline": roofline, "kernels": kernels,
         "hpsi_fp32": {"value": round(bps32, 2), "unit": "bands/s", "ms_per_step": round(ms32, 4),
                       "launch_shapes": {"bands_per_wg": tune32["bands_per_wg"],
                                         "items_per_launch": tune32["items_per_launch"],
-                                        "fused_yx": tune32["fused"]},
+                                        "fused_yx": tune32["fused"], "fused_density": tune32["fused_density"]},
                       "kernels": kernels32},
         "scf": scf_info, "cpu_baseline": cpu, "cpu_baseline_all_cores": cpu_all,
     }

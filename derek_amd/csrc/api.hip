@@ -106,6 +106,7 @@ derek_ctx::~derek_ctx() {
   P64.release();
   S32.release();
   P32.release();
+  rho_part.release();
   tA.release();
   tB.release();
   tC.release();
@@ -121,11 +122,11 @@ template <> const double* derek_ctx::vlocal<double>() const { return vloc; }
 template <> const float* derek_ctx::vlocal<float>() const { return vloc32; }
 template <> const double* derek_ctx::vtrans<double>() const { return vt64; }
 template <> const float* derek_ctx::vtrans<float>() const { return vt32; }
-template <typename R> bool derek_ctx::use_slab() const {
-  return fused && slab_supported<R>(dims[0], dims[1], dims[2], nax_max, nst_max);
+template <typename R> bool derek_ctx::use_slab(int part) const {
+  return (fused & part) && slab_supported<R>(dims[0], dims[1], dims[2], nax_max, nst_max);
 }
-template bool derek_ctx::use_slab<double>() const;
-template bool derek_ctx::use_slab<float>() const;
+template bool derek_ctx::use_slab<double>(int) const;
+template bool derek_ctx::use_slab<float>(int) const;
 
 void derek_ctx::vloc_changed(bool f32) {
   if (f32) convert_real_f(vloc32, vloc, N, s);
@@ -171,7 +172,7 @@ void derek_ctx::apply_h(int k0, int nkc, PwBlock<R> psi, PwBlock<R> hpsi, int nb
   const R* v = vlocal<R>();
   const double sc = sizeof(cx_t<R>);
   const int64_t nyz = (int64_t)dims[1] * dims[2];
-  const bool fz = use_slab<R>();
+  const bool fz = use_slab<R>(kFusedHpsi);
   for_item_launches(this, k0, nkc, nbands, [&](int ka, int kk, int kn, int b0, int nbb) {
     PwBatch B{d_maps, ka, kn, nbb, nst_max, nax_max};
     cx_t<R>* S = sticks<R>(kn * nbb);
@@ -211,11 +212,11 @@ void derek_ctx::density(int k0, int nkc, PwBlock<R> psi, int nbands, const doubl
   const Axis<R>* ax = axes<R>();
   const double sc = sizeof(cx_t<R>);
   const int64_t nyz = (int64_t)dims[1] * dims[2];
+  const bool fz = use_slab<R>(kFusedDensity);
   bool first = true;
   for_item_launches(this, k0, nkc, nbands, [&](int ka, int kk, int kn, int b0, int nbb) {
     PwBatch B{d_maps, ka, kn, nbb, nst_max, nax_max};
     cx_t<R>* S = sticks<R>(kn * nbb);
-    cx_t<R>* P = planes<R>(kn * nbb);
     double npw = 0, nst = 0, nax = 0;
     for (int k = ka; k < ka + kn; ++k) {
       npw += (double)ks[k].npw;
@@ -226,11 +227,20 @@ void derek_ctx::density(int k0, int nkc, PwBlock<R> psi, int nbands, const doubl
     const int kp = (keep || !first) ? 1 : 0;
     first = false;
     PwBlock<R> pi{psi.p + kk * psi.ks + (int64_t)b0 * psi.ld, psi.ld, psi.ks};
-    timed(KC_STICKS_Z_INV, pw_b + 12.0 * npw + st_b, 0, [&] { sticks_z_inv<R>(B, pi, ax, dims[2], S, false, s); });
-    timed(KC_PLANES_Y_INV, st_b + pl_b, 0, [&] { planes_y<R>(B, S, P, ax, dims[1], dims[2], +1, bpw, s); });
-    timed(KC_PLANES_X_DENSITY, pl_b + (1 + kp) * 8.0 * N, 0, [&] {
-      planes_x_density<R>(B, P, ax, nyz, occw + (int64_t)kk * occ_ks + b0, occ_ks, rho, kp, s);
-    });
+    const double* ow = occw + (int64_t)kk * occ_ks + b0;
+    timed(KC_STICKS_Z_INV, pw_b + 12.0 * npw + st_b, 0, [&] { sticks_z_inv<R>(B, pi, ax, dims[2], S, fz, s); });
+    if (fz) {
+      const int groups = slab_density_groups<R>(B, ax, dims[2]);
+      double* part = rho_part.get<double>((size_t)kn * groups * N);
+      // sticks in once, the partial planes out and back, rho read (keep) and written
+      timed(KC_SLAB_DENSITY, st_b + 2.0 * 8.0 * kn * groups * N + (1 + kp) * 8.0 * N, 0,
+            [&] { slab_density<R>(B, S, ax, dims[2], ow, occ_ks, part, groups, rho, kp, s); });
+    } else {
+      cx_t<R>* P = planes<R>(kn * nbb);
+      timed(KC_PLANES_Y_INV, st_b + pl_b, 0, [&] { planes_y<R>(B, S, P, ax, dims[1], dims[2], +1, bpw, s); });
+      timed(KC_PLANES_X_DENSITY, pl_b + (1 + kp) * 8.0 * N, 0,
+            [&] { planes_x_density<R>(B, P, ax, nyz, ow, occ_ks, rho, kp, s); });
+    }
   });
 }
 
@@ -1037,7 +1047,7 @@ int32_t derek_hip_set_launch(derek_ctx* c, int32_t bands_per_wg, int32_t items_p
     DEREK_REQUIRE(bands_per_wg == 1 || bands_per_wg == 2 || bands_per_wg == 4 || bands_per_wg == 8,
                   "bands per workgroup must be 1, 2, 4 or 8");
     DEREK_REQUIRE(items_per_launch >= 1, "items per launch must be positive");
-    DEREK_REQUIRE(fused == 0 || fused == 1, "fused must be 0 or 1");
+    DEREK_REQUIRE(fused >= 0 && fused <= 3, "fused must be 0..3 (bit 0: H|psi>, bit 1: density)");
     c->bpw = bands_per_wg;
     c->item_cap = items_per_launch;
     c->fused = fused;

@@ -69,6 +69,7 @@ enum KernelClass {
   KC_DENSE,
   KC_OTHER,
   KC_SLAB_VMUL,
+  KC_SLAB_DENSITY,
   KC_COUNT
 };
 
@@ -135,9 +136,12 @@ struct derek_ctx {
   // launch shapes (autotuned by the SCF engine when asked)
   int bpw = 2;         // bands per y/x-pass workgroup
   int item_cap = 256;  // (k-point, band) items per pipeline launch
-  int fused = 1;       // 1: fused y/x pass on z-major sticks where the grid allows it
+  // fused y/x passes on z-major sticks where the grid allows them: bit 0 for
+  // H|psi> (kFusedHpsi), bit 1 for the density (kFusedDensity)
+  int fused = 3;
   // pruned-FFT work buffers for item_cap items
   derek::DevBuf S64, P64, S32, P32;
+  derek::DevBuf rho_part;  // partial densities of the fused density pass
   template <typename R> derek::cx_t<R>* sticks(int items);
   template <typename R> derek::cx_t<R>* planes(int items);
   double2* cgrid = nullptr;
@@ -162,7 +166,8 @@ struct derek_ctx {
   template <typename R> const derek::Axis<R>* axes() const;
   template <typename R> const R* vlocal() const;
   template <typename R> const R* vtrans() const;
-  template <typename R> bool use_slab() const;
+  static constexpr int kFusedHpsi = 1, kFusedDensity = 2;
+  template <typename R> bool use_slab(int part) const;
   // after vloc changed: its FP32 copy (when f32) and the transposed copies
   void vloc_changed(bool f32);
   void require_basis(int k) const;

@@ -1035,6 +1035,8 @@ template <typename R, int N> struct Slab {
   static constexpr int NR = (N + LR - 1) / LR;
   static constexpr int LJ = NT / R2, LK = NT / R1;  // lines per pass of the R2- / R1-thread stages
   static_assert(NT % R1 == 0 && NT % R2 == 0, "workgroup size must keep each thread's digits");
+  static_assert(R1 % 7 && R2 % 7 && R1 % 11 && R2 % 11 && R1 % 13 && R2 % 13,
+                "the fused kernels use register DFTs with compile-time roots only");
   static_assert(LR <= LJ && LR <= LK, "one pass per stage of a round");
   static constexpr size_t lds(int nax_cap) {
     return sizeof(cx_t<R>) * (size_t)(nax_cap + LR) * LP + sizeof(int) * N +
@@ -1042,66 +1044,66 @@ template <typename R, int N> struct Slab {
   }
 };
 
-template <typename R, int N>
-__global__ __launch_bounds__((Slab<R, N>::NT)) void k_slab_vmul(PwBatch B, cx_t<R>* __restrict__ S, int nz, int bpw,
-                                                              const cx_t<R>* __restrict__ tw,
-                                                              const R* __restrict__ vt) {
+// Per-thread state of the fused kernels: LDS areas, the thread's digits
+// (j2 in the R2-thread stages, k1 in the R1-thread stages, and its line in
+// each), its twiddles exp(-2 pi i k1 r / N) and the LDS rows of the active
+// planes at x = j2 + R2 r (-1 if none).
+template <typename R, int N> struct SlabThread {
   using C = cx_t<R>;
   using SL = Slab<R, N>;
-  constexpr int R1 = SL::R1, R2 = SL::R2, R2P = SL::R2P, NT = SL::NT, LR = SL::LR, NR = SL::NR, LP = SL::LP;
-  constexpr int LJ = SL::LJ, LK = SL::LK;
-  extern __shared__ __align__(16) unsigned char smem[];
-  const PwMaps m = B.mt[B.k0 + blockIdx.z];
-  const int z = blockIdx.x, nax = m.nax, tid = threadIdx.x;
-  C* A = reinterpret_cast<C*>(smem);                           // [nax_cap][LP]
-  C* X = A + (size_t)B.nax_cap * LP;                           // [LR][LP]
-  int* xa = reinterpret_cast<int*>(X + LR * LP);               // active plane of x bin, -1 if none
-  int16_t* sty = reinterpret_cast<int16_t*>(xa + N);           // stick of (plane, y bin), -1 if none
-  const int j2 = tid % R2, lj = tid / R2, k1 = tid % R1, lk = tid / R1;
-  for (int x = tid; x < N; x += NT) xa[x] = -1;
-  for (int i = tid; i < nax * N; i += NT) sty[i] = (int16_t)m.pl_y[i];
-  C tk[R2];  // exp(-2 pi i k1 r / N), r >= 1
+  static constexpr int R1 = SL::R1, R2 = SL::R2, R2P = SL::R2P, NT = SL::NT, LR = SL::LR, LP = SL::LP;
+  static constexpr int LJ = SL::LJ, LK = SL::LK;
+  C* A;           // [nax_cap][LP]
+  C* X;           // [LR][LP]
+  int16_t* sty;   // stick of (plane, y bin), -1 if none
+  int nax, j2, lj, k1, lk;
+  C tk[R2];
+  int xr[R1];
+
+  // LDS tables of the k-point's maps (two barriers)
+  __device__ __forceinline__ void init(const PwBatch& B, const PwMaps& m, const C* __restrict__ tw,
+                                       unsigned char* smem) {
+    const int tid = threadIdx.x;
+    A = reinterpret_cast<C*>(smem);
+    X = A + (size_t)B.nax_cap * LP;
+    int* xa = reinterpret_cast<int*>(X + LR * LP);  // active plane of x bin, -1 if none
+    sty = reinterpret_cast<int16_t*>(xa + N);
+    nax = m.nax;
+    j2 = tid % R2, lj = tid / R2, k1 = tid % R1, lk = tid / R1;
+    for (int x = tid; x < N; x += NT) xa[x] = -1;
+    for (int i = tid; i < nax * N; i += NT) sty[i] = (int16_t)m.pl_y[i];
 #pragma unroll
-  for (int r = 1; r < R2; ++r) tk[r] = tw[(k1 * r) % N];
-  __syncthreads();
-  for (int a = tid; a < nax; a += NT) xa[m.ax_x[a]] = a;
-  __syncthreads();
-  int xr[R1];  // LDS row of the active plane of x = j2 + R2 r, -1 if none
+    for (int r = 1; r < R2; ++r) tk[r] = tw[(k1 * r) % N];
+    __syncthreads();
+    for (int a = tid; a < nax; a += NT) xa[m.ax_x[a]] = a;
+    __syncthreads();
 #pragma unroll
-  for (int r = 0; r < R1; ++r) {
-    const int a = xa[j2 + R2 * r];
-    xr[r] = a >= 0 ? a * LP : -1;
+    for (int r = 0; r < R1; ++r) {
+      const int a = xa[j2 + R2 * r];
+      xr[r] = a >= 0 ? a * LP : -1;
+    }
   }
-  auto pos = [](int y) { return (y % R1) * R2P + y / R1; };  // slot of index y
-  auto stick_off = [](int st) { return st >= 0 ? (uint32_t)st * (uint32_t)sizeof(C) : kOob; };
-  const R* vz = vt + (int64_t)z * N * N;  // V at (x, y, z) in vz[y*N + x]
-  const int bb0 = blockIdx.y * bpw, bb1 = min(B.nbb, bb0 + bpw);
-  const uint32_t row_bytes = (uint32_t)B.nst_cap * (uint32_t)sizeof(C);
-  auto row_rsrc = [&](int band) {
-    return buf_rsrc(S + (((int64_t)blockIdx.z * B.nbb + band) * nz + z) * B.nst_cap, row_bytes);
-  };
-  // the first A1 item's sticks of the next band are loaded during phase C
-  // (complex128 at one workgroup per CU, where the registers are free)
-  constexpr bool kPre = sizeof(R) == 8 && N >= 100;
-  C pre[kPre ? R1 : 1];
-  auto load_a1 = [&](const decltype(row_rsrc(0))& rs, int a, C* v) {
+  __device__ static int pos(int y) { return (y % R1) * R2P + y / R1; }  // slot of index y
+  __device__ static uint32_t stick_off(int st) { return st >= 0 ? (uint32_t)st * (uint32_t)sizeof(C) : kOob; }
+  // the sticks y = j2 + R2 r of plane a from a stick row
+  template <typename Rs> __device__ __forceinline__ void load_a1(const Rs& rs, int a, C* v) const {
     const int16_t* ty = sty + a * N + j2;
 #pragma unroll
     for (int r = 0; r < R1; ++r) v[r] = buf_ld<C>(rs, stick_off(ty[R2 * r]));
-  };
-  if (kPre && lj < nax) load_a1(row_rsrc(bb0), lj, pre);
-  for (int band = bb0; band < bb1; ++band) {
-    const auto rs = row_rsrc(band);
+  }
+  // y inverse of every active plane from a stick row into A (two barriers);
+  // with PRE, `pre` holds the first item's inputs
+  template <bool PRE, typename Rs> __device__ __forceinline__ void y_inverse(const Rs& rs, const C (&pre)[R1]) {
     // A1: y = j2 + R2 r from the sticks, output k in slot k R2P + j2
     for (int a = lj; a < nax; a += LJ) {
       C v[R1];
-      if (kPre && a == lj) {
+      if (PRE && a == lj) {
 #pragma unroll
         for (int r = 0; r < R1; ++r) v[r] = pre[r];
       } else {
         load_a1(rs, a, v);
       }
-      dftR<R1, +1>(v, tw, N);
+      dftR<R1, +1>(v, (const C*)nullptr, N);
       C* row = A + a * LP + j2;
 #pragma unroll
       for (int k = 0; k < R1; ++k) row[k * R2P] = v[k];
@@ -1113,11 +1115,65 @@ __global__ __launch_bounds__((Slab<R, N>::NT)) void k_slab_vmul(PwBatch B, cx_t<
       C u[R2];
 #pragma unroll
       for (int r = 0; r < R2; ++r) u[r] = r ? cmul(row[r], cconj(tk[r])) : row[r];
-      dftR<R2, +1>(u, tw, N);
+      dftR<R2, +1>(u, (const C*)nullptr, N);
 #pragma unroll
       for (int k = 0; k < R2; ++k) row[k] = u[k];
     }
     __syncthreads();
+  }
+  // B1: x line y0 + lj (lj < ln), active rows x = j2 + R2 r, first stage of
+  // the inverse into X (barrier after)
+  __device__ __forceinline__ void x_inverse1(int y0, int ln) {
+    if (lj < ln) {
+      const int py = pos(y0 + lj);
+      C v[R1];
+#pragma unroll
+      for (int r = 0; r < R1; ++r) v[r] = xr[r] >= 0 ? A[xr[r] + py] : cmk<C>(0, 0);
+      dftR<R1, +1>(v, (const C*)nullptr, N);
+      C* row = X + lj * LP + j2;
+#pragma unroll
+      for (int k = 0; k < R1; ++k) row[k * R2P] = v[k];
+    }
+    __syncthreads();
+  }
+  // second stage of the inverse of line lk: x = k1 + R1 k2 in u
+  __device__ __forceinline__ void x_inverse2(C* u) const {
+    const C* row = X + lk * LP + k1 * R2P;
+#pragma unroll
+    for (int r = 0; r < R2; ++r) u[r] = r ? cmul(row[r], cconj(tk[r])) : row[r];
+    dftR<R2, +1>(u, (const C*)nullptr, N);
+  }
+};
+
+template <typename R, int N>
+__global__ __launch_bounds__((Slab<R, N>::NT)) void k_slab_vmul(PwBatch B, cx_t<R>* __restrict__ S, int nz, int bpw,
+                                                              const cx_t<R>* __restrict__ tw,
+                                                              const R* __restrict__ vt) {
+  using C = cx_t<R>;
+  using ST = SlabThread<R, N>;
+  constexpr int R1 = ST::R1, R2 = ST::R2, LR = ST::LR, NR = Slab<R, N>::NR, LP = ST::LP, LJ = ST::LJ, LK = ST::LK;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const PwMaps m = B.mt[B.k0 + blockIdx.z];
+  const int z = blockIdx.x;
+  ST t;
+  t.init(B, m, tw, smem);
+  const int nax = t.nax, j2 = t.j2, lj = t.lj, k1 = t.k1, lk = t.lk;
+  C* A = t.A;
+  C* X = t.X;
+  const R* vz = vt + (int64_t)z * N * N;  // V at (x, y, z) in vz[y*N + x]
+  const int bb0 = blockIdx.y * bpw, bb1 = min(B.nbb, bb0 + bpw);
+  const uint32_t row_bytes = (uint32_t)B.nst_cap * (uint32_t)sizeof(C);
+  auto row_rsrc = [&](int band) {
+    return buf_rsrc(S + (((int64_t)blockIdx.z * B.nbb + band) * nz + z) * B.nst_cap, row_bytes);
+  };
+  // the first A1 item's sticks of the next band are loaded during phase C
+  // (complex128 at one workgroup per CU, where the registers are free)
+  constexpr bool kPre = sizeof(R) == 8 && N >= 100;
+  C pre[R1];
+  if (kPre && lj < nax) t.load_a1(row_rsrc(bb0), lj, pre);
+  for (int band = bb0; band < bb1; ++band) {
+    const auto rs = row_rsrc(band);
+    t.template y_inverse<kPre>(rs, pre);
 #pragma unroll 1
     for (int q = 0; q < NR; ++q) {
       const int y0 = q * LR, ln = min(LR, N - y0);
@@ -1129,73 +1185,144 @@ __global__ __launch_bounds__((Slab<R, N>::NT)) void k_slab_vmul(PwBatch B, cx_t<
 #pragma unroll
         for (int k = 0; k < R2; ++k) vq[k] = vy[R1 * k];
       }
-      // B1: line y = y0 + lj, active rows x = j2 + R2 r
-      if (lj < ln) {
-        const int py = pos(y0 + lj);
-        C v[R1];
-#pragma unroll
-        for (int r = 0; r < R1; ++r) v[r] = xr[r] >= 0 ? A[xr[r] + py] : cmk<C>(0, 0);
-        dftR<R1, +1>(v, tw, N);
-        C* row = X + lj * LP + j2;
-#pragma unroll
-        for (int k = 0; k < R1; ++k) row[k * R2P] = v[k];
-      }
-      __syncthreads();
+      t.x_inverse1(y0, ln);
       // B2: finish the inverse (x = k1 + R1 k2), * V, start the forward transform
       if (b2) {
-        C* row = X + lk * LP + k1 * R2P;
         C u[R2];
-#pragma unroll
-        for (int r = 0; r < R2; ++r) u[r] = r ? cmul(row[r], cconj(tk[r])) : row[r];
-        dftR<R2, +1>(u, tw, N);
+        t.x_inverse2(u);
 #pragma unroll
         for (int k = 0; k < R2; ++k) u[k] = cscale(u[k], vq[k]);
-        dftR<R2, -1>(u, tw, N);
+        dftR<R2, -1>(u, (const C*)nullptr, N);
+        C* row = X + lk * LP + k1 * ST::R2P;
 #pragma unroll
-        for (int k = 0; k < R2; ++k) row[k] = k ? cmul(u[k], tk[k]) : u[k];
+        for (int k = 0; k < R2; ++k) row[k] = k ? cmul(u[k], t.tk[k]) : u[k];
       }
       __syncthreads();
       // B3: finish the forward transform, x = j2 + R2 m1, active rows back into A
       if (lj < ln) {
-        const int py = pos(y0 + lj);
+        const int py = ST::pos(y0 + lj);
         const C* row = X + lj * LP + j2;
-        C t[R1];
+        C v[R1];
 #pragma unroll
-        for (int j = 0; j < R1; ++j) t[j] = row[j * R2P];
-        dftR<R1, -1>(t, tw, N);
+        for (int j = 0; j < R1; ++j) v[j] = row[j * ST::R2P];
+        dftR<R1, -1>(v, (const C*)nullptr, N);
 #pragma unroll
         for (int k = 0; k < R1; ++k)
-          if (xr[k] >= 0) A[xr[k] + py] = t[k];
+          if (t.xr[k] >= 0) A[t.xr[k] + py] = v[k];
       }
       __syncthreads();
     }
-    if (kPre && band + 1 < bb1 && lj < nax) load_a1(row_rsrc(band + 1), lj, pre);
+    if (kPre && band + 1 < bb1 && lj < nax) t.load_a1(row_rsrc(band + 1), lj, pre);
     // C1: y = j2 + R2 r, output k1 back into the slot of y = j2 + R2 k1
     for (int a = lj; a < nax; a += LJ) {
       C* row = A + a * LP;
       C v[R1];
 #pragma unroll
-      for (int r = 0; r < R1; ++r) v[r] = row[pos(j2 + R2 * r)];
-      dftR<R1, -1>(v, tw, N);
+      for (int r = 0; r < R1; ++r) v[r] = row[ST::pos(j2 + R2 * r)];
+      dftR<R1, -1>(v, (const C*)nullptr, N);
 #pragma unroll
-      for (int k = 0; k < R1; ++k) row[pos(j2 + R2 * k)] = v[k];
+      for (int k = 0; k < R1; ++k) row[ST::pos(j2 + R2 * k)] = v[k];
     }
     __syncthreads();
     // C2: y = k1 + R1 k2 to the sticks
     for (int a = lk; a < nax; a += LK) {
       const C* row = A + a * LP;
-      const int16_t* ty = sty + a * N + k1;
+      const int16_t* ty = t.sty + a * N + k1;
       C u[R2];
 #pragma unroll
       for (int j = 0; j < R2; ++j) {
-        const C w = row[pos(j + R2 * k1)];
-        u[j] = j ? cmul(w, tk[j]) : w;
+        const C w = row[ST::pos(j + R2 * k1)];
+        u[j] = j ? cmul(w, t.tk[j]) : w;
       }
-      dftR<R2, -1>(u, tw, N);
+      dftR<R2, -1>(u, (const C*)nullptr, N);
 #pragma unroll
-      for (int k = 0; k < R2; ++k) buf_st<C>(rs, stick_off(ty[R1 * k]), u[k]);
+      for (int k = 0; k < R2; ++k) buf_st<C>(rs, ST::stick_off(ty[R1 * k]), u[k]);
     }
     __syncthreads();
+  }
+}
+
+// Density on z-major sticks: per z plane and group of bands of one k-point,
+// the y and x inverse transforms as in k_slab_vmul, then
+// part[p][z][y][x] = sum over the group's bands of occ |psi(r)|^2 in FP64
+// (p = k-point slot * groups + group); k_slab_rho_sum adds the partials.
+template <typename R, int N>
+__global__ __launch_bounds__((Slab<R, N>::NT)) void k_slab_density(PwBatch B, const cx_t<R>* __restrict__ S, int nz,
+                                                                 int bpg, const cx_t<R>* __restrict__ tw,
+                                                                 const double* __restrict__ occ, int occ_ks,
+                                                                 double* __restrict__ part) {
+  using C = cx_t<R>;
+  using ST = SlabThread<R, N>;
+  constexpr int R1 = ST::R1, R2 = ST::R2, LR = ST::LR, NR = Slab<R, N>::NR;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const PwMaps m = B.mt[B.k0 + blockIdx.z];
+  const int z = blockIdx.x;
+  ST t;
+  t.init(B, m, tw, smem);
+  double acc[NR][R2];
+#pragma unroll
+  for (int q = 0; q < NR; ++q)
+#pragma unroll
+    for (int k = 0; k < R2; ++k) acc[q][k] = 0.0;
+  const int bb0 = blockIdx.y * bpg, bb1 = min(B.nbb, bb0 + bpg);
+  const uint32_t row_bytes = (uint32_t)B.nst_cap * (uint32_t)sizeof(C);
+  for (int band = bb0; band < bb1; ++band) {
+    const double w = occ[(int64_t)blockIdx.z * occ_ks + band];
+    if (w == 0.0) continue;  // uniform over the workgroup
+    const auto rs = buf_rsrc(S + (((int64_t)blockIdx.z * B.nbb + band) * nz + z) * B.nst_cap, row_bytes);
+    const C none[R1] = {};
+    t.template y_inverse<false>(rs, none);
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      const int y0 = q * LR, ln = min(LR, N - y0);
+      t.x_inverse1(y0, ln);
+      if (t.lk < ln) {
+        C u[R2];
+        t.x_inverse2(u);
+#pragma unroll
+        for (int k = 0; k < R2; ++k) {
+          const double re = (double)u[k].x, im = (double)u[k].y;
+          acc[q][k] += w * (re * re + im * im);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  double* pz = part + ((int64_t)(blockIdx.z * gridDim.y + blockIdx.y) * nz + z) * N * N;
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    const int y = q * LR + t.lk;
+    if (t.lk < min(LR, N - q * LR)) {
+#pragma unroll
+      for (int k = 0; k < R2; ++k) pz[(int64_t)y * N + t.k1 + R1 * k] = acc[q][k];
+    }
+  }
+}
+
+// rho[(x*ny + y)*nz + z] = keep*rho + sum_p part[p][z][y][x] (p in order),
+// 32 x 32 (x, z) tiles of one y through LDS
+__global__ __launch_bounds__(256) void k_slab_rho_sum(const double* __restrict__ part, int np, int nx, int ny, int nz,
+                                                      double* __restrict__ rho, int keep) {
+  __shared__ double tile[32][33];
+  const int x0 = blockIdx.x * 32, z0 = blockIdx.y * 32, y = blockIdx.z;
+  const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
+  const int64_t ng = (int64_t)nx * ny * nz;
+  for (int i = ty; i < 32; i += 8) {
+    const int z = z0 + i, x = x0 + tx;
+    double s = 0.0;
+    if (z < nz && x < nx) {
+      const double* p = part + ((int64_t)z * ny + y) * nx + x;
+      for (int j = 0; j < np; ++j) s += p[j * ng];
+    }
+    tile[i][tx] = s;
+  }
+  __syncthreads();
+  for (int i = ty; i < 32; i += 8) {
+    const int x = x0 + i, z = z0 + tx;
+    if (x < nx && z < nz) {
+      double* r = rho + ((int64_t)x * ny + y) * nz + z;
+      *r = (keep ? *r : 0.0) + tile[tx][i];
+    }
   }
 }
 
@@ -1243,6 +1370,81 @@ void slab_vmul(const PwBatch& B, cx_t<R>* S, const Axis<R>* ax, int nz, const R*
   DEREK_HIP(hipGetLastError());
 }
 
+template <typename R, int N> static int slab_density_groups_n(const PwBatch& B, int nz) {
+  static const int cus = [] {
+    int dev = 0;
+    DEREK_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    DEREK_HIP(hipGetDeviceProperties(&p, dev));
+    return std::max(1, p.multiProcessorCount);
+  }();
+  const size_t lds = Slab<R, N>::lds(B.nax_cap);
+  allow_lds_pw(k_slab_density<R, N>, lds);
+  int per = 0;
+  DEREK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(k_slab_density<R, N>),
+                                                         Slab<R, N>::NT, lds));
+  // about four rounds of resident workgroups
+  const int64_t want = 4LL * std::max(1, per) * cus;
+  const int64_t g = (want + (int64_t)nz * B.nkc - 1) / ((int64_t)nz * B.nkc);
+  const int bpg = (int)ceil_div(B.nbb, std::max<int64_t>(1, std::min<int64_t>(g, B.nbb)));
+  return (int)ceil_div(B.nbb, bpg);
+}
+
+template <typename R> int slab_density_groups(const PwBatch& B, const Axis<R>* ax, int nz) {
+  check_batch(B);
+  DEREK_REQUIRE(slab_supported<R>(ax[0].n, ax[1].n, nz, B.nax_cap, B.nst_cap),
+                "grid not supported by the fused y/x pass");
+  switch (ax[0].n) {
+    case 32: return slab_density_groups_n<R, 32>(B, nz);
+    case 72: return slab_density_groups_n<R, 72>(B, nz);
+    case 108: return slab_density_groups_n<R, 108>(B, nz);
+    case 144:
+      // complex128 planes of 144^3 do not fit the LDS (slab_supported is false)
+      if constexpr (sizeof(R) == 4) return slab_density_groups_n<R, 144>(B, nz);
+      throw ArgError("grid not supported by the fused y/x pass");
+    default: throw ArgError("grid not supported by the fused y/x pass");
+  }
+}
+
+template <typename R, int N>
+static void slab_density_n(const PwBatch& B, const cx_t<R>* S, const Axis<R>* ax, int nz, const double* occ, int occ_ks,
+                           double* part, int groups, hipStream_t s) {
+  using SL = Slab<R, N>;
+  const size_t lds = SL::lds(B.nax_cap);
+  check_lds_pw(lds, N);
+  auto kern = k_slab_density<R, N>;
+  allow_lds_pw(kern, lds);
+  dim3 grid((unsigned)nz, (unsigned)groups, B.nkc);
+  hipLaunchKernelGGL(kern, grid, dim3(SL::NT), lds, s, B, S, nz, (int)ceil_div(B.nbb, groups), ax[0].tw, occ, occ_ks,
+                     part);
+}
+
+template <typename R>
+void slab_density(const PwBatch& B, const cx_t<R>* S, const Axis<R>* ax, int nz, const double* occ, int occ_ks,
+                  double* part, int groups, double* rho, int keep, hipStream_t s) {
+  check_batch(B);
+  DEREK_REQUIRE(slab_supported<R>(ax[0].n, ax[1].n, nz, B.nax_cap, B.nst_cap),
+                "grid not supported by the fused y/x pass");
+  DEREK_REQUIRE(groups >= 1 && groups <= B.nbb && ceil_div(B.nbb, ceil_div(B.nbb, groups)) == groups,
+                "band groups must come from slab_density_groups");
+  switch (ax[0].n) {
+    case 32: slab_density_n<R, 32>(B, S, ax, nz, occ, occ_ks, part, groups, s); break;
+    case 72: slab_density_n<R, 72>(B, S, ax, nz, occ, occ_ks, part, groups, s); break;
+    case 108: slab_density_n<R, 108>(B, S, ax, nz, occ, occ_ks, part, groups, s); break;
+    case 144:
+      if constexpr (sizeof(R) == 4) {
+        slab_density_n<R, 144>(B, S, ax, nz, occ, occ_ks, part, groups, s);
+        break;
+      }
+      throw ArgError("grid not supported by the fused y/x pass");
+    default: throw ArgError("grid not supported by the fused y/x pass");
+  }
+  const int nx = ax[0].n, ny = ax[1].n;
+  dim3 grid((unsigned)ceil_div(nx, 32), (unsigned)ceil_div(nz, 32), (unsigned)ny);
+  hipLaunchKernelGGL(k_slab_rho_sum, grid, dim3(256), 0, s, part, B.nkc * groups, nx, ny, nz, rho, keep);
+  DEREK_HIP(hipGetLastError());
+}
+
 #define INST(R)                                                                                                   \
   template void sticks_z_inv<R>(const PwBatch&, PwBlock<R>, const Axis<R>*, int, cx_t<R>*, bool, hipStream_t);   \
   template void planes_y<R>(const PwBatch&, cx_t<R>*, cx_t<R>*, const Axis<R>*, int, int, int, int, hipStream_t); \
@@ -1250,6 +1452,9 @@ void slab_vmul(const PwBatch& B, cx_t<R>* S, const Axis<R>* ax, int nz, const R*
                                 bool, hipStream_t);                                                               \
   template bool slab_supported<R>(int, int, int, int, int);                                                       \
   template void slab_vmul<R>(const PwBatch&, cx_t<R>*, const Axis<R>*, int, const R*, int, hipStream_t);          \
+  template int slab_density_groups<R>(const PwBatch&, const Axis<R>*, int);                                       \
+  template void slab_density<R>(const PwBatch&, const cx_t<R>*, const Axis<R>*, int, const double*, int, double*,  \
+                                int, double*, int, hipStream_t);                                                  \
   template void planes_x_vmul<R>(const PwBatch&, cx_t<R>*, const Axis<R>*, int64_t, const R*, R, int, hipStream_t); \
   template void planes_x_density<R>(const PwBatch&, const cx_t<R>*, const Axis<R>*, int64_t, const double*, int,  \
                                     double*, int, hipStream_t);

@@ -36,6 +36,13 @@ void sticks_z_fwd(const PwBatch& B, const cx_t<R>* S, const Axis<R>* ax, int nz,
 template <typename R> bool slab_supported(int nx, int ny, int nz, int nax_cap, int nst_cap);
 template <typename R>
 void slab_vmul(const PwBatch& B, cx_t<R>* S, const Axis<R>* ax, int nz, const R* vt, int bands_per_wg, hipStream_t s);
+// Fused density on z-major sticks: rho = keep*rho + sum_items occ[kz*occ_ks + b] |psi(r)|^2.
+// The items are split into `groups` band groups per (z plane, k-point), each
+// writing an FP64 partial plane into part[nkc*groups][N], summed in order.
+template <typename R> int slab_density_groups(const PwBatch& B, const Axis<R>* ax, int nz);
+template <typename R>
+void slab_density(const PwBatch& B, const cx_t<R>* S, const Axis<R>* ax, int nz, const double* occ, int occ_ks,
+                  double* part, int groups, double* rho, int keep, hipStream_t s);
 // x pass: inverse, * vloc * vscale, forward, in place on the active rows of P.
 template <typename R>
 void planes_x_vmul(const PwBatch& B, cx_t<R>* P, const Axis<R>* ax, int64_t nyz, const R* vloc, R vscale,

@@ -334,10 +334,11 @@ void switch_to_fp64(derek_ctx* c) {
 }
 
 // Times H|psi> for the tunable launch shapes, starting from the current
-// ones: the separate or the fused y/x passes with their bands per
-// workgroup, then items per launch.  The timed sample is at most about 256
-// items (the first k-points' leading bands), so timings are noisy: another
-// shape replaces the current one only when it is more than 3 % faster.
+// ones: bands per workgroup on the current (fused or separate) y/x pass,
+// the other pass at the current bands per workgroup, then half and twice
+// the items per launch.  The timed sample is at most about 256 items (the
+// first k-points' leading bands), so timings are noisy: another shape
+// replaces the current one only when it is more than 3 % faster.
 template <typename R> void autotune(derek_ctx* c) {
   ScfEngine& E = *c->scf;
   constexpr int kTuneItems = 256;
@@ -355,7 +356,7 @@ template <typename R> void autotune(derek_ctx* c) {
   auto time_it = [&] {
     float best = 1e30f;
     c->apply_h<R>(0, nkc, x, hx, nbt);
-    for (int r = 0; r < 3; ++r) {
+    for (int r = 0; r < 2; ++r) {
       DEREK_HIP(hipEventRecord(a, c->s));
       c->apply_h<R>(0, nkc, x, hx, nbt);
       DEREK_HIP(hipEventRecord(b, c->s));
@@ -369,55 +370,46 @@ template <typename R> void autotune(derek_ctx* c) {
   // bytes of stick + plane buffers per item; the sweep stays within 8 GiB
   const double item_bytes =
       (double)sizeof(cx_t<R>) * ((double)c->nst_max * c->dims[2] + (double)c->nax_max * c->dims[1] * c->dims[2]);
-  const int all = nkc * nbt;
   const int cap_mem = (int)std::max(1.0, std::floor(8.0 * 1073741824.0 / item_bytes));
-  std::vector<int> caps;
-  for (int q : {8, 16, 32, 64, 128, 256, 512, all})
-    if (q <= all && q <= cap_mem) caps.push_back(q);
-  if (caps.empty()) caps.push_back(std::min(all, cap_mem));
-  std::sort(caps.begin(), caps.end());
-  caps.erase(std::unique(caps.begin(), caps.end()), caps.end());
-  c->item_cap = std::min(c->item_cap, cap_mem);
-  // bring the clocks up before the first timed shape: at least 50 ms of work
+  c->item_cap = std::max(1, std::min(c->item_cap, cap_mem));
+  // bring the clocks up before the first timed shape: at least 15 ms of work
   {
     const auto w0 = std::chrono::steady_clock::now();
-    for (int r = 0; r < 200; ++r) {
+    for (int r = 0; r < 100; ++r) {
       c->apply_h<R>(0, nkc, x, hx, nbt);
       c->sync();
-      if (std::chrono::steady_clock::now() - w0 > std::chrono::milliseconds(50)) break;
+      if (std::chrono::steady_clock::now() - w0 > std::chrono::milliseconds(15)) break;
     }
   }
   float best = time_it();
-  const int bpw0 = c->bpw, fz0 = c->fused;
-  int bpw = bpw0, fz = fz0;
-  for (int f : {0, 1}) {
-    c->fused = f;
-    if (f && !c->use_slab<R>()) continue;
-    for (int q : {1, 2, 4, 8}) {
-      if (q == bpw0 && f == fz0) continue;
-      c->bpw = q;
-      const float t = time_it();
-      if (t < kGain * best) {
-        best = t;
-        bpw = q;
-        fz = f;
-      }
-    }
-  }
-  c->bpw = bpw;
-  c->fused = fz;
-  const int cap0 = c->item_cap;
-  int cap = cap0;
-  for (int q : caps) {
-    if (q == cap0) continue;
-    c->item_cap = q;
+  auto try_shape = [&](int bpw, int fz, int cap) {
+    const int b0 = c->bpw, f0 = c->fused, c0 = c->item_cap;
+    c->bpw = bpw;
+    c->fused = fz;
+    c->item_cap = cap;
     const float t = time_it();
     if (t < kGain * best) {
       best = t;
-      cap = q;
+      return;
     }
-  }
-  c->item_cap = cap;
+    c->bpw = b0;
+    c->fused = f0;
+    c->item_cap = c0;
+  };
+  // bit 0 of `fused` (the H|psi> pass) is tuned; the density bit is kept
+  const int dens = c->fused & derek_ctx::kFusedDensity;
+  const int fz0 = c->fused & derek_ctx::kFusedHpsi;
+  const int bpw0 = c->bpw;
+  // whether this grid has the fused pass at all
+  c->fused = derek_ctx::kFusedHpsi | dens;
+  const bool can_fuse = c->use_slab<R>(derek_ctx::kFusedHpsi);
+  c->fused = fz0 | dens;
+  for (int q : {1, 2, 4, 8})
+    if (q != bpw0) try_shape(q, fz0 | dens, c->item_cap);
+  if (can_fuse) try_shape(c->bpw, (fz0 ^ derek_ctx::kFusedHpsi) | dens, c->item_cap);
+  const int all = nkc * nbt, cap0 = c->item_cap;
+  for (int q : {cap0 / 2, cap0 * 2})
+    if (q >= 1 && q <= cap_mem && q <= std::max(all, cap0)) try_shape(c->bpw, c->fused, q);
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   c->prof.on = prof;

@@ -36,7 +36,8 @@ EXPORTS = (
 )
 
 KERNEL_CLASSES = ("sticks_z_inv", "planes_y_inv", "planes_x_vmul", "planes_y_fwd", "sticks_z_fwd", "planes_x_density",
-                  "overlap", "rotate", "fft_z", "fft_y", "fft_x", "dense", "other", "slab_yx_vmul")
+                  "overlap", "rotate", "fft_z", "fft_y", "fft_x", "dense", "other", "slab_yx_vmul",
+                  "slab_yx_density")
 
 
 class DerekError(RuntimeError):
@@ -484,11 +485,15 @@ class Context:
         out = (C.c_int32 * 5)()
         check(self.lib.derek_hip_get_tuning(self.h, out))
         return {"bands_per_wg": out[0], "items_per_launch": out[1], "kbatch": out[2], "graph": bool(out[3]),
-                "fused": bool(out[4])}
+                "fused": bool(out[4] & 1), "fused_density": bool(out[4] & 2)}
 
-    def set_launch(self, bands_per_wg=2, items_per_launch=256, fused=True):
-        """H|psi> launch shapes; fused: the fused y/x pass where the grid allows it."""
-        check(self.lib.derek_hip_set_launch(self.h, bands_per_wg, items_per_launch, 1 if fused else 0))
+    def set_launch(self, bands_per_wg=2, items_per_launch=256, fused=True, fused_density=None):
+        """Launch shapes; fused / fused_density: the fused y/x passes for H|psi>
+        and for the density where the grid allows them (fused_density follows
+        fused unless given)."""
+        fd = fused if fused_density is None else fused_density
+        check(self.lib.derek_hip_set_launch(self.h, bands_per_wg, items_per_launch,
+                                            (1 if fused else 0) | (2 if fd else 0)))
 
     def scf_step(self):
         r = ScfReport()

@@ -287,7 +287,9 @@ int32_t derek_hip_apply_h_resident(derek_ctx* ctx, int32_t k, int32_t nb);
  * kinetic term, 5 x pass fused with the density accumulation, 6 overlap GEMM,
  * 7 rotation GEMM, 8/9/10 full-grid FFT z/y/x passes, 11 FP64 subspace
  * solves, 12 other SCF kernels (band statistics, TPA, potentials, mixing),
- * 13 fused y and x passes with the V(r) multiply (in place of 1, 2 and 3).
+ * 13 fused y and x passes with the V(r) multiply (in place of 1, 2 and 3),
+ * 14 fused y and x inverse passes with the density accumulation (in place of
+ * 1 and 5).
  * (The reference's equivalent is its trace/profile module, trace.f90:515-667.) */
 int32_t derek_hip_profile_enable(derek_ctx* ctx, int32_t on);
 int32_t derek_hip_profile_reset(derek_ctx* ctx);
@@ -296,10 +298,12 @@ int32_t derek_hip_profile_read(derek_ctx* ctx, int32_t cls, int64_t* launches, d
 /* Launch shapes in use: out[0] bands per x-pass workgroup (1, 2, 4, 8),
  * out[1] (k-point, band) items per pipeline launch, out[2] k-points per
  * batched launch, out[3] 1 if SCF steps are replayed from a hipGraph,
- * out[4] 1 if H|psi> uses the fused y/x pass where the grid allows it. */
+ * out[4] the fused y/x passes in use where the grid allows them: bit 0 for
+ * H|psi>, bit 1 for the density. */
 int32_t derek_hip_get_tuning(derek_ctx* ctx, int32_t out[5]);
 /* Set the H|psi> launch shapes (what scf_init's autotuner would choose):
- * bands per workgroup (1, 2, 4, 8), items per launch (>= 1), fused 0/1.
+ * bands per workgroup (1, 2, 4, 8), items per launch (>= 1), fused 0..3
+ * (bit 0: fused y/x pass for H|psi>, bit 1: for the density).
  * The fused y/x pass is used for nx = ny in {32, 72, 108, 144} and z lengths
  * in the same set when its LDS working set fits; otherwise the separate
  * passes run.  No counterpart in the reference (launch-shape control of the

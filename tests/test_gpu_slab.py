@@ -85,6 +85,33 @@ def test_fused_matches_separate_passes(precision):
     ctx.close()
 
 
+@pytest.mark.parametrize("n", [32, 72, 108, 144])
+@pytest.mark.parametrize("precision", [lib.DEREK_FP64, lib.DEREK_MIXED])
+def test_fused_density_matches_oracle(n, precision):
+    """rho = sum occ |psi(r)|^2 / volume through the fused pass, with zero
+    occupations skipped and the accumulate (keep) form."""
+    print("fused density", n, precision, flush=True)
+    su = gscf.Setup(grid_spec(n))
+    nb = 7
+    ctx = su.context(precision, nbands=nb)
+    ctx.set_launch(2, 5, fused=True)  # 7 bands in launches of 5 and 2
+    ctx.profile(True)
+    rng = np.random.default_rng(n + 1)
+    occ = np.array([2.0, 0.0, 1.5, 2.0, 0.25, 0.0, 1.0])
+    tol = 1e-12 if precision == lib.DEREK_FP64 else 5e-5
+    for k in range(su.nk):
+        psi = rnd_c(rng, nb, su.npw[k]) / np.sqrt(su.npw[k])
+        got = ctx.density(k, psi, occ)
+        ref = oks.density([psi], [occ], [su.weights[k]], [su.fft_index[k]], su.dims, su.volume)
+        assert_close(got, ref, tol)
+        assert_close(ctx.density(k, psi, occ, rho=got), 2 * ref, tol)
+    prof = ctx.profile_read()
+    fz = fused_expected(n, precision)
+    assert (prof["slab_yx_density"][0] > 0) == fz
+    assert (prof["planes_x_density"][0] > 0) == (not fz)
+    ctx.close()
+
+
 def test_fused_hermitian_and_kinetic_only():
     """<x|Hy> = conj(<y|Hx>) through the fused pass; V = 0 gives the kinetic term exactly."""
     su = gscf.Setup(config.benchmark(2))
@@ -105,7 +132,7 @@ def test_fused_hermitian_and_kinetic_only():
 def test_set_launch_rejects_bad_shapes():
     su = gscf.Setup(config.benchmark(1))
     ctx = su.context(nbands=2)
-    for bpw, items, fused in [(3, 16, 1), (4, 0, 1), (4, 16, 2)]:
+    for bpw, items, fused in [(3, 16, 1), (4, 0, 1), (4, 16, 4), (4, 16, -1)]:
         with pytest.raises(lib.DerekError) as e:
             lib.check(ctx.lib.derek_hip_set_launch(ctx.h, bpw, items, fused))
         assert e.value.code == lib.DEREK_ERR_ARG
