@@ -37,9 +37,30 @@ __device__ __forceinline__ double2 zconjmul(double2 a, double2 b) {  // conj(a) 
   return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
 }
 
+// One DPP move of a double (both 32-bit halves) with the given control and
+// row mask; lanes without a source (or outside the mask) read 0.
+template <int CTRL, int ROWS = 0xf> __device__ __forceinline__ double dpp_move(double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffull), CTRL, ROWS, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROWS, 0xf, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Sum over the 64 lanes of a wave (all lanes active), every lane gets it:
+// quad butterflies, row shifts by 4 and 8 (lane 15 of each row holds the row
+// sum), row broadcasts 15 and 31 (lane 63 holds the total), then a readlane.
+// Cross-lane moves on the VALU instead of LDS permutes.
 __device__ __forceinline__ double wave_sum(double x) {
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
-  return __shfl(x, 0, 64);
+  x += dpp_move<0xb1>(x);        // quad_perm [1,0,3,2]
+  x += dpp_move<0x4e>(x);        // quad_perm [2,3,0,1]
+  x += dpp_move<0x114>(x);       // row_shr:4
+  x += dpp_move<0x118>(x);       // row_shr:8
+  x += dpp_move<0x142, 0xa>(x);  // row_bcast:15 into rows 1 and 3
+  x += dpp_move<0x143, 0xc>(x);  // row_bcast:31 into rows 2 and 3
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffull), 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), 63);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 // Sum over the workgroup; every thread gets the result.
@@ -92,8 +113,14 @@ __device__ int sturm_count(const double* d, const double* e2, int n, double x, d
 
 // Solve (T - lambda I) x = b by LU with partial pivoting; T has diagonal d and
 // off-diagonal e.  u0/u1/u2/l/pv are scratch of length n; b is overwritten by x.
-__device__ void tri_solve(const double* d, const double* e, int n, double lambda, double pivmin, double* u0,
-                          double* u1, double* u2, double* l, double* pv, double* b) {
+// LU factorisation with partial pivoting of T - lambda I (T symmetric
+// tridiagonal: diagonal d, off-diagonal e), pivots below pivmin raised to it;
+// U has the three diagonals u0, u1, u2, L the multipliers l with row swaps pv.
+// The arrays are separate LDS regions (restrict), so the loads of the next
+// row are issued while the recurrence runs in registers.
+__device__ void tri_factor(const double* __restrict__ d, const double* __restrict__ e, int n, double lambda,
+                           double pivmin, double* __restrict__ u0, double* __restrict__ u1, double* __restrict__ u2,
+                           double* __restrict__ l, double* __restrict__ pv) {
   double cd = d[0] - lambda, cs = n > 1 ? e[0] : 0.0, cs2 = 0.0;
   for (int i = 0; i + 1 < n; ++i) {
     const double c = e[i];
@@ -123,17 +150,36 @@ __device__ void tri_solve(const double* d, const double* e, int n, double lambda
     }
   }
   u0[n - 1] = fabs(cd) < pivmin ? copysign(pivmin, cd == 0.0 ? 1.0 : cd) : cd;
+}
+
+// b <- (T - lambda I)^-1 b from tri_factor's factors; the values carried from
+// row to row stay in registers.
+__device__ void tri_solve(int n, const double* __restrict__ u0, const double* __restrict__ u1,
+                          const double* __restrict__ u2, const double* __restrict__ l, const double* __restrict__ pv,
+                          double* __restrict__ b) {
+  double bi = b[0];
   for (int i = 0; i + 1 < n; ++i) {
+    double bn = b[i + 1];
     if (pv[i] != 0.0) {
-      const double t = b[i];
-      b[i] = b[i + 1];
-      b[i + 1] = t;
+      const double t = bi;
+      bi = bn;
+      bn = t;
     }
-    b[i + 1] -= l[i] * b[i];
+    bn -= l[i] * bi;
+    b[i] = bi;
+    bi = bn;
   }
-  b[n - 1] /= u0[n - 1];
-  if (n > 1) b[n - 2] = (b[n - 2] - u1[n - 2] * b[n - 1]) / u0[n - 2];
-  for (int i = n - 3; i >= 0; --i) b[i] = (b[i] - u1[i] * b[i + 1] - u2[i] * b[i + 2]) / u0[i];
+  double b1 = bi / u0[n - 1];  // b[i+1]
+  b[n - 1] = b1;
+  if (n < 2) return;
+  double b0 = (b[n - 2] - u1[n - 2] * b1) / u0[n - 2];  // b[i+1] after the step below
+  b[n - 2] = b0;
+  for (int i = n - 3; i >= 0; --i) {
+    const double bi2 = (b[i] - u1[i] * b0 - u2[i] * b1) / u0[i];
+    b[i] = bi2;
+    b1 = b0;
+    b0 = bi2;
+  }
 }
 
 // Per-matrix workspace layout (doubles), see heev_work_doubles.
@@ -201,13 +247,42 @@ __device__ __forceinline__ void tri_reflector(int tid, int k, int n, double xn2,
   }
 }
 
-template <bool IN_LDS>
+// e = |sub| and the phases ph[0] = 1, ph[k+1] = ph[k] sub_k/|sub_k| that make
+// D^H T D real.  The sub-diagonal is staged in LDS (s_sub, n entries) by the
+// whole workgroup first, so that the one-thread product runs at LDS latency
+// instead of one global round trip per step.
+__device__ void tri_phases(HeevWork& W, int n, double2* s_sub) {
+  for (int k = threadIdx.x; k + 1 < n; k += blockDim.x) {
+    const double2 sk = W.sub[k];
+    s_sub[k] = sk;
+    W.e[k] = sqrt(sk.x * sk.x + sk.y * sk.y);
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  W.e[n - 1] = 0.0;
+  double2 p = make_double2(1.0, 0.0);
+  W.ph[0] = p;
+  for (int k = 0; k + 1 < n; ++k) {
+    const double2 sk = s_sub[k];
+    const double a = sqrt(sk.x * sk.x + sk.y * sk.y);
+    if (a > 0.0) p = zmul(p, make_double2(sk.x / a, sk.y / a));
+    W.ph[k + 1] = p;
+  }
+}
+
+// Householder reduction of the trailing block A[k0:, k0:] (nb = n - k0 rows
+// and columns) in LDS, one workgroup per matrix: the whole matrix for k0 = 0,
+// or the tail after the multi-workgroup steps 0..k0-1.  The LDS arrays are
+// addressed with global indices through offset bases.
 __global__ __launch_bounds__(DT) void k_tridiag(double2* __restrict__ Aall, int lda_g, int64_t strideA, int n,
-                                               int nev, double* __restrict__ work, int64_t strideWork) {
+                                               int k0, int nev, double* __restrict__ work, int64_t strideWork) {
   extern __shared__ __align__(16) unsigned char smem[];
-  double2* v = reinterpret_cast<double2*>(smem);  // [n]
-  double2* u = v + n;                              // [n]
-  double2* As = u + n;                             // [n * n] when IN_LDS
+  const int nb = n - k0;
+  double2* vs = reinterpret_cast<double2*>(smem);  // [nb]
+  double2* us = vs + nb;                           // [nb]
+  double2* As = us + nb;                           // [nb * nb]
+  double2* v = vs - k0;                            // v[i], i >= k0
+  double2* u = us - k0;
   // reflector scalars, double-buffered by the parity of the step: step k
   // reads slot k&1 while wave 0 writes slot (k+1)&1 for the next step
   __shared__ int s_skip[2];
@@ -216,23 +291,22 @@ __global__ __launch_bounds__(DT) void k_tridiag(double2* __restrict__ Aall, int 
   double2* Ag = Aall + mat * strideA;
   HeevWork W(work + mat * strideWork, n, nev);
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  double2* A = IN_LDS ? As : Ag;
-  const int lda = IN_LDS ? n : lda_g;
-  if (IN_LDS) {
-    for (int q = tid; q < n * n; q += DT) As[q] = Ag[(q % n) + (int64_t)(q / n) * lda_g];
-    __syncthreads();
-  }
-  if (n > 2 && wid == 0) {
+  const int lda = nb;
+  double2* A = As - k0 - (int64_t)k0 * lda;  // A[i + j*lda], i, j >= k0
+  for (int q = tid; q < nb * nb; q += DT) As[q] = Ag[(k0 + q % nb) + (int64_t)(k0 + q / nb) * lda_g];
+  __syncthreads();
+  if (nb > 2 && wid == 0) {
+    const double2* c0 = A + (int64_t)k0 * lda;
     double part = 0.0;
-    for (int i = 1 + lane; i < n; i += 64) {
-      const double2 a = A[i];
+    for (int i = k0 + 1 + lane; i < n; i += 64) {
+      const double2 a = c0[i];
       part += a.x * a.x + a.y * a.y;
     }
     part = wave_sum(part);
-    tri_reflector(tid, 0, n, part, A[1], &s_skip[0], s_scal[0], W);
+    tri_reflector(tid, k0, n, part, c0[k0 + 1], &s_skip[k0 & 1], s_scal[k0 & 1], W);
   }
   __syncthreads();
-  for (int k = 0; k + 2 < n; ++k) {
+  for (int k = k0; k + 2 < n; ++k) {
     const bool last = k + 3 >= n;  // no reflector for column k+1 after this step
     const int cur = k & 1, nxt = cur ^ 1;
     if (!s_skip[cur]) {
@@ -303,24 +377,13 @@ __global__ __launch_bounds__(DT) void k_tridiag(double2* __restrict__ Aall, int 
     }
     W.skip[n - 1] = 1;
   }
-  for (int i = tid; i < n; i += DT) W.d[i] = A[i + (int64_t)i * lda].x;
-  if (IN_LDS) {
-    // the reflectors (strict lower triangle) are read by the back-transformation
-    for (int q = tid; q < n * n; q += DT) Ag[(q % n) + (int64_t)(q / n) * lda_g] = As[q];
-  }
+  // the diagonal: rows < k0 were final after their own step
+  for (int i = tid; i < n; i += DT) W.d[i] = i < k0 ? Ag[i + (int64_t)i * lda_g].x : A[i + (int64_t)i * lda].x;
+  // the reflectors (strict lower triangle) are read by the back-transformation
+  for (int q = tid; q < nb * nb; q += DT) Ag[(k0 + q % nb) + (int64_t)(k0 + q / nb) * lda_g] = As[q];
   __syncthreads();
-  // phases: T' = D^H T D real with off-diagonal |sub|
-  if (tid == 0) {
-    double2 p = make_double2(1.0, 0.0);
-    W.ph[0] = p;
-    for (int k = 0; k + 1 < n; ++k) {
-      const double a = sqrt(W.sub[k].x * W.sub[k].x + W.sub[k].y * W.sub[k].y);
-      W.e[k] = a;
-      if (a > 0.0) p = zmul(p, make_double2(W.sub[k].x / a, W.sub[k].y / a));
-      W.ph[k + 1] = p;
-    }
-    W.e[n - 1] = 0.0;
-  }
+  // phases: T' = D^H T D real with off-diagonal |sub| (As is free now, nb^2 >= n)
+  tri_phases(W, n, As);
 }
 
 // ---- 1'. wide tridiagonalisation (matrices that do not fit LDS): two
@@ -467,29 +530,6 @@ __global__ __launch_bounds__(WT) void k_wide_update(double2* __restrict__ Aall, 
   }
 }
 
-// tail of the reduction: last sub-diagonal, diagonal, phases
-__global__ __launch_bounds__(64) void k_wide_last(const double2* __restrict__ Aall, int lda, int64_t strideA, int n,
-                                                  int nev, double* __restrict__ work, int64_t strideWork) {
-  const int mat = blockIdx.x;
-  const double2* A = Aall + mat * strideA;
-  HeevWork W(work + mat * strideWork, n, nev);
-  for (int i = threadIdx.x; i < n; i += 64) W.d[i] = A[i + (int64_t)i * lda].x;
-  if (threadIdx.x != 0) return;
-  if (n >= 2) {
-    W.sub[n - 2] = A[(n - 1) + (int64_t)(n - 2) * lda];
-    W.skip[n - 2] = 1;
-  }
-  W.skip[n - 1] = 1;
-  double2 p = make_double2(1.0, 0.0);
-  W.ph[0] = p;
-  for (int k = 0; k + 1 < n; ++k) {
-    const double a = sqrt(W.sub[k].x * W.sub[k].x + W.sub[k].y * W.sub[k].y);
-    W.e[k] = a;
-    if (a > 0.0) p = zmul(p, make_double2(W.sub[k].x / a, W.sub[k].y / a));
-    W.ph[k + 1] = p;
-  }
-  W.e[n - 1] = 0.0;
-}
 
 // ---- 2. eigenvalues by multisection: one wavefront per eigenvalue; the 64
 // lanes count eigenvalues below 64 interior points, so every step narrows the
@@ -551,17 +591,25 @@ __global__ __launch_bounds__(BSW * 64) void k_bisect(int n, int nev, double* __r
   if (lane == 0) W.lam[t] = 0.5 * (lo + hi);
 }
 
+// (d, e and the eigenvalues are staged in LDS, n + nev doubles, so that the
+// one-thread scans run at LDS latency)
 __global__ void k_clusters(int n, int nev, double* __restrict__ work, int64_t strideWork) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* tn = reinterpret_cast<double*>(smem);  // [n] |d_i| + 2|e_i|
+  double* lam = tn + n;                           // [nev]
   const int mat = blockIdx.x;
   HeevWork W(work + mat * strideWork, n, nev);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) tn[i] = fabs(W.d[i]) + 2.0 * (i + 1 < n ? fabs(W.e[i]) : 0.0);
+  for (int t = threadIdx.x; t < nev; t += blockDim.x) lam[t] = W.lam[t];
+  __syncthreads();
   if (threadIdx.x != 0) return;
   double tnorm = 0.0;
-  for (int i = 0; i < n; ++i) tnorm = fmax(tnorm, fabs(W.d[i]) + 2.0 * (i + 1 < n ? fabs(W.e[i]) : 0.0));
+  for (int i = 0; i < n; ++i) tnorm = fmax(tnorm, tn[i]);
   // eigenvalues closer than 1e-5 |T| are computed in order and re-orthogonalised
   const double ortol = 1e-5 * fmax(tnorm, 1e-300);
   int nc = 0;
   for (int t = 0; t < nev; ++t)
-    if (t == 0 || W.lam[t] - W.lam[t - 1] > ortol) W.cstart[nc++] = t;
+    if (t == 0 || lam[t] - lam[t - 1] > ortol) W.cstart[nc++] = t;
   W.cstart[nc] = nev;
   *W.ncl = nc;
 }
@@ -610,8 +658,9 @@ __global__ __launch_bounds__(IT) void k_inviter(int n, int nev, int cpw, double*
       h ^= h >> 29;
       x[i] = ((double)(h >> 11) * (1.0 / 9007199254740992.0)) - 0.5;
     }
+    tri_factor(dd, ee, n, lmb, pivmin, u0, u1, u2, lv, pv);
     for (int it = 0; it < 3; ++it) {
-      tri_solve(dd, ee, n, lmb, pivmin, u0, u1, u2, lv, pv, x);
+      tri_solve(n, u0, u1, u2, lv, pv, x);
       for (int q = t0; q < t; ++q) {  // orthogonalise against the earlier cluster members
         const double* y = W.xt + (size_t)q * n;
         double sdot = 0.0;
@@ -669,21 +718,24 @@ void heev_gpu(double2* A, int lda, int64_t strideA, int n, int nev, double* Wv, 
   DEREK_REQUIRE(n >= 1 && nev >= 1 && nev <= n, "heev: need 1 <= nev <= n");
   DEREK_REQUIRE(n <= 2048, "heev: n must be <= 2048");
   {
-    const size_t lds_g = sizeof(double2) * 2 * (size_t)n;
-    const size_t lds_l = lds_g + sizeof(double2) * (size_t)n * n;
-    if (lds_l <= kTridiagLds) {
-      allow_dynamic_lds(reinterpret_cast<const void*>(k_tridiag<true>), kTridiagLds);
-      hipLaunchKernelGGL(k_tridiag<true>, dim3(batch), dim3(DT), lds_l, s, A, lda, strideA, n, nev, work, strideWork);
-    } else {
-      // two launches per column over all of the trailing matrix
+    // the largest trailing block the LDS kernel holds (A, v and u)
+    auto lds_for = [](int nb) { return sizeof(double2) * ((size_t)nb * nb + 2 * (size_t)nb); };
+    int nbl = std::min(n, 98);
+    while (nbl > 1 && lds_for(nbl) > kTridiagLds) --nbl;
+    const int k0 = n - nbl;
+    // steps 0..k0-1 with two multi-workgroup launches per column while the
+    // trailing matrix is larger than that, then the rest in one workgroup
+    if (k0 > 0) {
       hipLaunchKernelGGL(k_wide_first, dim3(batch), dim3(64), 0, s, A, lda, strideA, n, nev, work, strideWork);
-      for (int k = 0; k + 2 < n; ++k) {
+      for (int k = 0; k < k0; ++k) {
         dim3 grid((unsigned)ceil_div(n - k - 1, WC), batch);
         hipLaunchKernelGGL(k_wide_gemv, grid, dim3(WT), 0, s, A, lda, strideA, n, nev, k, work, strideWork);
         hipLaunchKernelGGL(k_wide_update, grid, dim3(WT), 0, s, A, lda, strideA, n, nev, k, work, strideWork);
       }
-      hipLaunchKernelGGL(k_wide_last, dim3(batch), dim3(64), 0, s, A, lda, strideA, n, nev, work, strideWork);
     }
+    allow_dynamic_lds(reinterpret_cast<const void*>(k_tridiag), kTridiagLds);
+    hipLaunchKernelGGL(k_tridiag, dim3(batch), dim3(DT), lds_for(nbl), s, A, lda, strideA, n, k0, nev, work,
+                       strideWork);
     DEREK_HIP(hipGetLastError());
   }
   {
@@ -691,7 +743,8 @@ void heev_gpu(double2* A, int lda, int64_t strideA, int n, int nev, double* Wv, 
     hipLaunchKernelGGL(k_bisect, dim3((unsigned)ceil_div(nev, BSW), batch), dim3(BSW * 64), lds, s, n, nev, work,
                        strideWork);
     DEREK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_clusters, dim3(batch), dim3(64), 0, s, n, nev, work, strideWork);
+    hipLaunchKernelGGL(k_clusters, dim3(batch), dim3(64), sizeof(double) * ((size_t)n + nev), s, n, nev, work,
+                       strideWork);
     DEREK_HIP(hipGetLastError());
   }
   {
