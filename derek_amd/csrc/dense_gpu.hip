@@ -309,6 +309,7 @@ __global__ __launch_bounds__(DT) void k_tridiag(double2* __restrict__ Aall, int 
   for (int k = k0; k + 2 < n; ++k) {
     const bool last = k + 3 >= n;  // no reflector for column k+1 after this step
     const int cur = k & 1, nxt = cur ^ 1;
+    const int m = n - k - 1;  // trailing rows and columns
     if (!s_skip[cur]) {
       const double iv = s_scal[cur][0];
       for (int i = k + 1 + tid; i < n; i += DT) {
@@ -316,49 +317,93 @@ __global__ __launch_bounds__(DT) void k_tridiag(double2* __restrict__ Aall, int 
         v[i] = make_double2(a.x * iv, a.y * iv);
       }
       __syncthreads();
-      // u_i = sum_j A(i,j) v_j = sum_j conj(A(j,i)) v_j  (column i), one wave per column
-      for (int i = k + 1 + wid; i < n; i += DW) {
-        double sr = 0.0, si = 0.0;
-        const double2* col = A + (int64_t)i * lda;
-        for (int j = k + 1 + lane; j < n; j += 64) {
-          const double2 t = zconjmul(col[j], v[j]);
-          sr += t.x;
-          si += t.y;
+      // u_i = sum_j A(i,j) v_j along row i: one thread per row (consecutive
+      // rows across lanes, v_j a broadcast), no cross-lane reductions
+      for (int i = k + 1 + tid; i < n; i += DT) {
+        // eight columns per iteration, all loads issued before the first use
+        // (LDS latency paid once per eight), two accumulators
+        double sr = 0.0, si = 0.0, tr = 0.0, ti = 0.0;
+        const double2* row = A + i;
+        int j = k + 1;
+        for (; j + 7 < n; j += 8) {
+          double2 a[8], b[8];
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            a[q] = row[(int64_t)(j + q) * lda];
+            b[q] = v[j + q];
+          }
+#pragma unroll
+          for (int q = 0; q < 8; q += 2) {
+            sr += a[q].x * b[q].x - a[q].y * b[q].y;
+            si += a[q].x * b[q].y + a[q].y * b[q].x;
+            tr += a[q + 1].x * b[q + 1].x - a[q + 1].y * b[q + 1].y;
+            ti += a[q + 1].x * b[q + 1].y + a[q + 1].y * b[q + 1].x;
+          }
         }
-        sr = wave_sum(sr);
-        si = wave_sum(si);
-        if (lane == 0) u[i] = make_double2(sr, si);
+        for (; j < n; ++j) {
+          const double2 a0 = row[(int64_t)j * lda], v0 = v[j];
+          sr += a0.x * v0.x - a0.y * v0.y;
+          si += a0.x * v0.y + a0.y * v0.x;
+        }
+        u[i] = make_double2(sr + tr, si + ti);
       }
       __syncthreads();
       // gamma = v^H u, every wave on its own; w = 2u - 2 gamma v
       double gp = 0.0;
       for (int i = k + 1 + lane; i < n; i += 64) gp += zconjmul(v[i], u[i]).x;
       const double g2 = 2.0 * wave_sum(gp);
-      // A22 -= v w^H + w v^H, one wave per column; the reflector replaces column k
-      for (int j = k + 1 + wid; j < n; j += DW) {
-        const double2 vj = v[j], uj = u[j];
-        const double2 wj = make_double2(2.0 * uj.x - g2 * vj.x, 2.0 * uj.y - g2 * vj.y);
-        double2* col = A + (int64_t)j * lda;
-        double part = 0.0;
-        double2 x0 = make_double2(0.0, 0.0);
-        for (int i = k + 1 + lane; i < n; i += 64) {
-          const double2 vi = v[i], ui = u[i];
-          const double2 wi = make_double2(2.0 * ui.x - g2 * vi.x, 2.0 * ui.y - g2 * vi.y);
+      // A22 -= v w^H + w v^H: thread (row t, group g) updates row k+1+t of
+      // the columns k+1+g, k+1+g+cg, ...; group 0 also keeps the reflector in
+      // column k and, for the next reflector, the norm of column k+1 below row k+1
+      const int cg = DT / m;
+      const int t = tid % m, g = tid / m;
+      double part = 0.0;
+      double2 x0 = make_double2(0.0, 0.0);
+      if (g < cg) {
+        const int i = k + 1 + t;
+        const double2 vi = v[i], ui = u[i];
+        const double2 wi = make_double2(2.0 * ui.x - g2 * vi.x, 2.0 * ui.y - g2 * vi.y);
+        // two columns per iteration with their loads issued first
+        auto upd = [&](int j, double2 a, double2 vj, double2 uj) {
+          const double2 wj = make_double2(2.0 * uj.x - g2 * vj.x, 2.0 * uj.y - g2 * vj.y);
           const double2 t1 = zmul(vi, make_double2(wj.x, -wj.y));
           const double2 t2 = zmul(wi, make_double2(vj.x, -vj.y));
-          const double2 a = col[i];
           const double2 r = make_double2(a.x - t1.x - t2.x, a.y - t1.y - t2.y);
-          col[i] = r;
-          if (i > j) part += r.x * r.x + r.y * r.y;
-          if (i == j + 1) x0 = r;
+          A[i + (int64_t)j * lda] = r;
+          if (j == k + 1) {
+            if (i > j) part = r.x * r.x + r.y * r.y;
+            if (i == j + 1) x0 = r;
+          }
+        };
+        int j = k + 1 + g;
+        for (; j + cg < n; j += 2 * cg) {
+          const int j1 = j + cg;
+          const double2 a0 = A[i + (int64_t)j * lda], a1 = A[i + (int64_t)j1 * lda];
+          const double2 v0 = v[j], u0 = u[j], v1 = v[j1], u1 = u[j1];
+          upd(j, a0, v0, u0);
+          upd(j1, a1, v1, u1);
         }
-        if (j == k + 1 && !last) {
+        if (j < n) upd(j, A[i + (int64_t)j * lda], v[j], u[j]);
+        if (g == 0) A[i + (int64_t)k * lda] = vi;  // keep the reflector
+      }
+      if (!last) {
+        // column k+1 lies in wave 0 when m <= 64; otherwise it spans two waves
+        if (m > 64) __syncthreads();
+        if (wid == 0) {
+          if (m > 64) {
+            part = 0.0;
+            x0 = make_double2(0.0, 0.0);
+            for (int i = k + 2 + lane; i < n; i += 64) {
+              const double2 r = A[i + (int64_t)(k + 1) * lda];
+              part += r.x * r.x + r.y * r.y;
+              if (i == k + 2) x0 = r;
+            }
+          }
           part = wave_sum(part);
           x0 = make_double2(wave_sum(x0.x), wave_sum(x0.y));  // only one lane holds it
           tri_reflector(lane, k + 1, n, part, x0, &s_skip[nxt], s_scal[nxt], W);
         }
       }
-      for (int i = k + 1 + tid; i < n; i += DT) A[i + (int64_t)k * lda] = v[i];  // keep the reflector
     } else if (wid == 0 && !last) {
       double part = 0.0;
       for (int i = k + 2 + lane; i < n; i += 64) {
