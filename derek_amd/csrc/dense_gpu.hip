@@ -758,6 +758,107 @@ __global__ __launch_bounds__(BW * 64) void k_backtransform(const double2* __rest
   }
 }
 
+// The same with the eigenvector in registers (rows lane + 64 q, q < NQ,
+// n <= 64 NQ) and the reflector columns staged in LDS in panels of BTP
+// columns: the workgroup loads the next panel (coalesced over rows) while its
+// waves run the current one, and the skip flags sit in LDS, so no step waits
+// on global memory.  LDS: 2 BTP n complex + n ints.
+constexpr int BTP = 8;
+
+template <int NQ>
+__global__ __launch_bounds__(BW * 64) void k_backtransform_reg(const double2* __restrict__ Aall, int lda,
+                                                              int64_t strideA, int n, int nev,
+                                                              double* __restrict__ work, int64_t strideWork,
+                                                              double* __restrict__ Wall, int64_t strideW,
+                                                              double2* __restrict__ Zall, int ldz, int64_t strideZ) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double2* pan = reinterpret_cast<double2*>(smem);   // [2][BTP][n]
+  int* skip = reinterpret_cast<int*>(pan + 2 * BTP * n);  // [n]
+  const int mat = blockIdx.y;
+  const double2* A = Aall + mat * strideA;
+  HeevWork W(work + mat * strideWork, n, nev);
+  constexpr int NT = BW * 64;
+  constexpr int PL = (BTP * 64 * NQ + NT - 1) / NT;  // panel elements per thread (n <= 64 NQ)
+  const int tid = threadIdx.x;
+  for (int i = tid; i < n; i += NT) skip[i] = W.skip[i];
+  // panel p holds columns k = n-3 - BTP p - c, c < BTP (rows > k, zero above)
+  auto fetch = [&](int p, double2* r) {
+#pragma unroll
+    for (int e = 0; e < PL; ++e) {
+      const int q = tid + e * NT, c = q / n, i = q - c * n, k = n - 3 - BTP * p - c;
+      r[e] = (c < BTP && k >= 0 && i > k) ? A[i + (int64_t)k * lda] : make_double2(0.0, 0.0);
+    }
+  };
+  auto store = [&](int p, const double2* r) {
+    double2* b = pan + (size_t)(p & 1) * BTP * n;
+#pragma unroll
+    for (int e = 0; e < PL; ++e) {
+      const int q = tid + e * NT;
+      if (q < BTP * n) b[q] = r[e];
+    }
+  };
+  double2 pre[PL];
+  fetch(0, pre);
+  store(0, pre);
+  __syncthreads();
+  const int lane = tid & 63, wid = tid >> 6;
+  const int t = blockIdx.x * BW + wid;
+  const bool act = t < nev;
+  double2 z[NQ];
+  if (act) {
+    const double* xs = W.xt + (size_t)t * n;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int i = lane + 64 * q;
+      z[q] = i < n ? make_double2(W.ph[i].x * xs[i], W.ph[i].y * xs[i]) : make_double2(0.0, 0.0);
+    }
+    if (lane == 0) Wall[mat * strideW + t] = W.lam[t];
+  }
+  const int np = n >= 3 ? (n - 3) / BTP + 1 : 0;
+  for (int p = 0; p < np; ++p) {
+    if (p + 1 < np) fetch(p + 1, pre);
+    const double2* b = pan + (size_t)(p & 1) * BTP * n;
+    if (act) {
+      for (int c = 0; c < BTP; ++c) {
+        const int k = n - 3 - BTP * p - c;
+        if (k < 0) break;
+        if (skip[k]) continue;
+        const double2* col = b + (size_t)c * n;
+        double2 v[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          const int i = lane + 64 * q;
+          v[q] = i < n ? col[i] : make_double2(0.0, 0.0);
+        }
+        double sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          const double2 pr = zconjmul(v[q], z[q]);
+          sr += pr.x;
+          si += pr.y;
+        }
+        sr = 2.0 * wave_sum(sr);
+        si = 2.0 * wave_sum(si);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          const double2 pr = zmul(v[q], make_double2(sr, si));
+          z[q] = make_double2(z[q].x - pr.x, z[q].y - pr.y);
+        }
+      }
+    }
+    if (p + 1 < np) store(p + 1, pre);
+    __syncthreads();
+  }
+  if (act) {
+    double2* zo = Zall + mat * strideZ + (int64_t)t * ldz;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int i = lane + 64 * q;
+      if (i < n) zo[i] = z[q];
+    }
+  }
+}
+
 void heev_gpu(double2* A, int lda, int64_t strideA, int n, int nev, double* Wv, int64_t strideW, double2* Z, int ldz,
               int64_t strideZ, double* work, int64_t strideWork, int batch, hipStream_t s) {
   DEREK_REQUIRE(n >= 1 && nev >= 1 && nev <= n, "heev: need 1 <= nev <= n");
@@ -803,8 +904,22 @@ void heev_gpu(double2* A, int lda, int64_t strideA, int n, int nev, double* Wv, 
   }
   {
     dim3 grid((unsigned)ceil_div(nev, BW), batch);
-    hipLaunchKernelGGL(k_backtransform, grid, dim3(BW * 64), 0, s, A, lda, strideA, n, nev, work, strideWork, Wv,
-                       strideW, Z, ldz, strideZ);
+    const size_t lds = sizeof(double2) * 2 * BTP * (size_t)n + sizeof(int) * (size_t)n;
+#define BT_(NQ)                                                                                                  \
+  do {                                                                                                           \
+    if (lds > 65536) allow_dynamic_lds(reinterpret_cast<const void*>(k_backtransform_reg<NQ>), lds);             \
+    hipLaunchKernelGGL(k_backtransform_reg<NQ>, grid, dim3(BW * 64), lds, s, A, lda, strideA, n, nev, work,      \
+                       strideWork, Wv, strideW, Z, ldz, strideZ);                                                \
+  } while (0)
+    if (n <= 64) BT_(1);
+    else if (n <= 128) BT_(2);
+    else if (n <= 192) BT_(3);
+    else if (n <= 256) BT_(4);
+    else if (n <= 384) BT_(6);  // more rows per lane would not fit 128 registers
+    else
+      hipLaunchKernelGGL(k_backtransform, grid, dim3(BW * 64), 0, s, A, lda, strideA, n, nev, work, strideWork, Wv,
+                         strideW, Z, ldz, strideZ);
+#undef BT_
     DEREK_HIP(hipGetLastError());
   }
 }
