@@ -781,7 +781,7 @@ int32_t derek_hip_overlap(derek_ctx* c, int64_t npw, int32_t m, int32_t n, const
       convert<float, double>(fx, dx, (int64_t)m * npw, c->s);
       if (!herm) convert<float, double>(fy, dy, (int64_t)n * npw, c->s);
       c->timed(KC_OVERLAP, (double)(m + n) * npw * 8.0, 8.0 * npw * m * n, [&] {
-        overlap<float>(fx, npw, 0, fy, npw, 0, m, n, npw, nullptr, ds, m, 0, 1, w, wcap, herm, c->s);
+        overlap<float>(fx, npw, 0, fy, npw, 0, m, n, npw, nullptr, ds, m, 0, 1, w, wcap, herm, c->s, c->gemm_bf16);
       });
     }
     DEREK_HIP(hipMemcpyAsync(S, ds, (size_t)m * n * sizeof(double2), hipMemcpyDeviceToHost, c->s));
@@ -807,7 +807,7 @@ int32_t derek_hip_rotate(derek_ctx* c, int64_t npw, int32_t m, int32_t n, const 
       float2* fz = fx + (size_t)m * npw;
       convert<float, double>(fx, dx, (int64_t)m * npw, c->s);
       c->timed(KC_ROTATE, (double)(m + n) * npw * 8.0, 8.0 * npw * m * n,
-               [&] { rotate<float>(fx, npw, 0, du, m, 0, m, n, npw, nullptr, fz, npw, 0, 1, c->s); });
+               [&] { rotate<float>(fx, npw, 0, du, m, 0, m, n, npw, nullptr, fz, npw, 0, 1, c->s, c->gemm_bf16); });
       convert<double, float>(dz, fz, (int64_t)n * npw, c->s);
     }
     DEREK_HIP(hipMemcpyAsync(Z, dz, (size_t)n * npw * sizeof(double2), hipMemcpyDeviceToHost, c->s));
@@ -1051,6 +1051,14 @@ int32_t derek_hip_set_launch(derek_ctx* c, int32_t bands_per_wg, int32_t items_p
     c->bpw = bands_per_wg;
     c->item_cap = items_per_launch;
     c->fused = fused;
+  });
+}
+
+int32_t derek_hip_set_gemm(derek_ctx* c, int32_t bf16_terms) {
+  return guard([&] {
+    DEREK_REQUIRE(c, "null context");
+    DEREK_REQUIRE(bf16_terms >= 0 && bf16_terms <= 3, "bf16 terms must be 0 (FP32 matrix cores) or 1..3");
+    c->gemm_bf16 = bf16_terms;
   });
 }
 

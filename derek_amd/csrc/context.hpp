@@ -139,6 +139,9 @@ struct derek_ctx {
   // fused y/x passes on z-major sticks where the grid allows them: bit 0 for
   // H|psi> (kFusedHpsi), bit 1 for the density (kFusedDensity)
   int fused = 3;
+  // complex64 GEMMs (overlaps, rotations): 0 on the FP32 matrix cores,
+  // 1..3 on the BF16 ones with each value split into that many bf16 terms
+  int gemm_bf16 = 0;
   // pruned-FFT work buffers for item_cap items
   derek::DevBuf S64, P64, S32, P32;
   derek::DevBuf rho_part;  // partial densities of the fused density pass

@@ -5,12 +5,14 @@
 // the best of five HIP-event timings and the rate in TFLOP/s counted as
 // 8 m n npw per complex product (the four-real-product count; Hermitian
 // overlaps compute only the upper tiles, so their rate counts the full
-// product as well).
+// product as well).  complex64 runs on the FP32 matrix cores ("fp32") and
+// on the BF16 ones with 1, 2 and 3 bf16 terms per value ("bf16x1" ...).
 //
 //   derek_amd/gemm_bench          (built by make -C derek_amd/csrc)
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 
 #include "launch.hpp"
 
@@ -31,7 +33,7 @@ template <typename C> __global__ void k_fill(C* p, int64_t n) {
   }
 }
 
-template <typename R> void run(int64_t npw, int nb, int batch) {
+template <typename R> void run(int64_t npw, int nb, int batch, int terms = 0) {
   using C = cx_t<R>;
   const int64_t ld = ceil_div(npw, 64) * 64;
   const int m2 = 2 * nb;
@@ -63,23 +65,26 @@ template <typename R> void run(int64_t npw, int nb, int batch) {
       check(hipEventElapsedTime(&t, a, b), "hipEventElapsedTime");
       best = std::min(best, t);
     }
-    std::printf("%-5s npw %6lld b%-2d %-24s %9.3f ms %7.1f TF/s\n", sizeof(R) == 8 ? "fp64" : "fp32", (long long)npw,
-                batch, what, best, flops / best / 1e9);
+    char tag[16];
+    std::snprintf(tag, sizeof tag, "%s", sizeof(R) == 8 ? "fp64" : "fp32");
+    if (terms) std::snprintf(tag, sizeof tag, "bf16x%d", terms);
+    std::printf("%-6s npw %6lld b%-2d %-24s %9.3f ms %7.1f TF/s\n", tag, (long long)npw, batch, what, best,
+                flops / best / 1e9);
     std::fflush(stdout);
   };
   char w[64];
   for (int m : {nb, m2}) {
     std::snprintf(w, sizeof w, "overlap herm %dx%d", m, m);
     timeit(w, 8.0 * m * m * npw * batch, [&] {
-      overlap<R>(X, ld, ks, X, ld, ks, m, m, npw, nullptr, S, m, (int64_t)m * m, batch, work, cap, true, 0);
+      overlap<R>(X, ld, ks, X, ld, ks, m, m, npw, nullptr, S, m, (int64_t)m * m, batch, work, cap, true, 0, terms);
     });
     std::snprintf(w, sizeof w, "overlap full %dx%d", m, m);
     timeit(w, 8.0 * m * m * npw * batch, [&] {
-      overlap<R>(X, ld, ks, X, ld, ks, m, m, npw, nullptr, S, m, (int64_t)m * m, batch, work, cap, false, 0);
+      overlap<R>(X, ld, ks, X, ld, ks, m, m, npw, nullptr, S, m, (int64_t)m * m, batch, work, cap, false, 0, terms);
     });
     std::snprintf(w, sizeof w, "rotate %d->%d", m, nb);
     timeit(w, 8.0 * m * nb * npw * batch, [&] {
-      rotate<R>(X, ld, ks, U, m2, (int64_t)m2 * m2, m, nb, npw, nullptr, Z, ld, ks, batch, 0);
+      rotate<R>(X, ld, ks, U, m2, (int64_t)m2 * m2, m, nb, npw, nullptr, Z, ld, ks, batch, 0, terms);
     });
   }
   check(hipFree(X), "hipFree");
@@ -89,10 +94,12 @@ template <typename R> void run(int64_t npw, int nb, int batch) {
   check(hipFree(work), "hipFree");
 }
 
-int main() {
-  run<double>(161214, 300, 8);
-  run<float>(161214, 300, 8);
-  run<double>(62669, 80, 1);
-  run<float>(62669, 80, 1);
+int main(int argc, char** argv) {
+  // argument "bf16": the complex64 runs only, FP32 then BF16 with 1..3 terms
+  const bool bf_only = argc > 1 && std::string(argv[1]) == "bf16";
+  if (!bf_only) run<double>(161214, 300, 8);
+  for (int t = 0; t <= (bf_only ? 3 : 0); ++t) run<float>(161214, 300, 8, t);
+  if (!bf_only) run<double>(62669, 80, 1);
+  for (int t = 0; t <= (bf_only ? 3 : 0); ++t) run<float>(62669, 80, 1, t);
   return 0;
 }

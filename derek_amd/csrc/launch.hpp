@@ -59,15 +59,19 @@ void transpose_xz(const double* v, double* t64, float* t32, int nx, int ny, int 
 // C (m x n, column-major, ldc) = X^H Y; X: npw x m (ldx), Y: npw x n (ldy),
 // accumulated on MFMA in R, chunk partials summed in FP64.
 // `work` (work_cap double2, checked) must hold overlap_work_elems(m, n, npw, batch).
+// bf16_terms (complex64 data only): 0 multiplies on the FP32 matrix cores;
+// 1..3 on the BF16 ones with every FP32 value split into that many bf16
+// terms (1: plain BF16, 2: ~16 significant bits, 3: ~FP32), FP32 accumulation.
 int64_t overlap_work_elems(int m, int n, int64_t npw, int batch);
 template <typename R>
 void overlap(const cx_t<R>* X, int64_t ldx, int64_t sx, const cx_t<R>* Y, int64_t ldy, int64_t sy, int m, int n,
              int64_t npw, const int* npwk, double2* C, int ldc, int64_t sc, int batch, double2* work,
-             int64_t work_cap, bool hermitian, hipStream_t s);
+             int64_t work_cap, bool hermitian, hipStream_t s, int bf16_terms = 0);
 // Z (npw x n, ldz) = X (npw x m, ldx) * U (m x n, FP64, column-major, ldu),
-// U rounded to R.
+// U rounded to R; bf16_terms as for overlap.
 template <typename R>
 void rotate(const cx_t<R>* X, int64_t ldx, int64_t sx, const double2* U, int ldu, int64_t su, int m, int n,
-            int64_t npw, const int* npwk, cx_t<R>* Z, int64_t ldz, int64_t sz, int batch, hipStream_t s);
+            int64_t npw, const int* npwk, cx_t<R>* Z, int64_t ldz, int64_t sz, int batch, hipStream_t s,
+            int bf16_terms = 0);
 
 }  // namespace derek

@@ -99,6 +99,11 @@ template <typename R> void alloc_psi(derek_ctx* c, ScfEngine& E) {
   }
 }
 
+// bf16 terms of the complex64 GEMMs (complex128 data always uses FP64 MFMA)
+template <typename R> static int bf16_terms(const derek_ctx* c) {
+  return std::is_same<R, float>::value ? c->gemm_bf16 : 0;
+}
+
 // C[kz] (m x n, ld m, packed at m*n) = X^H Y for the batch k0..k0+nkc-1
 template <typename R>
 void ov(derek_ctx* c, const cx_t<R>* X, const cx_t<R>* Yp, int m, int n, int k0, int nkc, double2* C, bool herm) {
@@ -106,7 +111,7 @@ void ov(derek_ctx* c, const cx_t<R>* X, const cx_t<R>* Yp, int m, int n, int k0,
   const double npw = npw_sum(c, k0, nkc);
   c->timed(KC_OVERLAP, (double)(herm ? m : m + n) * npw * sizeof(cx_t<R>), 8.0 * npw * m * n, [&] {
     overlap<R>(X, E.ldk, E.kstride, Yp, E.ldk, E.kstride, m, n, c->max_npw, c->d_npw + k0, C, m, (int64_t)m * n, nkc,
-               E.ovw, E.ovw_cap, herm, c->s);
+               E.ovw, E.ovw_cap, herm, c->s, bf16_terms<R>(c));
   });
 }
 
@@ -130,7 +135,7 @@ void ov_block(derek_ctx* c, const cx_t<R>* X, int ca, const cx_t<R>* Yp, int cb,
   c->timed(KC_OVERLAP, (double)(herm ? nb : 2 * nb) * npw * sizeof(cx_t<R>), 8.0 * npw * nb * nb, [&] {
     overlap<R>(X + (int64_t)ca * E.ldk, E.ldk, E.kstride, Yp + (int64_t)cb * E.ldk, E.ldk, E.kstride, nb, nb,
                c->max_npw, c->d_npw + k0, C + ca + (int64_t)cb * m2, m2, (int64_t)m2 * m2, nkc, E.ovw, E.ovw_cap,
-               herm, c->s);
+               herm, c->s, bf16_terms<R>(c));
   });
 }
 
@@ -140,7 +145,8 @@ void rot(derek_ctx* c, const cx_t<R>* X, const double2* Uc, int m, int n, int64_
   ScfEngine& E = *c->scf;
   const double npw = npw_sum(c, k0, nkc);
   c->timed(KC_ROTATE, (double)(m + n) * npw * sizeof(cx_t<R>), 8.0 * npw * m * n, [&] {
-    rotate<R>(X, E.ldk, E.kstride, Uc, m, su, m, n, c->max_npw, c->d_npw + k0, Z, E.ldk, E.kstride, nkc, c->s);
+    rotate<R>(X, E.ldk, E.kstride, Uc, m, su, m, n, c->max_npw, c->d_npw + k0, Z, E.ldk, E.kstride, nkc, c->s,
+              bf16_terms<R>(c));
   });
 }
 

@@ -446,6 +446,13 @@ module derek_hip_m
        type(c_ptr), value :: ctx
        integer(c_int32_t), value :: bands_per_wg, items_per_launch, fused
      end function derek_hip_set_launch
+
+     ! complex64 overlaps/rotations: 0 FP32 MFMA, 1..3 BF16 MFMA with that many bf16 terms per value
+     integer(c_int32_t) function derek_hip_set_gemm(ctx, bf16_terms) bind(C, name='derek_hip_set_gemm')
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: bf16_terms
+     end function derek_hip_set_gemm
   end interface
 
   public :: derek_hip_version, derek_hip_last_error, derek_basis_grid_dims, derek_basis_inv_cell, &
@@ -458,7 +465,7 @@ module derek_hip_m
        derek_basis_std_index, derek_io_pot_write, derek_io_pot_read, derek_io_den_write, derek_io_den_read, &
        derek_io_wvfn_write, derek_io_wvfn_read, derek_hip_nscf, derek_info_read, derek_info_free, derek_info_num, &
        derek_info_str, derek_info_present, derek_info_cell, derek_info_wells, derek_info_log, derek_info_preset_pot, &
-       derek_info_help, derek_hip_get_tuning, derek_hip_set_launch
+       derek_info_help, derek_hip_get_tuning, derek_hip_set_launch, derek_hip_set_gemm
 
 contains
 

@@ -32,7 +32,7 @@ EXPORTS = (
     "derek_hip_get_occupations", "derek_hip_get_density", "derek_hip_get_potential", "derek_hip_get_psi", "derek_hip_put_psi",
     "derek_hip_apply_h_resident", "derek_hip_sync", "derek_hip_stream",
     "derek_hip_profile_enable", "derek_hip_profile_reset", "derek_hip_profile_read", "derek_hip_get_tuning",
-    "derek_hip_set_launch",
+    "derek_hip_set_launch", "derek_hip_set_gemm",
 )
 
 KERNEL_CLASSES = ("sticks_z_inv", "planes_y_inv", "planes_x_vmul", "planes_y_fwd", "sticks_z_fwd", "planes_x_density",
@@ -494,6 +494,11 @@ class Context:
         fd = fused if fused_density is None else fused_density
         check(self.lib.derek_hip_set_launch(self.h, bands_per_wg, items_per_launch,
                                             (1 if fused else 0) | (2 if fd else 0)))
+
+    def set_gemm(self, bf16_terms=0):
+        """Matrix cores of the complex64 overlaps and rotations: 0 FP32 MFMA;
+        1..3 BF16 MFMA with each FP32 value split into that many bf16 terms."""
+        check(self.lib.derek_hip_set_gemm(self.h, int(bf16_terms)))
 
     def scf_step(self):
         r = ScfReport()

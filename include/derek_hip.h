@@ -309,6 +309,14 @@ int32_t derek_hip_get_tuning(derek_ctx* ctx, int32_t out[5]);
  * passes run.  No counterpart in the reference (launch-shape control of the
  * GPU pipeline). */
 int32_t derek_hip_set_launch(derek_ctx* ctx, int32_t bands_per_wg, int32_t items_per_launch, int32_t fused);
+/* Matrix cores for the complex64 overlaps and rotations of a DEREK_MIXED
+ * context (derek_hip_overlap, derek_hip_rotate and the FP32 phase of the
+ * SCF): 0 (default) multiplies on the FP32 MFMAs; 1..3 on the BF16 MFMAs
+ * with FP32 accumulation, each FP32 operand split into that many bf16 terms
+ * (1: plain BF16, 8 significant bits; 2: three products per real product,
+ * about 16 bits; 3: six products, about FP32's 24).  FP64 data is unaffected.
+ * No counterpart in the reference (precision control of the GPU GEMMs). */
+int32_t derek_hip_set_gemm(derek_ctx* ctx, int32_t bf16_terms);
 /* Block until all work on the context's stream is done. */
 int32_t derek_hip_sync(derek_ctx* ctx);
 /* Host-visible stream handle (hipStream_t) for event timing. */

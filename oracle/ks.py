@@ -14,6 +14,14 @@
                   Gaussian charge (closed forms).
 * ``overlap`` / ``rotate`` - Psi^H Phi and Psi U (``wave_dot_*`` stubs,
                   Source/wave.f90:929-1036); pinned against explicit loops.
+* ``overlap_bf16`` / ``rotate_bf16`` - the same products with complex64
+                  operands on BF16 matrix cores: each FP32 value split into
+                  nt bf16 terms (round to nearest even), the term products
+                  with a + b < nt kept, three real products per complex one.
+                  No reference counterpart (a GPU precision option); pinned
+                  by bf16 rounding known answers, torch's bfloat16 cast, and
+                  the nt = 3 product sitting within FP32 error of the exact
+                  product while nt = 1 does not.
 * ``gaussian_wells`` - the benchmark's external potential at the reference's
                   ``real_fine_grid_points`` (basis.f90:175-188), wells as in
                   the ``$DATA gaussian_pot`` block (io.f90:1888-1922).
@@ -110,6 +118,64 @@ def overlap(X, Y):
 def rotate(X, U):
     """Y_j = sum_i U[i,j] X_i."""
     return U.T @ X
+
+
+def bf16_round(x):
+    """float32 -> nearest bfloat16 (ties to even), returned as float32."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def bf16_split(x, nt):
+    """x (float32) = t_0 + t_1 + ... + t_{nt-1} + remainder, t_j bfloat16."""
+    r = np.asarray(x, dtype=np.float32).copy()
+    terms = []
+    for _ in range(nt):
+        t = bf16_round(r)
+        terms.append(t)
+        r = (r - t).astype(np.float32)
+    return terms
+
+
+def _split_product(A, B, nt):
+    """sum over a + b < nt of A_a^T-style products, in float64:
+    A (p, K) and B (K, q) float32 -> (p, q)."""
+    As = bf16_split(A, nt)
+    Bs = bf16_split(B, nt)
+    out = 0.0
+    for s_ in range(nt):
+        for a in range(s_ + 1):
+            out = out + As[a].astype(np.float64) @ Bs[s_ - a].astype(np.float64)
+    return out
+
+
+def overlap_bf16(X, Y, nt):
+    """conj(X) Y^T with complex64 X (m, npw), Y (n, npw): planes
+    xr, xi, xr - xi and yr, yi, yr + yi (float32), each product split into
+    nt bf16 terms; Re = P1 + P2, Im = P3 - P1 + P2."""
+    X = np.asarray(X, dtype=np.complex64)
+    Y = np.asarray(Y, dtype=np.complex64)
+    xr, xi = X.real.astype(np.float32), X.imag.astype(np.float32)
+    yr, yi = Y.real.astype(np.float32), Y.imag.astype(np.float32)
+    p1 = _split_product(xr, yr.T, nt)
+    p2 = _split_product(xi, yi.T, nt)
+    p3 = _split_product((xr - xi).astype(np.float32), (yr + yi).astype(np.float32).T, nt)
+    return (p1 + p2) + 1j * (p3 - p1 + p2)
+
+
+def rotate_bf16(X, U, nt):
+    """U^T X with complex64 X (m, npw) and U (m, n) rounded to complex64:
+    planes xr, xi, xr + xi and ur, ui, ur + ui, nt bf16 terms;
+    Re = P1 - P2, Im = P3 - P1 - P2."""
+    X = np.asarray(X, dtype=np.complex64)
+    U = np.asarray(U, dtype=np.complex64)
+    xr, xi = X.real.astype(np.float32), X.imag.astype(np.float32)
+    ur, ui = U.real.astype(np.float32), U.imag.astype(np.float32)
+    p1 = _split_product(ur.T, xr, nt)
+    p2 = _split_product(ui.T, xi, nt)
+    p3 = _split_product((ur + ui).astype(np.float32).T, (xr + xi).astype(np.float32), nt)
+    return (p1 - p2) + 1j * (p3 - p1 - p2)
 
 
 EKIN_FLOOR = 0.1

@@ -154,6 +154,65 @@ def test_rotate(m, n, npw, precision):
     ctx.close()
 
 
+BF16_ACC = {1: 2e-2, 2: 1e-4, 3: 2e-6}  # against the exact product of the complex64 operands
+
+
+@pytest.mark.parametrize("m,n,npw", [(10, 10, 1503), (37, 53, 777), (160, 160, 5003), (80, 80, 62669)])
+@pytest.mark.parametrize("nt", [1, 2, 3])
+def test_overlap_bf16(m, n, npw, nt):
+    """BF16 MFMA overlaps of complex64 operands split into nt bf16 terms:
+    against the oracle's split product (FP32 accumulation: 1e-5 of the
+    largest entry), against the exact product to the split's accuracy, and
+    exactly Hermitian for X^H X."""
+    print("overlap_bf16", m, n, npw, nt, flush=True)
+    rng = np.random.default_rng(m * 1000 + n + nt)
+    ctx = lib.Context((8, 8, 8), 1, 1, 1.0, 1, lib.DEREK_MIXED)
+    ctx.set_gemm(nt)
+    X = rnd_c(rng, m, npw).astype(np.complex64)
+    Y = rnd_c(rng, n, npw).astype(np.complex64)
+    got = ctx.overlap(X, Y)
+    assert_close(got, oks.overlap_bf16(X, Y, nt), 1e-5)
+    assert_close(got, np.conj(X.astype(np.complex128)) @ Y.astype(np.complex128).T, BF16_ACC[nt])
+    if m == n:
+        # the Hermitian kernel multiplies the upper triangle and mirrors it
+        # (real diagonal); the split product of conj(x_j) x_i is not the
+        # conjugate of that of conj(x_i) x_j to the last bf16 bit
+        S = ctx.overlap(X, X)
+        R = oks.overlap_bf16(X, X, nt)
+        R = np.triu(R, 1) + np.triu(R, 1).conj().T + np.diag(np.diag(R).real)
+        assert_close(S, R, 1e-5)
+        assert np.array_equal(S, S.conj().T)
+    ctx.set_gemm(0)
+    Xr = X.astype(np.complex128)
+    assert_close(ctx.overlap(X, Y), np.conj(Xr) @ Y.astype(np.complex128).T, 1e-5)
+    ctx.close()
+
+
+@pytest.mark.parametrize("m,n,npw", [(10, 10, 1503), (37, 53, 777), (160, 80, 4003), (600, 300, 2048)])
+@pytest.mark.parametrize("nt", [1, 2, 3])
+def test_rotate_bf16(m, n, npw, nt):
+    """BF16 MFMA rotation Z = X U (U rounded to complex64, both split into nt
+    bf16 terms) against the oracle's split product and the exact product."""
+    print("rotate_bf16", m, n, npw, nt, flush=True)
+    rng = np.random.default_rng(m + n + nt)
+    ctx = lib.Context((8, 8, 8), 1, 1, 1.0, 1, lib.DEREK_MIXED)
+    ctx.set_gemm(nt)
+    X = rnd_c(rng, m, npw).astype(np.complex64)
+    U = rnd_c(rng, m, n).astype(np.complex64)
+    got = ctx.rotate(X, U)
+    assert_close(got, oks.rotate_bf16(X, U, nt), 1e-5)
+    assert_close(got, U.astype(np.complex128).T @ X.astype(np.complex128), BF16_ACC[nt])
+    ctx.close()
+
+
+def test_set_gemm_errors():
+    ctx = lib.Context((8, 8, 8), 1, 1, 1.0, 1, lib.DEREK_MIXED)
+    for bad in (-1, 4):
+        with pytest.raises(lib.DerekError):
+            ctx.set_gemm(bad)
+    ctx.close()
+
+
 @pytest.mark.parametrize("precision", [lib.DEREK_FP64, lib.DEREK_MIXED])
 def test_density(precision):
     spec = config.benchmark(1)

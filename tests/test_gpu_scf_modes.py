@@ -64,14 +64,18 @@ def test_autotune_keeps_results():
     assert np.max(np.abs(e1 - e0)) < 1e-10
 
 
-@pytest.mark.parametrize("name", ["cfg1", "kpoints"])
-def test_mixed_precision_reaches_fp64_oracle(name):
-    print("mixed", name, flush=True)
+@pytest.mark.parametrize("name,bf16", [("cfg1", 0), ("kpoints", 0), ("cfg1", 1), ("kpoints", 1), ("cfg1", 2),
+                                       ("kpoints", 3)])
+def test_mixed_precision_reaches_fp64_oracle(name, bf16):
+    """bf16 > 0: the FP32 phase's overlaps and rotations on the BF16 matrix
+    cores (bf16 terms per value); the FP64 refinement lands on the same
+    oracle values."""
+    print("mixed", name, "bf16 terms", bf16, flush=True)
     spec = config.benchmark(1) if name == "cfg1" else kspec()
     su = gscf.Setup(spec)
     ctx = su.context(lib.DEREK_MIXED)
     res = gscf.run(spec, max_scf=80, energy_tol=1e-11, rho_tol=1e-8, setup=su, ctx=ctx, log=print, kbatch=0,
-                   graph=True)
+                   graph=True, gemm_bf16=bf16)
     rmax = ctx.scf_polish(1000, 1e-10)
     eg = ctx.eigenvalues()
     ctx.close()

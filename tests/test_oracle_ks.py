@@ -98,3 +98,42 @@ def test_gaussian_wells_periodic_and_centred():
     v2 = ks.gaussian_wells(R, [8.0] * 3, [[0.0, 0.0, 0.0]], [-1.0], [1.5])
     a = v2.reshape(16, 16, 16)
     assert np.allclose(a[0, 15, 15], a[14, 15, 15])  # x = 0.5 and x = 7.5 are mirror images about 8
+
+
+def test_bf16_round_known_answers_and_torch():
+    """Round to nearest even at bfloat16's 8 significant bits: ties
+    1 + 2^-8 -> 1 and 1 + 3*2^-8 -> 1 + 2^-6; agrees with torch's cast."""
+    import torch
+    x = np.array([1.0, 1.0 + 2.0**-8, 1.0 + 3 * 2.0**-8, 1.0 + 2.0**-8 + 2.0**-20, -2.5, 3.0e-30, 0.0],
+                 dtype=np.float32)
+    want = np.array([1.0, 1.0, 1.0 + 2.0**-6, 1.0 + 2.0**-7, -2.5, np.nan, 0.0], dtype=np.float32)
+    got = ks.bf16_round(x)
+    assert np.array_equal(got[:5], want[:5]) and got[6] == 0.0
+    rng = np.random.default_rng(11)
+    r = (rng.normal(size=20000) * 10.0 ** rng.uniform(-6, 6, 20000)).astype(np.float32)
+    t = torch.from_numpy(r).to(torch.bfloat16).to(torch.float32).numpy()
+    assert np.array_equal(ks.bf16_round(r), t)
+    assert np.array_equal(ks.bf16_round(x[5:6]), torch.from_numpy(x[5:6]).to(torch.bfloat16).float().numpy())
+
+
+def test_bf16_split_products_accuracy():
+    """nt = 3 terms reproduce complex64 products to FP32-level error; nt = 2
+    to ~2^-16; nt = 1 (plain bf16) is ~2^-8 off.  Both products and both
+    Gauss sign conventions (conj(x) y and x u) are covered."""
+    rng = np.random.default_rng(4)
+    X = (rng.normal(size=(12, 900)) + 1j * rng.normal(size=(12, 900))).astype(np.complex64)
+    Y = (rng.normal(size=(9, 900)) + 1j * rng.normal(size=(9, 900))).astype(np.complex64)
+    U = (rng.normal(size=(12, 7)) + 1j * rng.normal(size=(12, 7))).astype(np.complex64)
+    S = np.conj(X.astype(np.complex128)) @ Y.astype(np.complex128).T
+    Z = U.astype(np.complex128).T @ X.astype(np.complex128)
+    # the term split reconstructs FP32 values exactly with 3 terms
+    v = X.real.ravel()
+    assert np.array_equal(sum(t.astype(np.float64) for t in ks.bf16_split(v, 3)), v.astype(np.float64))
+    errs = {}
+    for nt in (1, 2, 3):
+        eo = np.max(np.abs(ks.overlap_bf16(X, Y, nt) - S)) / np.max(np.abs(S))
+        er = np.max(np.abs(ks.rotate_bf16(X, U, nt) - Z)) / np.max(np.abs(Z))
+        errs[nt] = (eo, er)
+    assert all(e > 1e-4 for e in errs[1]) and all(e < 2e-2 for e in errs[1])
+    assert all(1e-9 < e < 1e-4 for e in errs[2])
+    assert all(e < 1e-6 for e in errs[3])
