@@ -16,13 +16,18 @@ timed region), the time to SCF convergence (|dE| < 1e-6 Ha) of
     bands, kT = 0.01 Ha) in plain FP64 (one k-point per launch, no graph),
     FP64 batched (all k-points per launch, graph replay) and mixed + batched
     + autotuned, with the speed-up of the last over the first;
+  * the mixed runs put their complex64 overlaps and rotations on the BF16
+    matrix cores with ``--bf16 N`` bf16 terms per value (default 2, the
+    library's default; FP32 accumulation), and each is repeated with them on
+    the FP32 matrix cores (``*_fp32_mfma``) for comparison (``--bf16 0``: the
+    FP32 ones only, no repeat);
 
 and a CPU baseline (the numpy oracle's H|psi> on a bounded sample of the
 headline system, timed on this host on one core and on every core of the
 host's share, before the GPU is touched).
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-scf] [--scf cfg2,cfg4] [--cpu-seconds S] [--no-cpu]
-                [--fixed-shapes | --shapes BPW,ITEMS,FUSED]
+                [--fixed-shapes | --shapes BPW,ITEMS,FUSED] [--bf16 N]
 """
 import argparse
 import json
@@ -141,6 +146,8 @@ def main():
     ap.add_argument("--no-scf", action="store_true")
     ap.add_argument("--scf", default="cfg2,cfg4")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--bf16", type=int, default=2,
+                    help="bf16 terms per value of the mixed runs' GEMMs (0: FP32 matrix cores, no repeat)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
                     help="run the batched SCF modes eagerly (no hipGraph replay), e.g. under a kernel tracer")
@@ -242,13 +249,21 @@ def main():
     if "cfg2" in todo:
         r64, e64 = scf_run(gscf, lib, spec, su, lib.DEREK_FP64, kbatch=1)
         log(f"# cfg2 FP64: {r64['steps']} steps, {r64['time_to_scf_s']:.3f} s, E = {r64['etot_ha']:.10f}")
-        rmx, emx = scf_run(gscf, lib, spec, su, lib.DEREK_MIXED, graph=graph)
+        rmx, emx = scf_run(gscf, lib, spec, su, lib.DEREK_MIXED, graph=graph, gemm_bf16=args.bf16)
         log(f"# cfg2 mixed: {rmx['steps']} steps ({rmx['fp32_steps']} FP32), {rmx['time_to_scf_s']:.3f} s, "
             f"E = {rmx['etot_ha']:.10f}")
         scf_info["cfg2_fp64"] = r64
-        scf_info["cfg2_mixed"] = dict(rmx, dE_vs_fp64_ha=abs(rmx["etot_ha"] - r64["etot_ha"]),
+        scf_info["cfg2_mixed"] = dict(rmx, bf16_terms=args.bf16, dE_vs_fp64_ha=abs(rmx["etot_ha"] - r64["etot_ha"]),
                                       max_deig_vs_fp64_ha=float(np.max(np.abs(emx - e64))),
                                       speedup_vs_fp64=round(r64["time_to_scf_s"] / rmx["time_to_scf_s"], 3))
+        if args.bf16:
+            rbf, ebf = scf_run(gscf, lib, spec, su, lib.DEREK_MIXED, graph=graph, gemm_bf16=0)
+            log(f"# cfg2 mixed, FP32 MFMA GEMMs: {rbf['steps']} steps ({rbf['fp32_steps']} FP32), "
+                f"{rbf['time_to_scf_s']:.3f} s")
+            scf_info["cfg2_mixed_fp32_mfma"] = dict(rbf, bf16_terms=0,
+                                                    dE_vs_fp64_ha=abs(rbf["etot_ha"] - r64["etot_ha"]),
+                                               max_deig_vs_fp64_ha=float(np.max(np.abs(ebf - e64))),
+                                               speedup_vs_fp64=round(r64["time_to_scf_s"] / rbf["time_to_scf_s"], 3))
     if "cfg4" in todo:
         spec4 = config.benchmark(4)
         su4 = gscf.Setup(spec4)
@@ -257,14 +272,25 @@ def main():
         log(f"# cfg4 FP64 plain: {p64['steps']} steps, {p64['time_to_scf_s']:.3f} s, E = {p64['etot_ha']:.10f}")
         b64, e_b = scf_run(gscf, lib, spec4, su4, lib.DEREK_FP64, kbatch=0, graph=graph)
         log(f"# cfg4 FP64 batched: {b64['steps']} steps, {b64['time_to_scf_s']:.3f} s")
-        bmx, e_m = scf_run(gscf, lib, spec4, su4, lib.DEREK_MIXED, kbatch=0, graph=graph, autotune=True)
+        bmx, e_m = scf_run(gscf, lib, spec4, su4, lib.DEREK_MIXED, kbatch=0, graph=graph, autotune=True,
+                           gemm_bf16=args.bf16)
         log(f"# cfg4 mixed batched: {bmx['steps']} steps ({bmx['fp32_steps']} FP32), {bmx['time_to_scf_s']:.3f} s")
         scf_info["cfg4_fp64_plain"] = p64
         scf_info["cfg4_fp64_batched"] = dict(b64, dE_vs_plain_ha=abs(b64["etot_ha"] - p64["etot_ha"]),
                                              speedup_vs_plain=round(p64["time_to_scf_s"] / b64["time_to_scf_s"], 3))
-        scf_info["cfg4_mixed_batched"] = dict(bmx, dE_vs_plain_ha=abs(bmx["etot_ha"] - p64["etot_ha"]),
+        scf_info["cfg4_mixed_batched"] = dict(bmx, bf16_terms=args.bf16,
+                                              dE_vs_plain_ha=abs(bmx["etot_ha"] - p64["etot_ha"]),
                                               max_deig_vs_plain_ha=float(np.max(np.abs(e_m - e_p))),
                                               speedup_vs_plain=round(p64["time_to_scf_s"] / bmx["time_to_scf_s"], 3))
+        if args.bf16:
+            bbf, e_bf = scf_run(gscf, lib, spec4, su4, lib.DEREK_MIXED, kbatch=0, graph=graph, autotune=True,
+                                gemm_bf16=0)
+            log(f"# cfg4 mixed batched, FP32 MFMA GEMMs: {bbf['steps']} steps ({bbf['fp32_steps']} FP32), "
+                f"{bbf['time_to_scf_s']:.3f} s")
+            scf_info["cfg4_mixed_batched_fp32_mfma"] = dict(
+                bbf, bf16_terms=0, dE_vs_plain_ha=abs(bbf["etot_ha"] - p64["etot_ha"]),
+                max_deig_vs_plain_ha=float(np.max(np.abs(e_bf - e_p))),
+                speedup_vs_plain=round(p64["time_to_scf_s"] / bbf["time_to_scf_s"], 3))
 
     if "cfg5" in todo:
         spec5 = config.benchmark(5)
@@ -273,12 +299,22 @@ def main():
         f64, e_f = scf_run(gscf, lib, spec5, su5, lib.DEREK_FP64, verbose=True, kbatch=0, graph=graph)
         log(f"# cfg5 FP64 batched: {f64['steps']} steps, {f64['time_to_scf_s']:.3f} s, E = {f64['etot_ha']:.10f}")
         mx5, e_x = scf_run(gscf, lib, spec5, su5, lib.DEREK_MIXED, verbose=True, kbatch=0, graph=graph,
-                           autotune=True)
+                           autotune=True, gemm_bf16=args.bf16)
         log(f"# cfg5 mixed batched: {mx5['steps']} steps ({mx5['fp32_steps']} FP32), {mx5['time_to_scf_s']:.3f} s")
         scf_info["cfg5_fp64_batched"] = f64
-        scf_info["cfg5_mixed_batched"] = dict(mx5, dE_vs_fp64_ha=abs(mx5["etot_ha"] - f64["etot_ha"]),
+        scf_info["cfg5_mixed_batched"] = dict(mx5, bf16_terms=args.bf16,
+                                              dE_vs_fp64_ha=abs(mx5["etot_ha"] - f64["etot_ha"]),
                                               max_deig_vs_fp64_ha=float(np.max(np.abs(e_x - e_f))),
                                               speedup_vs_fp64=round(f64["time_to_scf_s"] / mx5["time_to_scf_s"], 3))
+        if args.bf16:
+            bx5, e_bx = scf_run(gscf, lib, spec5, su5, lib.DEREK_MIXED, verbose=True, kbatch=0, graph=graph,
+                                autotune=True, gemm_bf16=0)
+            log(f"# cfg5 mixed batched, FP32 MFMA GEMMs: {bx5['steps']} steps ({bx5['fp32_steps']} FP32), "
+                f"{bx5['time_to_scf_s']:.3f} s")
+            scf_info["cfg5_mixed_batched_fp32_mfma"] = dict(
+                bx5, bf16_terms=0, dE_vs_fp64_ha=abs(bx5["etot_ha"] - f64["etot_ha"]),
+                max_deig_vs_fp64_ha=float(np.max(np.abs(e_bx - e_f))),
+                speedup_vs_fp64=round(f64["time_to_scf_s"] / bx5["time_to_scf_s"], 3))
 
     out = {
         "metric": METRIC, "value": round(bands_per_s, 2), "unit": "H|psi> bands/s", "n_gpus": 1,

@@ -591,6 +591,9 @@ int32_t derek_hip_init(derek_ctx** out, const derek_dims* d) {
     c->nk = d->nk;
     c->nb = d->nbands;
     c->prec = d->precision;
+    // complex64 GEMMs: BF16 matrix cores with two bf16 terms per value
+    // (about 16 significant bits) by default; derek_hip_set_gemm changes it
+    c->gemm_bf16 = c->prec == DEREK_MIXED ? 2 : 0;
     c->volume = d->volume;
     c->max_npw = d->max_npw;
     DEREK_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));

@@ -141,6 +141,7 @@ struct derek_ctx {
   int fused = 3;
   // complex64 GEMMs (overlaps, rotations): 0 on the FP32 matrix cores,
   // 1..3 on the BF16 ones with each value split into that many bf16 terms
+  // (set to 2 at init for DEREK_MIXED contexts)
   int gemm_bf16 = 0;
   // pruned-FFT work buffers for item_cap items
   derek::DevBuf S64, P64, S32, P32;

@@ -495,9 +495,10 @@ class Context:
         check(self.lib.derek_hip_set_launch(self.h, bands_per_wg, items_per_launch,
                                             (1 if fused else 0) | (2 if fd else 0)))
 
-    def set_gemm(self, bf16_terms=0):
+    def set_gemm(self, bf16_terms=2):
         """Matrix cores of the complex64 overlaps and rotations: 0 FP32 MFMA;
-        1..3 BF16 MFMA with each FP32 value split into that many bf16 terms."""
+        1..3 BF16 MFMA with each FP32 value split into that many bf16 terms
+        (a DEREK_MIXED context starts at 2)."""
         check(self.lib.derek_hip_set_gemm(self.h, int(bf16_terms)))
 
     def scf_step(self):

@@ -53,18 +53,20 @@ class Setup:
 
 def run(spec, max_scf=40, energy_tol=1e-7, rho_tol=None, precision=lib.DEREK_FP64, log=print,
         n_david_first=4, n_david=2, beta=0.5, q0=1.0, history=8, setup=None, ctx=None, kbatch=0, graph=False,
-        autotune=False, switch_tol=0.0, refine_steps=0, gemm_bf16=0):
+        autotune=False, switch_tol=0.0, refine_steps=0, gemm_bf16=None):
     """Run the SCF; returns a dict with the final energy, eigenvalues,
     per-step reports, wall time (init included in ``wall_init``) and the
     converged flag.  In DEREK_MIXED mode a step may only end the run once
     its report's ``refine_left`` is 0 (the FP64 refinement is done).
     ``gemm_bf16`` (1..3) runs the complex64 overlaps and rotations on the
-    BF16 matrix cores with that many bf16 terms per value (0: FP32 MFMA)."""
+    BF16 matrix cores with that many bf16 terms per value, 0 on the FP32 ones
+    (None: the context's setting, 2 for DEREK_MIXED)."""
     su = setup if setup is not None else Setup(spec)
     own = ctx is None
     if ctx is None:
         ctx = su.context(precision)
-    ctx.set_gemm(gemm_bf16)
+    if gemm_bf16 is not None:
+        ctx.set_gemm(gemm_bf16)
     ti = time.perf_counter()
     ctx.scf_init(spec.nelec, kT=spec.kT, beta=beta, q0=q0, n_david_first=n_david_first, n_david=n_david,
                  history=history, seed=spec.seed, kbatch=kbatch, graph=graph, autotune=autotune,

@@ -33,9 +33,10 @@ extern "C" {
 #define DEREK_ERR_IO 5      /* file cannot be opened / read / written, or malformed record */
 
 #define DEREK_FP64 0  /* complex128 wavefunctions, FP64 everywhere */
-#define DEREK_MIXED 1 /* complex64 wavefunctions, FFTs, H|psi> and density; FP32 MFMA overlaps and
-                        rotations; FP64 subspace solves, reductions and energies; the last SCF
-                        steps refined in FP64 from the FP32-converged state */
+#define DEREK_MIXED 1 /* complex64 wavefunctions, FFTs, H|psi> and density; BF16-in / FP32-accumulate
+                        MFMA overlaps and rotations (derek_hip_set_gemm); FP64 subspace solves,
+                        reductions and energies; the last SCF steps refined in FP64 from the
+                        FP32-converged state */
 
 typedef struct derek_ctx derek_ctx;
 
@@ -311,10 +312,11 @@ int32_t derek_hip_get_tuning(derek_ctx* ctx, int32_t out[5]);
 int32_t derek_hip_set_launch(derek_ctx* ctx, int32_t bands_per_wg, int32_t items_per_launch, int32_t fused);
 /* Matrix cores for the complex64 overlaps and rotations of a DEREK_MIXED
  * context (derek_hip_overlap, derek_hip_rotate and the FP32 phase of the
- * SCF): 0 (default) multiplies on the FP32 MFMAs; 1..3 on the BF16 MFMAs
- * with FP32 accumulation, each FP32 operand split into that many bf16 terms
- * (1: plain BF16, 8 significant bits; 2: three products per real product,
- * about 16 bits; 3: six products, about FP32's 24).  FP64 data is unaffected.
+ * SCF): 0 multiplies on the FP32 MFMAs; 1..3 on the BF16 MFMAs with FP32
+ * accumulation, each FP32 operand split into that many bf16 terms (1: plain
+ * BF16, 8 significant bits; 2 (the default of a DEREK_MIXED context): three
+ * products per real product, about 16 bits; 3: six products, about FP32's
+ * 24).  FP64 data is unaffected.
  * No counterpart in the reference (precision control of the GPU GEMMs). */
 int32_t derek_hip_set_gemm(derek_ctx* ctx, int32_t bf16_terms);
 /* Block until all work on the context's stream is done. */

@@ -117,12 +117,14 @@ def test_apply_h_hermitian_full_size():
 @pytest.mark.parametrize("m,n,npw", [(10, 10, 1503), (37, 53, 777), (160, 160, 5003), (80, 80, 62669)])
 @pytest.mark.parametrize("precision", [lib.DEREK_FP64, lib.DEREK_MIXED])
 def test_overlap(m, n, npw, precision):
-    """FP64: f64 MFMA against numpy to 1e-13.  MIXED: operands rounded to
-    complex64, f32 MFMA accumulation; checked against the FP64 product of the
-    rounded operands (tolerance 1e-5 of the largest entry)."""
+    """FP64: f64 MFMA against numpy to 1e-13.  MIXED with FP32 matrix cores:
+    operands rounded to complex64, f32 MFMA accumulation; checked against the
+    FP64 product of the rounded operands (tolerance 1e-5 of the largest
+    entry)."""
     print("overlap", m, n, npw, precision, flush=True)
     rng = np.random.default_rng(m * 1000 + n)
     ctx = lib.Context((8, 8, 8), 1, 1, 1.0, 1, precision)
+    ctx.set_gemm(0)
     X = rnd_c(rng, m, npw)
     Y = rnd_c(rng, n, npw)
     tol = 1e-13 if precision == lib.DEREK_FP64 else 1e-5
@@ -139,10 +141,12 @@ def test_overlap(m, n, npw, precision):
 @pytest.mark.parametrize("m,n,npw", [(10, 10, 1503), (37, 53, 777), (160, 80, 4003)])
 @pytest.mark.parametrize("precision", [lib.DEREK_FP64, lib.DEREK_MIXED])
 def test_rotate(m, n, npw, precision):
-    """MIXED: X and U rounded to complex64, f32 MFMA accumulation."""
+    """MIXED with FP32 matrix cores: X and U rounded to complex64, f32 MFMA
+    accumulation."""
     print("rotate", m, n, npw, precision, flush=True)
     rng = np.random.default_rng(m + n)
     ctx = lib.Context((8, 8, 8), 1, 1, 1.0, 1, precision)
+    ctx.set_gemm(0)
     X = rnd_c(rng, m, npw)
     U = rnd_c(rng, m, n)
     if precision == lib.DEREK_FP64:
@@ -206,7 +210,13 @@ def test_rotate_bf16(m, n, npw, nt):
 
 
 def test_set_gemm_errors():
+    """Out-of-range term counts are refused; a DEREK_MIXED context starts on
+    the two-term BF16 GEMMs (its overlap equals the oracle's nt = 2 split)."""
     ctx = lib.Context((8, 8, 8), 1, 1, 1.0, 1, lib.DEREK_MIXED)
+    rng = np.random.default_rng(2)
+    X = rnd_c(rng, 20, 900).astype(np.complex64)
+    Y = rnd_c(rng, 12, 900).astype(np.complex64)
+    assert_close(ctx.overlap(X, Y), oks.overlap_bf16(X, Y, 2), 1e-5)
     for bad in (-1, 4):
         with pytest.raises(lib.DerekError):
             ctx.set_gemm(bad)

@@ -65,11 +65,11 @@ def test_autotune_keeps_results():
 
 
 @pytest.mark.parametrize("name,bf16", [("cfg1", 0), ("kpoints", 0), ("cfg1", 1), ("kpoints", 1), ("cfg1", 2),
-                                       ("kpoints", 3)])
+                                       ("kpoints", 2), ("kpoints", 3)])
 def test_mixed_precision_reaches_fp64_oracle(name, bf16):
     """bf16 > 0: the FP32 phase's overlaps and rotations on the BF16 matrix
-    cores (bf16 terms per value); the FP64 refinement lands on the same
-    oracle values."""
+    cores (bf16 terms per value; 2 is the default), 0 on the FP32 ones; the
+    FP64 refinement lands on the same oracle values."""
     print("mixed", name, "bf16 terms", bf16, flush=True)
     spec = config.benchmark(1) if name == "cfg1" else kspec()
     su = gscf.Setup(spec)
