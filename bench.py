@@ -146,6 +146,8 @@ def main():
     ap.add_argument("--no-scf", action="store_true")
     ap.add_argument("--scf", default="cfg2,cfg4")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--refine", type=int, default=0,
+                    help="FP64 refinement steps of the mixed runs (0: the library default)")
     ap.add_argument("--bf16", type=int, default=2,
                     help="bf16 terms per value of the mixed runs' GEMMs (0: FP32 matrix cores, no repeat)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -249,7 +251,8 @@ def main():
     if "cfg2" in todo:
         r64, e64 = scf_run(gscf, lib, spec, su, lib.DEREK_FP64, kbatch=1)
         log(f"# cfg2 FP64: {r64['steps']} steps, {r64['time_to_scf_s']:.3f} s, E = {r64['etot_ha']:.10f}")
-        rmx, emx = scf_run(gscf, lib, spec, su, lib.DEREK_MIXED, graph=graph, gemm_bf16=args.bf16)
+        rmx, emx = scf_run(gscf, lib, spec, su, lib.DEREK_MIXED, graph=graph, refine_steps=args.refine,
+                           gemm_bf16=args.bf16)
         log(f"# cfg2 mixed: {rmx['steps']} steps ({rmx['fp32_steps']} FP32), {rmx['time_to_scf_s']:.3f} s, "
             f"E = {rmx['etot_ha']:.10f}")
         scf_info["cfg2_fp64"] = r64
@@ -257,7 +260,8 @@ def main():
                                       max_deig_vs_fp64_ha=float(np.max(np.abs(emx - e64))),
                                       speedup_vs_fp64=round(r64["time_to_scf_s"] / rmx["time_to_scf_s"], 3))
         if args.bf16:
-            rbf, ebf = scf_run(gscf, lib, spec, su, lib.DEREK_MIXED, graph=graph, gemm_bf16=0)
+            rbf, ebf = scf_run(gscf, lib, spec, su, lib.DEREK_MIXED, graph=graph, refine_steps=args.refine,
+                               gemm_bf16=0)
             log(f"# cfg2 mixed, FP32 MFMA GEMMs: {rbf['steps']} steps ({rbf['fp32_steps']} FP32), "
                 f"{rbf['time_to_scf_s']:.3f} s")
             scf_info["cfg2_mixed_fp32_mfma"] = dict(rbf, bf16_terms=0,
@@ -273,7 +277,7 @@ def main():
         b64, e_b = scf_run(gscf, lib, spec4, su4, lib.DEREK_FP64, kbatch=0, graph=graph)
         log(f"# cfg4 FP64 batched: {b64['steps']} steps, {b64['time_to_scf_s']:.3f} s")
         bmx, e_m = scf_run(gscf, lib, spec4, su4, lib.DEREK_MIXED, kbatch=0, graph=graph, autotune=True,
-                           gemm_bf16=args.bf16)
+                           refine_steps=args.refine, gemm_bf16=args.bf16)
         log(f"# cfg4 mixed batched: {bmx['steps']} steps ({bmx['fp32_steps']} FP32), {bmx['time_to_scf_s']:.3f} s")
         scf_info["cfg4_fp64_plain"] = p64
         scf_info["cfg4_fp64_batched"] = dict(b64, dE_vs_plain_ha=abs(b64["etot_ha"] - p64["etot_ha"]),
@@ -284,7 +288,7 @@ def main():
                                               speedup_vs_plain=round(p64["time_to_scf_s"] / bmx["time_to_scf_s"], 3))
         if args.bf16:
             bbf, e_bf = scf_run(gscf, lib, spec4, su4, lib.DEREK_MIXED, kbatch=0, graph=graph, autotune=True,
-                                gemm_bf16=0)
+                                refine_steps=args.refine, gemm_bf16=0)
             log(f"# cfg4 mixed batched, FP32 MFMA GEMMs: {bbf['steps']} steps ({bbf['fp32_steps']} FP32), "
                 f"{bbf['time_to_scf_s']:.3f} s")
             scf_info["cfg4_mixed_batched_fp32_mfma"] = dict(
@@ -299,7 +303,7 @@ def main():
         f64, e_f = scf_run(gscf, lib, spec5, su5, lib.DEREK_FP64, verbose=True, kbatch=0, graph=graph)
         log(f"# cfg5 FP64 batched: {f64['steps']} steps, {f64['time_to_scf_s']:.3f} s, E = {f64['etot_ha']:.10f}")
         mx5, e_x = scf_run(gscf, lib, spec5, su5, lib.DEREK_MIXED, verbose=True, kbatch=0, graph=graph,
-                           autotune=True, gemm_bf16=args.bf16)
+                           refine_steps=args.refine, autotune=True, gemm_bf16=args.bf16)
         log(f"# cfg5 mixed batched: {mx5['steps']} steps ({mx5['fp32_steps']} FP32), {mx5['time_to_scf_s']:.3f} s")
         scf_info["cfg5_fp64_batched"] = f64
         scf_info["cfg5_mixed_batched"] = dict(mx5, bf16_terms=args.bf16,
@@ -308,7 +312,7 @@ def main():
                                               speedup_vs_fp64=round(f64["time_to_scf_s"] / mx5["time_to_scf_s"], 3))
         if args.bf16:
             bx5, e_bx = scf_run(gscf, lib, spec5, su5, lib.DEREK_MIXED, verbose=True, kbatch=0, graph=graph,
-                                autotune=True, gemm_bf16=0)
+                                refine_steps=args.refine, autotune=True, gemm_bf16=0)
             log(f"# cfg5 mixed batched, FP32 MFMA GEMMs: {bx5['steps']} steps ({bx5['fp32_steps']} FP32), "
                 f"{bx5['time_to_scf_s']:.3f} s")
             scf_info["cfg5_mixed_batched_fp32_mfma"] = dict(

@@ -35,7 +35,7 @@ struct ScfEngine {
   // the wavefunctions may be off orthonormal (just promoted from complex64):
   // the next FP64 step forms the full subspace overlaps, which restores it
   bool full_overlaps = false;
-  int refine = 3, refine_left = 0;
+  int refine = 2, refine_left = 0;
   double switch_tol = 1e-5, e_prev = 0.0, de_prev = 1e300;
   bool have_prev = false, have_rho_out = false;
   void *Y = nullptr, *HY = nullptr, *Y2 = nullptr, *HY2 = nullptr;
@@ -454,7 +454,9 @@ void scf_init(derek_ctx* c, const derek_scf_params& p) {
   E.graphs = p.use_graph != 0;
   E.fp32 = c->prec == DEREK_MIXED;
   E.switch_tol = p.fp32_switch_tol > 0.0 ? p.fp32_switch_tol : 1e-5;
-  E.refine = p.refine_steps > 0 ? p.refine_steps : 3;
+  // two FP64 steps: the first restores orthonormality (full overlaps), the
+  // stop test then compares two consecutive FP64 energies
+  E.refine = p.refine_steps > 0 ? p.refine_steps : 2;
   E.refine_left = E.fp32 ? E.refine : 0;
   if (E.fp32) alloc_psi<float>(c, E);
   else alloc_psi<double>(c, E);

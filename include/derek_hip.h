@@ -61,7 +61,8 @@ typedef struct {
   uint64_t seed;         /* initial-wavefunction seed */
   double fp32_switch_tol; /* DEREK_MIXED: the FP32 phase ends once |E_i - E_{i-1}| < max(this, 2e-7 |E_i|)
                              (0: 1e-5 Ha), or once |dE| grows again after falling below 1e-3 Ha */
-  int32_t refine_steps;   /* DEREK_MIXED: FP64 steps after the switch before convergence may be declared (0: 3) */
+  int32_t refine_steps;   /* DEREK_MIXED: FP64 steps after the switch before convergence may be declared (0: 2,
+                             i.e. the stop test compares two consecutive FP64 energies) */
   int32_t kbatch;         /* k-points per batched launch (0: all k-points in one launch) */
   int32_t use_graph;      /* 1: capture the SCF step in a hipGraph once and replay it */
   int32_t autotune;       /* 1: time the tunable launch shapes at scf_init and keep the fastest */
