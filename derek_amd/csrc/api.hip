@@ -684,6 +684,9 @@ int32_t derek_hip_set_basis(derek_ctx* c, int32_t k, int64_t npw, const int32_t*
         c->nst_max = std::max(c->nst_max, s.maps.nst);
         c->nax_max = std::max(c->nax_max, s.maps.nax);
       }
+    // stick capacity per item: a multiple of 16, so that z-major stick rows
+    // start on 128-byte lines
+    c->nst_max = (int)(ceil_div(c->nst_max, 16) * 16);
   });
 }
 

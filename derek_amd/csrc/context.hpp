@@ -124,7 +124,7 @@ struct derek_ctx {
   derek::PwMaps* d_maps = nullptr;  // device table of every k-point's maps
   int* d_npw = nullptr;             // [nk]
   double* d_wk = nullptr;           // [nk] k-point weights
-  int nst_max = 0, nax_max = 0;     // largest stick / plane count over the k-points
+  int nst_max = 0, nax_max = 0;     // largest stick count (rounded up to 16) / plane count over the k-points
   double* gsq = nullptr;
   double* vext = nullptr;
   double* vloc = nullptr;

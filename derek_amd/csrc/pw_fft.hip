@@ -647,20 +647,27 @@ __global__ __launch_bounds__(Split<N>::NT) void k_py_fwd2(PwBatch B, const cx_t<
 // TPL = max(R1, R2) threads per stick; the sticks are padded lines of one
 // LDS buffer (the plane-wave scatter / gather needs them there anyway), the
 // two DFT stages run in registers with one transpose through that buffer.
-template <int N> struct StickSplit {
+// SL sticks per workgroup, 16 (FP64) or 32 (FP32): with the stick capacity
+// a multiple of 16, every workgroup's segment of a z-major row is two whole
+// 128-byte lines, so no line is written by two workgroups.  (32 FP64 sticks
+// need 57 KB of LDS at N = 108 and cut the z passes to two workgroups per
+// CU, which was slower.)
+template <int N, int SL_> struct StickSplit {
   static constexpr int R1 = Split<N>::R1, R2 = Split<N>::R2;
   static constexpr int TPL = R1 > R2 ? R1 : R2;
-  static constexpr int NT = 256;
-  static constexpr int SL = NT / TPL;  // sticks per workgroup
+  static constexpr int SL = SL_;       // sticks per workgroup
+  static constexpr int NT = SL * TPL;
   static constexpr int LS = N + 1;     // padded line pitch
   static constexpr size_t lds(size_t csize) { return csize * ((size_t)N + (size_t)SL * LS); }
 };
+
+template <typename R, int N> using StickSplitR = StickSplit<N, sizeof(R) == 8 ? 16 : 32>;
 
 // S1 + S2 of one transform on a padded LDS line, in place; thread (line, j)
 // with j < TPL, `act` false for lines past the block's last stick
 template <int SGN, typename C, int N>
 __device__ __forceinline__ void stick_two_stage(C* line, const C* twl, int j, bool act) {
-  using SS = StickSplit<N>;
+  using SS = StickSplit<N, 1>;
   constexpr int R1 = SS::R1, R2 = SS::R2;
   C v[R1];
   if (act && j < R2) {
@@ -693,11 +700,11 @@ __device__ __forceinline__ void stick_two_stage(C* line, const C* twl, int j, bo
 
 // psi[b][perm[p]] -> S[item][s][:] for the block's sticks
 template <typename R, int N, bool ZM>
-__global__ __launch_bounds__(StickSplit<N>::NT) void k_sticks_z_inv2(PwBatch B, PwBlock<R> psi,
+__global__ __launch_bounds__((StickSplitR<R, N>::NT)) void k_sticks_z_inv2(PwBatch B, PwBlock<R> psi,
                                                                      const cx_t<R>* __restrict__ tw,
                                                                      cx_t<R>* __restrict__ S) {
   using C = cx_t<R>;
-  using SS = StickSplit<N>;
+  using SS = StickSplitR<R, N>;
   constexpr int NT = SS::NT, SL = SS::SL, LS = SS::LS, TPL = SS::TPL;
   extern __shared__ __align__(16) unsigned char smem[];
   const PwMaps m = B.mt[B.k0 + blockIdx.z];
@@ -734,11 +741,11 @@ __global__ __launch_bounds__(StickSplit<N>::NT) void k_sticks_z_inv2(PwBatch B, 
 
 // hpsi[b][perm[p]] = (z FFT of the stick)[pw_z[p]] + kin[perm[p]] psi[b][perm[p]]
 template <typename R, int N, bool ZM>
-__global__ __launch_bounds__(StickSplit<N>::NT) void k_sticks_z_fwd2(PwBatch B, const cx_t<R>* __restrict__ S,
+__global__ __launch_bounds__((StickSplitR<R, N>::NT)) void k_sticks_z_fwd2(PwBatch B, const cx_t<R>* __restrict__ S,
                                                                      const cx_t<R>* __restrict__ tw, PwBlock<R> psi,
                                                                      PwBlock<R> hpsi) {
   using C = cx_t<R>;
-  using SS = StickSplit<N>;
+  using SS = StickSplitR<R, N>;
   constexpr int NT = SS::NT, SL = SS::SL, LS = SS::LS, TPL = SS::TPL;
   extern __shared__ __align__(16) unsigned char smem[];
   const PwMaps m = B.mt[B.k0 + blockIdx.z];
@@ -802,7 +809,7 @@ template <typename R> static void check_offsets(const PwBatch& B, int64_t nyz, i
 template <typename R, int N>
 static void sticks_z_inv2(const PwBatch& B, PwBlock<R> psi, const Axis<R>* ax, cx_t<R>* S, bool zmajor,
                           hipStream_t s) {
-  using SS = StickSplit<N>;
+  using SS = StickSplitR<R, N>;
   const size_t lds = SS::lds(sizeof(cx_t<R>));
   auto kern = zmajor ? k_sticks_z_inv2<R, N, true> : k_sticks_z_inv2<R, N, false>;
   allow_lds_pw(kern, lds);
@@ -813,7 +820,7 @@ static void sticks_z_inv2(const PwBatch& B, PwBlock<R> psi, const Axis<R>* ax, c
 template <typename R, int N>
 static void sticks_z_fwd2(const PwBatch& B, const cx_t<R>* S, const Axis<R>* ax, PwBlock<R> psi, PwBlock<R> hpsi,
                           bool zmajor, hipStream_t s) {
-  using SS = StickSplit<N>;
+  using SS = StickSplitR<R, N>;
   const size_t lds = SS::lds(sizeof(cx_t<R>));
   auto kern = zmajor ? k_sticks_z_fwd2<R, N, true> : k_sticks_z_fwd2<R, N, false>;
   allow_lds_pw(kern, lds);
