@@ -410,27 +410,45 @@ int64_t overlap_work_elems(int m, int n, int64_t npw, int batch) {
   return (int64_t)batch * chunks * m * n;
 }
 
-// C[kz] = sum over chunks, columns spread over the workgroups of blockIdx.x
-__global__ void k_overlap_reduce(const double2* __restrict__ part, OvGeom g, double2* __restrict__ C, int ldc,
-                                 int64_t sc) {
+// C[kz] = sum over chunks.  A workgroup takes RD_E consecutive elements;
+// its RD_G thread groups each sum every RD_G-th chunk (in increasing
+// order), and the group sums are added in group order through LDS, so the
+// result does not depend on the launch.
+constexpr int RD_E = 32, RD_G = 8;
+__global__ __launch_bounds__(RD_E * RD_G) void k_overlap_reduce(const double2* __restrict__ part, OvGeom g,
+                                                                double2* __restrict__ C, int ldc, int64_t sc) {
+  __shared__ double2 red[RD_G][RD_E];
   const int kz = blockIdx.y;
   const int64_t mn = (int64_t)g.m * g.n;
   const double2* pk = part + (int64_t)kz * g.nchunk * mn;
   double2* Ck = C + kz * sc;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < mn; e += (int64_t)gridDim.x * blockDim.x) {
-    const int i = (int)(e % g.m), j = (int)(e / g.m);
+  const int el = threadIdx.x % RD_E, grp = threadIdx.x / RD_E;
+  for (int64_t e0 = (int64_t)blockIdx.x * RD_E; e0 < mn; e0 += (int64_t)gridDim.x * RD_E) {
+    const int64_t e = e0 + el;
+    const bool ok = e < mn;
+    const int i = ok ? (int)(e % g.m) : 0, j = ok ? (int)(e / g.m) : 0;
     // Hermitian: the strict lower triangle is the conjugate of the upper one
     // (exactly Hermitian output, real diagonal)
     const bool mirror = g.herm && i > j;
     const int64_t src = mirror ? (int64_t)i * g.m + j : (int64_t)j * g.m + i;
     double sr = 0.0, si = 0.0;
-    for (int c = 0; c < g.nchunk; ++c) {
-      const double2 v = pk[(int64_t)c * mn + src];
-      sr += v.x;
-      si += v.y;
+    if (ok)
+      for (int c = grp; c < g.nchunk; c += RD_G) {
+        const double2 v = pk[(int64_t)c * mn + src];
+        sr += v.x;
+        si += v.y;
+      }
+    red[grp][el] = make_double2(sr, si);
+    __syncthreads();
+    if (grp == 0 && ok) {
+      for (int q = 1; q < RD_G; ++q) {
+        sr += red[q][el].x;
+        si += red[q][el].y;
+      }
+      if (g.herm && i == j) si = 0.0;
+      Ck[(int64_t)j * ldc + i] = mirror ? make_double2(sr, -si) : make_double2(sr, si);
     }
-    if (g.herm && i == j) si = 0.0;
-    Ck[(int64_t)j * ldc + i] = mirror ? make_double2(sr, -si) : make_double2(sr, si);
+    __syncthreads();
   }
 }
 
@@ -463,8 +481,8 @@ void overlap(const cx_t<R>* X, int64_t ldx, int64_t sx, const cx_t<R>* Y, int64_
     hipLaunchKernelGGL(k_overlap<R>, grid, dim3(256), 0, s, X, ldx, sx, Y, ldy, sy, npwk, g, work);
   DEREK_HIP(hipGetLastError());
   const int64_t mn = (int64_t)m * n;
-  hipLaunchKernelGGL(k_overlap_reduce, dim3((unsigned)std::min<int64_t>(512, ceil_div(mn, 256)), batch), dim3(256), 0,
-                     s, work, g, C, ldc, sc);
+  hipLaunchKernelGGL(k_overlap_reduce, dim3((unsigned)std::min<int64_t>(4096, ceil_div(mn, RD_E)), batch),
+                     dim3(RD_E * RD_G), 0, s, work, g, C, ldc, sc);
   DEREK_HIP(hipGetLastError());
 }
 
