@@ -1039,9 +1039,14 @@ template <typename R, int N> struct Slab {
   // unpadded A is 3-4 % faster for complex64 at 108^3, FP64 is indifferent)
   static constexpr int LPA = L;
   static constexpr int LPX = L % 2 ? L : L + 1;
-  static constexpr int NT = N == 32 ? 256 : (N == 72 ? 288 : 504);
-  // x lines per round: the complex64 108 case is sized for two workgroups per CU
-  static constexpr int LR = N == 32 ? 32 : (N == 72 ? 24 : (N == 108 && sizeof(R) == 4 ? 27 : 36));
+  // 360 threads at N = 72 (not 288): the R1- and R2-thread stages then take
+  // 40 and 45 planes per pass, so the 33 active planes of a 16 bohr cell at
+  // Ecut 20 Ha need one pass per y stage instead of two
+  static constexpr int NT = N == 32 ? 256 : (N == 72 ? 360 : 504);
+  // x lines per round: the complex64 108 case is sized for two workgroups per
+  // CU; complex64 at 72 takes the 72 lines in two rounds
+  static constexpr int LR =
+      N == 32 ? 32 : (N == 72 ? (sizeof(R) == 4 ? 36 : 24) : (N == 108 && sizeof(R) == 4 ? 27 : 36));
   static constexpr int NR = (N + LR - 1) / LR;
   static constexpr int LJ = NT / R2, LK = NT / R1;  // lines per pass of the R2- / R1-thread stages
   static_assert(NT % R1 == 0 && NT % R2 == 0, "workgroup size must keep each thread's digits");
