@@ -1031,7 +1031,9 @@ void planes_x_density(const PwBatch& B, const cx_t<R>* P, const Axis<R>* ax, int
 // x rows in registers for the whole launch.  Compared with the separate y and
 // x passes the active planes stay on chip: HBM sees the stick rows once each
 // way and V once per band group.
-template <typename R, int N> struct Slab {
+// VM: parameters of k_slab_vmul (false: k_slab_density, whose FP64
+// accumulators need more registers per thread)
+template <typename R, int N, bool VM = false> struct Slab {
   static constexpr int R1 = Split<N>::R1, R2 = Split<N>::R2;
   static constexpr int R2P = R2 % 2 ? R2 : R2 + 1;
   static constexpr int L = R1 * R2P;              // slots of a line
@@ -1042,7 +1044,10 @@ template <typename R, int N> struct Slab {
   // 360 threads at N = 72 (not 288): the R1- and R2-thread stages then take
   // 40 and 45 planes per pass, so the 33 active planes of a 16 bohr cell at
   // Ecut 20 Ha need one pass per y stage instead of two
-  static constexpr int NT = N == 32 ? 256 : (N == 72 ? 360 : 504);
+  // complex64 H|psi> at 144 runs 840 threads (70 planes per pass: one pass
+  // per y stage for the 69 active planes of a 30 bohr cell at Ecut 25 Ha,
+  // and 14 waves on the CU that its LDS working set allows)
+  static constexpr int NT = N == 32 ? 256 : (N == 72 ? 360 : (VM && N == 144 && sizeof(R) == 4 ? 840 : 504));
   // x lines per round: the complex64 108 case is sized for two workgroups per
   // CU; complex64 at 72 takes the 72 lines in two rounds
   static constexpr int LR =
@@ -1063,9 +1068,9 @@ template <typename R, int N> struct Slab {
 // (j2 in the R2-thread stages, k1 in the R1-thread stages, and its line in
 // each), its twiddles exp(-2 pi i k1 r / N) and the LDS rows of the active
 // planes at x = j2 + R2 r (-1 if none).
-template <typename R, int N> struct SlabThread {
+template <typename R, int N, bool VM = false> struct SlabThread {
   using C = cx_t<R>;
-  using SL = Slab<R, N>;
+  using SL = Slab<R, N, VM>;
   static constexpr int R1 = SL::R1, R2 = SL::R2, R2P = SL::R2P, NT = SL::NT, LR = SL::LR;
   static constexpr int LPA = SL::LPA, LPX = SL::LPX;
   static constexpr int LJ = SL::LJ, LK = SL::LK;
@@ -1162,12 +1167,12 @@ template <typename R, int N> struct SlabThread {
 };
 
 template <typename R, int N>
-__global__ __launch_bounds__((Slab<R, N>::NT)) void k_slab_vmul(PwBatch B, cx_t<R>* __restrict__ S, int nz, int bpw,
+__global__ __launch_bounds__((Slab<R, N, true>::NT)) void k_slab_vmul(PwBatch B, cx_t<R>* __restrict__ S, int nz, int bpw,
                                                               const cx_t<R>* __restrict__ tw,
                                                               const R* __restrict__ vt) {
   using C = cx_t<R>;
-  using ST = SlabThread<R, N>;
-  constexpr int R1 = ST::R1, R2 = ST::R2, LR = ST::LR, NR = Slab<R, N>::NR, LJ = ST::LJ, LK = ST::LK;
+  using ST = SlabThread<R, N, true>;
+  constexpr int R1 = ST::R1, R2 = ST::R2, LR = ST::LR, NR = Slab<R, N, true>::NR, LJ = ST::LJ, LK = ST::LK;
   constexpr int LPA = ST::LPA, LPX = ST::LPX;
   extern __shared__ __align__(16) unsigned char smem[];
   const PwMaps m = B.mt[B.k0 + blockIdx.z];
@@ -1361,7 +1366,7 @@ template <typename R> bool slab_supported(int nx, int ny, int nz, int nax_cap, i
 
 template <typename R, int N>
 static void slab_vmul_n(const PwBatch& B, cx_t<R>* S, const Axis<R>* ax, int nz, const R* vt, int bpw, hipStream_t s) {
-  using SL = Slab<R, N>;
+  using SL = Slab<R, N, true>;
   const size_t lds = SL::lds(B.nax_cap);
   check_lds_pw(lds, N);
   auto kern = k_slab_vmul<R, N>;
