@@ -1047,11 +1047,15 @@ template <typename R, int N, bool VM = false> struct Slab {
   // complex64 H|psi> at 144 runs 840 threads (70 planes per pass: one pass
   // per y stage for the 69 active planes of a 30 bohr cell at Ecut 25 Ha,
   // and 14 waves on the CU that its LDS working set allows)
-  static constexpr int NT = N == 32 ? 256 : (N == 72 ? 360 : (VM && N == 144 && sizeof(R) == 4 ? 840 : 504));
+  // FP64 H|psi> at 72 runs 720 threads with all 72 x lines in one round
+  // (7 barrier phases per band instead of 13; one 136 KB workgroup per CU)
+  static constexpr bool kWide72 = VM && N == 72 && sizeof(R) == 8;
+  static constexpr int NT =
+      N == 32 ? 256 : (N == 72 ? (kWide72 ? 720 : 360) : (VM && N == 144 && sizeof(R) == 4 ? 840 : 504));
   // x lines per round: the complex64 108 case is sized for two workgroups per
   // CU; complex64 at 72 takes the 72 lines in two rounds
   static constexpr int LR =
-      N == 32 ? 32 : (N == 72 ? (sizeof(R) == 4 ? 36 : 24) : (N == 108 && sizeof(R) == 4 ? 27 : 36));
+      N == 32 ? 32 : (N == 72 ? (kWide72 ? 72 : (sizeof(R) == 4 ? 36 : 24)) : (N == 108 && sizeof(R) == 4 ? 27 : 36));
   static constexpr int NR = (N + LR - 1) / LR;
   static constexpr int LJ = NT / R2, LK = NT / R1;  // lines per pass of the R2- / R1-thread stages
   static_assert(NT % R1 == 0 && NT % R2 == 0, "workgroup size must keep each thread's digits");
@@ -1348,8 +1352,9 @@ __global__ __launch_bounds__(256) void k_slab_rho_sum(const double* __restrict__
   }
 }
 
+// both fused kernels (H|psi> and density) fit the LDS for nax_cap planes
 template <typename R, int N> static bool slab_fits(int nax_cap) {
-  return Slab<R, N>::lds(nax_cap) <= 160 * 1024;
+  return Slab<R, N>::lds(nax_cap) <= 160 * 1024 && Slab<R, N, true>::lds(nax_cap) <= 160 * 1024;
 }
 
 template <typename R> bool slab_supported(int nx, int ny, int nz, int nax_cap, int nst_cap) {
