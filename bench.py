@@ -26,7 +26,7 @@ and a CPU baseline (the numpy oracle's H|psi> on a bounded sample of the
 headline system, timed on this host on one core and on every core of the
 host's share, before the GPU is touched).
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-scf] [--scf cfg2,cfg4] [--cpu-seconds S] [--no-cpu]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-scf] [--scf gamma108,kmesh72] [--cpu-seconds S] [--no-cpu]
                 [--fixed-shapes | --shapes BPW,ITEMS,FUSED] [--bf16 N]
 """
 import argparse
@@ -144,7 +144,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-scf", action="store_true")
-    ap.add_argument("--scf", default="cfg2,cfg4")
+    ap.add_argument("--scf", default="gamma108,kmesh72")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--refine", type=int, default=0,
                     help="FP64 refinement steps of the mixed runs (0: the library default)")
@@ -173,7 +173,7 @@ def main():
     from derek_amd import config, lib
     from derek_amd import scf as gscf
 
-    spec = config.benchmark(2)
+    spec = config.workload("gamma108")
     t0 = time.perf_counter()
     su = gscf.Setup(spec)  # host-side discretisation: no GPU call yet
     cpu = cpu_all = None
@@ -248,77 +248,77 @@ def main():
     scf_info = {}
     todo = [] if args.no_scf else [x.strip() for x in args.scf.split(",") if x.strip()]
     graph = not args.no_graph
-    if "cfg2" in todo:
+    if "gamma108" in todo:
         r64, e64 = scf_run(gscf, lib, spec, su, lib.DEREK_FP64, kbatch=1)
-        log(f"# cfg2 FP64: {r64['steps']} steps, {r64['time_to_scf_s']:.3f} s, E = {r64['etot_ha']:.10f}")
+        log(f"# gamma108 FP64: {r64['steps']} steps, {r64['time_to_scf_s']:.3f} s, E = {r64['etot_ha']:.10f}")
         rmx, emx = scf_run(gscf, lib, spec, su, lib.DEREK_MIXED, graph=graph, refine_steps=args.refine,
                            gemm_bf16=args.bf16)
-        log(f"# cfg2 mixed: {rmx['steps']} steps ({rmx['fp32_steps']} FP32), {rmx['time_to_scf_s']:.3f} s, "
+        log(f"# gamma108 mixed: {rmx['steps']} steps ({rmx['fp32_steps']} FP32), {rmx['time_to_scf_s']:.3f} s, "
             f"E = {rmx['etot_ha']:.10f}")
-        scf_info["cfg2_fp64"] = r64
-        scf_info["cfg2_mixed"] = dict(rmx, bf16_terms=args.bf16, dE_vs_fp64_ha=abs(rmx["etot_ha"] - r64["etot_ha"]),
+        scf_info["gamma108_fp64"] = r64
+        scf_info["gamma108_mixed"] = dict(rmx, bf16_terms=args.bf16, dE_vs_fp64_ha=abs(rmx["etot_ha"] - r64["etot_ha"]),
                                       max_deig_vs_fp64_ha=float(np.max(np.abs(emx - e64))),
                                       speedup_vs_fp64=round(r64["time_to_scf_s"] / rmx["time_to_scf_s"], 3))
         if args.bf16:
             rbf, ebf = scf_run(gscf, lib, spec, su, lib.DEREK_MIXED, graph=graph, refine_steps=args.refine,
                                gemm_bf16=0)
-            log(f"# cfg2 mixed, FP32 MFMA GEMMs: {rbf['steps']} steps ({rbf['fp32_steps']} FP32), "
+            log(f"# gamma108 mixed, FP32 MFMA GEMMs: {rbf['steps']} steps ({rbf['fp32_steps']} FP32), "
                 f"{rbf['time_to_scf_s']:.3f} s")
-            scf_info["cfg2_mixed_fp32_mfma"] = dict(rbf, bf16_terms=0,
+            scf_info["gamma108_mixed_fp32_mfma"] = dict(rbf, bf16_terms=0,
                                                     dE_vs_fp64_ha=abs(rbf["etot_ha"] - r64["etot_ha"]),
                                                max_deig_vs_fp64_ha=float(np.max(np.abs(ebf - e64))),
                                                speedup_vs_fp64=round(r64["time_to_scf_s"] / rbf["time_to_scf_s"], 3))
-    if "cfg4" in todo:
-        spec4 = config.benchmark(4)
-        su4 = gscf.Setup(spec4)
-        log(f"# cfg4 grid {su4.dims} nk {su4.nk} npw {min(su4.npw)}..{max(su4.npw)}")
-        p64, e_p = scf_run(gscf, lib, spec4, su4, lib.DEREK_FP64, kbatch=1)
-        log(f"# cfg4 FP64 plain: {p64['steps']} steps, {p64['time_to_scf_s']:.3f} s, E = {p64['etot_ha']:.10f}")
-        b64, e_b = scf_run(gscf, lib, spec4, su4, lib.DEREK_FP64, kbatch=0, graph=graph)
-        log(f"# cfg4 FP64 batched: {b64['steps']} steps, {b64['time_to_scf_s']:.3f} s")
-        bmx, e_m = scf_run(gscf, lib, spec4, su4, lib.DEREK_MIXED, kbatch=0, graph=graph, autotune=True,
+    if "kmesh72" in todo:
+        spec72 = config.workload("kmesh72")
+        su72 = gscf.Setup(spec72)
+        log(f"# kmesh72 grid {su72.dims} nk {su72.nk} npw {min(su72.npw)}..{max(su72.npw)}")
+        p64, e_p = scf_run(gscf, lib, spec72, su72, lib.DEREK_FP64, kbatch=1)
+        log(f"# kmesh72 FP64 plain: {p64['steps']} steps, {p64['time_to_scf_s']:.3f} s, E = {p64['etot_ha']:.10f}")
+        b64, e_b = scf_run(gscf, lib, spec72, su72, lib.DEREK_FP64, kbatch=0, graph=graph)
+        log(f"# kmesh72 FP64 batched: {b64['steps']} steps, {b64['time_to_scf_s']:.3f} s")
+        bmx, e_m = scf_run(gscf, lib, spec72, su72, lib.DEREK_MIXED, kbatch=0, graph=graph, autotune=True,
                            refine_steps=args.refine, gemm_bf16=args.bf16)
-        log(f"# cfg4 mixed batched: {bmx['steps']} steps ({bmx['fp32_steps']} FP32), {bmx['time_to_scf_s']:.3f} s")
-        scf_info["cfg4_fp64_plain"] = p64
-        scf_info["cfg4_fp64_batched"] = dict(b64, dE_vs_plain_ha=abs(b64["etot_ha"] - p64["etot_ha"]),
+        log(f"# kmesh72 mixed batched: {bmx['steps']} steps ({bmx['fp32_steps']} FP32), {bmx['time_to_scf_s']:.3f} s")
+        scf_info["kmesh72_fp64_plain"] = p64
+        scf_info["kmesh72_fp64_batched"] = dict(b64, dE_vs_plain_ha=abs(b64["etot_ha"] - p64["etot_ha"]),
                                              speedup_vs_plain=round(p64["time_to_scf_s"] / b64["time_to_scf_s"], 3))
-        scf_info["cfg4_mixed_batched"] = dict(bmx, bf16_terms=args.bf16,
+        scf_info["kmesh72_mixed_batched"] = dict(bmx, bf16_terms=args.bf16,
                                               dE_vs_plain_ha=abs(bmx["etot_ha"] - p64["etot_ha"]),
                                               max_deig_vs_plain_ha=float(np.max(np.abs(e_m - e_p))),
                                               speedup_vs_plain=round(p64["time_to_scf_s"] / bmx["time_to_scf_s"], 3))
         if args.bf16:
-            bbf, e_bf = scf_run(gscf, lib, spec4, su4, lib.DEREK_MIXED, kbatch=0, graph=graph, autotune=True,
+            bbf, e_bf = scf_run(gscf, lib, spec72, su72, lib.DEREK_MIXED, kbatch=0, graph=graph, autotune=True,
                                 refine_steps=args.refine, gemm_bf16=0)
-            log(f"# cfg4 mixed batched, FP32 MFMA GEMMs: {bbf['steps']} steps ({bbf['fp32_steps']} FP32), "
+            log(f"# kmesh72 mixed batched, FP32 MFMA GEMMs: {bbf['steps']} steps ({bbf['fp32_steps']} FP32), "
                 f"{bbf['time_to_scf_s']:.3f} s")
-            scf_info["cfg4_mixed_batched_fp32_mfma"] = dict(
+            scf_info["kmesh72_mixed_batched_fp32_mfma"] = dict(
                 bbf, bf16_terms=0, dE_vs_plain_ha=abs(bbf["etot_ha"] - p64["etot_ha"]),
                 max_deig_vs_plain_ha=float(np.max(np.abs(e_bf - e_p))),
                 speedup_vs_plain=round(p64["time_to_scf_s"] / bbf["time_to_scf_s"], 3))
 
-    if "cfg5" in todo:
-        spec5 = config.benchmark(5)
-        su5 = gscf.Setup(spec5)
-        log(f"# cfg5 grid {su5.dims} nk {su5.nk} npw {min(su5.npw)}..{max(su5.npw)} bands {spec5.nbands}")
-        f64, e_f = scf_run(gscf, lib, spec5, su5, lib.DEREK_FP64, verbose=True, kbatch=0, graph=graph)
-        log(f"# cfg5 FP64 batched: {f64['steps']} steps, {f64['time_to_scf_s']:.3f} s, E = {f64['etot_ha']:.10f}")
-        mx5, e_x = scf_run(gscf, lib, spec5, su5, lib.DEREK_MIXED, verbose=True, kbatch=0, graph=graph,
+    if "kmesh144" in todo:
+        spec144 = config.workload("kmesh144")
+        su144 = gscf.Setup(spec144)
+        log(f"# kmesh144 grid {su144.dims} nk {su144.nk} npw {min(su144.npw)}..{max(su144.npw)} bands {spec144.nbands}")
+        f64, e_f = scf_run(gscf, lib, spec144, su144, lib.DEREK_FP64, verbose=True, kbatch=0, graph=graph)
+        log(f"# kmesh144 FP64 batched: {f64['steps']} steps, {f64['time_to_scf_s']:.3f} s, E = {f64['etot_ha']:.10f}")
+        mx144, e_x = scf_run(gscf, lib, spec144, su144, lib.DEREK_MIXED, verbose=True, kbatch=0, graph=graph,
                            refine_steps=args.refine, autotune=True, gemm_bf16=args.bf16)
-        log(f"# cfg5 mixed batched: {mx5['steps']} steps ({mx5['fp32_steps']} FP32), {mx5['time_to_scf_s']:.3f} s")
-        scf_info["cfg5_fp64_batched"] = f64
-        scf_info["cfg5_mixed_batched"] = dict(mx5, bf16_terms=args.bf16,
-                                              dE_vs_fp64_ha=abs(mx5["etot_ha"] - f64["etot_ha"]),
+        log(f"# kmesh144 mixed batched: {mx144['steps']} steps ({mx144['fp32_steps']} FP32), {mx144['time_to_scf_s']:.3f} s")
+        scf_info["kmesh144_fp64_batched"] = f64
+        scf_info["kmesh144_mixed_batched"] = dict(mx144, bf16_terms=args.bf16,
+                                              dE_vs_fp64_ha=abs(mx144["etot_ha"] - f64["etot_ha"]),
                                               max_deig_vs_fp64_ha=float(np.max(np.abs(e_x - e_f))),
-                                              speedup_vs_fp64=round(f64["time_to_scf_s"] / mx5["time_to_scf_s"], 3))
+                                              speedup_vs_fp64=round(f64["time_to_scf_s"] / mx144["time_to_scf_s"], 3))
         if args.bf16:
-            bx5, e_bx = scf_run(gscf, lib, spec5, su5, lib.DEREK_MIXED, verbose=True, kbatch=0, graph=graph,
+            bx144, e_bx = scf_run(gscf, lib, spec144, su144, lib.DEREK_MIXED, verbose=True, kbatch=0, graph=graph,
                                 refine_steps=args.refine, autotune=True, gemm_bf16=0)
-            log(f"# cfg5 mixed batched, FP32 MFMA GEMMs: {bx5['steps']} steps ({bx5['fp32_steps']} FP32), "
-                f"{bx5['time_to_scf_s']:.3f} s")
-            scf_info["cfg5_mixed_batched_fp32_mfma"] = dict(
-                bx5, bf16_terms=0, dE_vs_fp64_ha=abs(bx5["etot_ha"] - f64["etot_ha"]),
+            log(f"# kmesh144 mixed batched, FP32 MFMA GEMMs: {bx144['steps']} steps ({bx144['fp32_steps']} FP32), "
+                f"{bx144['time_to_scf_s']:.3f} s")
+            scf_info["kmesh144_mixed_batched_fp32_mfma"] = dict(
+                bx144, bf16_terms=0, dE_vs_fp64_ha=abs(bx144["etot_ha"] - f64["etot_ha"]),
                 max_deig_vs_fp64_ha=float(np.max(np.abs(e_bx - e_f))),
-                speedup_vs_fp64=round(f64["time_to_scf_s"] / bx5["time_to_scf_s"], 3))
+                speedup_vs_fp64=round(f64["time_to_scf_s"] / bx144["time_to_scf_s"], 3))
 
     out = {
         "metric": METRIC, "value": round(bands_per_s, 2), "unit": "H|psi> bands/s", "n_gpus": 1,

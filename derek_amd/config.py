@@ -118,19 +118,30 @@ class SystemSpec:
         return "\n".join(lines) + "\n"
 
 
-def benchmark(i):
-    """The five benchmark systems (1-based index as in the benchmark list)."""
-    if i in (1, 3):
-        return SystemSpec("cfg1", 10.0, 10.0, 2.15, (1, 1, 1), 8, 10, nwells=4, seed=1)
-    if i == 2:
+WORKLOADS = ("gamma32", "gamma108", "kmesh72", "kmesh144")
+
+
+def workload(name):
+    """The benchmark systems, all cubic cells with seeded Gaussian wells:
+
+    gamma32   10 bohr, Ecut 10 Ha, Gamma, 32^3 fine grid, 8 electrons, 10 bands, 4 wells
+    gamma108  20 bohr, Ecut 30 Ha, Gamma, 108^3, 128 electrons, 80 bands, 16 wells
+              + 0.2 Ha cos(2 pi x / L), kT 0.002 Ha
+    kmesh72   16 bohr, Ecut 20 Ha, 4x4x4 mesh, 72^3, 32 electrons, 24 bands, 8 wells, kT 0.01 Ha
+    kmesh144  30 bohr, Ecut 25 Ha, 2x2x2 mesh, 144^3, 512 electrons, 300 bands, 64 wells, kT 0.002 Ha
+    """
+    if name == "gamma32":
+        return SystemSpec("gamma32", 10.0, 10.0, 2.15, (1, 1, 1), 8, 10, nwells=4, seed=1)
+    if name == "gamma108":
         # metallic: the gap between bands 64 and 65 closes, fixed occupations
         # slosh; a 0.002 Ha Fermi-Dirac smearing (16 kT below the top band) converges
-        return SystemSpec("cfg2", 20.0, 30.0, 2.15, (1, 1, 1), 128, 80, kT=0.002, nwells=16, cos_amp=0.2, seed=2)
-    if i == 4:
-        return SystemSpec("cfg4", 16.0, 20.0, 2.2, (4, 4, 4), 32, 24, kT=0.01, nwells=8, seed=4)
-    if i == 5:
-        return SystemSpec("cfg5", 30.0, 25.0, 2.125, (2, 2, 2), 512, 300, kT=0.002, nwells=64, seed=5)
-    raise ValueError("benchmark systems are numbered 1..5")
+        return SystemSpec("gamma108", 20.0, 30.0, 2.15, (1, 1, 1), 128, 80, kT=0.002, nwells=16, cos_amp=0.2,
+                          seed=2)
+    if name == "kmesh72":
+        return SystemSpec("kmesh72", 16.0, 20.0, 2.2, (4, 4, 4), 32, 24, kT=0.01, nwells=8, seed=4)
+    if name == "kmesh144":
+        return SystemSpec("kmesh144", 30.0, 25.0, 2.125, (2, 2, 2), 512, 300, kT=0.002, nwells=64, seed=5)
+    raise ValueError("benchmark systems: " + ", ".join(WORKLOADS))
 
 
 def small_system(seed=11, length=8.0, ecut=6.0, g=2.0, mp=(1, 1, 1), nelec=4, nbands=6,

@@ -25,9 +25,9 @@ def write_info(tmp_path, spec, name=None, **kw):
     return seed
 
 
-@pytest.mark.parametrize("i", [1, 4])
-def test_check_mode_matches_library_setup(lib, tmp_path, i):
-    spec = config.benchmark(i)
+@pytest.mark.parametrize("name", ["gamma32", "kmesh72"])
+def test_check_mode_matches_library_setup(lib, tmp_path, name):
+    spec = config.workload(name)
     out = subprocess.run([driver(), write_info(tmp_path, spec), "-c"], capture_output=True, text=True,
                          check=True).stdout
     su = Setup(spec)
@@ -69,7 +69,7 @@ def test_command_line_help(tmp_path):
 
 
 def test_spell_check_autocorrects(tmp_path):
-    spec = config.benchmark(1)
+    spec = config.workload("gamma32")
     text = spec.to_info().replace("cut_off_energy", "cut_of_energy")
     seed = str(tmp_path / "typo")
     open(seed + ".info", "w").write(text)
@@ -86,7 +86,7 @@ def test_fortran_writes_and_reads_interchange_files(tmp_path):
     second run whose external potential comes from a .potex file."""
     import numpy as np
     from derek_amd import interchange, lib
-    spec = config.benchmark(1)
+    spec = config.workload("gamma32")
     su = Setup(spec)
     flags = [("write_potential", "true"), ("write_density", "true"), ("write_wvfn", "true")]
     seed = write_info(tmp_path, spec, energy_tol=1e-10, max_scf=60, extra=flags)
@@ -114,7 +114,7 @@ def test_fortran_writes_and_reads_interchange_files(tmp_path):
 
 
 def test_unknown_task_is_an_error(tmp_path):
-    spec = config.benchmark(1)
+    spec = config.workload("gamma32")
     seed = write_info(tmp_path, spec, extra=[("task", "phonons")])
     r = subprocess.run([driver(), seed, "-c"], capture_output=True, text=True)
     assert r.returncode != 0 and "Unknown task: phonons" in r.stdout
@@ -127,7 +127,7 @@ def test_fortran_spectral_tasks(tmp_path, task):
     library's non-self-consistent bands at the potential the driver wrote."""
     import numpy as np
     from derek_amd import interchange, spectral
-    spec = config.benchmark(1)
+    spec = config.workload("gamma32")
     su = Setup(spec)
     extra = [("task", task), ("write_potential", "true"), ("dos_kpt_mp_grid", "2 2 2")]
     seed = write_info(tmp_path, spec, energy_tol=1e-10, max_scf=60, extra=extra)
@@ -152,7 +152,7 @@ def test_fortran_spectral_tasks(tmp_path, task):
 def test_fortran_scf_matches_oracle(tmp_path):
     from oracle import scf as oscf
     from tests.conftest import oracle_system
-    spec = config.benchmark(1)
+    spec = config.workload("gamma32")
     out = subprocess.run([driver(), write_info(tmp_path, spec, energy_tol=1e-10, max_scf=60)],
                          capture_output=True, text=True, check=True, timeout=300).stdout
     print(out)

@@ -59,7 +59,7 @@ def make_setup(spec):
     return su
 
 
-@pytest.mark.parametrize("name", ["small", "small_k", "cfg1"])
+@pytest.mark.parametrize("name", ["small", "small_k", "gamma32"])
 @pytest.mark.parametrize("precision", [lib.DEREK_FP64, lib.DEREK_MIXED])
 def test_apply_h(name, precision):
     print("apply_h", name, precision, flush=True)
@@ -68,7 +68,7 @@ def test_apply_h(name, precision):
     elif name == "small_k":
         spec = config.small_system(seed=4, mp=(2, 1, 2), ecut=5.0)
     else:
-        spec = config.benchmark(1)
+        spec = config.workload("gamma32")
     su = make_setup(spec)
     ctx = su.context(precision, nbands=11)
     rng = np.random.default_rng(7)
@@ -97,8 +97,8 @@ def test_apply_h_state_errors():
 
 
 def test_apply_h_hermitian_full_size():
-    """cfg2 size (108^3 grid, 62669 plane waves): <x|Hy> = conj(<y|Hx>)."""
-    su = make_setup(config.benchmark(2))
+    """108^3 grid, 62669 plane waves: <x|Hy> = conj(<y|Hx>)."""
+    su = make_setup(config.workload("gamma108"))
     ctx = su.context(nbands=2)
     ctx.set_vloc(su.vext)
     rng = np.random.default_rng(11)
@@ -225,7 +225,7 @@ def test_set_gemm_errors():
 
 @pytest.mark.parametrize("precision", [lib.DEREK_FP64, lib.DEREK_MIXED])
 def test_density(precision):
-    spec = config.benchmark(1)
+    spec = config.workload("gamma32")
     su = make_setup(spec)
     ctx = su.context(precision)
     rng = np.random.default_rng(5)
@@ -258,7 +258,7 @@ def test_lda_matches_libxc_fixture_and_oracle():
 
 
 def test_hartree():
-    su = make_setup(config.benchmark(1))
+    su = make_setup(config.workload("gamma32"))
     ctx = su.context()
     rng = np.random.default_rng(9)
     rho = 0.01 + 0.001 * rng.random(su.N)

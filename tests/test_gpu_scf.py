@@ -29,11 +29,11 @@ def run_both(spec, max_scf=60):
     return res, eg, rho, ro
 
 
-@pytest.mark.parametrize("name", ["cfg1", "small_smeared_k", "many_bands"])
+@pytest.mark.parametrize("name", ["gamma32", "small_smeared_k", "many_bands"])
 def test_scf_matches_oracle(name):
     print("scf", name, flush=True)
-    if name == "cfg1":
-        spec = config.benchmark(1)
+    if name == "gamma32":
+        spec = config.workload("gamma32")
     elif name == "many_bands":
         # 128 bands: the Davidson step multiplies the X^H R / R^H R blocks
         # only (X^H X = I, X^H H X = diag(eps))

@@ -64,14 +64,14 @@ def test_autotune_keeps_results():
     assert np.max(np.abs(e1 - e0)) < 1e-10
 
 
-@pytest.mark.parametrize("name,bf16", [("cfg1", 0), ("kpoints", 0), ("cfg1", 1), ("kpoints", 1), ("cfg1", 2),
+@pytest.mark.parametrize("name,bf16", [("gamma32", 0), ("kpoints", 0), ("gamma32", 1), ("kpoints", 1), ("gamma32", 2),
                                        ("kpoints", 2), ("kpoints", 3)])
 def test_mixed_precision_reaches_fp64_oracle(name, bf16):
     """bf16 > 0: the FP32 phase's overlaps and rotations on the BF16 matrix
     cores (bf16 terms per value; 2 is the default), 0 on the FP32 ones; the
     FP64 refinement lands on the same oracle values."""
     print("mixed", name, "bf16 terms", bf16, flush=True)
-    spec = config.benchmark(1) if name == "cfg1" else kspec()
+    spec = config.workload("gamma32") if name == "gamma32" else kspec()
     su = gscf.Setup(spec)
     ctx = su.context(lib.DEREK_MIXED)
     res = gscf.run(spec, max_scf=80, energy_tol=1e-11, rho_tol=1e-8, setup=su, ctx=ctx, log=print, kbatch=0,
@@ -95,7 +95,7 @@ def test_mixed_precision_reaches_fp64_oracle(name, bf16):
 def test_fp32_step_tracks_fp64_step():
     """The first SCF step from the same seeded start: FP32 bulk vs FP64."""
     print("fp32 step", flush=True)
-    spec = config.benchmark(1)
+    spec = config.workload("gamma32")
     e64, _, _ = energies(spec, 2)
     e32, r32, _ = energies(spec, 2, precision=lib.DEREK_MIXED)
     assert r32["history"][0]["fp32"] == 1

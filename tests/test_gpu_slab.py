@@ -26,11 +26,11 @@ def rnd_c(rng, *shape):
 def grid_spec(n):
     """Systems on an n^3 grid: the benchmark cells, with one or two k-points."""
     if n == 32:
-        return config.benchmark(1)
+        return config.workload("gamma32")
     if n == 72:
         return config.SystemSpec("g72", 16.0, 20.0, 2.2, (2, 1, 1), 32, 24, kT=0.01, nwells=8, seed=4)
     if n == 108:
-        return config.benchmark(2)
+        return config.workload("gamma108")
     return config.SystemSpec("g144", 30.0, 25.0, 2.125, (1, 1, 1), 512, 300, kT=0.002, nwells=64, seed=5)
 
 
@@ -70,7 +70,7 @@ def test_fused_apply_h_matches_oracle(n, precision):
 def test_fused_matches_separate_passes(precision):
     """108^3, 62669 plane waves, 19 bands: the same H|psi> for every launch shape."""
     print("fused vs separate", precision, flush=True)
-    su = gscf.Setup(config.benchmark(2))
+    su = gscf.Setup(config.workload("gamma108"))
     nb = 19
     ctx = su.context(precision, nbands=nb)
     rng = np.random.default_rng(5)
@@ -114,7 +114,7 @@ def test_fused_density_matches_oracle(n, precision):
 
 def test_fused_hermitian_and_kinetic_only():
     """<x|Hy> = conj(<y|Hx>) through the fused pass; V = 0 gives the kinetic term exactly."""
-    su = gscf.Setup(config.benchmark(2))
+    su = gscf.Setup(config.workload("gamma108"))
     ctx = su.context(nbands=2)
     ctx.set_launch(4, 16, fused=True)
     ctx.set_vloc(su.vext)
@@ -130,7 +130,7 @@ def test_fused_hermitian_and_kinetic_only():
 
 
 def test_set_launch_rejects_bad_shapes():
-    su = gscf.Setup(config.benchmark(1))
+    su = gscf.Setup(config.workload("gamma32"))
     ctx = su.context(nbands=2)
     for bpw, items, fused in [(3, 16, 1), (4, 0, 1), (4, 16, 4), (4, 16, -1)]:
         with pytest.raises(lib.DerekError) as e:

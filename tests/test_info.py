@@ -29,7 +29,7 @@ def err(tmp_path, text, safety="Normal"):
 
 
 def test_benchmark_file_round_trip(tmp_path):
-    spec = config.benchmark(4)
+    spec = config.workload("kmesh72")
     inf = lib.Info(write(tmp_path, spec.to_info(energy_tol=1e-8, max_scf=33)))
     assert inf.num("cut_off_energy") == spec.ecut and inf.num("g_fine_scale") == spec.g_fine_scale
     assert list(inf.num("kpt_mp_grid", 3)) == list(spec.mp)

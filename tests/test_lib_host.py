@@ -30,9 +30,9 @@ def test_exports_every_declared_symbol(lib):
     assert L.derek_hip_version() >= 1
 
 
-@pytest.mark.parametrize("i", [1, 2, 4])
-def test_grid_dims_match(lib, i):
-    sp = config.benchmark(i)
+@pytest.mark.parametrize("name", ["gamma32", "gamma108", "kmesh72"])
+def test_grid_dims_match(lib, name):
+    sp = config.workload(name)
     cell = np.diag(sp.lengths)
     assert lib.grid_dims(cell, sp.ecut, sp.g_fine_scale) == tuple(ob.grid_dims(cell, sp.ecut, sp.g_fine_scale)) or \
         list(lib.grid_dims(cell, sp.ecut, sp.g_fine_scale)) == list(ob.grid_dims(cell, sp.ecut, sp.g_fine_scale))
@@ -57,9 +57,9 @@ def test_kpoints_bit_exact(lib, mp):
     assert np.array_equal(lib.kpoints(mp), ob.kpoint_grid(mp))
 
 
-@pytest.mark.parametrize("i", [1, 4])
-def test_sphere_bit_exact(lib, i):
-    sp = config.benchmark(i)
+@pytest.mark.parametrize("name", ["gamma32", "kmesh72"])
+def test_sphere_bit_exact(lib, name):
+    sp = config.workload(name)
     cell = np.diag(sp.lengths)
     inv = ob.inv_cell_orthorhombic(cell)
     _, dims = ob.grid_dims(cell, sp.ecut, sp.g_fine_scale)
@@ -71,7 +71,7 @@ def test_sphere_bit_exact(lib, i):
 
 
 def test_gaussian_pot_matches_oracle(lib):
-    sp = config.benchmark(2)
+    sp = config.workload("gamma108")
     sp2 = config.small_system(seed=7, cos_amp=0.2)
     cell = np.diag(sp2.lengths)
     _, dims = ob.grid_dims(cell, sp2.ecut, sp2.g_fine_scale)

@@ -22,14 +22,15 @@ def test_prime_fact_bruteforce():
         assert basis.prime_fact(n) == m
 
 
-@pytest.mark.parametrize("i,std,fine", [(1, 15, 32), (2, 50, 108), (4, 33, 72), (5, 70, 144)])
-def test_benchmark_grids(i, std, fine):
-    sp = config.benchmark(i)
+@pytest.mark.parametrize("name,std,fine", [("gamma32", 15, 32), ("gamma108", 50, 108), ("kmesh72", 33, 72),
+                                           ("kmesh144", 70, 144)])
+def test_benchmark_grids(name, std, fine):
+    sp = config.workload(name)
     s, f = basis.grid_dims(np.diag(sp.lengths), sp.ecut, sp.g_fine_scale)
     assert s == [std] * 3 and f == [fine] * 3
     # default g_fine_scale = 2.0 gives the reference's own fine grids
     _, f2 = basis.grid_dims(np.diag(sp.lengths), sp.ecut, 2.0)
-    assert f2 == [{1: 30, 2: 99, 4: 65, 5: 140}[i]] * 3
+    assert f2 == [{"gamma32": 30, "gamma108": 99, "kmesh72": 65, "kmesh144": 140}[name]] * 3
 
 
 def test_g_map_order_and_values():
@@ -70,9 +71,9 @@ def test_fft_bin_roundtrip():
     assert sorted(b.tolist()) == list(range(120))
 
 
-@pytest.mark.parametrize("i,npw", [(1, 1503), (2, 62669)])
-def test_sphere_counts(i, npw):
-    sp = config.benchmark(i)
+@pytest.mark.parametrize("name,npw", [("gamma32", 1503), ("gamma108", 62669)])
+def test_sphere_counts(name, npw):
+    sp = config.workload(name)
     cell = np.diag(sp.lengths)
     inv = basis.inv_cell_orthorhombic(cell)
     _, dims = basis.grid_dims(cell, sp.ecut, sp.g_fine_scale)
@@ -83,8 +84,8 @@ def test_sphere_counts(i, npw):
     assert len(set(idx.tolist())) == npw
 
 
-def test_sphere_kpoint_counts_cfg4():
-    sp = config.benchmark(4)
+def test_sphere_kpoint_counts_kmesh72():
+    sp = config.workload("kmesh72")
     cell = np.diag(sp.lengths)
     inv = basis.inv_cell_orthorhombic(cell)
     _, dims = basis.grid_dims(cell, sp.ecut, sp.g_fine_scale)
