@@ -218,12 +218,13 @@ int64_t heev_work_doubles(int n, int nev) {
   return (w + 31) / 32 * 32;
 }
 
-// ---- 1. Householder tridiagonalisation + phases (one workgroup per matrix)
-// Wave 0 forms the reflector of the next column while it applies the
-// rank-2 update to that column (the column stays in its registers), so one
-// step costs three barriers: reflector -> v, v -> u = A v, u -> update.
-// IN_LDS: the whole matrix is copied to LDS first (n <= 99), else it is
-// updated in place in global memory (L2 resident).
+// ---- 1. Householder tridiagonalisation + phases
+// Columns 0 .. n-99 (n > 98) take two multi-workgroup launches each
+// (k_wide_gemv: u = A v; k_wide_update: rank-2 update, and the wavefront of
+// the next column forms its reflector); the trailing block of at most 98
+// columns is copied to LDS and reduced by one workgroup per matrix
+// (k_tridiag), where the wavefront holding column k+1 forms the next
+// reflector while it applies the rank-2 update to that column.
 __device__ __forceinline__ void tri_reflector(int tid, int k, int n, double xn2, double2 x0, int* s_skip,
                                               double* s_scal, HeevWork& W) {
   if (tid != 0) return;
