@@ -1052,9 +1052,11 @@ __global__ __launch_bounds__(DT) void k_chol_drop(const double2* __restrict__ Sa
 // Inverse of a lower-triangular m x m matrix, one wavefront per column j:
 // forward substitution on e_j with the solution and the current column of L
 // in registers (row i lives in lane i % 64, slot i / 64); the pivot and x_k
-// are read across lanes with readlane, and the next column of L is loaded
-// while the current one is applied.
-constexpr int TR = 10;  // rows per lane: m <= 640
+// are read across lanes with readlane.  The columns of L are L2 reads: D of
+// them are kept in flight in a register ring (slot d holds column k0 + d and
+// is refilled with column k0 + d + D once used), so a step waits for a load
+// issued D steps earlier instead of one.  TR = rows per lane (m <= 64 TR).
+constexpr int TR = 10;  // largest rows per lane: m <= 640
 
 __device__ __forceinline__ double readlane_d(double v, int l) {
   const long long b = __double_as_longlong(v);
@@ -1063,52 +1065,63 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// rows i >= c of column c of L (zero elsewhere, and for c >= m)
+template <int NR>
+__device__ __forceinline__ void trinv_load(const double2* __restrict__ L, int m, int c, int lane, double2 (&col)[NR]) {
+#pragma unroll
+  for (int t = 0; t < NR; ++t) {
+    const int i = lane + 64 * t;
+    col[t] = (c < m && i >= c && i < m) ? L[i + (int64_t)c * m] : make_double2(0.0, 0.0);
+  }
+}
+
+template <int NR, int D>
 __global__ __launch_bounds__(64) void k_trinv(const double2* __restrict__ Lall, int m,
                                               double2* __restrict__ Linvall) {
   const int j = blockIdx.x, lane = threadIdx.x;
   const int64_t mm = (int64_t)m * m;
   const double2* L = Lall + blockIdx.y * mm;
-  double2 x[TR], cur[TR];
+  double2 x[NR], ring[D][NR];
 #pragma unroll
-  for (int t = 0; t < TR; ++t) {
+  for (int t = 0; t < NR; ++t) {
     const int i = lane + 64 * t;
     x[t] = make_double2(i == j ? 1.0 : 0.0, 0.0);
-    cur[t] = (i >= j && i < m) ? L[i + (int64_t)j * m] : make_double2(0.0, 0.0);
   }
-  for (int k = j; k < m; ++k) {
-    double2 nxt[TR];
 #pragma unroll
-    for (int t = 0; t < TR; ++t) {
-      const int i = lane + 64 * t;
-      nxt[t] = (i > k && i < m) ? L[i + (int64_t)(k + 1) * m] : make_double2(0.0, 0.0);
-    }
-    const int tk = k >> 6, lk = k & 63;
-    double2 xo = make_double2(0.0, 0.0), dg = make_double2(1.0, 0.0);
+  for (int d = 0; d < D; ++d) trinv_load<NR>(L, m, j + d, lane, ring[d]);
+  for (int k0 = j; k0 < m; k0 += D) {
 #pragma unroll
-    for (int t = 0; t < TR; ++t)
-      if (t == tk) {
-        xo = x[t];
-        dg = cur[t];
+    for (int d = 0; d < D; ++d) {
+      const int k = k0 + d;
+      if (k >= m) break;  // uniform over the wavefront
+      const int tk = k >> 6, lk = k & 63;
+      double2 xo = make_double2(0.0, 0.0), dg = make_double2(1.0, 0.0);
+#pragma unroll
+      for (int t = 0; t < NR; ++t)
+        if (t == tk) {
+          xo = x[t];
+          dg = ring[d][t];
+        }
+      xo = make_double2(readlane_d(xo.x, lk), readlane_d(xo.y, lk));
+      dg = make_double2(readlane_d(dg.x, lk), readlane_d(dg.y, lk));
+      const double den = 1.0 / (dg.x * dg.x + dg.y * dg.y);
+      const double2 xk = make_double2((xo.x * dg.x + xo.y * dg.y) * den, (xo.y * dg.x - xo.x * dg.y) * den);
+#pragma unroll
+      for (int t = 0; t < NR; ++t) {
+        const int i = lane + 64 * t;
+        if (i > k) {
+          const double2 u = zmul(ring[d][t], xk);
+          x[t] = make_double2(x[t].x - u.x, x[t].y - u.y);
+        } else if (i == k) {
+          x[t] = xk;
+        }
       }
-    xo = make_double2(readlane_d(xo.x, lk), readlane_d(xo.y, lk));
-    dg = make_double2(readlane_d(dg.x, lk), readlane_d(dg.y, lk));
-    const double den = 1.0 / (dg.x * dg.x + dg.y * dg.y);
-    const double2 xk = make_double2((xo.x * dg.x + xo.y * dg.y) * den, (xo.y * dg.x - xo.x * dg.y) * den);
-#pragma unroll
-    for (int t = 0; t < TR; ++t) {
-      const int i = lane + 64 * t;
-      if (i > k) {
-        const double2 u = zmul(cur[t], xk);
-        x[t] = make_double2(x[t].x - u.x, x[t].y - u.y);
-      } else if (i == k) {
-        x[t] = xk;
-      }
-      cur[t] = nxt[t];
+      trinv_load<NR>(L, m, k + D, lane, ring[d]);
     }
   }
   double2* out = Linvall + blockIdx.y * mm + (int64_t)j * m;
 #pragma unroll
-  for (int t = 0; t < TR; ++t) {
+  for (int t = 0; t < NR; ++t) {
     const int i = lane + 64 * t;
     if (i < m) out[i] = x[t];
   }
@@ -1234,7 +1247,13 @@ void chol_drop_gpu(const double2* S, int m, double tol, int nev, double2* Wm, do
 
 void trinv_gpu(const double2* L, int m, double2* Linv, int batch, hipStream_t s) {
   DEREK_REQUIRE(m <= 64 * TR, "trinv: matrix too large");
-  hipLaunchKernelGGL(k_trinv, dim3((unsigned)m, batch), dim3(64), 0, s, L, m, Linv);
+  const dim3 grid((unsigned)m, batch);
+  if (m <= 64)
+    hipLaunchKernelGGL((k_trinv<1, 8>), grid, dim3(64), 0, s, L, m, Linv);
+  else if (m <= 192)
+    hipLaunchKernelGGL((k_trinv<3, 8>), grid, dim3(64), 0, s, L, m, Linv);
+  else
+    hipLaunchKernelGGL((k_trinv<TR, 4>), grid, dim3(64), 0, s, L, m, Linv);
   DEREK_HIP(hipGetLastError());
 }
 
