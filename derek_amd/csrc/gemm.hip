@@ -135,6 +135,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const int64_t r1 = std::min<int64_t>(npw, r0 + g.kch);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wi = wave & 1, wj = wave >> 1;
+  // 16 x 16 blocks wholly outside C (their operands are zero padding and
+  // their results are never stored) skip the matrix cores; wave-uniform
+  bool live[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) live[a][b] = i0 + wi * 32 + a * 16 < g.m && j0 + wj * 32 + b * 16 < g.n;
   A4 p1[2][2], p2[2][2], p3[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
@@ -189,11 +196,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
       for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          p1[a][b] = M::op(ar[a], br[b], p1[a][b]);
-          p2[a][b] = M::op(ai[a], bi[b], p2[a][b]);
-          p3[a][b] = M::op(as[a], bs[b], p3[a][b]);
-        }
+        for (int b = 0; b < 2; ++b)
+          if (live[a][b]) {
+            p1[a][b] = M::op(ar[a], br[b], p1[a][b]);
+            p2[a][b] = M::op(ai[a], bi[b], p2[a][b]);
+            p3[a][b] = M::op(as[a], bs[b], p3[a][b]);
+          }
     }
     __syncthreads();
   }
@@ -303,6 +311,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const int64_t r1 = std::min<int64_t>(npw, r0 + g.kch);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wi = wave & 1, wj = wave >> 1;
+  // 16 x 16 blocks wholly outside C (their operands are zero padding and
+  // their results are never stored) skip the matrix cores; wave-uniform
+  bool live[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) live[a][b] = i0 + wi * 32 + a * 16 < g.m && j0 + wj * 32 + b * 16 < g.n;
   f4 p1[2][2], p2[2][2], p3[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
@@ -361,11 +376,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
         for (int q = 0; q < NT; ++q) A[p][q] = lx[p][q][ii][kg];
 #pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        p1[a][b] = bf_products<NT>(A[0], B[b][0], p1[a][b]);
-        p2[a][b] = bf_products<NT>(A[1], B[b][1], p2[a][b]);
-        p3[a][b] = bf_products<NT>(A[2], B[b][2], p3[a][b]);
-      }
+      for (int b = 0; b < 2; ++b)
+        if (live[a][b]) {
+          p1[a][b] = bf_products<NT>(A[0], B[b][0], p1[a][b]);
+          p2[a][b] = bf_products<NT>(A[1], B[b][1], p2[a][b]);
+          p3[a][b] = bf_products<NT>(A[2], B[b][2], p3[a][b]);
+        }
     }
     __syncthreads();
   }
@@ -517,6 +533,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   U += kz * su;
   Z += kz * sz;
   const int j0 = (int)(w % ncb) * RT_COLS;
+  // 16-column blocks past n, and a wavefront whose 16 rows are all past the
+  // k-point's plane waves, skip the matrix cores (wave-uniform)
+  bool live[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) live[b] = j0 + b * 16 < n && rb * RT_ROWS + wave * 16 < nrow;
   const int64_t grow = g0 + (lane & 15);
   const bool rok = grow < nrow;
   A4 p1[4], p2[4], p3[4];
@@ -558,18 +579,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     for (int kk = 0; kk < RT_KS / 4; ++kk) xv[kk] = pxv[kk];
     __syncthreads();
     if (k0 + RT_KS < m) gload(k0 + RT_KS);
+    R xs[RT_KS / 4];
 #pragma unroll
-    for (int kk = 0; kk < RT_KS / 4; ++kk) {
-      const int k = kk * 4 + (lane >> 4);
-      const R xs = xv[kk].x + xv[kk].y;
+    for (int kk = 0; kk < RT_KS / 4; ++kk) xs[kk] = xv[kk].x + xv[kk].y;
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
+    for (int b = 0; b < 4; ++b)
+      if (live[b]) {
         const int c = b * 16 + (lane & 15);
-        p1[b] = M::op(xv[kk].x, ur[k][c], p1[b]);
-        p2[b] = M::op(xv[kk].y, ui[k][c], p2[b]);
-        p3[b] = M::op(xs, us[k][c], p3[b]);
+#pragma unroll
+        for (int kk = 0; kk < RT_KS / 4; ++kk) {
+          const int k = kk * 4 + (lane >> 4);
+          p1[b] = M::op(xv[kk].x, ur[k][c], p1[b]);
+          p2[b] = M::op(xv[kk].y, ui[k][c], p2[b]);
+          p3[b] = M::op(xs[kk], us[k][c], p3[b]);
+        }
       }
-    }
     __syncthreads();
   }
 #pragma unroll
@@ -605,6 +629,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   U += kz * su;
   Z += kz * sz;
   const int j0 = (int)(w % ncb) * RT_COLS;
+  // 16-column blocks past n, and a wavefront whose 16 rows are all past the
+  // k-point's plane waves, skip the matrix cores (wave-uniform)
+  bool live[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) live[b] = j0 + b * 16 < n && rb * RT_ROWS + wave * 16 < nrow;
   const int64_t grow = rb * RT_ROWS + wave * 16 + (lane & 15);
   const bool rok = grow < nrow;
   const int kg = lane >> 4;
@@ -661,17 +690,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     __syncthreads();
     if (k0 + BF_KS < m) gload(k0 + BF_KS);
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int c = b * 16 + (lane & 15);
-      bf8 B[3][NT];
+    for (int b = 0; b < 4; ++b)
+      if (live[b]) {
+        const int c = b * 16 + (lane & 15);
+        bf8 B[3][NT];
 #pragma unroll
-      for (int p = 0; p < 3; ++p)
+        for (int p = 0; p < 3; ++p)
 #pragma unroll
-        for (int q = 0; q < NT; ++q) B[p][q] = lu[p][q][c][kg];
-      p1[b] = bf_products<NT>(A[0], B[0], p1[b]);
-      p2[b] = bf_products<NT>(A[1], B[1], p2[b]);
-      p3[b] = bf_products<NT>(A[2], B[2], p3[b]);
-    }
+          for (int q = 0; q < NT; ++q) B[p][q] = lu[p][q][c][kg];
+        p1[b] = bf_products<NT>(A[0], B[0], p1[b]);
+        p2[b] = bf_products<NT>(A[1], B[1], p2[b]);
+        p3[b] = bf_products<NT>(A[2], B[2], p3[b]);
+      }
     __syncthreads();
   }
 #pragma unroll
