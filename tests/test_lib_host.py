@@ -94,3 +94,14 @@ def test_host_argument_errors(lib):
     with pytest.raises(lib.DerekError) as e:
         lib.inv_cell(cell)
     assert "orthorhombic" in str(e.value)
+
+
+def test_null_context_refused(lib):
+    """Entry points taking a context refuse a null one on the host, before
+    any HIP call (here: the launch-shape and GEMM-precision setters)."""
+    L = lib.load()
+    assert L.derek_hip_set_gemm(None, 2) == lib.DEREK_ERR_ARG
+    assert L.derek_hip_set_launch(None, 2, 256, 3) == lib.DEREK_ERR_ARG
+    buf = C.create_string_buffer(256)
+    L.derek_hip_last_error(buf, C.c_int64(256))
+    assert b"null" in buf.value
