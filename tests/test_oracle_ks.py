@@ -137,3 +137,42 @@ def test_bf16_split_products_accuracy():
     assert all(e > 1e-4 for e in errs[1]) and all(e < 2e-2 for e in errs[1])
     assert all(1e-9 < e < 1e-4 for e in errs[2])
     assert all(e < 1e-6 for e in errs[3])
+
+
+def test_bf16_split_products_explicit_loops():
+    """overlap_bf16 / rotate_bf16 against explicit loops over rows, terms and
+    Gauss planes on a tiny case (2 x 3 x 5), exact in float64."""
+    rng = np.random.default_rng(21)
+    X = (rng.normal(size=(2, 5)) + 1j * rng.normal(size=(2, 5))).astype(np.complex64)
+    Y = (rng.normal(size=(3, 5)) + 1j * rng.normal(size=(3, 5))).astype(np.complex64)
+    U = (rng.normal(size=(2, 3)) + 1j * rng.normal(size=(2, 3))).astype(np.complex64)
+
+    def terms(v, nt):
+        out, r = [], np.float32(v)
+        for _ in range(nt):
+            t = ks.bf16_round(np.array([r], dtype=np.float32))[0]
+            out.append(float(t))
+            r = np.float32(r - t)
+        return out
+
+    def prod(a, b, nt):
+        ta, tb = terms(a, nt), terms(b, nt)
+        return sum(ta[p] * tb[q] for p in range(nt) for q in range(nt) if p + q < nt)
+
+    for nt in (1, 2, 3):
+        S = ks.overlap_bf16(X, Y, nt)
+        for i in range(2):
+            for j in range(3):
+                p1 = sum(prod(X[i, g].real, Y[j, g].real, nt) for g in range(5))
+                p2 = sum(prod(X[i, g].imag, Y[j, g].imag, nt) for g in range(5))
+                p3 = sum(prod(np.float32(X[i, g].real - X[i, g].imag), np.float32(Y[j, g].real + Y[j, g].imag), nt)
+                         for g in range(5))
+                assert abs(S[i, j] - complex(p1 + p2, p3 - p1 + p2)) < 1e-12
+        Z = ks.rotate_bf16(X, U, nt)
+        for j in range(3):
+            for g in range(5):
+                p1 = sum(prod(U[i, j].real, X[i, g].real, nt) for i in range(2))
+                p2 = sum(prod(U[i, j].imag, X[i, g].imag, nt) for i in range(2))
+                p3 = sum(prod(np.float32(U[i, j].real + U[i, j].imag), np.float32(X[i, g].real + X[i, g].imag), nt)
+                         for i in range(2))
+                assert abs(Z[j, g] - complex(p1 - p2, p3 - p1 - p2)) < 1e-12
