@@ -85,7 +85,10 @@ Profiler::~Profiler() {
 }  // namespace derek
 
 // ------------------------------------------------------------------ context
-derek_ctx::derek_ctx() = default;
+derek_ctx::derek_ctx() {
+  for (derek::DevBuf* b : {&S64, &P64, &S32, &P32, &rho_part, &tA, &tB, &tC, &tD, &ovwork, &partial})
+    b->epoch = &graph_epoch;
+}
 
 derek_ctx::~derek_ctx() {
   if (s) (void)hipStreamSynchronize(s);
@@ -737,6 +740,52 @@ int32_t derek_hip_fft3d(derek_ctx* c, double* grid, int32_t howmany, int32_t dir
   });
 }
 
+int32_t derek_hip_fft1d(derek_ctx* c, double* data, int32_t n, int64_t howmany, int32_t dir) {
+  return guard([&] {
+    DEREK_REQUIRE(c && data, "null argument");
+    DEREK_REQUIRE(dir == 1 || dir == -1, "direction must be +1 or -1");
+    DEREK_REQUIRE(howmany >= 1, "howmany must be >= 1");
+    int a = -1;
+    for (int q = 2; q >= 0; --q)
+      if (c->dims[q] == n) a = q;
+    DEREK_REQUIRE(a >= 0, "fft1d length " + std::to_string(n) + " is not one of the context's grid lengths");
+    const size_t len = (size_t)howmany * n;
+    const int sign = dir == 1 ? -1 : +1;
+    double2* d = c->tA.get<double2>(len);
+    DEREK_HIP(hipMemcpyAsync(d, data, len * sizeof(double2), hipMemcpyHostToDevice, c->s));
+    if (c->prec == DEREK_FP64) {
+      c->timed(KC_FFT_Z, 2.0 * len * sizeof(double2), 0, [&] { fft_z<double>(d, howmany, c->ax64[a], sign, c->s); });
+    } else {
+      float2* f = c->tB.get<float2>(len);
+      convert<float, double>(f, d, (int64_t)len, c->s);
+      c->timed(KC_FFT_Z, 2.0 * len * sizeof(float2), 0, [&] { fft_z<float>(f, howmany, c->ax32[a], sign, c->s); });
+      convert<double, float>(d, f, (int64_t)len, c->s);
+    }
+    DEREK_HIP(hipMemcpyAsync(data, d, len * sizeof(double2), hipMemcpyDeviceToHost, c->s));
+    c->sync();
+  });
+}
+
+int32_t derek_hip_wave_axpby(derek_ctx* c, int64_t n, const double a[2], const double* x, const double b[2],
+                             const double* y, double* z) {
+  return guard([&] {
+    DEREK_REQUIRE(c && a && b && x && z, "null argument");
+    DEREK_REQUIRE(n >= 1, "n must be >= 1");
+    const bool use_y = b[0] != 0.0 || b[1] != 0.0;
+    DEREK_REQUIRE(!use_y || y, "y is needed when b != 0");
+    double2* dx = c->tA.get<double2>(n);
+    double2* dy = use_y ? c->tB.get<double2>(n) : nullptr;
+    double2* dz = c->tC.get<double2>(n);
+    DEREK_HIP(hipMemcpyAsync(dx, x, n * sizeof(double2), hipMemcpyHostToDevice, c->s));
+    if (use_y) DEREK_HIP(hipMemcpyAsync(dy, y, n * sizeof(double2), hipMemcpyHostToDevice, c->s));
+    c->timed(KC_OTHER, (use_y ? 3.0 : 2.0) * n * sizeof(double2), 0, [&] {
+      wave_axpby(dz, make_double2(a[0], a[1]), dx, make_double2(b[0], b[1]), dy, n, c->s);
+    });
+    DEREK_HIP(hipMemcpyAsync(z, dz, n * sizeof(double2), hipMemcpyDeviceToHost, c->s));
+    c->sync();
+  });
+}
+
 int32_t derek_hip_apply_h(derek_ctx* c, int32_t k, int32_t nb, const double* psi, double* hpsi) {
   return guard([&] {
     DEREK_REQUIRE(c && psi && hpsi, "null argument");
@@ -1057,6 +1106,7 @@ int32_t derek_hip_set_launch(derek_ctx* c, int32_t bands_per_wg, int32_t items_p
     c->bpw = bands_per_wg;
     c->item_cap = items_per_launch;
     c->fused = fused;
+    ++c->graph_epoch;
   });
 }
 
@@ -1065,6 +1115,7 @@ int32_t derek_hip_set_gemm(derek_ctx* c, int32_t bf16_terms) {
     DEREK_REQUIRE(c, "null context");
     DEREK_REQUIRE(bf16_terms >= 0 && bf16_terms <= 3, "bf16 terms must be 0 (FP32 matrix cores) or 1..3");
     c->gemm_bf16 = bf16_terms;
+    ++c->graph_epoch;
   });
 }
 

@@ -15,16 +15,22 @@
 
 namespace derek {
 
-// Device buffer that grows on demand.
+// Device buffer that grows on demand.  Every free or reallocation bumps
+// *epoch (when set): captured SCF graphs hold these addresses, and the
+// engine drops its graphs when the epoch it captured them at has moved.
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  uint64_t* epoch = nullptr;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      (void)hipFree(p);
+      if (epoch) ++*epoch;
+    }
     p = nullptr;
     bytes = 0;
   }
@@ -34,6 +40,7 @@ struct DevBuf {
       release();
       DEREK_HIP(hipMalloc(&p, need));
       bytes = need;
+      if (epoch) ++*epoch;
     }
     return reinterpret_cast<T*>(p);
   }
@@ -143,6 +150,10 @@ struct derek_ctx {
   // 1..3 on the BF16 ones with each value split into that many bf16 terms
   // (set to 2 at init for DEREK_MIXED contexts)
   int gemm_bf16 = 0;
+  // Moves whenever a work buffer below is freed or reallocated, or a launch
+  // setting changes (set_launch, set_gemm): a graph captured at another
+  // epoch may hold freed addresses or stale shapes and is not replayed.
+  uint64_t graph_epoch = 0;
   // pruned-FFT work buffers for item_cap items
   derek::DevBuf S64, P64, S32, P32;
   derek::DevBuf rho_part;  // partial densities of the fused density pass

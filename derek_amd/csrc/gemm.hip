@@ -20,6 +20,7 @@
 //   D (16x16), f64: register r holds D[l/16 + 4*r][l%16]
 //   D (16x16), f32: register r holds D[4*(l/16) + r][l%16]
 #include "launch.hpp"
+#include "ov_geom.hpp"
 
 namespace derek {
 
@@ -42,15 +43,7 @@ template <> struct Mfma<float> {
   __device__ static __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-constexpr int OV_T = 64;     // C tile edge
-constexpr int OV_KS = 16;    // rows per LDS slab
 constexpr int OV_PAD = 66;   // LDS row pitch
-
-struct OvGeom {
-  int m, n, tm, tn, ntiles, nchunk;
-  int64_t npw, kch;
-  bool herm;
-};
 
 constexpr int kXcds = 8;  // MI355X: 8 XCDs, each with its own L2
 
@@ -62,39 +55,6 @@ __device__ __forceinline__ int64_t xcd_item(int64_t L, int64_t total) {
   const int64_t q = total / kXcds, r = total % kXcds;
   const int64_t x = L % kXcds, slot = L / kXcds;
   return x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
-}
-
-// Row chunks per tile: at least two full rounds of workgroups over the
-// chip's resident slots (`slots` = resident workgroups per CU x CUs), then,
-// among the next few counts, the one that leaves the smallest idle tail in
-// the last round; every chunk keeps at least 256 rows.
-static OvGeom ov_geom(int m, int n, int64_t npw, bool herm, int batch, int slots, int ks = OV_KS) {
-  OvGeom g{};
-  g.m = m;
-  g.n = n;
-  g.npw = npw;
-  g.herm = herm;
-  g.tm = (int)ceil_div(m, OV_T);
-  g.tn = (int)ceil_div(n, OV_T);
-  g.ntiles = herm ? g.tm * (g.tm + 1) / 2 : g.tm * g.tn;
-  const int64_t units = (int64_t)g.ntiles * std::max(1, batch);
-  const int64_t cmax = std::max<int64_t>(1, npw / 256);
-  const int64_t c0 = std::min(cmax, std::max<int64_t>(1, ceil_div(2 * (int64_t)slots, units)));
-  int64_t best = c0;
-  double best_eff = -1.0;
-  for (int64_t c = c0; c <= std::min(cmax, c0 + 16); ++c) {
-    const int64_t w = units * c;
-    const double eff = (double)w / (double)(ceil_div(w, (int64_t)slots) * slots);
-    if (eff > best_eff + 1e-9) {
-      best_eff = eff;
-      best = c;
-    }
-  }
-  int64_t kch = ceil_div(std::max<int64_t>(npw, 1), best);
-  kch = std::max<int64_t>(ks, ceil_div(kch, ks) * ks);
-  g.kch = kch;
-  g.nchunk = (int)std::max<int64_t>(1, ceil_div(npw, kch));
-  return g;
 }
 
 // at least two waves per SIMD (register budget 256 per lane incl. accumulators)
@@ -233,7 +193,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 // slot], slots contiguous, so a lane's 8 values are one 16-byte read).
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
-constexpr int BF_KS = 32;  // rows (k) per slab = one MFMA
 constexpr int BF_P = 5;    // LDS pitch of a column in bf8 units (80 bytes: 16 lanes hit 64 distinct banks)
 
 __device__ __forceinline__ f4 mfma_bf16(bf8 a, bf8 b, f4 c) {
@@ -419,11 +378,25 @@ template <int NT> static int ov_slots_bf16() {
   return slots;
 }
 
+// resident slots of every overlap kernel (FP64, FP32, BF16 with 1..3 terms)
+static void overlap_slots(int out[5]) {
+  out[0] = ov_slots<double>();
+  out[1] = ov_slots<float>();
+  out[2] = ov_slots_bf16<1>();
+  out[3] = ov_slots_bf16<2>();
+  out[4] = ov_slots_bf16<3>();
+}
+
 int64_t overlap_work_elems(int m, int n, int64_t npw, int batch) {
-  int64_t chunks = 1;
-  for (int slots : {ov_slots<double>(), ov_slots<float>(), ov_slots_bf16<1>(), ov_slots_bf16<2>(), ov_slots_bf16<3>()})
-    for (bool h : {false, m == n}) chunks = std::max<int64_t>(chunks, ov_geom(m, n, npw, h, batch, slots).nchunk);
-  return (int64_t)batch * chunks * m * n;
+  int slots[5];
+  overlap_slots(slots);
+  return ov_work_elems(m, n, npw, batch, slots, 5);
+}
+
+int64_t engine_overlap_work_elems(int nb, int nk, int kc, int64_t npw) {
+  int slots[5];
+  overlap_slots(slots);
+  return ov_engine_work_elems(nb, nk, kc, npw, slots, 5);
 }
 
 // C[kz] = sum over chunks.  A workgroup takes RD_E consecutive elements;

@@ -46,6 +46,7 @@ void dot_partials(const double* a, const double* b, double* partial, int64_t n, 
 int dot_nblocks(int64_t n);
 void axpy(double* y, const double* x, double a, int64_t n, hipStream_t s);
 void sub(double* out, const double* a, const double* b, int64_t n, hipStream_t s);
+void wave_axpby(double2* z, double2 a, const double2* x, double2 b, const double2* y, int64_t n, hipStream_t s);
 
 template <typename Rd, typename Rs>
 void convert(cx_t<Rd>* dst, const cx_t<Rs>* src, int64_t n, hipStream_t s);
@@ -63,6 +64,9 @@ void transpose_xz(const double* v, double* t64, float* t32, int nx, int ny, int 
 // 1..3 on the BF16 ones with every FP32 value split into that many bf16
 // terms (1: plain BF16, 2: ~16 significant bits, 3: ~FP32), FP32 accumulation.
 int64_t overlap_work_elems(int m, int n, int64_t npw, int batch);
+// the SCF engine's partial-sum buffer: every nb x nb and 2nb x 2nb overlap
+// over batches of kc (and the last nk mod kc) k-points
+int64_t engine_overlap_work_elems(int nb, int nk, int kc, int64_t npw);
 template <typename R>
 void overlap(const cx_t<R>* X, int64_t ldx, int64_t sx, const cx_t<R>* Y, int64_t ldy, int64_t sy, int m, int n,
              int64_t npw, const int* npwk, double2* C, int ldc, int64_t sc, int batch, double2* work,

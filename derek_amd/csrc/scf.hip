@@ -49,12 +49,21 @@ struct ScfEngine {
   double *hin = nullptr, *hres = nullptr, *mixA = nullptr, *coef = nullptr, *mix_partial = nullptr;
   double* rep = nullptr;
   double* rep_h = nullptr;  // pinned
+  // One executable graph per step shape, with derek_ctx::graph_epoch at the
+  // time it was made; a graph whose epoch differs from the current one may
+  // address freed work buffers or launch stale shapes and is not replayed.
   struct Graph {
     int nit;
     bool fp32, full;
     hipGraphExec_t exec;
+    uint64_t epoch;
   };
   std::vector<Graph> cache;
+  void drop_graphs() {
+    for (auto& g : cache)
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    cache.clear();
+  }
   std::vector<void*> owned;
   template <typename T> T* alloc(size_t n) {
     T* p = dmalloc<T>(n);
@@ -68,8 +77,7 @@ struct ScfEngine {
     }
   }
   ~ScfEngine() {
-    for (auto& g : cache)
-      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    drop_graphs();
     free_psi();
     for (void* p : owned) dfree(p);
     if (rep_h) (void)hipHostFree(rep_h);
@@ -294,12 +302,30 @@ void run_step(derek_ctx* c, int nit) {
   const bool full = E.full_overlaps;
   for (auto& e : E.cache)
     if (e.nit == nit && e.fp32 == E.fp32 && e.full == full) g = &e;
-  if (!g) {
+  if (g && g->exec && g->epoch != c->graph_epoch) {
+    // a work buffer was reallocated or freed, or a launch setting changed,
+    // since this graph was made: make it again
+    (void)hipGraphExecDestroy(g->exec);
+    g->exec = nullptr;
+    g->epoch = ~(uint64_t)0;
+  }
+  // Eager step: the first use of a shape, or its first use since its graph
+  // went stale.  It allocates what the shape needs, so the capture at the
+  // shape's next use sees no allocation.
+  auto eager = [&] {
     enqueue();
-    E.cache.push_back({nit, E.fp32, full, nullptr});
+    if (!g) {
+      E.cache.push_back({nit, E.fp32, full, nullptr, 0});
+      g = &E.cache.back();
+    }
+    g->epoch = c->graph_epoch;
+  };
+  if (!g || (!g->exec && g->epoch == ~(uint64_t)0)) {
+    eager();
     return;
   }
   if (!g->exec) {
+    const uint64_t e0 = c->graph_epoch;
     c->prof.paused = true;
     hipGraph_t graph = nullptr;
     DEREK_HIP(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
@@ -314,9 +340,17 @@ void run_step(derek_ctx* c, int nit) {
     }
     DEREK_HIP(hipStreamEndCapture(c->s, &graph));
     c->prof.paused = false;
+    if (c->graph_epoch != e0) {
+      // a work buffer was (re)allocated while the step was recorded
+      // (nothing ran): run the step eagerly, capture at the next use
+      (void)hipGraphDestroy(graph);
+      eager();
+      return;
+    }
     const hipError_t e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     DEREK_HIP(e);
+    g->epoch = e0;
   }
   DEREK_HIP(hipGraphLaunch(g->exec, c->s));
 }
@@ -436,7 +470,18 @@ void scf_init(derek_ctx* c, const derek_scf_params& p) {
   DEREK_REQUIRE(p.kT >= 0.0, "kT must be >= 0");
   DEREK_REQUIRE(p.history >= 1 && p.history <= 16, "Pulay history must be 1..16");
   DEREK_REQUIRE(p.n_david_first >= 0 && p.n_david >= 0, "Davidson step counts must be >= 0");
-  DEREK_REQUIRE(p.kT > 0.0 || std::lround(p.nelec / 2.0) <= c->nb, "more occupied bands than bands");
+  // fixed occupations fill nelec/2 bands with two electrons each; Fermi-Dirac
+  // occupations need room for nelec in 2 * nbands * sum_k w_k
+  if (p.kT == 0.0) {
+    DEREK_REQUIRE(std::fabs(p.nelec / 2.0 - std::nearbyint(p.nelec / 2.0)) < 1e-9,
+                  "fixed occupations (kT = 0) need an even number of electrons");
+    DEREK_REQUIRE(std::lround(p.nelec / 2.0) <= c->nb, "more occupied bands than bands");
+  } else {
+    double wsum = 0.0;
+    for (const KState& K : c->ks) wsum += K.weight;
+    DEREK_REQUIRE(p.nelec <= 2.0 * c->nb * wsum * (1.0 + 1e-12),
+                  "nelec exceeds what the bands hold (2 * nbands * sum of k-point weights)");
+  }
   DEREK_REQUIRE(p.fp32_switch_tol >= 0.0 && p.refine_steps >= 0 && p.kbatch >= 0, "negative tuning parameter");
   DEREK_REQUIRE(2 * c->nb <= 640, "2 * nbands must be <= 640");
   if (c->s) c->sync();
@@ -467,9 +512,7 @@ void scf_init(derek_ctx* c, const derek_scf_params& p) {
   E.dw.ensure(E.m2, E.m2, E.kc);
   // partial sums of every overlap shape the engine launches: nb x nb and
   // 2nb x 2nb, for full batches and the last, shorter one
-  for (int b : {E.kc, nk % E.kc})
-    if (b > 0)
-      for (int m : {nb, E.m2}) E.ovw_cap = std::max(E.ovw_cap, overlap_work_elems(m, m, c->max_npw, b));
+  E.ovw_cap = engine_overlap_work_elems(nb, nk, E.kc, c->max_npw);
   E.ovw = E.alloc<double2>(E.ovw_cap);
   for (double** q : {&E.eps, &E.ekin, &E.rn, &E.occ, &E.occw}) *q = E.alloc<double>((size_t)nk * nb);
   E.stats_partial = E.alloc<double>((size_t)E.kc * nb * band_stats_chunks(c->max_npw) * 2);

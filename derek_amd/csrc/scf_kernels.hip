@@ -225,17 +225,35 @@ __global__ __launch_bounds__(ONT) void k_occupations(const double* __restrict__ 
     }
     return;
   }
-  if (tid == 0) {
+  {
+    // bracket [min eps - 50 kT, max eps + 50 kT]: strided min / max per
+    // thread, then over the wavefront and the workgroup
+    __shared__ double s_min[OW], s_max[OW];
     double lo = 1e300, hi = -1e300;
-    for (int i = 0; i < n; ++i) {
+    for (int i = tid; i < n; i += ONT) {
       lo = fmin(lo, eps[i]);
       hi = fmax(hi, eps[i]);
     }
-    s_lo = lo - 50 * kT;
-    s_hi = hi + 50 * kT;
-    s_done = 0;
+    for (int o = 32; o > 0; o >>= 1) {
+      lo = fmin(lo, __shfl_xor(lo, o, 64));
+      hi = fmax(hi, __shfl_xor(hi, o, 64));
+    }
+    if (lane == 0) {
+      s_min[wid] = lo;
+      s_max[wid] = hi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      for (int w = 1; w < OW; ++w) {
+        lo = fmin(lo, s_min[w]);
+        hi = fmax(hi, s_max[w]);
+      }
+      s_lo = lo - 50 * kT;
+      s_hi = hi + 50 * kT;
+      s_done = 0;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   // count(lo) < nelec <= count(hi); count increases with mu
   for (int it = 0; it < 64; ++it) {
     const double lo = s_lo, hi = s_hi;

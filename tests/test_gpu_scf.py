@@ -50,6 +50,30 @@ def test_scf_matches_oracle(name):
     assert np.max(np.abs(rho - ro["rho_out"])) < 1e-6 * np.max(ro["rho_out"])
 
 
+def test_smearing_terms_match_oracle():
+    """The device's occupations, Fermi level and -TS (reported with every
+    step) against the oracle's fermi_occupations, whose -TS is pinned by the
+    grand-potential identity, evaluated on the device's own eigenvalues."""
+    from oracle import ks as oks
+    print("smearing terms", flush=True)
+    spec = config.small_system(seed=13, mp=(2, 2, 1), kT=0.01, nelec=6, nbands=8)
+    su = gscf.Setup(spec)
+    ctx = su.context()
+    ctx.scf_init(spec.nelec, kT=spec.kT, seed=spec.seed)
+    for _ in range(3):
+        r = ctx.scf_step()
+    eg = ctx.eigenvalues()
+    occ = ctx.occupations()
+    ctx.close()
+    occ_o, mu_o, mts_o = oks.fermi_occupations(eg, su.weights, spec.nelec, spec.kT)
+    print(f"-TS device {r['minus_ts']:.15f} oracle {mts_o:.15f}; mu {r['fermi']:.12f} / {mu_o:.12f}")
+    assert r["minus_ts"] < 0.0
+    assert abs(r["minus_ts"] - mts_o) < 1e-12
+    assert abs(r["fermi"] - mu_o) < 1e-9
+    assert np.max(np.abs(occ - occ_o)) < 1e-9
+    assert abs(float(np.sum(su.weights[:, None] * occ)) - spec.nelec) < 1e-10
+
+
 def test_scf_errors():
     spec = config.small_system(seed=3)
     su = gscf.Setup(spec)
@@ -60,4 +84,14 @@ def test_scf_errors():
     with pytest.raises(lib.DerekError) as e:
         ctx.scf_init(nelec=100.0)
     assert e.value.code == lib.DEREK_ERR_ARG
+    # fixed occupations need an even electron count (3 electrons would fill 2 bands)
+    with pytest.raises(lib.DerekError) as e:
+        ctx.scf_init(nelec=3.0)
+    assert e.value.code == lib.DEREK_ERR_ARG and "even" in str(e.value)
+    # smeared occupations: at most 2 electrons per band (6 bands, one k-point of weight 1)
+    with pytest.raises(lib.DerekError) as e:
+        ctx.scf_init(nelec=12.5, kT=0.01)
+    assert e.value.code == lib.DEREK_ERR_ARG
+    ctx.scf_init(nelec=12.0, kT=0.01)
+    ctx.scf_init(nelec=3.0, kT=0.01)
     ctx.close()

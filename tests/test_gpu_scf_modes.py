@@ -54,6 +54,41 @@ def test_graph_replay_matches_eager():
     assert np.max(np.abs(e1 - e0)) < 1e-11
 
 
+def test_graph_survives_buffer_growth_and_setting_changes():
+    """Graph-replayed steps interleaved with calls that reallocate the
+    pipeline's work buffers (H|psi> on more bands than the SCF launches) and
+    change launch settings: every step must equal the eager run's, i.e. no
+    replay may use a freed buffer or a stale shape."""
+    print("graph + buffer growth", flush=True)
+    spec = kspec()
+
+    def run(graph):
+        su = gscf.Setup(spec)
+        ctx = su.context()
+        ctx.scf_init(spec.nelec, kT=spec.kT, seed=spec.seed, kbatch=0, graph=graph)
+        rng = np.random.default_rng(5)
+        e = []
+        for step in range(12):
+            e.append(ctx.scf_step()["etot"])
+            if step == 3:
+                # 4 k x 8 bands per SCF launch; 96 bands of one k-point grow the stick buffers
+                psi = rng.normal(size=(96, su.npw[0])) + 1j * rng.normal(size=(96, su.npw[0]))
+                ctx.apply_h(0, psi)
+            if step == 6:
+                ctx.set_launch(bands_per_wg=1, items_per_launch=8, fused=True)
+            if step == 8:
+                ctx.set_launch(bands_per_wg=2, items_per_launch=512, fused=False)
+        t = ctx.tuning()
+        ctx.close()
+        return np.array(e), t
+
+    e0, _ = run(False)
+    e1, t = run(True)
+    assert t["graph"]
+    print("largest step energy difference", np.max(np.abs(e1 - e0)))
+    assert np.max(np.abs(e1 - e0)) < 1e-11
+
+
 def test_autotune_keeps_results():
     print("autotune", flush=True)
     spec = kspec()
