@@ -1,5 +1,6 @@
 """Kohn-Sham pieces of the oracle against brute force and closed forms."""
 import numpy as np
+import pytest
 
 from derek_amd import config
 from oracle import basis, fft, ks
@@ -86,6 +87,43 @@ def test_fermi_occupations():
     assert -0.4 < mu < 0.0 and mts <= 0.0
     occ0, _, mts0 = ks.fermi_occupations(eig, w, 4.0, 0.0)
     assert np.all(occ0[:, :2] == 2.0) and np.all(occ0[:, 2:] == 0.0) and mts0 == 0.0
+
+
+def _grand_potential(eig, w, mu, kT, g=2.0):
+    """mu N - kT sum_k w_k g sum_n ln(1 + exp((mu - eps)/kT)), written with
+    logaddexp so that deep levels do not overflow."""
+    x = (mu - np.asarray(eig)) / kT
+    return -kT * float(np.sum(np.asarray(w)[:, None] * g * np.logaddexp(0.0, x)))
+
+
+@pytest.mark.parametrize("kT,nelec", [(0.01, 4.0), (0.05, 5.3), (0.002, 7.0)])
+def test_fermi_entropy_grand_potential_identity(kT, nelec):
+    """-TS pinned by the grand-potential identity at fixed eigenvalues:
+    E_band - TS = mu N_e - kT sum_k w_k 2 sum_n ln(1 + e^{(mu - eps_nk)/kT}).
+    A dropped spin factor, a missing (1 - f) ln(1 - f) term or a wrong sign
+    of kT each break it by O(kT)."""
+    rng = np.random.default_rng(int(kT * 1e4) + 3)
+    eig = np.sort(rng.uniform(-1.0, 0.5, size=(3, 6)), axis=1)
+    w = np.array([0.25, 0.25, 0.5])
+    occ, mu, mts = ks.fermi_occupations(eig, w, nelec, kT)
+    e_band = float(np.sum(w[:, None] * occ * eig))
+    lhs = e_band + mts
+    rhs = mu * nelec + _grand_potential(eig, w, mu, kT)
+    assert abs(lhs - rhs) < 1e-12 * max(1.0, abs(rhs)), (lhs, rhs)
+    assert mts < 0.0
+
+
+def test_fermi_entropy_half_filled_level():
+    """Two k-points, each with one level exactly at mu (by symmetry) between
+    a deep filled and a high empty one: f = 1/2 on the middle level, so
+    -TS = -kT * 2 ln 2 (spin factor 2, weights summing to 1)."""
+    kT = 0.01
+    eig = np.array([[-2.0, 0.0, 2.0], [-2.0, 0.0, 2.0]])
+    w = np.array([0.5, 0.5])
+    occ, mu, mts = ks.fermi_occupations(eig, w, 3.0, kT)
+    assert abs(mu) < 1e-12
+    assert np.allclose(occ[:, 1], 1.0, atol=1e-12)
+    assert abs(mts - (-kT * 2.0 * np.log(2.0))) < 1e-14
 
 
 def test_gaussian_wells_periodic_and_centred():
