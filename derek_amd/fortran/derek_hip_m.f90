@@ -309,6 +309,24 @@ module derek_hip_m
        integer(c_int32_t), value :: howmany, dir
      end function derek_hip_fft3d
 
+     ! fft_1d (fft.f90:305-376): howmany lines of length n, in place
+     integer(c_int32_t) function derek_hip_fft1d(ctx, lines, n, howmany, dir) bind(C, name='derek_hip_fft1d')
+       import :: c_int32_t, c_int64_t, c_double_complex, c_ptr
+       type(c_ptr), value :: ctx
+       complex(c_double_complex), dimension(*) :: lines
+       integer(c_int32_t), value :: n, dir
+       integer(c_int64_t), value :: howmany
+     end function derek_hip_fft1d
+
+     ! wavefunction + - * and copies (wave.f90:226-855): z = a x + b y
+     integer(c_int32_t) function derek_hip_wave_axpby(ctx, n, a, x, b, y, z) bind(C, name='derek_hip_wave_axpby')
+       import :: c_int32_t, c_int64_t, c_double, c_double_complex, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), value :: n
+       real(c_double), dimension(2) :: a, b
+       complex(c_double_complex), dimension(*) :: x, y, z
+     end function derek_hip_wave_axpby
+
      ! pot_apply_wfn (pot.f90:344-382) + kinetic term
      integer(c_int32_t) function derek_hip_apply_h(ctx, k, nb, psi, hpsi) bind(C, name='derek_hip_apply_h')
        import :: c_int32_t, c_double_complex, c_ptr
@@ -458,7 +476,8 @@ module derek_hip_m
   public :: derek_hip_version, derek_hip_last_error, derek_basis_grid_dims, derek_basis_inv_cell, &
        derek_basis_g_map, derek_basis_real_map, derek_basis_kpoints, derek_basis_sphere, derek_basis_gsq, &
        derek_gaussian_pot, derek_hip_init, derek_hip_destroy, derek_hip_set_basis, derek_hip_set_gsq, &
-       derek_hip_set_vloc, derek_hip_set_vext, derek_hip_fft3d, derek_hip_apply_h, derek_hip_overlap, &
+       derek_hip_set_vloc, derek_hip_set_vext, derek_hip_fft3d, derek_hip_fft1d, derek_hip_wave_axpby, &
+       derek_hip_apply_h, derek_hip_overlap, &
        derek_hip_rotate, derek_hip_density, derek_hip_lda, derek_hip_hartree, derek_hip_scf_init, &
        derek_hip_scf_step, derek_hip_scf_polish, derek_hip_get_eigenvalues, derek_hip_get_occupations, &
        derek_hip_get_density, derek_hip_get_psi, derek_hip_put_psi, derek_hip_sync, derek_hip_get_potential, &

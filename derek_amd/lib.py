@@ -26,7 +26,7 @@ EXPORTS = (
     "derek_info_cell", "derek_info_wells", "derek_info_log", "derek_info_preset_pot", "derek_info_help",
     "derek_hip_init", "derek_hip_destroy", "derek_hip_set_basis", "derek_hip_set_gsq",
     "derek_hip_set_vloc", "derek_hip_set_vext",
-    "derek_hip_fft3d", "derek_hip_apply_h", "derek_hip_overlap", "derek_hip_rotate",
+    "derek_hip_fft3d", "derek_hip_fft1d", "derek_hip_wave_axpby", "derek_hip_apply_h", "derek_hip_overlap", "derek_hip_rotate",
     "derek_hip_density", "derek_hip_lda", "derek_hip_hartree", "derek_hip_heev", "derek_hip_hegv",
     "derek_hip_scf_init", "derek_hip_scf_step", "derek_hip_scf_polish", "derek_hip_nscf", "derek_hip_get_eigenvalues",
     "derek_hip_get_occupations", "derek_hip_get_density", "derek_hip_get_potential", "derek_hip_get_psi", "derek_hip_put_psi",
@@ -386,6 +386,33 @@ class Context:
         check(self.lib.derek_hip_fft3d(self.h, a.ctypes.data_as(C.POINTER(C.c_double)),
                                        C.c_int32(a.shape[0]), C.c_int32(direction)))
         return a.reshape(shape)
+
+    def fft1d(self, lines, direction):
+        """In-place transforms of each row of ``lines`` (howmany x n complex),
+        n one of the grid's lengths; reference sign and normalisation."""
+        a = np.array(lines, dtype=np.complex128, copy=True)
+        shape = a.shape
+        n = shape[-1]
+        a = np.ascontiguousarray(a.reshape(-1, n))
+        check(self.lib.derek_hip_fft1d(self.h, a.ctypes.data_as(C.POINTER(C.c_double)), C.c_int32(n),
+                                       C.c_int64(a.shape[0]), C.c_int32(direction)))
+        return a.reshape(shape)
+
+    def wave_axpby(self, a, x, b=0.0, y=None):
+        """a x + b y for complex arrays of one shape (b = 0: a x, y unread)."""
+        xa = np.ascontiguousarray(np.asarray(x, dtype=np.complex128))
+        ca = (C.c_double * 2)(complex(a).real, complex(a).imag)
+        cb = (C.c_double * 2)(complex(b).real, complex(b).imag)
+        yp = None
+        if complex(b) != 0:
+            ya = np.ascontiguousarray(np.asarray(y, dtype=np.complex128))
+            if ya.shape != xa.shape:
+                raise ValueError("x and y shapes differ")
+            yp = ya.ctypes.data_as(C.POINTER(C.c_double))
+        z = np.empty_like(xa)
+        check(self.lib.derek_hip_wave_axpby(self.h, C.c_int64(xa.size), ca, xa.ctypes.data_as(C.POINTER(C.c_double)),
+                                            cb, yp, z.ctypes.data_as(C.POINTER(C.c_double))))
+        return z
 
     def apply_h(self, k, psi):
         """psi: (nb, npw_k) complex (each band contiguous) -> H psi."""

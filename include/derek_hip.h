@@ -209,6 +209,18 @@ int32_t derek_hip_set_vext(derek_ctx* ctx, const double* v);
 /* In-place 3D FFT of howmany grids (complex, N_g each); replaces fft_3d,
  * fft.f90:222-303.  dir must be +1 or -1. */
 int32_t derek_hip_fft3d(derek_ctx* ctx, double* grid, int32_t howmany, int32_t dir);
+/* In-place 1D FFTs of howmany contiguous lines of length n (complex), n one
+ * of the context's grid lengths; replaces fft_1d (fft.f90:305-376: the
+ * length-nz transforms of the STD or FINE grid, dir +1 FFTW_FORWARD, -1
+ * FFTW_BACKWARD, unnormalised). */
+int32_t derek_hip_fft1d(derek_ctx* ctx, double* data, int32_t n, int64_t howmany, int32_t dir);
+/* z = a x + b y over n complex values, a and b complex ({re, im}); y is not
+ * read when b = 0, and z may alias x or y.  The wavefunction operators of
+ * wave.f90: + (wave_add_*, :226-406, a = b = 1), - (wave_sub_*, :409-589,
+ * a = 1, b = -1), * by a real or complex scalar (wave_scale_*, :591-736,
+ * b = 0) and the copies (wave_copy_*, :740-855, a = 1, b = 0). */
+int32_t derek_hip_wave_axpby(derek_ctx* ctx, int64_t n, const double a[2], const double* x, const double b[2],
+                             const double* y, double* z);
 /* H|psi> for nb bands of k-point k: hpsi = 1/2|G+k|^2 psi + FFT[V psi(r)]/N.
  * psi, hpsi: npw_k x nb complex.  Replaces pot_apply_wfn (pot.f90:344-382)
  * plus the kinetic term. */

@@ -54,6 +54,62 @@ def test_fft3d_roundtrip_and_bad_direction():
     ctx.close()
 
 
+@pytest.mark.parametrize("dims,n", [((8, 12, 10), 8), ((8, 12, 10), 12), ((8, 12, 10), 10), ((14, 22, 26), 26),
+                                    ((14, 22, 26), 22), ((14, 22, 26), 14), ((32, 32, 32), 32), ((72, 72, 72), 72),
+                                    ((108, 108, 108), 108), ((144, 144, 144), 144)])
+@pytest.mark.parametrize("precision", [lib.DEREK_FP64, lib.DEREK_MIXED])
+def test_fft1d(dims, n, precision):
+    """fft_1d (fft.f90:305-376) against the oracle, which the brute-force DFT
+    pins: 1D lines of one of the grid's lengths, both directions."""
+    print("fft1d", dims, n, precision, flush=True)
+    ctx = lib.Context(dims, 1, 1, 1.0, 1, precision)
+    rng = np.random.default_rng(n)
+    for howmany in (1, 7, 1000):
+        a = rnd_c(rng, howmany, n)
+        tol = 1e-12 if precision == lib.DEREK_FP64 else 5e-5
+        for d in (1, -1):
+            assert_close(ctx.fft1d(a, d), offt.fft1d(a, d), tol, f"fft1d n={n} howmany={howmany} dir={d}")
+    with pytest.raises(lib.DerekError) as e:
+        ctx.fft1d(rnd_c(rng, 2, 9), 1)
+    assert e.value.code == lib.DEREK_ERR_ARG
+    with pytest.raises(lib.DerekError) as e:
+        ctx.fft1d(rnd_c(rng, 2, n), 0)
+    assert e.value.code == lib.DEREK_ERR_ARG
+    ctx.close()
+
+
+def test_wave_axpby():
+    """The wavefunction operators of wave.f90 (+, -, real and complex *,
+    copies) against the oracle's restatement, on a wavefunction-shaped
+    array (k-points x bands x plane waves) and a slice."""
+    from oracle import wave as ow
+    print("wave algebra", flush=True)
+    ctx = lib.Context((8, 8, 8), 1, 1, 1.0, 1)
+    rng = np.random.default_rng(9)
+    for shape in [(3, 5, 1001), (7, 333)]:
+        x, y = rnd_c(rng, *shape), rnd_c(rng, *shape)
+        cases = [
+            (ctx.wave_axpby(1.0, x, 1.0, y), ow.add(x, y)),
+            (ctx.wave_axpby(1.0, x, -1.0, y), ow.sub(x, y)),
+            (ctx.wave_axpby(-2.5, x), ow.scale(-2.5, x)),
+            (ctx.wave_axpby(0.3 - 1.7j, x), ow.scale(0.3 - 1.7j, x)),
+            (ctx.wave_axpby(1.0, x), ow.copy(x)),
+        ]
+        for got, ref in cases:
+            assert got.shape == ref.shape
+            assert_close(got, ref, 1e-15)
+        # exact cases: a copy and a sum of dyadic values are bit-exact
+        assert np.array_equal(ctx.wave_axpby(1.0, x), x)
+        xd = np.round(x * 64) / 64
+        yd = np.round(y * 64) / 64
+        assert np.array_equal(ctx.wave_axpby(1.0, xd, 1.0, yd), ow.add(xd, yd))
+    with pytest.raises(lib.DerekError) as e:
+        lib.check(ctx.lib.derek_hip_wave_axpby(ctx.h, 0, (lib.C.c_double * 2)(1, 0), None,
+                                               (lib.C.c_double * 2)(0, 0), None, None))
+    assert e.value.code == lib.DEREK_ERR_ARG
+    ctx.close()
+
+
 def make_setup(spec):
     su = Setup(spec)
     return su
