@@ -5,29 +5,40 @@ cell, Ecut 30 Ha (108^3 FFT grid, 62669 plane waves at Gamma), 128
 electrons, 80 bands, 16 seeded Gaussian wells plus a 0.2 Ha cosine term.
 A step is one H|psi> over all 80 resident bands (sphere-pruned inverse 3D
 FFT, V(r) multiply, forward 3D FFT, gather + kinetic); value = bands/s over
-the K timed steps with the inputs resident in HBM.  The same run reports the
-roofline of the dominant kernel (HIP events on the library's stream over the
-timed region), the time to SCF convergence (|dE| < 1e-6 Ha) of
+the K timed steps with the inputs resident in HBM.  ``roofline`` is the
+dominant kernel of that step against HBM, with the path's compulsory bytes
+per band (2 npw s_psi + 12 npw + N_g s_r / B) times the bands of one launch
+over its duration; ``roofline_valu`` is the same kernel against the FP64
+vector ALU (its FFT flops).  The same run reports the time to SCF
+convergence (|dE| < 1e-6 Ha) of
 
-  * that system in plain FP64 and in the mixed form (complex64 bulk, FP64
-    subspace solves and energies, FP64 refinement), with dE_tot and the
-    largest eigenvalue difference between the two;
+  * the 32^3 system (10 bohr, Ecut 10 Ha, 10 bands) in FP64, beside the
+    numpy oracle's SCF of the same system on one CPU core;
+  * the headline system in plain FP64 and in the mixed form (complex64 bulk,
+    FP64 subspace solves and energies, FP64 refinement), with dE_tot and the
+    largest eigenvalue difference between the two, and the FP64 step against
+    its own roofline (compulsory bytes at 8 TB/s plus GEMM flops at the FP64
+    matrix-core peak);
   * the 64-k-point system (16 bohr, Ecut 20 Ha, 4x4x4 mesh, 72^3 grid, 24
     bands, kT = 0.01 Ha) in plain FP64 (one k-point per launch, no graph),
     FP64 batched (all k-points per launch, graph replay) and mixed + batched
     + autotuned, with the speed-up of the last over the first;
+  * the 8-k-point system (30 bohr, Ecut 25 Ha, 2x2x2 mesh, 144^3 grid,
+    161214 plane waves per k, 300 bands) in FP64 batched and mixed batched;
   * the mixed runs put their complex64 overlaps and rotations on the BF16
     matrix cores with ``--bf16 N`` bf16 terms per value (default 2, the
     library's default; FP32 accumulation), and each is repeated with them on
     the FP32 matrix cores (``*_fp32_mfma``) for comparison (``--bf16 0``: the
     FP32 ones only, no repeat);
 
-and a CPU baseline (the numpy oracle's H|psi> on a bounded sample of the
-headline system, timed on this host on one core and on every core of the
-host's share, before the GPU is touched).
+and CPU baselines, timed on this host before the GPU is touched: the numpy
+oracle's H|psi> on a bounded sample of the headline system (one core and
+every core of the host's share), one oracle FP64 SCF step of the headline
+system (one core and every core) and the oracle's whole SCF of the 32^3
+system (one core).
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-scf] [--scf gamma108,kmesh72] [--cpu-seconds S] [--no-cpu]
-                [--fixed-shapes | --shapes BPW,ITEMS,FUSED] [--bf16 N]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-scf] [--scf gamma32,gamma108,kmesh72,kmesh144]
+                [--cpu-seconds S] [--no-cpu] [--fixed-shapes | --shapes BPW,ITEMS,FUSED] [--bf16 N]
 """
 import argparse
 import json
@@ -124,6 +135,73 @@ def cpu_baseline(su, seconds):
     return serial, allc
 
 
+def cpu_scf_baseline(ncores):
+    """The numpy oracle's FP64 SCF on this host, before the GPU is touched:
+    one SCF step of the headline system (initial wavefunctions, V_in, H X,
+    Rayleigh-Ritz and two block-Davidson steps, occupations, density,
+    energies, Pulay/Kerker mixing) on one core and on ``ncores`` threads
+    (pocketfft workers and OpenBLAS threads), and the whole SCF of the
+    32^3 system (10 bands) to |dE| < 1e-6 Ha on one core."""
+    from threadpoolctl import threadpool_limits
+
+    from derek_amd import config
+    from oracle import fft as offt
+    from oracle import scf as oscf
+
+    def system(spec):
+        return oscf.System(spec.lengths, spec.ecut, spec.g_fine_scale, spec.mp, spec.nelec, spec.nbands, spec.kT,
+                           spec.centres, spec.heights, spec.widths, spec.cos_amp, spec.seed)
+
+    out = {}
+    s32 = system(config.workload("gamma32"))
+    offt.set_workers(1)
+    with threadpool_limits(1):
+        t = time.perf_counter()
+        r = oscf.scf(s32, max_scf=80, energy_tol=1e-6)
+        dt = time.perf_counter() - t
+    out["gamma32_time_to_scf"] = {"value": round(dt, 3), "unit": "s", "cores": 1, "kind": "port",
+                                  "steps": r["iterations"], "converged": r["converged"],
+                                  "etot_ha": float(r["etot"]),
+                                  "sample": "numpy oracle FP64 SCF of the 32^3 system to |dE| < 1e-6 Ha"}
+    s108 = system(config.workload("gamma108"))
+    for n in sorted({1, ncores}):
+        offt.set_workers(n)
+        with threadpool_limits(n):
+            t = time.perf_counter()
+            oscf.scf(s108, max_scf=1, energy_tol=0.0, n_david_first=2)
+            dt = time.perf_counter() - t
+        out[f"gamma108_scf_step_{n}core"] = {
+            "value": round(dt, 3), "unit": "s per SCF step", "cores": n, "kind": "port",
+            "sample": "one numpy oracle FP64 SCF step (two Davidson steps) of the 108^3 system, 80 bands"}
+        log(f"# CPU FP64 SCF step (108^3, 80 bands) on {n} core(s): {dt:.2f} s")
+    offt.set_workers(1)
+    return out
+
+
+def scf_step_roofline(su, nb, n_david, s_psi, t_step_s):
+    """Lower bound on one SCF step of a Gamma system from its compulsory work:
+    HBM bytes of H|psi> (nb (1 + n_david) band applications, each
+    2 npw s_psi + 12 npw + N_g 8 / nb), of the density (nb npw s_psi in,
+    2 N_g 8 for rho) and of the grid stages (Hartree and LDA of rho_in and
+    rho_out, Kerker, Pulay with a full 8-deep history: 16+16+24+24+16+(4+32)*8
+    N_g bytes), at 8 TB/s, plus the overlap and rotation flops
+    (8 npw m n per complex GEMM: Y^H HY of X, nb^2; per Davidson step the
+    2nb x 2nb S and H and two 2nb -> nb rotations; two nb -> nb rotations
+    after Rayleigh-Ritz) at the FP64 matrix-core peak.  The dense 2nb x 2nb
+    solves are latency-bound and not counted."""
+    npw, N = float(su.npw[0]), float(su.N)
+    hp = nb * (1 + n_david) * (2 * npw * s_psi + 12 * npw + N * 8.0 / nb)
+    dens = nb * npw * s_psi + 2 * N * 8.0
+    grid = (16 + 16 + 24 + 24 + 16 + (4 + 32) * 8) * N
+    flops = 8.0 * npw * nb * nb * (3 + 12 * n_david)
+    t_bytes = (hp + dens + grid) / (HBM_PEAK_GBS * 1e9)
+    t_flops = flops / (FP64_PEAK_TFS * 1e12)
+    t_roof = t_bytes + t_flops
+    return {"bytes": hp + dens + grid, "gemm_flops": flops, "t_bytes_ms": round(t_bytes * 1e3, 4),
+            "t_flops_ms": round(t_flops * 1e3, 4), "t_roofline_ms": round(t_roof * 1e3, 4),
+            "t_step_ms": round(t_step_s * 1e3, 3), "frac": round(t_roof / t_step_s, 4)}
+
+
 def scf_run(gscf, lib, spec, su, precision, verbose=False, **kw):
     """Time to |dE| < 1e-6 Ha from a fresh engine (init included)."""
     ctx = su.context(precision)
@@ -144,7 +222,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-scf", action="store_true")
-    ap.add_argument("--scf", default="gamma108,kmesh72")
+    ap.add_argument("--scf", default="gamma32,gamma108,kmesh72,kmesh144")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--refine", type=int, default=0,
                     help="FP64 refinement steps of the mixed runs (0: the library default)")
@@ -176,11 +254,12 @@ def main():
     spec = config.workload("gamma108")
     t0 = time.perf_counter()
     su = gscf.Setup(spec)  # host-side discretisation: no GPU call yet
-    cpu = cpu_all = None
+    cpu = cpu_all = cpu_scf = None
     if not args.no_cpu:
         cpu, cpu_all = cpu_baseline(su, args.cpu_seconds)
         log(f"# CPU baseline: {cpu['value']} bands/s on 1 core, {cpu_all['value']} bands/s on "
             f"{cpu_all['cores']} cores")
+        cpu_scf = cpu_scf_baseline(cpu_all["cores"])
     if args.copy_calib > 0:
         copy_calibration(args.copy_calib)
     shapes = [int(x) for x in args.shapes.split(",")] if args.shapes else None
@@ -231,25 +310,43 @@ def main():
         if per:
             items = min(tune64["items_per_launch"], nb)
             traffic = round(items * (per["fetch"] + per["write"]))
-    if dom == "slab_yx_vmul":
-        # the fused y/x pass keeps the planes in LDS: its HBM bytes are the
-        # stick rows only, and the bound is the FP64 vector pipe (VALU + LDS);
-        # achieved = FFT flops (5 n log2 n per line) per launch / duration
-        tf = fl / n_l / avg_s / 1e12
-        roofline = {"bound": "valu", "achieved": round(tf, 2), "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
-                    "frac": round(tf / FP64_PEAK_TFS, 4), "traffic": traffic, "kernel": dom,
-                    "flops_per_launch": fl / n_l, "bytes_per_launch": by / n_l, "avg_launch_us": avg_s * 1e6,
-                    "hbm_GB_s": round(achieved, 1), "hbm_frac": round(achieved / HBM_PEAK_GBS, 4)}
-    else:
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
-                    "bytes_per_launch": by / n_l, "avg_launch_us": avg_s * 1e6}
+    # The path's roofline (HBM): the compulsory bytes of one band's H|psi> at
+    # one k, 2 npw s_psi (psi in, H psi out) + 12 npw (FFT index and kinetic
+    # energy) + N_g s_r / B (V read once per launch of B bands), times the
+    # bands of one launch of the dominant kernel, over its launch duration.
+    npw0, s_psi, s_r = float(su.npw[0]), 16.0, 8.0
+    units = min(tune64["items_per_launch"], nb)
+    band_bytes = 2 * npw0 * s_psi + 12 * npw0 + su.N * s_r / units
+    hbm = band_bytes * units / avg_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(hbm, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(hbm / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
+                "unit_of_work": "one band's H|psi> at one k: 2 npw s_psi + 12 npw + N_g s_r / B",
+                "bytes_per_unit": round(band_bytes), "units_per_launch": units, "avg_launch_us": avg_s * 1e6}
+    # the same kernel against the pipe that bounds it: the FFT flops
+    # (5 n log2 n per line) and the V multiply on the FP64 vector ALU, and
+    # the bytes the kernel itself moves (stick rows in and out, V once)
+    tf = fl / n_l / avg_s / 1e12 if fl > 0 else 0.0
+    roofline_valu = {"bound": "valu", "achieved": round(tf, 2), "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
+                     "frac": round(tf / FP64_PEAK_TFS, 4), "kernel": dom, "flops_per_launch": fl / n_l,
+                     "kernel_bytes_per_launch": by / n_l, "kernel_GB_s": round(achieved, 1),
+                     "kernel_hbm_frac": round(achieved / HBM_PEAK_GBS, 4)}
 
     scf_info = {}
     todo = [] if args.no_scf else [x.strip() for x in args.scf.split(",") if x.strip()]
     graph = not args.no_graph
+    if "gamma32" in todo:
+        spec32 = config.workload("gamma32")
+        su32 = gscf.Setup(spec32)
+        r32, _ = scf_run(gscf, lib, spec32, su32, lib.DEREK_FP64, kbatch=1)
+        log(f"# gamma32 FP64: {r32['steps']} steps, {r32['time_to_scf_s']:.3f} s, E = {r32['etot_ha']:.10f}")
+        if cpu_scf:
+            c32 = cpu_scf["gamma32_time_to_scf"]
+            r32 = dict(r32, cpu_time_to_scf_s=c32["value"], dE_vs_cpu_oracle_ha=abs(r32["etot_ha"] - c32["etot_ha"]),
+                       speedup_vs_cpu_1core=round(c32["value"] / r32["time_to_scf_s"], 2))
+        scf_info["gamma32_fp64"] = r32
     if "gamma108" in todo:
         r64, e64 = scf_run(gscf, lib, spec, su, lib.DEREK_FP64, kbatch=1)
+        r64["step_roofline"] = scf_step_roofline(su, nb, 2, 16.0, r64["ms_per_step"] * 1e-3)
         log(f"# gamma108 FP64: {r64['steps']} steps, {r64['time_to_scf_s']:.3f} s, E = {r64['etot_ha']:.10f}")
         rmx, emx = scf_run(gscf, lib, spec, su, lib.DEREK_MIXED, graph=graph, refine_steps=args.refine,
                            gemm_bf16=args.bf16)
@@ -331,13 +428,13 @@ def main():
                    "launch_shapes": {"bands_per_wg": tune64["bands_per_wg"],
                                      "items_per_launch": tune64["items_per_launch"],
                                      "fused_yx": tune64["fused"], "fused_density": tune64["fused_density"]}},
-        "roofline": roofline, "kernels": kernels,
+        "roofline": roofline, "roofline_valu": roofline_valu, "kernels": kernels,
         "hpsi_fp32": {"value": round(bps32, 2), "unit": "bands/s", "ms_per_step": round(ms32, 4),
                       "launch_shapes": {"bands_per_wg": tune32["bands_per_wg"],
                                         "items_per_launch": tune32["items_per_launch"],
                                         "fused_yx": tune32["fused"], "fused_density": tune32["fused_density"]},
                       "kernels": kernels32},
-        "scf": scf_info, "cpu_baseline": cpu, "cpu_baseline_all_cores": cpu_all,
+        "scf": scf_info, "cpu_baseline": cpu, "cpu_baseline_all_cores": cpu_all, "cpu_baseline_scf": cpu_scf,
     }
     print(json.dumps(out))
 
