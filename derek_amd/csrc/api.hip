@@ -599,7 +599,9 @@ int32_t derek_hip_init(derek_ctx** out, const derek_dims* d) {
     c->gemm_bf16 = c->prec == DEREK_MIXED ? 2 : 0;
     c->volume = d->volume;
     c->max_npw = d->max_npw;
+    DEREK_HIP(hipGetDevice(&c->device));
     DEREK_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    noalloc_setup(c->device);
     for (int a = 0; a < 3; ++a) {
       c->tw64[a] = reinterpret_cast<double2*>(make_twiddles<double>(c->dims[a]));
       c->tw32[a] = reinterpret_cast<float2*>(make_twiddles<float>(c->dims[a]));
@@ -841,6 +843,17 @@ int32_t derek_hip_overlap(derek_ctx* c, int64_t npw, int32_t m, int32_t n, const
     }
     DEREK_HIP(hipMemcpyAsync(S, ds, (size_t)m * n * sizeof(double2), hipMemcpyDeviceToHost, c->s));
     c->sync();
+  });
+}
+
+int32_t derek_hip_overlap_noalloc(derek_ctx* c, int64_t npw, int32_t m, int32_t n, const double* X, const double* Y,
+                                  double* S) {
+  return guard([&] {
+    DEREK_REQUIRE(c && X && Y && S, "null argument");
+    DEREK_REQUIRE(npw >= 1 && m >= 1 && n >= 1, "empty overlap");
+    DEREK_REQUIRE(c->prec == DEREK_FP64, "overlap_noalloc multiplies complex128 operands: it needs a DEREK_FP64 context");
+    overlap_noalloc(c, npw, m, n, reinterpret_cast<const double2*>(X), reinterpret_cast<const double2*>(Y),
+                    reinterpret_cast<double2*>(S));
   });
 }
 

@@ -58,6 +58,10 @@ inline void dfree(void* p) {
 
 inline int64_t round_up(int64_t a, int64_t b) { return ceil_div(a, b) * b; }
 
+// Allocation-free overlap (noalloc.hip): a file-scope device array of this
+// many double2 (32 MiB per GPU) holds its split-K partial sums.
+constexpr int64_t kNoAllocScratchElems = (int64_t)1 << 21;
+
 // Per-kernel-class timers: HIP events recorded on the context's stream around
 // each launch while enabled, resolved lazily; algorithmic bytes and flops of
 // each launch are accumulated beside the times.
@@ -116,6 +120,7 @@ struct EngineDeleter {
 }  // namespace derek
 
 struct derek_ctx {
+  int device = 0;  // HIP device current at derek_hip_init
   int dims[3] = {0, 0, 0};
   int64_t N = 0;
   int nk = 0, nb = 0, prec = DEREK_FP64;
@@ -205,3 +210,12 @@ struct derek_ctx {
   void dot_into(const double* a, const double* b, int64_t n, double scale, bool root, double* rep, int slot);
   void kerker(const double* f, double* out, double q0);
 };
+
+namespace derek {
+// resolves the allocation-free overlap's scratch address on device `dev`
+// (current); once per device
+void noalloc_setup(int dev);
+// S = X^H Y (device pointers, complex128) on the context's stream, partial
+// sums in the scratch array; no allocation, no synchronisation
+void overlap_noalloc(derek_ctx* c, int64_t npw, int m, int n, const double2* X, const double2* Y, double2* S);
+}  // namespace derek

@@ -54,6 +54,16 @@ inline OvGeom ov_geom(int m, int n, int64_t npw, bool herm, int batch, int slots
   return g;
 }
 
+// At most max_chunks row chunks (a fixed partial-sum buffer): each chunk
+// takes more rows, still a whole number of `ks`-row slabs.
+inline void ov_limit_chunks(OvGeom& g, int64_t max_chunks, int ks) {
+  if (max_chunks < 1 || g.nchunk <= max_chunks) return;
+  int64_t kch = ov_ceil_div(std::max<int64_t>(g.npw, 1), max_chunks);
+  kch = std::max<int64_t>(ks, ov_ceil_div(kch, ks) * ks);
+  g.kch = kch;
+  g.nchunk = (int)std::max<int64_t>(1, ov_ceil_div(g.npw, kch));
+}
+
 // double2 elements of the partial sums of an m x n overlap of `batch`
 // k-points, for any of the kernels whose resident slots are listed (both
 // slab depths, Hermitian or not).

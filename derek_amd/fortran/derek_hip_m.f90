@@ -327,6 +327,17 @@ module derek_hip_m
        complex(c_double_complex), dimension(*) :: x, y, z
      end function derek_hip_wave_axpby
 
+     ! S = X^H Y on device memory with no allocation (no reference counterpart;
+     ! for callers that cannot allocate: HIP graph capture, serving loops);
+     ! X, Y, S are device addresses, m * n <= 2**21
+     integer(c_int32_t) function derek_hip_overlap_noalloc(ctx, npw, m, n, x, y, s) &
+          bind(C, name='derek_hip_overlap_noalloc')
+       import :: c_int32_t, c_int64_t, c_ptr
+       type(c_ptr), value :: ctx, x, y, s
+       integer(c_int64_t), value :: npw
+       integer(c_int32_t), value :: m, n
+     end function derek_hip_overlap_noalloc
+
      ! pot_apply_wfn (pot.f90:344-382) + kinetic term
      integer(c_int32_t) function derek_hip_apply_h(ctx, k, nb, psi, hpsi) bind(C, name='derek_hip_apply_h')
        import :: c_int32_t, c_double_complex, c_ptr
@@ -477,7 +488,7 @@ module derek_hip_m
        derek_basis_g_map, derek_basis_real_map, derek_basis_kpoints, derek_basis_sphere, derek_basis_gsq, &
        derek_gaussian_pot, derek_hip_init, derek_hip_destroy, derek_hip_set_basis, derek_hip_set_gsq, &
        derek_hip_set_vloc, derek_hip_set_vext, derek_hip_fft3d, derek_hip_fft1d, derek_hip_wave_axpby, &
-       derek_hip_apply_h, derek_hip_overlap, &
+       derek_hip_apply_h, derek_hip_overlap, derek_hip_overlap_noalloc, &
        derek_hip_rotate, derek_hip_density, derek_hip_lda, derek_hip_hartree, derek_hip_scf_init, &
        derek_hip_scf_step, derek_hip_scf_polish, derek_hip_get_eigenvalues, derek_hip_get_occupations, &
        derek_hip_get_density, derek_hip_get_psi, derek_hip_put_psi, derek_hip_sync, derek_hip_get_potential, &

@@ -26,7 +26,8 @@ EXPORTS = (
     "derek_info_cell", "derek_info_wells", "derek_info_log", "derek_info_preset_pot", "derek_info_help",
     "derek_hip_init", "derek_hip_destroy", "derek_hip_set_basis", "derek_hip_set_gsq",
     "derek_hip_set_vloc", "derek_hip_set_vext",
-    "derek_hip_fft3d", "derek_hip_fft1d", "derek_hip_wave_axpby", "derek_hip_apply_h", "derek_hip_overlap", "derek_hip_rotate",
+    "derek_hip_fft3d", "derek_hip_fft1d", "derek_hip_wave_axpby", "derek_hip_apply_h", "derek_hip_overlap", "derek_hip_overlap_noalloc",
+    "derek_hip_rotate",
     "derek_hip_density", "derek_hip_lda", "derek_hip_hartree", "derek_hip_heev", "derek_hip_hegv",
     "derek_hip_scf_init", "derek_hip_scf_step", "derek_hip_scf_polish", "derek_hip_nscf", "derek_hip_get_eigenvalues",
     "derek_hip_get_occupations", "derek_hip_get_density", "derek_hip_get_potential", "derek_hip_get_psi", "derek_hip_put_psi",

@@ -233,6 +233,19 @@ int32_t derek_hip_apply_h(derek_ctx* ctx, int32_t k, int32_t nb, const double* p
  * derek_hip_rotate likewise. */
 int32_t derek_hip_overlap(derek_ctx* ctx, int64_t npw, int32_t m, int32_t n, const double* X, const double* Y,
                           double* S);
+/* S = X^H Y as derek_hip_overlap, with X, Y and S in device memory
+ * (complex128; X == Y with m == n takes the Hermitian path), for callers
+ * that cannot allocate: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  No counterpart in the reference.  It
+ * allocates nothing: the split-K partial sums go to a file-scope device
+ * array of 2^21 complex128 values (32 MiB per GPU, held until the process
+ * ends), whose address derek_hip_init resolves once per device; m * n above
+ * 2^21 is refused with DEREK_ERR_ARG, and the row chunks are capped so their
+ * partials fit.  Needs a DEREK_FP64 context.  Asynchronous on the context's
+ * stream (derek_hip_stream; derek_hip_sync waits).  Every call on one GPU
+ * shares the array: calls that overlap in time on one GPU race. */
+int32_t derek_hip_overlap_noalloc(derek_ctx* ctx, int64_t npw, int32_t m, int32_t n, const double* X,
+                                  const double* Y, double* S);
 /* Z = X U; X: npw x m, U: m x n, Z: npw x n (complex). */
 int32_t derek_hip_rotate(derek_ctx* ctx, int64_t npw, int32_t m, int32_t n, const double* X, const double* U,
                          double* Z);
