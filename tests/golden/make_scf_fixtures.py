@@ -1,0 +1,119 @@
+"""Writes the oracle's converged results for the three large benchmark systems
+as fixtures for the GPU SCF tests (run here, on the CPU; takes tens of
+minutes for all three):
+
+  scf_gamma108.json   20 bohr cell, Ecut 30 Ha, Gamma, 108^3 grid, 128
+                      electrons, 80 bands, 16 wells + 0.2 Ha cos, kT 0.002 Ha:
+                      the FP64 SCF converged to |dE| < 1e-11 Ha and
+                      ||rho_out - rho_in|| < 1e-8, then eigenvectors polished
+                      to residual 1e-9 at the final potential.
+  scf_kmesh72.json    16 bohr, Ecut 20 Ha, 4x4x4 mesh (64 k), 72^3, 32
+                      electrons, 24 bands, 8 wells, kT 0.01 Ha: the same.
+  eigs_kmesh144_k0.json  30 bohr, Ecut 25 Ha, 2x2x2 mesh, 144^3, 300 bands,
+                      64 wells: the 300 lowest eigenvalues at the first
+                      k-point of the first SCF step's potential (uniform
+                      density: V_ext + V_H[n0] + V_xc[n0], n0 = N_e/Omega),
+                      residual 1e-9.  The full oracle SCF of this system does
+                      not fit a test budget; these eigenvalues depend on the
+                      Hamiltonian only, not on the algorithm that finds them.
+
+Each fixture holds the energies and eigenvalues as 17-digit decimals and, for
+the SCF systems, the density at every (N // 2048)-th grid point.
+
+usage: python tests/golden/make_scf_fixtures.py [gamma108] [kmesh72] [kmesh144]
+"""
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from derek_amd import config  # noqa: E402
+from oracle import fft as offt  # noqa: E402
+from oracle import ks as oks  # noqa: E402
+from oracle import scf as oscf  # noqa: E402
+
+ENERGY_TOL = 1e-11
+RHO_TOL = 1e-8
+RES_TOL = 1e-9
+NSAMPLE = 2048
+
+
+def system(spec):
+    return oscf.System(spec.lengths, spec.ecut, spec.g_fine_scale, spec.mp, spec.nelec, spec.nbands, spec.kT,
+                       spec.centres, spec.heights, spec.widths, spec.cos_amp, spec.seed)
+
+
+def floats(a):
+    return [float(x) for x in np.asarray(a, dtype=np.float64).ravel()]
+
+
+def scf_fixture(name):
+    spec = config.workload(name)
+    t0 = time.time()
+    s = system(spec)
+    print(f"{name}: dims {s.dims}, npw {[len(i) for i in s.fft_index][:4]}..., setup {time.time() - t0:.1f} s",
+          flush=True)
+    r = oscf.scf(s, max_scf=80, energy_tol=ENERGY_TOL, rho_tol=RHO_TOL, final_res_tol=RES_TOL,
+                 log=lambda m: print(m, flush=True))
+    if not r["converged"]:
+        raise SystemExit(f"{name}: oracle SCF did not converge")
+    stride = s.N // NSAMPLE
+    idx = np.arange(0, s.N, stride)[:NSAMPLE]
+    out = dict(system=name, dims=list(s.dims), nk=s.nk, nbands=s.nbands, nelec=s.nelec, kT=s.kT,
+               energy_tol=ENERGY_TOL, rho_tol=RHO_TOL, res_tol=RES_TOL, iterations=r["iterations"],
+               etot=float(r["etot"]), mu=float(r["mu"]), eigs_shape=list(r["eigs_final"].shape),
+               eigs=floats(r["eigs_final"]), occ=floats(r["occ"]),
+               energies=[float(h["etot"]) for h in r["history"]],
+               rho_index=[int(i) for i in idx], rho=floats(r["rho_out"][idx]),
+               rho_integral=float(np.sum(r["rho_out"]) * s.volume / s.N),
+               seconds=time.time() - t0)
+    path = os.path.join(HERE, f"scf_{name}.json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=0)
+    print(f"{name}: E = {out['etot']:.12f} Ha after {out['iterations']} steps, {out['seconds']:.0f} s -> {path}",
+          flush=True)
+
+
+def first_step_eigs(name="kmesh144", k=0, chunk=24):
+    spec = config.workload(name)
+    t0 = time.time()
+    s = system(spec)
+    print(f"{name}: dims {s.dims}, npw {len(s.fft_index[k])}, setup {time.time() - t0:.1f} s", flush=True)
+    rho0 = np.full(s.N, s.nelec / s.volume)
+    vh, _ = oks.hartree(rho0, s.gsq, s.dims, s.volume)
+    vxc, _, _ = oks.xc_on_grid(rho0, s.dims, s.volume)
+    v = s.vext + vh + vxc
+
+    def apply(P):
+        P = np.atleast_2d(P)
+        return np.vstack([oks.apply_h(P[i:i + chunk], s.fft_index[k], s.kin[k], v, s.dims)
+                          for i in range(0, P.shape[0], chunk)])
+
+    X = s.initial_psi(k)
+    X, _, eps, rn = oscf.davidson(X, apply(X), s.kin[k], apply, 2000, res_tol=RES_TOL)
+    print(f"{name}: largest residual {rn.max():.2e}, {time.time() - t0:.0f} s", flush=True)
+    if rn.max() >= RES_TOL:
+        raise SystemExit(f"{name}: Davidson stopped at residual {rn.max():.2e}")
+    out = dict(system=name, k=k, kfrac=floats(s.kfrac[k]), dims=list(s.dims), npw=len(s.fft_index[k]),
+               nbands=s.nbands, res_tol=RES_TOL, max_residual=float(rn.max()), eigs=floats(eps),
+               v_mean=float(np.mean(v)), seconds=time.time() - t0)
+    path = os.path.join(HERE, f"eigs_{name}_k{k}.json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=0)
+    print(f"{name}: -> {path}", flush=True)
+
+
+if __name__ == "__main__":
+    offt.set_workers(os.cpu_count() or 1)
+    todo = sys.argv[1:] or ["gamma108", "kmesh72", "kmesh144"]
+    for n in todo:
+        if n == "kmesh144":
+            first_step_eigs(n)
+        else:
+            scf_fixture(n)

@@ -1,0 +1,146 @@
+"""The benchmark systems at full size on the GPU against the oracle's
+converged results, committed as fixtures by tests/golden/make_scf_fixtures.py:
+
+* 20 bohr cell, Ecut 30 Ha, Gamma, 108^3 grid, 80 bands, kT 0.002 Ha: plain
+  FP64 (one k-point per launch, eager) and the mixed form (complex64 bulk,
+  BF16-split matrix-core GEMMs, FP64 refinement, graph replay);
+* 16 bohr cell, Ecut 20 Ha, 4x4x4 mesh (64 k-points with their own
+  plane-wave counts), 72^3 grid, 24 bands, kT 0.01 Ha: plain FP64, batched
+  FP64 with graph replay, and mixed + batched + graph + autotuned shapes;
+* 30 bohr cell, Ecut 25 Ha, 2x2x2 mesh, 144^3 grid, 161214 plane waves,
+  300 bands: the eigenvalues at the first k-point of the first step's
+  potential (fixture), and the full FP64 and mixed SCF against each other.
+
+Targets: FP64 total energy within 1e-7 Ha of the oracle and polished
+eigenvalues within 1e-8 Ha; the mixed form within 1e-6 Ha in energy (its
+FP64 refinement then lands the polished eigenvalues within 1e-8 Ha as
+well); the density at 2048 sampled grid points within 1e-6 of its maximum.
+Both sides converge to |dE| < 1e-11 Ha and ||rho_out - rho_in|| < 1e-8
+before the comparison, then polish to residual 1e-9.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from derek_amd import config, lib
+from derek_amd import scf as gscf
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def fixture(name):
+    path = os.path.join(GOLDEN, name)
+    if not os.path.exists(path):
+        pytest.skip(f"fixture {name} not generated (tests/golden/make_scf_fixtures.py)")
+    with open(path) as f:
+        return json.load(f)
+
+
+def converge(spec, precision, **kw):
+    su = gscf.Setup(spec)
+    ctx = su.context(precision)
+    res = gscf.run(spec, max_scf=80, energy_tol=1e-11, rho_tol=1e-8, setup=su, ctx=ctx, log=print, **kw)
+    rmax = ctx.scf_polish(1000, 1e-9)
+    eg = ctx.eigenvalues()
+    rho = ctx.get_density()
+    ctx.close()
+    print(f"converged {res['converged']} in {res['iterations']} steps ({res['fp32_steps']} FP32), "
+          f"polished residual {rmax:.2e}")
+    return res, eg, rho, rmax
+
+
+def compare(fx, res, eg, rho, rmax, e_tol):
+    ref_e = np.array(fx["eigs"]).reshape(fx["eigs_shape"])
+    de = abs(res["etot"] - fx["etot"])
+    deig = np.abs(eg - ref_e)
+    idx = np.array(fx["rho_index"])
+    drho = np.max(np.abs(rho[idx] - np.array(fx["rho"]))) / np.max(np.abs(fx["rho"]))
+    print(f"E {res['etot']:.12f} vs oracle {fx['etot']:.12f}: dE = {de:.2e} Ha; "
+          f"max |d eps| = {deig.max():.2e} Ha at {np.unravel_index(deig.argmax(), deig.shape)}; "
+          f"sampled rho rel {drho:.2e}")
+    assert res["converged"]
+    assert rmax < 1e-9
+    assert de < e_tol
+    assert deig.max() < 1e-8
+    assert drho < 1e-6
+
+
+@pytest.mark.parametrize("mode", ["fp64_plain", "mixed_graph"])
+def test_gamma108_matches_oracle_fixture(mode):
+    print("gamma108", mode, flush=True)
+    fx = fixture("scf_gamma108.json")
+    spec = config.workload("gamma108")
+    if mode == "fp64_plain":
+        out = converge(spec, lib.DEREK_FP64, kbatch=1)
+        compare(fx, *out, e_tol=1e-7)
+    else:
+        out = converge(spec, lib.DEREK_MIXED, kbatch=0, graph=True)
+        assert out[0]["fp32_steps"] >= 2
+        compare(fx, *out, e_tol=1e-6)
+
+
+@pytest.mark.parametrize("mode", ["fp64_plain", "fp64_batched_graph", "mixed_batched_graph_autotuned"])
+def test_kmesh72_matches_oracle_fixture(mode):
+    print("kmesh72", mode, flush=True)
+    fx = fixture("scf_kmesh72.json")
+    spec = config.workload("kmesh72")
+    if mode == "fp64_plain":
+        out = converge(spec, lib.DEREK_FP64, kbatch=1)
+        compare(fx, *out, e_tol=1e-7)
+    elif mode == "fp64_batched_graph":
+        out = converge(spec, lib.DEREK_FP64, kbatch=0, graph=True)
+        assert out[0]["tuning"]["kbatch"] == 64 and out[0]["tuning"]["graph"]
+        compare(fx, *out, e_tol=1e-7)
+    else:
+        out = converge(spec, lib.DEREK_MIXED, kbatch=0, graph=True, autotune=True)
+        assert out[0]["fp32_steps"] >= 2
+        compare(fx, *out, e_tol=1e-6)
+
+
+def test_kmesh144_first_step_eigenvalues_match_oracle_fixture():
+    """300 bands at 161214 plane waves: the first SCF step's potential
+    (uniform density) taken from the device, held fixed, and the lowest 300
+    eigenvalues at the first k-point converged to residual 1e-9."""
+    print("kmesh144 eigenvalues at k0", flush=True)
+    fx = fixture("eigs_kmesh144_k0.json")
+    spec = config.workload("kmesh144")
+    kfrac = np.array([fx["kfrac"]])
+    su = gscf.Setup(spec, kfrac=kfrac)
+    assert su.npw[0] == fx["npw"]
+    ctx = su.context(lib.DEREK_FP64)
+    ctx.scf_init(spec.nelec, kT=spec.kT, seed=spec.seed)
+    ctx.scf_step()  # V_in of the first step: V_ext + V_H + V_xc of the uniform density
+    v = ctx.get_potential()
+    print(f"mean potential {np.mean(v):.12f} (oracle {fx['v_mean']:.12f})")
+    assert abs(np.mean(v) - fx["v_mean"]) < 1e-10
+    ctx.set_vloc(v)
+    rmax = ctx.nscf(max_iter=400, res_tol=1e-9)
+    eg = ctx.eigenvalues()[0]
+    ctx.close()
+    ref = np.array(fx["eigs"])
+    d = np.abs(eg - ref)
+    print(f"residual {rmax:.2e}; max |d eps| {d.max():.2e} Ha at band {d.argmax()}")
+    assert rmax < 1e-9
+    assert d.max() < 1e-8
+
+
+def test_kmesh144_mixed_lands_on_fp64():
+    """The full 8-k-point, 300-band SCF in FP64 (batched, graph replay) and in
+    the mixed form (BF16-split GEMMs, autotuned, graph replay), each to the
+    benchmark's stop test |dE| < 1e-6 Ha: the mixed total energy within
+    1e-6 Ha of the FP64 one."""
+    print("kmesh144 FP64 vs mixed", flush=True)
+    spec = config.workload("kmesh144")
+    su = gscf.Setup(spec)
+    r64 = gscf.run(spec, max_scf=60, energy_tol=1e-6, setup=su, log=print, kbatch=0, graph=True)
+    rmx = gscf.run(spec, max_scf=60, energy_tol=1e-6, setup=su, precision=lib.DEREK_MIXED, log=print, kbatch=0,
+                   graph=True, autotune=True)
+    de = abs(r64["etot"] - rmx["etot"])
+    print(f"FP64 {r64['etot']:.10f} ({r64['iterations']} steps, {r64['wall']:.2f} s); mixed {rmx['etot']:.10f} "
+          f"({rmx['iterations']} steps, {rmx['fp32_steps']} FP32, {rmx['wall']:.2f} s); dE = {de:.2e}")
+    assert r64["converged"] and rmx["converged"] and rmx["fp32_steps"] >= 2
+    assert de < 1e-6
