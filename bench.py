@@ -38,7 +38,8 @@ system (one core and every core) and the oracle's whole SCF of the 32^3
 system (one core).
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-scf] [--scf gamma32,gamma108,kmesh72,kmesh144]
-                [--cpu-seconds S] [--no-cpu] [--fixed-shapes | --shapes BPW,ITEMS,FUSED] [--bf16 N]
+                [--cpu-seconds S] [--no-cpu] [--fixed-shapes | --shapes BPW,ITEMS,FUSED] [--tiles ZS,ROUNDS]
+                [--bf16 N]
 """
 import argparse
 import json
@@ -237,6 +238,9 @@ def main():
     ap.add_argument("--shapes", default=None,
                     help="BPW,ITEMS,FUSED: time H|psi> at these launch shapes (bands per workgroup, items per "
                          "launch, fused passes: bit 0 H|psi>, bit 1 density) instead of autotuned ones")
+    ap.add_argument("--tiles", default=None,
+                    help="ZS,ROUNDS: time H|psi> with these FFT/GEMM tiles (sticks per z-pass workgroup 16/32, "
+                         "overlap chunk rounds 1/2/4) instead of autotuned ones")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -263,12 +267,15 @@ def main():
     if args.copy_calib > 0:
         copy_calibration(args.copy_calib)
     shapes = [int(x) for x in args.shapes.split(",")] if args.shapes else None
-    tuned = not (args.fixed_shapes or shapes)
+    tiles = [int(x) for x in args.tiles.split(",")] if args.tiles else None
+    tuned = not (args.fixed_shapes or shapes or tiles)
     ctx = su.context()
     ctx.scf_init(spec.nelec, kT=spec.kT, seed=spec.seed, autotune=tuned)
     ctx.set_vloc(su.vext)
     if shapes:
         ctx.set_launch(shapes[0], shapes[1], bool(shapes[2] & 1), bool(shapes[2] & 2))
+    if tiles:
+        ctx.set_tiles(tiles[0], tiles[1])
     tune64 = ctx.tuning()
     nb = spec.nbands
     log(f"# setup {time.perf_counter() - t0:.2f}s grid {su.dims} npw {su.npw[0]} bands {nb}")
@@ -281,6 +288,8 @@ def main():
     ctx.set_vloc(su.vext)
     if shapes:
         ctx.set_launch(shapes[0], shapes[1], bool(shapes[2] & 1), bool(shapes[2] & 2))
+    if tiles:
+        ctx.set_tiles(tiles[0], tiles[1])
     tune32 = ctx.tuning()
     bps32, ms32, prof32 = hpsi_rate(ctx, nb, args.steps, args.warmup)
     log(f"# H|psi> FP32: {ms32:.3f} ms per {nb} bands, {bps32:.1f} bands/s, shapes {tune32}")
@@ -427,12 +436,14 @@ def main():
                    "grid": su.dims, "parallelism": "single GPU",
                    "launch_shapes": {"bands_per_wg": tune64["bands_per_wg"],
                                      "items_per_launch": tune64["items_per_launch"],
-                                     "fused_yx": tune64["fused"], "fused_density": tune64["fused_density"]}},
+                                     "fused_yx": tune64["fused"], "fused_density": tune64["fused_density"],
+                                     "z_sticks": tune64["z_sticks"], "gemm_rounds": tune64["gemm_rounds"]}},
         "roofline": roofline, "roofline_valu": roofline_valu, "kernels": kernels,
         "hpsi_fp32": {"value": round(bps32, 2), "unit": "bands/s", "ms_per_step": round(ms32, 4),
                       "launch_shapes": {"bands_per_wg": tune32["bands_per_wg"],
                                         "items_per_launch": tune32["items_per_launch"],
-                                        "fused_yx": tune32["fused"], "fused_density": tune32["fused_density"]},
+                                        "fused_yx": tune32["fused"], "fused_density": tune32["fused_density"],
+                                        "z_sticks": tune32["z_sticks"], "gemm_rounds": tune32["gemm_rounds"]},
                       "kernels": kernels32},
         "scf": scf_info, "cpu_baseline": cpu, "cpu_baseline_all_cores": cpu_all, "cpu_baseline_scf": cpu_scf,
     }

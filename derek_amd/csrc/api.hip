@@ -177,7 +177,7 @@ void derek_ctx::apply_h(int k0, int nkc, PwBlock<R> psi, PwBlock<R> hpsi, int nb
   const int64_t nyz = (int64_t)dims[1] * dims[2];
   const bool fz = use_slab<R>(kFusedHpsi);
   for_item_launches(this, k0, nkc, nbands, [&](int ka, int kk, int kn, int b0, int nbb) {
-    PwBatch B{d_maps, ka, kn, nbb, nst_max, nax_max};
+    PwBatch B{d_maps, ka, kn, nbb, nst_max, nax_max, zsl};
     cx_t<R>* S = sticks<R>(kn * nbb);
     double npw = 0, nst = 0, nax = 0;
     for (int k = ka; k < ka + kn; ++k) {
@@ -218,7 +218,7 @@ void derek_ctx::density(int k0, int nkc, PwBlock<R> psi, int nbands, const doubl
   const bool fz = use_slab<R>(kFusedDensity);
   bool first = true;
   for_item_launches(this, k0, nkc, nbands, [&](int ka, int kk, int kn, int b0, int nbb) {
-    PwBatch B{d_maps, ka, kn, nbb, nst_max, nax_max};
+    PwBatch B{d_maps, ka, kn, nbb, nst_max, nax_max, zsl};
     cx_t<R>* S = sticks<R>(kn * nbb);
     double npw = 0, nst = 0, nax = 0;
     for (int k = ka; k < ka + kn; ++k) {
@@ -1099,13 +1099,16 @@ int32_t derek_hip_apply_h_resident(derek_ctx* c, int32_t k, int32_t nb) {
   });
 }
 
-int32_t derek_hip_get_tuning(derek_ctx* c, int32_t out[5]) {
+int32_t derek_hip_get_tuning(derek_ctx* c, int32_t out[8]) {
   return guard([&] {
     DEREK_REQUIRE(c && out, "null argument");
     out[0] = c->bpw;
     out[1] = c->item_cap;
     scf_tuning(c, out + 2);
     out[4] = c->fused;
+    out[5] = c->zsl ? c->zsl : (c->prec == DEREK_FP64 ? 16 : 32);
+    out[6] = c->ov_rounds;
+    out[7] = 0;
   });
 }
 
@@ -1119,6 +1122,17 @@ int32_t derek_hip_set_launch(derek_ctx* c, int32_t bands_per_wg, int32_t items_p
     c->bpw = bands_per_wg;
     c->item_cap = items_per_launch;
     c->fused = fused;
+    ++c->graph_epoch;
+  });
+}
+
+int32_t derek_hip_set_tiles(derek_ctx* c, int32_t z_sticks, int32_t gemm_rounds) {
+  return guard([&] {
+    DEREK_REQUIRE(c, "null context");
+    DEREK_REQUIRE(z_sticks == 0 || z_sticks == 16 || z_sticks == 32, "sticks per z-pass workgroup must be 0, 16 or 32");
+    DEREK_REQUIRE(gemm_rounds == 1 || gemm_rounds == 2 || gemm_rounds == 4, "GEMM rounds must be 1, 2 or 4");
+    c->zsl = z_sticks;
+    c->ov_rounds = gemm_rounds;
     ++c->graph_epoch;
   });
 }

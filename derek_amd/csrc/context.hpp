@@ -155,6 +155,11 @@ struct derek_ctx {
   // 1..3 on the BF16 ones with each value split into that many bf16 terms
   // (set to 2 at init for DEREK_MIXED contexts)
   int gemm_bf16 = 0;
+  // sticks per workgroup of the z passes (16 or 32; 0: 16 for complex128,
+  // 32 for complex64) and the rounds of resident workgroups the overlaps'
+  // row chunks fill (1, 2 or 4); both autotuned by the SCF engine when asked
+  int zsl = 0;
+  int ov_rounds = 2;
   // Moves whenever a work buffer below is freed or reallocated, or a launch
   // setting changes (set_launch, set_gemm): a graph captured at another
   // epoch may hold freed addresses or stale shapes and is not replayed.

@@ -444,7 +444,7 @@ __global__ __launch_bounds__(RD_E * RD_G) void k_overlap_reduce(const double2* _
 template <typename R>
 void overlap(const cx_t<R>* X, int64_t ldx, int64_t sx, const cx_t<R>* Y, int64_t ldy, int64_t sy, int m, int n,
              int64_t npw, const int* npwk, double2* C, int ldc, int64_t sc, int batch, double2* work,
-             int64_t work_cap, bool hermitian, hipStream_t s, int bf16_terms, int64_t max_chunks) {
+             int64_t work_cap, bool hermitian, hipStream_t s, int bf16_terms, int64_t max_chunks, int rounds) {
   DEREK_REQUIRE(m > 0 && n > 0 && batch > 0, "overlap: empty block");
   DEREK_REQUIRE(!hermitian || m == n, "overlap: hermitian needs a square result");
   DEREK_REQUIRE(bf16_terms == 0 || (std::is_same<R, float>::value && bf16_terms >= 1 && bf16_terms <= 3),
@@ -453,7 +453,8 @@ void overlap(const cx_t<R>* X, int64_t ldx, int64_t sx, const cx_t<R>* Y, int64_
                     : bf16_terms == 2 ? ov_slots_bf16<2>()
                     : bf16_terms == 3 ? ov_slots_bf16<3>()
                                       : ov_slots<R>();
-  OvGeom g = ov_geom(m, n, npw, hermitian, batch, slots, bf16_terms ? BF_KS : OV_KS);
+  DEREK_REQUIRE(rounds == 1 || rounds == 2 || rounds == 4, "overlap: rounds must be 1, 2 or 4");
+  OvGeom g = ov_geom(m, n, npw, hermitian, batch, slots, bf16_terms ? BF_KS : OV_KS, rounds);
   ov_limit_chunks(g, max_chunks, bf16_terms ? BF_KS : OV_KS);
   DEREK_REQUIRE((int64_t)batch * g.nchunk * m * n <= work_cap, "overlap: partial-sum buffer too small");
   const int64_t items = (int64_t)g.ntiles * g.nchunk * batch;
@@ -715,10 +716,10 @@ void rotate(const cx_t<R>* X, int64_t ldx, int64_t sx, const double2* U, int ldu
 
 template void overlap<double>(const double2*, int64_t, int64_t, const double2*, int64_t, int64_t, int, int, int64_t,
                               const int*, double2*, int, int64_t, int, double2*, int64_t, bool, hipStream_t, int,
-                              int64_t);
+                              int64_t, int);
 template void overlap<float>(const float2*, int64_t, int64_t, const float2*, int64_t, int64_t, int, int, int64_t,
                              const int*, double2*, int, int64_t, int, double2*, int64_t, bool, hipStream_t, int,
-                             int64_t);
+                             int64_t, int);
 template void rotate<double>(const double2*, int64_t, int64_t, const double2*, int, int64_t, int, int, int64_t,
                              const int*, double2*, int64_t, int64_t, int, hipStream_t, int);
 template void rotate<float>(const float2*, int64_t, int64_t, const double2*, int, int64_t, int, int, int64_t,

@@ -20,12 +20,13 @@ struct OvGeom {
 
 inline int64_t ov_ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
-// Row chunks per tile: at least two full rounds of workgroups over the
+// Row chunks per tile: at least `rounds` full rounds of workgroups over the
 // chip's resident slots (`slots` = resident workgroups per CU x CUs), then,
 // among the next few counts, the one that leaves the smallest idle tail in
 // the last round; every chunk keeps at least 256 rows and a whole number of
-// `ks`-row slabs.
-inline OvGeom ov_geom(int m, int n, int64_t npw, bool herm, int batch, int slots, int ks = OV_KS) {
+// `ks`-row slabs.  The SCF autotuner picks `rounds` among kOvRounds.
+constexpr int kOvRounds[3] = {1, 2, 4};
+inline OvGeom ov_geom(int m, int n, int64_t npw, bool herm, int batch, int slots, int ks = OV_KS, int rounds = 2) {
   OvGeom g{};
   g.m = m;
   g.n = n;
@@ -36,7 +37,7 @@ inline OvGeom ov_geom(int m, int n, int64_t npw, bool herm, int batch, int slots
   g.ntiles = herm ? g.tm * (g.tm + 1) / 2 : g.tm * g.tn;
   const int64_t units = (int64_t)g.ntiles * std::max(1, batch);
   const int64_t cmax = std::max<int64_t>(1, npw / 256);
-  const int64_t c0 = std::min(cmax, std::max<int64_t>(1, ov_ceil_div(2 * (int64_t)slots, units)));
+  const int64_t c0 = std::min(cmax, std::max<int64_t>(1, ov_ceil_div((int64_t)rounds * slots, units)));
   int64_t best = c0;
   double best_eff = -1.0;
   for (int64_t c = c0; c <= std::min(cmax, c0 + 16); ++c) {
@@ -66,12 +67,14 @@ inline void ov_limit_chunks(OvGeom& g, int64_t max_chunks, int ks) {
 
 // double2 elements of the partial sums of an m x n overlap of `batch`
 // k-points, for any of the kernels whose resident slots are listed (both
-// slab depths, Hermitian or not).
+// slab depths, Hermitian or not, every choice of rounds).
 inline int64_t ov_work_elems(int m, int n, int64_t npw, int batch, const int* slots, int nslots) {
   int64_t chunks = 1;
   for (int i = 0; i < nslots; ++i)
     for (int ks : {OV_KS, BF_KS})
-      for (bool h : {false, m == n}) chunks = std::max<int64_t>(chunks, ov_geom(m, n, npw, h, batch, slots[i], ks).nchunk);
+      for (bool h : {false, m == n})
+        for (int rounds : kOvRounds)
+          chunks = std::max<int64_t>(chunks, ov_geom(m, n, npw, h, batch, slots[i], ks, rounds).nchunk);
   return (int64_t)batch * chunks * m * n;
 }
 

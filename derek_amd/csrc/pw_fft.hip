@@ -661,7 +661,8 @@ template <int N, int SL_> struct StickSplit {
   static constexpr size_t lds(size_t csize) { return csize * ((size_t)N + (size_t)SL * LS); }
 };
 
-template <typename R, int N> using StickSplitR = StickSplit<N, sizeof(R) == 8 ? 16 : 32>;
+// sticks per workgroup of a batch: its choice, or 16 (FP64) / 32 (FP32)
+template <typename R> int stick_block(const PwBatch& B) { return B.zsl ? B.zsl : (sizeof(R) == 8 ? 16 : 32); }
 
 // S1 + S2 of one transform on a padded LDS line, in place; thread (line, j)
 // with j < TPL, `act` false for lines past the block's last stick
@@ -699,12 +700,12 @@ __device__ __forceinline__ void stick_two_stage(C* line, const C* twl, int j, bo
 }
 
 // psi[b][perm[p]] -> S[item][s][:] for the block's sticks
-template <typename R, int N, bool ZM>
-__global__ __launch_bounds__((StickSplitR<R, N>::NT)) void k_sticks_z_inv2(PwBatch B, PwBlock<R> psi,
+template <typename R, int N, bool ZM, int SL_>
+__global__ __launch_bounds__((StickSplit<N, SL_>::NT)) void k_sticks_z_inv2(PwBatch B, PwBlock<R> psi,
                                                                      const cx_t<R>* __restrict__ tw,
                                                                      cx_t<R>* __restrict__ S) {
   using C = cx_t<R>;
-  using SS = StickSplitR<R, N>;
+  using SS = StickSplit<N, SL_>;
   constexpr int NT = SS::NT, SL = SS::SL, LS = SS::LS, TPL = SS::TPL;
   extern __shared__ __align__(16) unsigned char smem[];
   const PwMaps m = B.mt[B.k0 + blockIdx.z];
@@ -740,12 +741,12 @@ __global__ __launch_bounds__((StickSplitR<R, N>::NT)) void k_sticks_z_inv2(PwBat
 }
 
 // hpsi[b][perm[p]] = (z FFT of the stick)[pw_z[p]] + kin[perm[p]] psi[b][perm[p]]
-template <typename R, int N, bool ZM>
-__global__ __launch_bounds__((StickSplitR<R, N>::NT)) void k_sticks_z_fwd2(PwBatch B, const cx_t<R>* __restrict__ S,
+template <typename R, int N, bool ZM, int SL_>
+__global__ __launch_bounds__((StickSplit<N, SL_>::NT)) void k_sticks_z_fwd2(PwBatch B, const cx_t<R>* __restrict__ S,
                                                                      const cx_t<R>* __restrict__ tw, PwBlock<R> psi,
                                                                      PwBlock<R> hpsi) {
   using C = cx_t<R>;
-  using SS = StickSplitR<R, N>;
+  using SS = StickSplit<N, SL_>;
   constexpr int NT = SS::NT, SL = SS::SL, LS = SS::LS, TPL = SS::TPL;
   extern __shared__ __align__(16) unsigned char smem[];
   const PwMaps m = B.mt[B.k0 + blockIdx.z];
@@ -806,26 +807,42 @@ template <typename R> static void check_offsets(const PwBatch& B, int64_t nyz, i
                 "one item's sticks or planes exceed 2 GiB");
 }
 
-template <typename R, int N>
-static void sticks_z_inv2(const PwBatch& B, PwBlock<R> psi, const Axis<R>* ax, cx_t<R>* S, bool zmajor,
-                          hipStream_t s) {
-  using SS = StickSplitR<R, N>;
+template <typename R, int N, int SL>
+static void sticks_z_inv2_sl(const PwBatch& B, PwBlock<R> psi, const Axis<R>* ax, cx_t<R>* S, bool zmajor,
+                             hipStream_t s) {
+  using SS = StickSplit<N, SL>;
   const size_t lds = SS::lds(sizeof(cx_t<R>));
-  auto kern = zmajor ? k_sticks_z_inv2<R, N, true> : k_sticks_z_inv2<R, N, false>;
+  check_lds_pw(lds, N);
+  auto kern = zmajor ? k_sticks_z_inv2<R, N, true, SL> : k_sticks_z_inv2<R, N, false, SL>;
   allow_lds_pw(kern, lds);
   dim3 grid((unsigned)ceil_div(B.nst_cap, SS::SL), B.nbb, B.nkc);
   hipLaunchKernelGGL(kern, grid, dim3(SS::NT), lds, s, B, psi, ax[2].tw, S);
 }
 
-template <typename R, int N>
-static void sticks_z_fwd2(const PwBatch& B, const cx_t<R>* S, const Axis<R>* ax, PwBlock<R> psi, PwBlock<R> hpsi,
-                          bool zmajor, hipStream_t s) {
-  using SS = StickSplitR<R, N>;
+template <typename R, int N, int SL>
+static void sticks_z_fwd2_sl(const PwBatch& B, const cx_t<R>* S, const Axis<R>* ax, PwBlock<R> psi, PwBlock<R> hpsi,
+                             bool zmajor, hipStream_t s) {
+  using SS = StickSplit<N, SL>;
   const size_t lds = SS::lds(sizeof(cx_t<R>));
-  auto kern = zmajor ? k_sticks_z_fwd2<R, N, true> : k_sticks_z_fwd2<R, N, false>;
+  check_lds_pw(lds, N);
+  auto kern = zmajor ? k_sticks_z_fwd2<R, N, true, SL> : k_sticks_z_fwd2<R, N, false, SL>;
   allow_lds_pw(kern, lds);
   dim3 grid((unsigned)ceil_div(B.nst_cap, SS::SL), B.nbb, B.nkc);
   hipLaunchKernelGGL(kern, grid, dim3(SS::NT), lds, s, B, S, ax[2].tw, psi, hpsi);
+}
+
+template <typename R, int N>
+static void sticks_z_inv2(const PwBatch& B, PwBlock<R> psi, const Axis<R>* ax, cx_t<R>* S, bool zmajor,
+                          hipStream_t s) {
+  if (stick_block<R>(B) == 32) sticks_z_inv2_sl<R, N, 32>(B, psi, ax, S, zmajor, s);
+  else sticks_z_inv2_sl<R, N, 16>(B, psi, ax, S, zmajor, s);
+}
+
+template <typename R, int N>
+static void sticks_z_fwd2(const PwBatch& B, const cx_t<R>* S, const Axis<R>* ax, PwBlock<R> psi, PwBlock<R> hpsi,
+                          bool zmajor, hipStream_t s) {
+  if (stick_block<R>(B) == 32) sticks_z_fwd2_sl<R, N, 32>(B, S, ax, psi, hpsi, zmajor, s);
+  else sticks_z_fwd2_sl<R, N, 16>(B, S, ax, psi, hpsi, zmajor, s);
 }
 
 template <typename R>

@@ -31,6 +31,9 @@ struct PwBatch {
   const PwMaps* mt = nullptr;
   int k0 = 0, nkc = 1, nbb = 1;
   int nst_cap = 0, nax_cap = 0;
+  // sticks per workgroup of the compiled z passes (16 or 32; 0: 16 for
+  // complex128, 32 for complex64)
+  int zsl = 0;
 };
 
 // Host construction of the maps from flat FFT-bin indices (z fastest).

@@ -22,6 +22,7 @@
 #include <chrono>
 #include <cmath>
 
+#include "ov_geom.hpp"
 #include "scf.hpp"
 #include "subspace.hpp"
 
@@ -119,7 +120,7 @@ void ov(derek_ctx* c, const cx_t<R>* X, const cx_t<R>* Yp, int m, int n, int k0,
   const double npw = npw_sum(c, k0, nkc);
   c->timed(KC_OVERLAP, (double)(herm ? m : m + n) * npw * sizeof(cx_t<R>), 8.0 * npw * m * n, [&] {
     overlap<R>(X, E.ldk, E.kstride, Yp, E.ldk, E.kstride, m, n, c->max_npw, c->d_npw + k0, C, m, (int64_t)m * n, nkc,
-               E.ovw, E.ovw_cap, herm, c->s, bf16_terms<R>(c));
+               E.ovw, E.ovw_cap, herm, c->s, bf16_terms<R>(c), 0, c->ov_rounds);
   });
 }
 
@@ -143,7 +144,7 @@ void ov_block(derek_ctx* c, const cx_t<R>* X, int ca, const cx_t<R>* Yp, int cb,
   c->timed(KC_OVERLAP, (double)(herm ? nb : 2 * nb) * npw * sizeof(cx_t<R>), 8.0 * npw * nb * nb, [&] {
     overlap<R>(X + (int64_t)ca * E.ldk, E.ldk, E.kstride, Yp + (int64_t)cb * E.ldk, E.ldk, E.kstride, nb, nb,
                c->max_npw, c->d_npw + k0, C + ca + (int64_t)cb * m2, m2, (int64_t)m2 * m2, nkc, E.ovw, E.ovw_cap,
-               herm, c->s, bf16_terms<R>(c));
+               herm, c->s, bf16_terms<R>(c), 0, c->ov_rounds);
   });
 }
 
@@ -450,6 +451,75 @@ template <typename R> void autotune(derek_ctx* c) {
   const int all = nkc * nbt, cap0 = c->item_cap;
   for (int q : {cap0 / 2, cap0 * 2})
     if (q >= 1 && q <= cap_mem && q <= std::max(all, cap0)) try_shape(c->bpw, c->fused, q);
+  // FFT tile: the other number of sticks per z-pass workgroup
+  {
+    const int zprev = c->zsl;
+    const int z0 = zprev ? zprev : (sizeof(R) == 8 ? 16 : 32);
+    c->zsl = z0 == 16 ? 32 : 16;
+    const float t = time_it();
+    if (t < kGain * best) best = t;
+    else c->zsl = zprev;
+  }
+  // One launch per pass over every (k-point, band) item of the first batch
+  // (the whole SCF batch), against launches of item_cap items, timed on that
+  // whole batch; the pipeline buffers may then take up to 24 GiB.
+  {
+    const int full = std::min(E.kc, c->nk) * c->nb;
+    const int cap_all = (int)std::max(1.0, std::floor(24.0 * 1073741824.0 / item_bytes));
+    if (full > c->item_cap && full <= cap_all) {
+      const int nkf = std::min(E.kc, c->nk), nbf = c->nb;
+      auto time_full = [&] {
+        float tb = 1e30f;
+        c->apply_h<R>(0, nkf, x, hx, nbf);
+        for (int r = 0; r < 2; ++r) {
+          DEREK_HIP(hipEventRecord(a, c->s));
+          c->apply_h<R>(0, nkf, x, hx, nbf);
+          DEREK_HIP(hipEventRecord(b, c->s));
+          DEREK_HIP(hipEventSynchronize(b));
+          float t = 0.f;
+          DEREK_HIP(hipEventElapsedTime(&t, a, b));
+          tb = std::min(tb, t);
+        }
+        return tb;
+      };
+      const int c0 = c->item_cap;
+      const float t0 = time_full();
+      c->item_cap = full;
+      const float t1 = time_full();
+      if (!(t1 < kGain * t0)) c->item_cap = c0;
+    }
+  }
+  // GEMM chunks: the rounds of resident workgroups the split-K row chunks of
+  // the overlaps fill, timed on the Davidson step's 2nb x 2nb Hermitian
+  // overlap of the first batch
+  {
+    const int nkg = std::min(E.kc, c->nk);
+    cx_t<R>* Yb = reinterpret_cast<cx_t<R>*>(E.Y);
+    auto time_ov = [&] {
+      float tb = 1e30f;
+      ov<R>(c, Yb, Yb, E.m2, E.m2, 0, nkg, E.S, true);
+      for (int r = 0; r < 2; ++r) {
+        DEREK_HIP(hipEventRecord(a, c->s));
+        ov<R>(c, Yb, Yb, E.m2, E.m2, 0, nkg, E.S, true);
+        DEREK_HIP(hipEventRecord(b, c->s));
+        DEREK_HIP(hipEventSynchronize(b));
+        float t = 0.f;
+        DEREK_HIP(hipEventElapsedTime(&t, a, b));
+        tb = std::min(tb, t);
+      }
+      return tb;
+    };
+    const int r0 = c->ov_rounds;
+    float bestg = time_ov();
+    for (int q : kOvRounds) {
+      if (q == r0) continue;
+      const int prev = c->ov_rounds;
+      c->ov_rounds = q;
+      const float t = time_ov();
+      if (t < kGain * bestg) bestg = t;
+      else c->ov_rounds = prev;
+    }
+  }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   c->prof.on = prof;

@@ -466,8 +466,15 @@ module derek_hip_m
      integer(c_int32_t) function derek_hip_get_tuning(ctx, out) bind(C, name='derek_hip_get_tuning')
        import :: c_int32_t, c_ptr
        type(c_ptr), value :: ctx
-       integer(c_int32_t), dimension(5) :: out
+       integer(c_int32_t), dimension(8) :: out
      end function derek_hip_get_tuning
+
+     ! z-pass sticks per workgroup (0, 16, 32) and overlap chunk rounds (1, 2, 4)
+     integer(c_int32_t) function derek_hip_set_tiles(ctx, z_sticks, gemm_rounds) bind(C, name='derek_hip_set_tiles')
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: z_sticks, gemm_rounds
+     end function derek_hip_set_tiles
 
      integer(c_int32_t) function derek_hip_set_launch(ctx, bands_per_wg, items_per_launch, fused) &
           bind(C, name='derek_hip_set_launch')
@@ -495,7 +502,7 @@ module derek_hip_m
        derek_basis_std_index, derek_io_pot_write, derek_io_pot_read, derek_io_den_write, derek_io_den_read, &
        derek_io_wvfn_write, derek_io_wvfn_read, derek_hip_nscf, derek_info_read, derek_info_free, derek_info_num, &
        derek_info_str, derek_info_present, derek_info_cell, derek_info_wells, derek_info_log, derek_info_preset_pot, &
-       derek_info_help, derek_hip_get_tuning, derek_hip_set_launch, derek_hip_set_gemm
+       derek_info_help, derek_hip_get_tuning, derek_hip_set_launch, derek_hip_set_gemm, derek_hip_set_tiles
 
 contains
 

@@ -33,7 +33,7 @@ EXPORTS = (
     "derek_hip_get_occupations", "derek_hip_get_density", "derek_hip_get_potential", "derek_hip_get_psi", "derek_hip_put_psi",
     "derek_hip_apply_h_resident", "derek_hip_sync", "derek_hip_stream",
     "derek_hip_profile_enable", "derek_hip_profile_reset", "derek_hip_profile_read", "derek_hip_get_tuning",
-    "derek_hip_set_launch", "derek_hip_set_gemm",
+    "derek_hip_set_launch", "derek_hip_set_gemm", "derek_hip_set_tiles",
 )
 
 KERNEL_CLASSES = ("sticks_z_inv", "planes_y_inv", "planes_x_vmul", "planes_y_fwd", "sticks_z_fwd", "planes_x_density",
@@ -509,11 +509,17 @@ class Context:
         check(self.lib.derek_hip_scf_init(self.h, C.byref(p)))
 
     def tuning(self):
-        """{bands_per_wg, items_per_launch, kbatch, graph, fused}"""
-        out = (C.c_int32 * 5)()
+        """{bands_per_wg, items_per_launch, kbatch, graph, fused, fused_density, z_sticks, gemm_rounds}"""
+        out = (C.c_int32 * 8)()
         check(self.lib.derek_hip_get_tuning(self.h, out))
         return {"bands_per_wg": out[0], "items_per_launch": out[1], "kbatch": out[2], "graph": bool(out[3]),
-                "fused": bool(out[4] & 1), "fused_density": bool(out[4] & 2)}
+                "fused": bool(out[4] & 1), "fused_density": bool(out[4] & 2), "z_sticks": out[5],
+                "gemm_rounds": out[6]}
+
+    def set_tiles(self, z_sticks=0, gemm_rounds=2):
+        """Sticks per z-pass workgroup (16, 32; 0: the precision's default) and
+        the rounds of resident workgroups the overlaps' row chunks fill (1, 2, 4)."""
+        check(self.lib.derek_hip_set_tiles(self.h, int(z_sticks), int(gemm_rounds)))
 
     def set_launch(self, bands_per_wg=2, items_per_launch=256, fused=True, fused_density=None):
         """Launch shapes; fused / fused_density: the fused y/x passes for H|psi>

@@ -40,12 +40,15 @@ def fused_expected(n, precision):
 
 @pytest.mark.parametrize("n", [32, 72, 108, 144])
 @pytest.mark.parametrize("precision", [lib.DEREK_FP64, lib.DEREK_MIXED])
-def test_fused_apply_h_matches_oracle(n, precision):
-    print("fused apply_h", n, precision, flush=True)
+@pytest.mark.parametrize("z_sticks", [16, 32])
+def test_fused_apply_h_matches_oracle(n, precision, z_sticks):
+    """z_sticks: sticks per workgroup of the z passes."""
+    print("fused apply_h", n, precision, "z sticks", z_sticks, flush=True)
     su = gscf.Setup(grid_spec(n))
     assert tuple(su.dims) == (n, n, n)
     nb = 5
     ctx = su.context(precision, nbands=nb)
+    ctx.set_tiles(z_sticks, 2)
     rng = np.random.default_rng(n)
     v = su.vext + 0.3 * rng.normal(size=su.N)
     ctx.set_vloc(v)
