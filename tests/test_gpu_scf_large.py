@@ -40,10 +40,12 @@ def fixture(name):
         return json.load(f)
 
 
-def converge(spec, precision, **kw):
+def converge(spec, precision, fx, **kw):
+    """The SCF to the fixture's stop test, with its Davidson steps per SCF step."""
     su = gscf.Setup(spec)
     ctx = su.context(precision)
-    res = gscf.run(spec, max_scf=80, energy_tol=1e-11, rho_tol=1e-8, setup=su, ctx=ctx, log=print, **kw)
+    res = gscf.run(spec, max_scf=80, energy_tol=1e-11, rho_tol=1e-8, setup=su, ctx=ctx, log=print,
+                   n_david_first=fx.get("n_david_first", 4), n_david=fx.get("n_david", 2), **kw)
     rmax = ctx.scf_polish(1000, 1e-9)
     eg = ctx.eigenvalues()
     rho = ctx.get_density()
@@ -75,10 +77,10 @@ def test_gamma108_matches_oracle_fixture(mode):
     fx = fixture("scf_gamma108.json")
     spec = config.workload("gamma108")
     if mode == "fp64_plain":
-        out = converge(spec, lib.DEREK_FP64, kbatch=1)
+        out = converge(spec, lib.DEREK_FP64, fx, kbatch=1)
         compare(fx, *out, e_tol=1e-7)
     else:
-        out = converge(spec, lib.DEREK_MIXED, kbatch=0, graph=True)
+        out = converge(spec, lib.DEREK_MIXED, fx, kbatch=0, graph=True)
         assert out[0]["fp32_steps"] >= 2
         compare(fx, *out, e_tol=1e-6)
 
@@ -89,14 +91,14 @@ def test_kmesh72_matches_oracle_fixture(mode):
     fx = fixture("scf_kmesh72.json")
     spec = config.workload("kmesh72")
     if mode == "fp64_plain":
-        out = converge(spec, lib.DEREK_FP64, kbatch=1)
+        out = converge(spec, lib.DEREK_FP64, fx, kbatch=1)
         compare(fx, *out, e_tol=1e-7)
     elif mode == "fp64_batched_graph":
-        out = converge(spec, lib.DEREK_FP64, kbatch=0, graph=True)
+        out = converge(spec, lib.DEREK_FP64, fx, kbatch=0, graph=True)
         assert out[0]["tuning"]["kbatch"] == 64 and out[0]["tuning"]["graph"]
         compare(fx, *out, e_tol=1e-7)
     else:
-        out = converge(spec, lib.DEREK_MIXED, kbatch=0, graph=True, autotune=True)
+        out = converge(spec, lib.DEREK_MIXED, fx, kbatch=0, graph=True, autotune=True)
         assert out[0]["fp32_steps"] >= 2
         compare(fx, *out, e_tol=1e-6)
 
