@@ -203,6 +203,38 @@ def scf_step_roofline(su, nb, n_david, s_psi, t_step_s):
             "t_step_ms": round(t_step_s * 1e3, 3), "frac": round(t_roof / t_step_s, 4)}
 
 
+def items_sweep(gscf, lib, config, name, caps, precision, steps=5):
+    """H|psi> over every (k-point, band) item of a system's resident bands (all
+    k-points in one batch) with at most `cap` items per pipeline launch, for
+    each cap (0: every item in one launch per pass): ms per application and
+    the launches per pass."""
+    import numpy as np
+    spec = config.workload(name)
+    su = gscf.Setup(spec)
+    ctx = su.context(precision)
+    ctx.scf_init(spec.nelec, kT=spec.kT, seed=spec.seed)
+    ctx.set_vloc(su.vext)
+    nk, nb = su.nk, spec.nbands
+    out = []
+    t0 = ctx.tuning()
+    for cap in caps:
+        c = cap if cap > 0 else nk * nb
+        ctx.set_launch(t0["bands_per_wg"], c, t0["fused"], t0["fused_density"])
+        for _ in range(2):
+            ctx.apply_h_batch()
+        ctx.sync()
+        t = time.perf_counter()
+        for _ in range(steps):
+            ctx.apply_h_batch()
+        ctx.sync()
+        ms = (time.perf_counter() - t) / steps * 1e3
+        out.append({"items_per_launch": c, "launches_per_pass": int(np.ceil(nk * nb / c)), "ms": round(ms, 3),
+                    "items": nk * nb, "us_per_item": round(ms * 1e3 / (nk * nb), 3)})
+        log(f"# {name} H|psi> over {nk * nb} items, {c} per launch: {ms:.3f} ms")
+    ctx.close()
+    return out
+
+
 def scf_run(gscf, lib, spec, su, precision, verbose=False, **kw):
     """Time to |dE| < 1e-6 Ha from a fresh engine (init included)."""
     ctx = su.context(precision)
@@ -230,6 +262,9 @@ def main():
     ap.add_argument("--bf16", type=int, default=2,
                     help="bf16 terms per value of the mixed runs' GEMMs (0: FP32 matrix cores, no repeat)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--items-sweep", default="",
+                    help="SYSTEM: also time H|psi> over every resident (k-point, band) item of that system at 64, "
+                         "128, 256, 512 items per launch and all items in one launch (e.g. kmesh72)")
     ap.add_argument("--no-graph", action="store_true",
                     help="run the batched SCF modes eagerly (no hipGraph replay), e.g. under a kernel tracer")
     ap.add_argument("--fixed-shapes", action="store_true",
@@ -426,6 +461,13 @@ def main():
                 max_deig_vs_fp64_ha=float(np.max(np.abs(e_bx - e_f))),
                 speedup_vs_fp64=round(f64["time_to_scf_s"] / bx144["time_to_scf_s"], 3))
 
+    sweep = None
+    if args.items_sweep:
+        sweep = {"system": args.items_sweep,
+                 "fp64": items_sweep(gscf, lib, config, args.items_sweep, [64, 128, 256, 512, 0], lib.DEREK_FP64),
+                 "complex64": items_sweep(gscf, lib, config, args.items_sweep, [64, 128, 256, 512, 0],
+                                          lib.DEREK_MIXED)}
+
     out = {
         "metric": METRIC, "value": round(bands_per_s, 2), "unit": "H|psi> bands/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
@@ -446,6 +488,7 @@ def main():
                                         "z_sticks": tune32["z_sticks"], "gemm_rounds": tune32["gemm_rounds"]},
                       "kernels": kernels32},
         "scf": scf_info, "cpu_baseline": cpu, "cpu_baseline_all_cores": cpu_all, "cpu_baseline_scf": cpu_scf,
+        "items_sweep": sweep,
     }
     print(json.dumps(out))
 

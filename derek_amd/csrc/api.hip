@@ -1092,7 +1092,8 @@ int32_t derek_hip_put_psi(derek_ctx* c, int32_t k, const double* psi) {
 int32_t derek_hip_apply_h_resident(derek_ctx* c, int32_t k, int32_t nb) {
   return guard([&] {
     DEREK_REQUIRE(c, "null context");
-    c->require_basis(k);
+    if (k == -1) c->require_all_bases();
+    else c->require_basis(k);
     if (!c->vloc_set) throw StateError("local potential not set");
     DEREK_REQUIRE(nb >= 1 && nb <= c->nb, "nb must be in 1..nbands");
     scf_apply_h_resident(c, k, nb);

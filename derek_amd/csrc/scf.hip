@@ -780,14 +780,16 @@ void scf_psi(derek_ctx* c, int k, double* psi, bool put) {
 
 void scf_apply_h_resident(derek_ctx* c, int k, int nb) {
   ScfEngine& E = engine(c);
+  // k = -1: every k-point in one batch
+  const int k0 = k < 0 ? 0 : k, nkc = k < 0 ? c->nk : 1;
   if (E.fp32) {
-    float2* y = reinterpret_cast<float2*>(E.Y) + k * E.kstride;
-    float2* hy = reinterpret_cast<float2*>(E.HY) + k * E.kstride;
-    c->apply_h<float>(k, 1, PwBlock<float>{y, E.ldk, E.kstride}, PwBlock<float>{hy, E.ldk, E.kstride}, nb);
+    float2* y = reinterpret_cast<float2*>(E.Y) + k0 * E.kstride;
+    float2* hy = reinterpret_cast<float2*>(E.HY) + k0 * E.kstride;
+    c->apply_h<float>(k0, nkc, PwBlock<float>{y, E.ldk, E.kstride}, PwBlock<float>{hy, E.ldk, E.kstride}, nb);
   } else {
-    double2* y = reinterpret_cast<double2*>(E.Y) + k * E.kstride;
-    double2* hy = reinterpret_cast<double2*>(E.HY) + k * E.kstride;
-    c->apply_h<double>(k, 1, PwBlock<double>{y, E.ldk, E.kstride}, PwBlock<double>{hy, E.ldk, E.kstride}, nb);
+    double2* y = reinterpret_cast<double2*>(E.Y) + k0 * E.kstride;
+    double2* hy = reinterpret_cast<double2*>(E.HY) + k0 * E.kstride;
+    c->apply_h<double>(k0, nkc, PwBlock<double>{y, E.ldk, E.kstride}, PwBlock<double>{hy, E.ldk, E.kstride}, nb);
   }
 }
 

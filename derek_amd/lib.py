@@ -583,6 +583,10 @@ class Context:
     def apply_h_resident(self, k, nb):
         check(self.lib.derek_hip_apply_h_resident(self.h, C.c_int32(k), C.c_int32(nb)))
 
+    def apply_h_batch(self):
+        """H on every resident band of every k-point, all k-points in one batch."""
+        check(self.lib.derek_hip_apply_h_resident(self.h, C.c_int32(-1), C.c_int32(self.nbands)))
+
     def profile(self, on=True):
         check(self.lib.derek_hip_profile_enable(self.h, C.c_int32(1 if on else 0)))
 

@@ -303,7 +303,8 @@ int32_t derek_hip_get_potential(derek_ctx* ctx, double* v);
 /* resident wavefunctions of k (npw_k x nbands complex, FP64 on the host) */
 int32_t derek_hip_get_psi(derek_ctx* ctx, int32_t k, double* psi);
 int32_t derek_hip_put_psi(derek_ctx* ctx, int32_t k, const double* psi);
-/* H applied to the resident wavefunctions of k (for timing; result kept on device). */
+/* H applied to the first nb resident wavefunctions of k (for timing; result
+ * kept on device); k = -1: of every k-point, all k-points in one batch. */
 int32_t derek_hip_apply_h_resident(derek_ctx* ctx, int32_t k, int32_t nb);
 /* Per-kernel-class timers: while enabled, HIP events on the context's
  * stream bracket every launch (not inside a captured graph); read returns
