@@ -657,7 +657,13 @@ template <int N, int SL_> struct StickSplit {
   static constexpr int TPL = R1 > R2 ? R1 : R2;
   static constexpr int SL = SL_;       // sticks per workgroup
   static constexpr int NT = SL * TPL;
-  static constexpr int LS = N + 1;     // padded line pitch
+  // intermediate pitch between the stages: sub-transform j's outputs start
+  // at slot j (R1 + 1), so the lanes' stage-1 stores stride by an odd count
+  // (a stride of R1 = 12 complex128 is 48 dwords: lanes j and j + 4 would
+  // share banks)
+  static constexpr int IP = R1 + 1;
+  // padded line pitch (odd), room for the padded intermediate
+  static constexpr int LS = ((N + 1 > R2 * IP ? N + 1 : R2 * IP)) | 1;
   static constexpr size_t lds(size_t csize) { return csize * ((size_t)N + (size_t)SL * LS); }
 };
 
@@ -669,7 +675,7 @@ template <typename R> int stick_block(const PwBatch& B) { return B.zsl ? B.zsl :
 template <int SGN, typename C, int N>
 __device__ __forceinline__ void stick_two_stage(C* line, const C* twl, int j, bool act) {
   using SS = StickSplit<N, 1>;
-  constexpr int R1 = SS::R1, R2 = SS::R2;
+  constexpr int R1 = SS::R1, R2 = SS::R2, IP = SS::IP;
   C v[R1];
   if (act && j < R2) {
 #pragma unroll
@@ -679,14 +685,14 @@ __device__ __forceinline__ void stick_two_stage(C* line, const C* twl, int j, bo
   __syncthreads();
   if (act && j < R2) {
 #pragma unroll
-    for (int k = 0; k < R1; ++k) line[j * R1 + k] = v[k];
+    for (int k = 0; k < R1; ++k) line[j * IP + k] = v[k];
   }
   __syncthreads();
   C u[R2];
   if (act && j < R1) {
 #pragma unroll
     for (int r = 0; r < R2; ++r) {
-      const C y = line[j + R1 * r];
+      const C y = line[r * IP + j];
       u[r] = r ? cmul(y, SGN > 0 ? cconj(twl[(j * r) % N]) : twl[(j * r) % N]) : y;
     }
     dftR<R2, SGN>(u, twl, N);
@@ -1054,10 +1060,19 @@ template <typename R, int N, bool VM = false> struct Slab {
   static constexpr int R1 = Split<N>::R1, R2 = Split<N>::R2;
   static constexpr int R2P = R2 % 2 ? R2 : R2 + 1;
   static constexpr int L = R1 * R2P;              // slots of a line
-  // line pitches of the plane area A and the x-line area X (measured: an
-  // unpadded A is 3-4 % faster for complex64 at 108^3, FP64 is indifferent)
-  static constexpr int LPA = L;
-  static constexpr int LPX = L % 2 ? L : L + 1;
+  // line pitches of the plane area A and the x-line area X.  A 16-byte LDS
+  // access of 16 lanes is conflict-free when their slot indices differ mod
+  // 16; the lanes of one access span several planes or x lines, so the
+  // pitches decide the conflicts.  For complex128 at 108 the pitches 113 / 108
+  // came from a model of every access of the kernel (phase by phase, 16
+  // lanes per pass over 64 four-byte banks): 25 % fewer LDS passes than
+  // 109 / 109 and 22 % fewer than 108 / 109, within the LDS budget of 49
+  // active planes.  (Measured: an unpadded A is 3-4 % faster for complex64
+  // at 108^3.)
+  static constexpr bool kPitch108 = sizeof(R) == 8 && N == 108;
+  static constexpr int LPA = kPitch108 ? 113 : (sizeof(R) == 8 ? (L % 2 ? L : L + 1) : L);
+  static constexpr int LPX = kPitch108 ? 108 : (L % 2 ? L : L + 1);
+  static_assert(LPA >= L && LPX >= L, "line pitches below the slots of a line");
   // 360 threads at N = 72 (not 288): the R1- and R2-thread stages then take
   // 40 and 45 planes per pass, so the 33 active planes of a 16 bohr cell at
   // Ecut 20 Ha need one pass per y stage instead of two
