@@ -38,7 +38,7 @@ system (one core and every core) and the oracle's whole SCF of the 32^3
 system (one core).
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-scf] [--scf gamma32,gamma108,kmesh72,kmesh144]
-                [--cpu-seconds S] [--no-cpu] [--fixed-shapes | --shapes BPW,ITEMS,FUSED] [--tiles ZS,ROUNDS]
+                [--cpu-seconds S] [--no-cpu] [--fixed-shapes | --shapes BPW,ITEMS,FUSED] [--tiles ZS,ROUNDS[,TILE]]
                 [--bf16 N]
 """
 import argparse
@@ -274,8 +274,8 @@ def main():
                     help="BPW,ITEMS,FUSED: time H|psi> at these launch shapes (bands per workgroup, items per "
                          "launch, fused passes: bit 0 H|psi>, bit 1 density) instead of autotuned ones")
     ap.add_argument("--tiles", default=None,
-                    help="ZS,ROUNDS: time H|psi> with these FFT/GEMM tiles (sticks per z-pass workgroup 16/32, "
-                         "overlap chunk rounds 1/2/4) instead of autotuned ones")
+                    help="ZS,ROUNDS[,TILE]: time H|psi> with these FFT/GEMM tiles (sticks per z-pass workgroup "
+                         "16/32, overlap chunk rounds 1/2/4, BF16 overlap tile edge 64/128) instead of autotuned ones")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -310,7 +310,7 @@ def main():
     if shapes:
         ctx.set_launch(shapes[0], shapes[1], bool(shapes[2] & 1), bool(shapes[2] & 2))
     if tiles:
-        ctx.set_tiles(tiles[0], tiles[1])
+        ctx.set_tiles(tiles[0], tiles[1], tiles[2] if len(tiles) > 2 else 64)
     tune64 = ctx.tuning()
     nb = spec.nbands
     log(f"# setup {time.perf_counter() - t0:.2f}s grid {su.dims} npw {su.npw[0]} bands {nb}")
@@ -324,7 +324,7 @@ def main():
     if shapes:
         ctx.set_launch(shapes[0], shapes[1], bool(shapes[2] & 1), bool(shapes[2] & 2))
     if tiles:
-        ctx.set_tiles(tiles[0], tiles[1])
+        ctx.set_tiles(tiles[0], tiles[1], tiles[2] if len(tiles) > 2 else 64)
     tune32 = ctx.tuning()
     bps32, ms32, prof32 = hpsi_rate(ctx, nb, args.steps, args.warmup)
     log(f"# H|psi> FP32: {ms32:.3f} ms per {nb} bands, {bps32:.1f} bands/s, shapes {tune32}")
@@ -479,13 +479,15 @@ def main():
                    "launch_shapes": {"bands_per_wg": tune64["bands_per_wg"],
                                      "items_per_launch": tune64["items_per_launch"],
                                      "fused_yx": tune64["fused"], "fused_density": tune64["fused_density"],
-                                     "z_sticks": tune64["z_sticks"], "gemm_rounds": tune64["gemm_rounds"]}},
+                                     "z_sticks": tune64["z_sticks"], "gemm_rounds": tune64["gemm_rounds"],
+                                     "gemm_tile": tune64["gemm_tile"]}},
         "roofline": roofline, "roofline_valu": roofline_valu, "kernels": kernels,
         "hpsi_fp32": {"value": round(bps32, 2), "unit": "bands/s", "ms_per_step": round(ms32, 4),
                       "launch_shapes": {"bands_per_wg": tune32["bands_per_wg"],
                                         "items_per_launch": tune32["items_per_launch"],
                                         "fused_yx": tune32["fused"], "fused_density": tune32["fused_density"],
-                                        "z_sticks": tune32["z_sticks"], "gemm_rounds": tune32["gemm_rounds"]},
+                                        "z_sticks": tune32["z_sticks"], "gemm_rounds": tune32["gemm_rounds"],
+                                        "gemm_tile": tune32["gemm_tile"]},
                       "kernels": kernels32},
         "scf": scf_info, "cpu_baseline": cpu, "cpu_baseline_all_cores": cpu_all, "cpu_baseline_scf": cpu_scf,
         "items_sweep": sweep,

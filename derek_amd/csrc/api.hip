@@ -838,7 +838,8 @@ int32_t derek_hip_overlap(derek_ctx* c, int64_t npw, int32_t m, int32_t n, const
       convert<float, double>(fx, dx, (int64_t)m * npw, c->s);
       if (!herm) convert<float, double>(fy, dy, (int64_t)n * npw, c->s);
       c->timed(KC_OVERLAP, (double)(m + n) * npw * 8.0, 8.0 * npw * m * n, [&] {
-        overlap<float>(fx, npw, 0, fy, npw, 0, m, n, npw, nullptr, ds, m, 0, 1, w, wcap, herm, c->s, c->gemm_bf16);
+        overlap<float>(fx, npw, 0, fy, npw, 0, m, n, npw, nullptr, ds, m, 0, 1, w, wcap, herm, c->s, c->gemm_bf16, 0,
+                       c->ov_rounds, c->ov_tile);
       });
     }
     DEREK_HIP(hipMemcpyAsync(S, ds, (size_t)m * n * sizeof(double2), hipMemcpyDeviceToHost, c->s));
@@ -1109,7 +1110,7 @@ int32_t derek_hip_get_tuning(derek_ctx* c, int32_t out[8]) {
     out[4] = c->fused;
     out[5] = c->zsl ? c->zsl : (c->prec == DEREK_FP64 ? 16 : 32);
     out[6] = c->ov_rounds;
-    out[7] = 0;
+    out[7] = c->ov_tile;
   });
 }
 
@@ -1127,13 +1128,15 @@ int32_t derek_hip_set_launch(derek_ctx* c, int32_t bands_per_wg, int32_t items_p
   });
 }
 
-int32_t derek_hip_set_tiles(derek_ctx* c, int32_t z_sticks, int32_t gemm_rounds) {
+int32_t derek_hip_set_tiles(derek_ctx* c, int32_t z_sticks, int32_t gemm_rounds, int32_t gemm_tile) {
   return guard([&] {
     DEREK_REQUIRE(c, "null context");
     DEREK_REQUIRE(z_sticks == 0 || z_sticks == 16 || z_sticks == 32, "sticks per z-pass workgroup must be 0, 16 or 32");
     DEREK_REQUIRE(gemm_rounds == 1 || gemm_rounds == 2 || gemm_rounds == 4, "GEMM rounds must be 1, 2 or 4");
+    DEREK_REQUIRE(gemm_tile == 64 || gemm_tile == 128, "GEMM tile must be 64 or 128");
     c->zsl = z_sticks;
     c->ov_rounds = gemm_rounds;
+    c->ov_tile = gemm_tile;
     ++c->graph_epoch;
   });
 }

@@ -160,6 +160,8 @@ struct derek_ctx {
   // row chunks fill (1, 2 or 4); both autotuned by the SCF engine when asked
   int zsl = 0;
   int ov_rounds = 2;
+  // C tile edge of the BF16 overlaps (64 or 128), autotuned as well
+  int ov_tile = 64;
   // Moves whenever a work buffer below is freed or reallocated, or a launch
   // setting changes (set_launch, set_gemm): a graph captured at another
   // epoch may hold freed addresses or stale shapes and is not replayed.

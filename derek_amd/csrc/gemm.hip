@@ -224,8 +224,8 @@ template <int NT> __device__ __forceinline__ f4 bf_products(const bf8 (&A)[NT], 
 // re + sgn*im), each split into NT terms, into the adjacent k slots 2*slot
 // and 2*slot + 1 of LDS column `col` (one 4-byte store per plane and term).
 typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
-template <int NT>
-__device__ __forceinline__ void bf_stage2(float2 v0, float2 v1, float sgn, bf8 (*L)[NT][64][BF_P], int col, int slot) {
+template <int NT, int TT>
+__device__ __forceinline__ void bf_stage2(float2 v0, float2 v1, float sgn, bf8 (*L)[NT][TT][BF_P], int col, int slot) {
   float a[3][2] = {{v0.x, v1.x}, {v0.y, v1.y}, {v0.x + sgn * v0.y, v1.x + sgn * v1.y}};
 #pragma unroll
   for (int p = 0; p < 3; ++p)
@@ -242,13 +242,20 @@ __device__ __forceinline__ void bf_stage2(float2 v0, float2 v1, float sgn, bf8 (
     }
 }
 
-template <int NT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_overlap_bf16(
+// TT = 64: 4 wavefronts, each a 32 x 32 block of the tile; TT = 128: 8
+// wavefronts, each 64 rows x 32 columns.  The larger tile reads every operand
+// row half as often per matrix-core operation (fewer tiles share a row chunk)
+// and splits and stages half as many values per product.
+template <int TT> constexpr int kBfWaves = TT == 64 ? 4 : 8;
+template <int NT, int TT>
+__global__ __launch_bounds__(64 * kBfWaves<TT>) __attribute__((amdgpu_waves_per_eu(2))) void k_overlap_bf16(
     const float2* __restrict__ X, int64_t ldx, int64_t sx, const float2* __restrict__ Y, int64_t ldy, int64_t sy,
     const int* __restrict__ npwk, OvGeom g, double2* __restrict__ part) {
+  static_assert(TT == 64 || TT == 128, "tile edge 64 or 128");
+  constexpr int AB = TT / 32;  // 16-row blocks per wavefront
   // planes: conj(x) y = P1 + P2 + i (P3 - P1 + P2), P1 = xr yr, P2 = xi yi,
   // P3 = (xr - xi)(yr + yi)
-  __shared__ bf8 lx[3][NT][64][BF_P], ly[3][NT][64][BF_P];
+  __shared__ bf8 lx[3][NT][TT][BF_P], ly[3][NT][TT][BF_P];
   const int64_t w = xcd_item(blockIdx.x, gridDim.x);
   const int chunk = (int)((w / g.ntiles) % g.nchunk);
   const int kz = (int)(w / ((int64_t)g.ntiles * g.nchunk));
@@ -265,21 +272,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     ti = t % g.tm;
     tj = t / g.tm;
   }
-  const int i0 = ti * OV_T, j0 = tj * OV_T;
+  const int i0 = ti * TT, j0 = tj * TT;
   const int64_t r0 = (int64_t)chunk * g.kch;
   const int64_t r1 = std::min<int64_t>(npw, r0 + g.kch);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wi = wave & 1, wj = wave >> 1;
+  const int ri = wi * (TT / 2), cj = wj * 32;  // the wavefront's rows and columns in the tile
   // 16 x 16 blocks wholly outside C (their operands are zero padding and
   // their results are never stored) skip the matrix cores; wave-uniform
-  bool live[2][2];
+  bool live[AB][2];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < AB; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) live[a][b] = i0 + wi * 32 + a * 16 < g.m && j0 + wj * 32 + b * 16 < g.n;
-  f4 p1[2][2], p2[2][2], p3[2][2];
+    for (int b = 0; b < 2; ++b) live[a][b] = i0 + ri + a * 16 < g.m && j0 + cj + b * 16 < g.n;
+  f4 p1[AB][2], p2[AB][2], p3[AB][2];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < AB; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
       p1[a][b] = f4{0, 0, 0, 0};
@@ -312,23 +320,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int c = 16 * wave + 4 * q + (lane >> 4);
-      bf_stage2<NT>(px[q][0], px[q][1], -1.f, lx, c, sr);
-      bf_stage2<NT>(py[q][0], py[q][1], 1.f, ly, c, sr);
+      bf_stage2<NT, TT>(px[q][0], px[q][1], -1.f, lx, c, sr);
+      bf_stage2<NT, TT>(py[q][0], py[q][1], 1.f, ly, c, sr);
     }
     __syncthreads();
     if (k0 + BF_KS < r1) gload(k0 + BF_KS);
     bf8 B[2][3][NT];
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
-      const int jj = wj * 32 + b * 16 + (lane & 15);
+      const int jj = cj + b * 16 + (lane & 15);
 #pragma unroll
       for (int p = 0; p < 3; ++p)
 #pragma unroll
         for (int q = 0; q < NT; ++q) B[b][p][q] = ly[p][q][jj][kg];
     }
 #pragma unroll
-    for (int a = 0; a < 2; ++a) {
-      const int ii = wi * 32 + a * 16 + (lane & 15);
+    for (int a = 0; a < AB; ++a) {
+      const int ii = ri + a * 16 + (lane & 15);
       bf8 A[3][NT];
 #pragma unroll
       for (int p = 0; p < 3; ++p)
@@ -346,13 +354,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   }
   double2* P = part + ((int64_t)kz * g.nchunk + chunk) * g.m * g.n;
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < AB; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int i = i0 + wi * 32 + a * 16 + Mfma<float>::row(lane, r);
-        const int j = j0 + wj * 32 + b * 16 + (lane & 15);
+        const int i = i0 + ri + a * 16 + Mfma<float>::row(lane, r);
+        const int j = j0 + cj + b * 16 + (lane & 15);
         const double q1 = (double)p1[a][b][r], q2 = (double)p2[a][b][r], q3 = (double)p3[a][b][r];
         if (i < g.m && j < g.n) P[(int64_t)j * g.m + i] = make_double2(q1 + q2, q3 - q1 + q2);
       }
@@ -360,43 +368,48 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 
 // resident 256-thread workgroups of `kern` on the current device (CUs x
 // per-CU occupancy)
-static int resident_slots(const void* kern) {
+static int resident_slots(const void* kern, int threads) {
   int dev = 0;
   DEREK_HIP(hipGetDevice(&dev));
   hipDeviceProp_t p;
   DEREK_HIP(hipGetDeviceProperties(&p, dev));
   int per = 0;
-  DEREK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, 256, 0));
+  DEREK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, threads, 0));
   return std::max(1, per) * std::max(1, p.multiProcessorCount);
 }
 template <typename R> static int ov_slots() {
-  static const int slots = resident_slots(reinterpret_cast<const void*>(k_overlap<R>));
+  static const int slots = resident_slots(reinterpret_cast<const void*>(k_overlap<R>), 256);
   return slots;
 }
-template <int NT> static int ov_slots_bf16() {
-  static const int slots = resident_slots(reinterpret_cast<const void*>(k_overlap_bf16<NT>));
+template <int NT, int TT> static int ov_slots_bf16() {
+  static const int slots = resident_slots(reinterpret_cast<const void*>(k_overlap_bf16<NT, TT>), 64 * kBfWaves<TT>);
   return slots;
 }
 
-// resident slots of every overlap kernel (FP64, FP32, BF16 with 1..3 terms)
-static void overlap_slots(int out[5]) {
+// resident slots of every overlap kernel (FP64, FP32, BF16 with 1..3 terms
+// on 64- and 128-wide tiles)
+constexpr int kOvKernels = 8;
+static void overlap_slots(int out[kOvKernels]) {
   out[0] = ov_slots<double>();
   out[1] = ov_slots<float>();
-  out[2] = ov_slots_bf16<1>();
-  out[3] = ov_slots_bf16<2>();
-  out[4] = ov_slots_bf16<3>();
+  out[2] = ov_slots_bf16<1, 64>();
+  out[3] = ov_slots_bf16<2, 64>();
+  out[4] = ov_slots_bf16<3, 64>();
+  out[5] = ov_slots_bf16<1, 128>();
+  out[6] = ov_slots_bf16<2, 128>();
+  out[7] = out[6];
 }
 
 int64_t overlap_work_elems(int m, int n, int64_t npw, int batch) {
-  int slots[5];
+  int slots[kOvKernels];
   overlap_slots(slots);
-  return ov_work_elems(m, n, npw, batch, slots, 5);
+  return ov_work_elems(m, n, npw, batch, slots, kOvKernels);
 }
 
 int64_t engine_overlap_work_elems(int nb, int nk, int kc, int64_t npw) {
-  int slots[5];
+  int slots[kOvKernels];
   overlap_slots(slots);
-  return ov_engine_work_elems(nb, nk, kc, npw, slots, 5);
+  return ov_engine_work_elems(nb, nk, kc, npw, slots, kOvKernels);
 }
 
 // C[kz] = sum over chunks.  A workgroup takes RD_E consecutive elements;
@@ -444,29 +457,39 @@ __global__ __launch_bounds__(RD_E * RD_G) void k_overlap_reduce(const double2* _
 template <typename R>
 void overlap(const cx_t<R>* X, int64_t ldx, int64_t sx, const cx_t<R>* Y, int64_t ldy, int64_t sy, int m, int n,
              int64_t npw, const int* npwk, double2* C, int ldc, int64_t sc, int batch, double2* work,
-             int64_t work_cap, bool hermitian, hipStream_t s, int bf16_terms, int64_t max_chunks, int rounds) {
+             int64_t work_cap, bool hermitian, hipStream_t s, int bf16_terms, int64_t max_chunks, int rounds,
+             int tile) {
   DEREK_REQUIRE(m > 0 && n > 0 && batch > 0, "overlap: empty block");
   DEREK_REQUIRE(!hermitian || m == n, "overlap: hermitian needs a square result");
   DEREK_REQUIRE(bf16_terms == 0 || (std::is_same<R, float>::value && bf16_terms >= 1 && bf16_terms <= 3),
                 "overlap: bf16 terms must be 0, or 1..3 for complex64 data");
-  const int slots = bf16_terms == 1   ? ov_slots_bf16<1>()
-                    : bf16_terms == 2 ? ov_slots_bf16<2>()
-                    : bf16_terms == 3 ? ov_slots_bf16<3>()
+  DEREK_REQUIRE(tile == 64 || tile == 128, "overlap: tile must be 64 or 128");
+  // the 128-wide tile exists for the BF16 kernels with 1 or 2 terms (three
+  // terms of a 128-wide tile need 180 KB of LDS)
+  const int tt = bf16_terms && bf16_terms < 3 ? tile : 64;
+  const int slots = bf16_terms == 1   ? (tt == 128 ? ov_slots_bf16<1, 128>() : ov_slots_bf16<1, 64>())
+                    : bf16_terms == 2 ? (tt == 128 ? ov_slots_bf16<2, 128>() : ov_slots_bf16<2, 64>())
+                    : bf16_terms == 3 ? ov_slots_bf16<3, 64>()
                                       : ov_slots<R>();
   DEREK_REQUIRE(rounds == 1 || rounds == 2 || rounds == 4, "overlap: rounds must be 1, 2 or 4");
-  OvGeom g = ov_geom(m, n, npw, hermitian, batch, slots, bf16_terms ? BF_KS : OV_KS, rounds);
+  OvGeom g = ov_geom(m, n, npw, hermitian, batch, slots, bf16_terms ? BF_KS : OV_KS, rounds, tt);
   ov_limit_chunks(g, max_chunks, bf16_terms ? BF_KS : OV_KS);
   DEREK_REQUIRE((int64_t)batch * g.nchunk * m * n <= work_cap, "overlap: partial-sum buffer too small");
   const int64_t items = (int64_t)g.ntiles * g.nchunk * batch;
   DEREK_REQUIRE(items < (int64_t)1 << 31, "overlap: too many workgroups");
   const dim3 grid((unsigned)items);
   if constexpr (std::is_same<R, float>::value) {
-    if (bf16_terms == 1)
-      hipLaunchKernelGGL(k_overlap_bf16<1>, grid, dim3(256), 0, s, X, ldx, sx, Y, ldy, sy, npwk, g, work);
+    const dim3 b64(64 * kBfWaves<64>), b128(64 * kBfWaves<128>);
+    if (bf16_terms == 1 && tt == 64)
+      hipLaunchKernelGGL((k_overlap_bf16<1, 64>), grid, b64, 0, s, X, ldx, sx, Y, ldy, sy, npwk, g, work);
+    else if (bf16_terms == 2 && tt == 64)
+      hipLaunchKernelGGL((k_overlap_bf16<2, 64>), grid, b64, 0, s, X, ldx, sx, Y, ldy, sy, npwk, g, work);
+    else if (bf16_terms == 3 && tt == 64)
+      hipLaunchKernelGGL((k_overlap_bf16<3, 64>), grid, b64, 0, s, X, ldx, sx, Y, ldy, sy, npwk, g, work);
+    else if (bf16_terms == 1)
+      hipLaunchKernelGGL((k_overlap_bf16<1, 128>), grid, b128, 0, s, X, ldx, sx, Y, ldy, sy, npwk, g, work);
     else if (bf16_terms == 2)
-      hipLaunchKernelGGL(k_overlap_bf16<2>, grid, dim3(256), 0, s, X, ldx, sx, Y, ldy, sy, npwk, g, work);
-    else if (bf16_terms == 3)
-      hipLaunchKernelGGL(k_overlap_bf16<3>, grid, dim3(256), 0, s, X, ldx, sx, Y, ldy, sy, npwk, g, work);
+      hipLaunchKernelGGL((k_overlap_bf16<2, 128>), grid, b128, 0, s, X, ldx, sx, Y, ldy, sy, npwk, g, work);
   }
   if (bf16_terms == 0)
     hipLaunchKernelGGL(k_overlap<R>, grid, dim3(256), 0, s, X, ldx, sx, Y, ldy, sy, npwk, g, work);
@@ -646,7 +669,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int c = 16 * wave + 4 * q + (lane >> 4);
-      bf_stage2<NT>(make_float2((float)pu[q][0].x, (float)pu[q][0].y),
+      bf_stage2<NT, 64>(make_float2((float)pu[q][0].x, (float)pu[q][0].y),
                     make_float2((float)pu[q][1].x, (float)pu[q][1].y), 1.f, lu, c, sr);
     }
     bf8 A[3][NT];
@@ -716,10 +739,10 @@ void rotate(const cx_t<R>* X, int64_t ldx, int64_t sx, const double2* U, int ldu
 
 template void overlap<double>(const double2*, int64_t, int64_t, const double2*, int64_t, int64_t, int, int, int64_t,
                               const int*, double2*, int, int64_t, int, double2*, int64_t, bool, hipStream_t, int,
-                              int64_t, int);
+                              int64_t, int, int);
 template void overlap<float>(const float2*, int64_t, int64_t, const float2*, int64_t, int64_t, int, int, int64_t,
                              const int*, double2*, int, int64_t, int, double2*, int64_t, bool, hipStream_t, int,
-                             int64_t, int);
+                             int64_t, int, int);
 template void rotate<double>(const double2*, int64_t, int64_t, const double2*, int, int64_t, int, int, int64_t,
                              const int*, double2*, int64_t, int64_t, int, hipStream_t, int);
 template void rotate<float>(const float2*, int64_t, int64_t, const double2*, int, int64_t, int, int, int64_t,

@@ -8,7 +8,7 @@
 // product as well).  complex64 runs on the FP32 matrix cores ("fp32") and
 // on the BF16 ones with 1, 2 and 3 bf16 terms per value ("bf16x1" ...).
 //
-//   derek_amd/gemm_bench          (built by make -C derek_amd/csrc)
+//   derek_amd/gemm_bench [bf16]   (built by make -C derek_amd/csrc)
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -33,7 +33,7 @@ template <typename C> __global__ void k_fill(C* p, int64_t n) {
   }
 }
 
-template <typename R> void run(int64_t npw, int nb, int batch, int terms = 0) {
+template <typename R> void run(int64_t npw, int nb, int batch, int terms = 0, int tile = 64) {
   using C = cx_t<R>;
   const int64_t ld = ceil_div(npw, 64) * 64;
   const int m2 = 2 * nb;
@@ -67,7 +67,7 @@ template <typename R> void run(int64_t npw, int nb, int batch, int terms = 0) {
     }
     char tag[16];
     std::snprintf(tag, sizeof tag, "%s", sizeof(R) == 8 ? "fp64" : "fp32");
-    if (terms) std::snprintf(tag, sizeof tag, "bf16x%d", terms);
+    if (terms) std::snprintf(tag, sizeof tag, "bf16x%d%s", terms, tile == 128 ? "/128" : "");
     std::printf("%-6s npw %6lld b%-2d %-24s %9.3f ms %7.1f TF/s\n", tag, (long long)npw, batch, what, best,
                 flops / best / 1e9);
     std::fflush(stdout);
@@ -76,11 +76,13 @@ template <typename R> void run(int64_t npw, int nb, int batch, int terms = 0) {
   for (int m : {nb, m2}) {
     std::snprintf(w, sizeof w, "overlap herm %dx%d", m, m);
     timeit(w, 8.0 * m * m * npw * batch, [&] {
-      overlap<R>(X, ld, ks, X, ld, ks, m, m, npw, nullptr, S, m, (int64_t)m * m, batch, work, cap, true, 0, terms);
+      overlap<R>(X, ld, ks, X, ld, ks, m, m, npw, nullptr, S, m, (int64_t)m * m, batch, work, cap, true, 0, terms, 0,
+                 2, tile);
     });
     std::snprintf(w, sizeof w, "overlap full %dx%d", m, m);
     timeit(w, 8.0 * m * m * npw * batch, [&] {
-      overlap<R>(X, ld, ks, X, ld, ks, m, m, npw, nullptr, S, m, (int64_t)m * m, batch, work, cap, false, 0, terms);
+      overlap<R>(X, ld, ks, X, ld, ks, m, m, npw, nullptr, S, m, (int64_t)m * m, batch, work, cap, false, 0, terms, 0,
+                 2, tile);
     });
     std::snprintf(w, sizeof w, "rotate %d->%d", m, nb);
     timeit(w, 8.0 * m * nb * npw * batch, [&] {
@@ -95,10 +97,14 @@ template <typename R> void run(int64_t npw, int nb, int batch, int terms = 0) {
 }
 
 int main(int argc, char** argv) {
-  // argument "bf16": the complex64 runs only, FP32 then BF16 with 1..3 terms
+  // argument "bf16": the complex64 runs only, FP32 then BF16 with 1..3 terms,
+  // then BF16 overlaps with 1 and 2 terms on 128-wide tiles
   const bool bf_only = argc > 1 && std::string(argv[1]) == "bf16";
   if (!bf_only) run<double>(161214, 300, 8);
   for (int t = 0; t <= (bf_only ? 3 : 0); ++t) run<float>(161214, 300, 8, t);
+  // BF16 overlaps on 128-wide tiles (1 and 2 terms)
+  if (bf_only)
+    for (int t = 1; t <= 2; ++t) run<float>(161214, 300, 8, t, 128);
   if (!bf_only) run<double>(62669, 80, 1);
   for (int t = 0; t <= (bf_only ? 3 : 0); ++t) run<float>(62669, 80, 1, t);
   return 0;

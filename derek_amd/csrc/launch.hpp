@@ -61,7 +61,8 @@ void transpose_xz(const double* v, double* t64, float* t32, int nx, int ny, int 
 // accumulated on MFMA in R, chunk partials summed in FP64.
 // `work` (work_cap double2, checked) must hold overlap_work_elems(m, n, npw, batch),
 // or batch * max_chunks * m * n when max_chunks > 0 caps the row chunks;
-// `rounds` (1, 2, 4): rounds of resident workgroups the row chunks fill at least.
+// `rounds` (1, 2, 4): rounds of resident workgroups the row chunks fill at least;
+// `tile` (64, 128): the C tile edge of the BF16 kernels (the others take 64).
 // bf16_terms (complex64 data only): 0 multiplies on the FP32 matrix cores;
 // 1..3 on the BF16 ones with every FP32 value split into that many bf16
 // terms (1: plain BF16, 2: ~16 significant bits, 3: ~FP32), FP32 accumulation.
@@ -73,7 +74,7 @@ template <typename R>
 void overlap(const cx_t<R>* X, int64_t ldx, int64_t sx, const cx_t<R>* Y, int64_t ldy, int64_t sy, int m, int n,
              int64_t npw, const int* npwk, double2* C, int ldc, int64_t sc, int batch, double2* work,
              int64_t work_cap, bool hermitian, hipStream_t s, int bf16_terms = 0, int64_t max_chunks = 0,
-             int rounds = 2);
+             int rounds = 2, int tile = 64);
 // Z (npw x n, ldz) = X (npw x m, ldx) * U (m x n, FP64, column-major, ldu),
 // U rounded to R; bf16_terms as for overlap.
 template <typename R>

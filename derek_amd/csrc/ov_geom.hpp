@@ -8,7 +8,7 @@
 
 namespace derek {
 
-constexpr int OV_T = 64;    // C tile edge
+constexpr int OV_T = 64;    // C tile edge (the BF16 kernels also take 128)
 constexpr int OV_KS = 16;   // rows per LDS slab (FP64 / FP32 matrix cores)
 constexpr int BF_KS = 32;   // rows per slab on the BF16 matrix cores (one MFMA)
 
@@ -16,6 +16,7 @@ struct OvGeom {
   int m, n, tm, tn, ntiles, nchunk;
   int64_t npw, kch;
   bool herm;
+  int t;  // tile edge
 };
 
 inline int64_t ov_ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
@@ -26,14 +27,16 @@ inline int64_t ov_ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 // the last round; every chunk keeps at least 256 rows and a whole number of
 // `ks`-row slabs.  The SCF autotuner picks `rounds` among kOvRounds.
 constexpr int kOvRounds[3] = {1, 2, 4};
-inline OvGeom ov_geom(int m, int n, int64_t npw, bool herm, int batch, int slots, int ks = OV_KS, int rounds = 2) {
+inline OvGeom ov_geom(int m, int n, int64_t npw, bool herm, int batch, int slots, int ks = OV_KS, int rounds = 2,
+                      int tile = OV_T) {
   OvGeom g{};
   g.m = m;
   g.n = n;
   g.npw = npw;
   g.herm = herm;
-  g.tm = (int)ov_ceil_div(m, OV_T);
-  g.tn = (int)ov_ceil_div(n, OV_T);
+  g.t = tile;
+  g.tm = (int)ov_ceil_div(m, tile);
+  g.tn = (int)ov_ceil_div(n, tile);
   g.ntiles = herm ? g.tm * (g.tm + 1) / 2 : g.tm * g.tn;
   const int64_t units = (int64_t)g.ntiles * std::max(1, batch);
   const int64_t cmax = std::max<int64_t>(1, npw / 256);
@@ -67,14 +70,15 @@ inline void ov_limit_chunks(OvGeom& g, int64_t max_chunks, int ks) {
 
 // double2 elements of the partial sums of an m x n overlap of `batch`
 // k-points, for any of the kernels whose resident slots are listed (both
-// slab depths, Hermitian or not, every choice of rounds).
+// slab depths, Hermitian or not, every choice of rounds, both tile edges).
 inline int64_t ov_work_elems(int m, int n, int64_t npw, int batch, const int* slots, int nslots) {
   int64_t chunks = 1;
   for (int i = 0; i < nslots; ++i)
     for (int ks : {OV_KS, BF_KS})
       for (bool h : {false, m == n})
         for (int rounds : kOvRounds)
-          chunks = std::max<int64_t>(chunks, ov_geom(m, n, npw, h, batch, slots[i], ks, rounds).nchunk);
+          for (int tile : {OV_T, 2 * OV_T})
+            chunks = std::max<int64_t>(chunks, ov_geom(m, n, npw, h, batch, slots[i], ks, rounds, tile).nchunk);
   return (int64_t)batch * chunks * m * n;
 }
 

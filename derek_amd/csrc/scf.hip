@@ -120,7 +120,7 @@ void ov(derek_ctx* c, const cx_t<R>* X, const cx_t<R>* Yp, int m, int n, int k0,
   const double npw = npw_sum(c, k0, nkc);
   c->timed(KC_OVERLAP, (double)(herm ? m : m + n) * npw * sizeof(cx_t<R>), 8.0 * npw * m * n, [&] {
     overlap<R>(X, E.ldk, E.kstride, Yp, E.ldk, E.kstride, m, n, c->max_npw, c->d_npw + k0, C, m, (int64_t)m * n, nkc,
-               E.ovw, E.ovw_cap, herm, c->s, bf16_terms<R>(c), 0, c->ov_rounds);
+               E.ovw, E.ovw_cap, herm, c->s, bf16_terms<R>(c), 0, c->ov_rounds, c->ov_tile);
   });
 }
 
@@ -144,7 +144,7 @@ void ov_block(derek_ctx* c, const cx_t<R>* X, int ca, const cx_t<R>* Yp, int cb,
   c->timed(KC_OVERLAP, (double)(herm ? nb : 2 * nb) * npw * sizeof(cx_t<R>), 8.0 * npw * nb * nb, [&] {
     overlap<R>(X + (int64_t)ca * E.ldk, E.ldk, E.kstride, Yp + (int64_t)cb * E.ldk, E.ldk, E.kstride, nb, nb,
                c->max_npw, c->d_npw + k0, C + ca + (int64_t)cb * m2, m2, (int64_t)m2 * m2, nkc, E.ovw, E.ovw_cap,
-               herm, c->s, bf16_terms<R>(c), 0, c->ov_rounds);
+               herm, c->s, bf16_terms<R>(c), 0, c->ov_rounds, c->ov_tile);
   });
 }
 
@@ -489,9 +489,9 @@ template <typename R> void autotune(derek_ctx* c) {
       if (!(t1 < kGain * t0)) c->item_cap = c0;
     }
   }
-  // GEMM chunks: the rounds of resident workgroups the split-K row chunks of
-  // the overlaps fill, timed on the Davidson step's 2nb x 2nb Hermitian
-  // overlap of the first batch
+  // GEMM tiles and chunks: the C tile edge of the BF16 overlaps and the
+  // rounds of resident workgroups their split-K row chunks fill, timed on
+  // the Davidson step's 2nb x 2nb Hermitian overlap of the first batch
   {
     const int nkg = std::min(E.kc, c->nk);
     cx_t<R>* Yb = reinterpret_cast<cx_t<R>*>(E.Y);
@@ -509,8 +509,16 @@ template <typename R> void autotune(derek_ctx* c) {
       }
       return tb;
     };
-    const int r0 = c->ov_rounds;
     float bestg = time_ov();
+    // tile edge first (BF16 overlaps only), then the rounds at that edge
+    if (bf16_terms<R>(c) > 0) {
+      const int t0 = c->ov_tile;
+      c->ov_tile = t0 == 64 ? 128 : 64;
+      const float t = time_ov();
+      if (t < kGain * bestg) bestg = t;
+      else c->ov_tile = t0;
+    }
+    const int r0 = c->ov_rounds;
     for (int q : kOvRounds) {
       if (q == r0) continue;
       const int prev = c->ov_rounds;

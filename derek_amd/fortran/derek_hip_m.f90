@@ -469,11 +469,13 @@ module derek_hip_m
        integer(c_int32_t), dimension(8) :: out
      end function derek_hip_get_tuning
 
-     ! z-pass sticks per workgroup (0, 16, 32) and overlap chunk rounds (1, 2, 4)
-     integer(c_int32_t) function derek_hip_set_tiles(ctx, z_sticks, gemm_rounds) bind(C, name='derek_hip_set_tiles')
+     ! z-pass sticks per workgroup (0, 16, 32), overlap chunk rounds (1, 2, 4),
+     ! BF16 overlap tile edge (64, 128)
+     integer(c_int32_t) function derek_hip_set_tiles(ctx, z_sticks, gemm_rounds, gemm_tile) &
+          bind(C, name='derek_hip_set_tiles')
        import :: c_int32_t, c_ptr
        type(c_ptr), value :: ctx
-       integer(c_int32_t), value :: z_sticks, gemm_rounds
+       integer(c_int32_t), value :: z_sticks, gemm_rounds, gemm_tile
      end function derek_hip_set_tiles
 
      integer(c_int32_t) function derek_hip_set_launch(ctx, bands_per_wg, items_per_launch, fused) &

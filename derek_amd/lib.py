@@ -514,12 +514,13 @@ class Context:
         check(self.lib.derek_hip_get_tuning(self.h, out))
         return {"bands_per_wg": out[0], "items_per_launch": out[1], "kbatch": out[2], "graph": bool(out[3]),
                 "fused": bool(out[4] & 1), "fused_density": bool(out[4] & 2), "z_sticks": out[5],
-                "gemm_rounds": out[6]}
+                "gemm_rounds": out[6], "gemm_tile": out[7]}
 
-    def set_tiles(self, z_sticks=0, gemm_rounds=2):
-        """Sticks per z-pass workgroup (16, 32; 0: the precision's default) and
-        the rounds of resident workgroups the overlaps' row chunks fill (1, 2, 4)."""
-        check(self.lib.derek_hip_set_tiles(self.h, int(z_sticks), int(gemm_rounds)))
+    def set_tiles(self, z_sticks=0, gemm_rounds=2, gemm_tile=64):
+        """Sticks per z-pass workgroup (16, 32; 0: the precision's default),
+        the rounds of resident workgroups the overlaps' row chunks fill (1, 2,
+        4) and the C tile edge of the BF16 overlaps (64, 128)."""
+        check(self.lib.derek_hip_set_tiles(self.h, int(z_sticks), int(gemm_rounds), int(gemm_tile)))
 
     def set_launch(self, bands_per_wg=2, items_per_launch=256, fused=True, fused_density=None):
         """Launch shapes; fused / fused_density: the fused y/x passes for H|psi>

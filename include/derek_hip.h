@@ -329,14 +329,15 @@ int32_t derek_hip_profile_read(derek_ctx* ctx, int32_t cls, int64_t* launches, d
  * out[4] the fused y/x passes in use where the grid allows them: bit 0 for
  * H|psi>, bit 1 for the density, out[5] sticks per z-pass workgroup (16,
  * 32), out[6] rounds of resident workgroups the overlaps' row chunks fill
- * (1, 2, 4), out[7] reserved (0). */
+ * (1, 2, 4), out[7] the C tile edge of the BF16 overlaps (64, 128). */
 int32_t derek_hip_get_tuning(derek_ctx* ctx, int32_t out[8]);
 /* Set the FFT and GEMM tile choices (what scf_init's autotuner would
  * choose): sticks per workgroup of the z passes (16 or 32; 0 = 16 for
- * complex128, 32 for complex64) and the rounds of resident workgroups that
- * the split-K overlaps' row chunks fill (1, 2 or 4).  Both regroup the same
- * operations.  No counterpart in the reference. */
-int32_t derek_hip_set_tiles(derek_ctx* ctx, int32_t z_sticks, int32_t gemm_rounds);
+ * complex128, 32 for complex64), the rounds of resident workgroups that the
+ * split-K overlaps' row chunks fill (1, 2 or 4) and the C tile edge of the
+ * BF16 overlaps (64 or 128).  All of them regroup the same operations.  No
+ * counterpart in the reference. */
+int32_t derek_hip_set_tiles(derek_ctx* ctx, int32_t z_sticks, int32_t gemm_rounds, int32_t gemm_tile);
 /* Set the H|psi> launch shapes (what scf_init's autotuner would choose):
  * bands per workgroup (1, 2, 4, 8), items per launch (>= 1), fused 0..3
  * (bit 0: fused y/x pass for H|psi>, bit 1: for the density).

@@ -10,8 +10,9 @@
 //                      R^H R, R^H (HR)  nb x nb, Hermitian
 // each over a full batch of kc k-points and the last batch of nk mod kc,
 // on any of the five overlap kernels (FP64, FP32, BF16 with 1..3 terms:
-// slabs of 16 or 32 rows), with each choice of chunk rounds (1, 2, 4), for
-// resident-slot counts of 1..8 workgroups per CU on 256 CUs.  Prints one line per failure; exit status 1 if any.
+// slabs of 16 or 32 rows), with each choice of chunk rounds (1, 2, 4) and
+// tile edge (64, 128), for resident-slot counts of 1..8 workgroups per CU
+// on 256 CUs.  Prints one line per failure; exit status 1 if any.
 #include <cstdio>
 #include <vector>
 
@@ -52,14 +53,15 @@ int main() {
           for (const L& l : launches)
             for (int sl : slots)
               for (int ks : {OV_KS, BF_KS})
-                for (int rounds : kOvRounds) {
-                const OvGeom g = ov_geom(l.m, l.n, s.npw, l.herm, b, sl, ks, rounds);
+                for (int rounds : kOvRounds)
+                  for (int tile : {64, 128}) {
+                const OvGeom g = ov_geom(l.m, l.n, s.npw, l.herm, b, sl, ks, rounds, tile);
                 const long long need = (long long)b * g.nchunk * l.m * l.n;
                 ++checked;
                 if (need > cap) {
                   ++bad;
-                  std::printf("FAIL %s: %dx%d herm=%d batch=%d slots=%d ks=%d rounds=%d needs %lld > %lld\n", s.name,
-                              l.m, l.n, (int)l.herm, b, sl, ks, rounds, need, cap);
+                  std::printf("FAIL %s: %dx%d herm=%d batch=%d slots=%d ks=%d rounds=%d tile=%d needs %lld > %lld\n",
+                              s.name, l.m, l.n, (int)l.herm, b, sl, ks, rounds, tile, need, cap);
                 }
               }
         }

@@ -217,17 +217,21 @@ def test_rotate(m, n, npw, precision):
 BF16_ACC = {1: 2e-2, 2: 1e-4, 3: 2e-6}  # against the exact product of the complex64 operands
 
 
-@pytest.mark.parametrize("m,n,npw", [(10, 10, 1503), (37, 53, 777), (160, 160, 5003), (80, 80, 62669)])
+@pytest.mark.parametrize("m,n,npw", [(10, 10, 1503), (37, 53, 777), (160, 160, 5003), (80, 80, 62669),
+                                     (300, 260, 3001)])
 @pytest.mark.parametrize("nt", [1, 2, 3])
-def test_overlap_bf16(m, n, npw, nt):
-    """BF16 MFMA overlaps of complex64 operands split into nt bf16 terms:
-    against the oracle's split product (FP32 accumulation: 1e-5 of the
-    largest entry), against the exact product to the split's accuracy, and
-    exactly Hermitian for X^H X."""
-    print("overlap_bf16", m, n, npw, nt, flush=True)
+@pytest.mark.parametrize("tile", [64, 128])
+def test_overlap_bf16(m, n, npw, nt, tile):
+    """BF16 MFMA overlaps of complex64 operands split into nt bf16 terms, on
+    64- or 128-wide C tiles (three terms always take 64): against the
+    oracle's split product (FP32 accumulation: 1e-5 of the largest entry),
+    against the exact product to the split's accuracy, and exactly Hermitian
+    for X^H X."""
+    print("overlap_bf16", m, n, npw, nt, tile, flush=True)
     rng = np.random.default_rng(m * 1000 + n + nt)
     ctx = lib.Context((8, 8, 8), 1, 1, 1.0, 1, lib.DEREK_MIXED)
     ctx.set_gemm(nt)
+    ctx.set_tiles(0, 2, tile)
     X = rnd_c(rng, m, npw).astype(np.complex64)
     Y = rnd_c(rng, n, npw).astype(np.complex64)
     got = ctx.overlap(X, Y)

@@ -100,24 +100,24 @@ def test_autotune_keeps_results():
     print("tuned", t)
     assert t["bands_per_wg"] in (1, 2, 4, 8)
     assert t["items_per_launch"] >= 1
-    assert t["z_sticks"] in (16, 32) and t["gemm_rounds"] in (1, 2, 4)
+    assert t["z_sticks"] in (16, 32) and t["gemm_rounds"] in (1, 2, 4) and t["gemm_tile"] in (64, 128)
     assert np.max(np.abs(e1 - e0)) < 1e-10
 
 
 @pytest.mark.parametrize("precision", [lib.DEREK_FP64, lib.DEREK_MIXED])
 def test_tile_choices_keep_results(precision):
-    """Sticks per z-pass workgroup and the overlaps' chunk rounds regroup the
-    same operations: H|psi> agrees to 1e-14 (FP64) / 1e-6 (complex64)
+    """Sticks per z-pass workgroup, the overlaps' chunk rounds and the BF16
+    overlaps' tile edge regroup the same operations: H|psi> agrees to 1e-14 (FP64) / 1e-6 (complex64)
     relative, SCF energies to 1e-10 (FP64; the complex64 phase to 1e-7 of
     |E|)."""
     print("tiles", precision, flush=True)
     # a 32^3 grid (a compiled z length, so the stick blocks apply), 4 k-points
     spec = config.small_system(seed=17, length=10.0, ecut=10.0, g=2.15, mp=(2, 2, 1), kT=0.01, nelec=6, nbands=8)
-    e0, _, t0 = energies(spec, 5, precision=precision, tiles=(16, 2), kbatch=0)
-    assert (t0["z_sticks"], t0["gemm_rounds"]) == (16, 2)
-    for tiles in [(32, 1), (32, 4), (16, 4)]:
+    e0, _, t0 = energies(spec, 5, precision=precision, tiles=(16, 2, 64), kbatch=0)
+    assert (t0["z_sticks"], t0["gemm_rounds"], t0["gemm_tile"]) == (16, 2, 64)
+    for tiles in [(32, 1, 128), (32, 4, 64), (16, 4, 128)]:
         e1, _, t1 = energies(spec, 5, precision=precision, tiles=tiles, kbatch=0)
-        assert (t1["z_sticks"], t1["gemm_rounds"]) == tiles
+        assert (t1["z_sticks"], t1["gemm_rounds"], t1["gemm_tile"]) == tiles
         d = np.max(np.abs(e1 - e0))
         print("tiles", tiles, "largest step energy difference", d)
         assert d < (1e-10 if precision == lib.DEREK_FP64 else 1e-7 * np.max(np.abs(e0)))
@@ -127,9 +127,9 @@ def test_tile_choices_keep_results(precision):
     ctx.set_vloc(su.vext)
     rng = np.random.default_rng(3)
     psi = rng.normal(size=(8, su.npw[1])) + 1j * rng.normal(size=(8, su.npw[1]))
-    ctx.set_tiles(16, 2)
+    ctx.set_tiles(16, 2, 64)
     h16 = ctx.apply_h(1, psi)
-    ctx.set_tiles(32, 2)
+    ctx.set_tiles(32, 2, 64)
     h32 = ctx.apply_h(1, psi)
     ctx.close()
     # the same operations per stick in separately compiled kernels (FMA
@@ -140,6 +140,8 @@ def test_tile_choices_keep_results(precision):
         lib.Context((8, 8, 8), 1, 1, 1.0, 1).set_tiles(24, 2)
     with pytest.raises(lib.DerekError):
         lib.Context((8, 8, 8), 1, 1, 1.0, 1).set_tiles(16, 3)
+    with pytest.raises(lib.DerekError):
+        lib.Context((8, 8, 8), 1, 1, 1.0, 1).set_tiles(16, 2, 96)
 
 
 @pytest.mark.parametrize("name,bf16", [("gamma32", 0), ("kpoints", 0), ("gamma32", 1), ("kpoints", 1), ("gamma32", 2),

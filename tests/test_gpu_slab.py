@@ -48,7 +48,7 @@ def test_fused_apply_h_matches_oracle(n, precision, z_sticks):
     assert tuple(su.dims) == (n, n, n)
     nb = 5
     ctx = su.context(precision, nbands=nb)
-    ctx.set_tiles(z_sticks, 2)
+    ctx.set_tiles(z_sticks, 2, 64)
     rng = np.random.default_rng(n)
     v = su.vext + 0.3 * rng.normal(size=su.N)
     ctx.set_vloc(v)
