@@ -19,7 +19,10 @@ def ctx():
     c.close()
 
 
-@pytest.mark.parametrize("n,nev", [(1, 1), (2, 2), (3, 2), (17, 17), (80, 80), (98, 98), (99, 99), (160, 80), (300, 150),
+# sizes on both sides of the switch from the in-LDS reduction (up to 98
+# columns) to multi-workgroup steps before a 98-column tail
+@pytest.mark.parametrize("n,nev", [(1, 1), (2, 2), (3, 2), (16, 16), (17, 17), (33, 20), (64, 64), (80, 80), (97, 97),
+                                   (98, 98), (99, 99), (128, 128), (160, 80), (161, 161), (200, 100), (300, 150),
                                    (600, 300)])
 def test_heev_random(ctx, n, nev):
     print("heev", n, nev, flush=True)
@@ -51,7 +54,7 @@ def test_heev_degenerate_and_diagonal(ctx):
     assert np.max(np.abs(D @ Z - Z * w[None, :])) < 1e-12
 
 
-@pytest.mark.parametrize("n", [60, 150])
+@pytest.mark.parametrize("n", [60, 150, 400])
 def test_heev_columns_without_reflector(ctx, n):
     """Columns already zero below the subdiagonal take no reflector, both in
     the in-LDS reduction (n <= 98) and in the multi-workgroup one."""
