@@ -705,6 +705,25 @@ __device__ __forceinline__ void stick_two_stage(C* line, const C* twl, int j, bo
   __syncthreads();
 }
 
+// (major, minor) task of a workgroup of a (gridDim.x majors x gridDim.y
+// minors) grid, for workgroups that share data through their major index
+// (the complex128 fused pass: the V rows of a z plane).  The dispatcher deals
+// workgroups to the 8 XCDs in turn (linear id mod 8); the tasks are numbered
+// major-first and each XCD takes a contiguous range of them, so the
+// workgroups of one major index run on one XCD at about the same time and
+// read the shared data through its L2 once.  (The z passes keep the stick
+// block fastest: grouping the bands of a stick block also removed their
+// re-read maps, but their stick-row writes, one item stride apart, then
+// landed together and the passes ran 5-10 % slower.)
+__device__ __forceinline__ void xcd_task(int& major, int& minor) {
+  const int T = (int)(gridDim.x * gridDim.y);
+  const int L = (int)(blockIdx.x + gridDim.x * blockIdx.y);
+  const int x = L % 8, s = L / 8;
+  const int t = x * (T / 8) + min(x, T % 8) + s;
+  major = t / (int)gridDim.y;
+  minor = t % (int)gridDim.y;
+}
+
 // psi[b][perm[p]] -> S[item][s][:] for the block's sticks
 template <typename R, int N, bool ZM, int SL_>
 __global__ __launch_bounds__((StickSplit<N, SL_>::NT)) void k_sticks_z_inv2(PwBatch B, PwBlock<R> psi,
@@ -1212,14 +1231,19 @@ __global__ __launch_bounds__((Slab<R, N, true>::NT)) void k_slab_vmul(PwBatch B,
   constexpr int LPA = ST::LPA, LPX = ST::LPX;
   extern __shared__ __align__(16) unsigned char smem[];
   const PwMaps m = B.mt[B.k0 + blockIdx.z];
-  const int z = blockIdx.x;
+  // complex128 runs one workgroup per CU and waits on its V rows: planes
+  // grouped per XCD (measured 4-5 % faster at 108^3; complex64, two
+  // workgroups per CU, keeps the plane-fastest order, which measured 1-2 %
+  // faster for it)
+  int z = blockIdx.x, grp = blockIdx.y;
+  if constexpr (sizeof(R) == 8) xcd_task(z, grp);
   ST t;
   t.init(B, m, tw, smem);
   const int nax = t.nax, j2 = t.j2, lj = t.lj, k1 = t.k1, lk = t.lk;
   C* A = t.A;
   C* X = t.X;
   const R* vz = vt + (int64_t)z * N * N;  // V at (x, y, z) in vz[y*N + x]
-  const int bb0 = blockIdx.y * bpw, bb1 = min(B.nbb, bb0 + bpw);
+  const int bb0 = grp * bpw, bb1 = min(B.nbb, bb0 + bpw);
   const uint32_t row_bytes = (uint32_t)B.nst_cap * (uint32_t)sizeof(C);
   auto row_rsrc = [&](int band) {
     return buf_rsrc(S + (((int64_t)blockIdx.z * B.nbb + band) * nz + z) * B.nst_cap, row_bytes);
