@@ -8,7 +8,8 @@ minutes for all three):
                       ||rho_out - rho_in|| < 1e-8, then eigenvectors polished
                       to residual 1e-9 at the final potential.
   scf_kmesh72.json    16 bohr, Ecut 20 Ha, 4x4x4 mesh (64 k), 72^3, 32
-                      electrons, 24 bands, 8 wells, kT 0.01 Ha: the same.
+                      electrons, 24 bands, 8 wells, kT 0.01 Ha: the same with
+                      |dE| < 1e-8 Ha and ||rho_out - rho_in|| < 1e-6 (TOLS).
   eigs_kmesh144_k0.json  30 bohr, Ecut 25 Ha, 2x2x2 mesh, 144^3, 300 bands,
                       64 wells: the 300 lowest eigenvalues at the first
                       k-point of the first SCF step's potential (uniform
@@ -38,9 +39,13 @@ from oracle import fft as offt  # noqa: E402
 from oracle import ks as oks  # noqa: E402
 from oracle import scf as oscf  # noqa: E402
 
-ENERGY_TOL = 1e-11
-RHO_TOL = 1e-8
 RES_TOL = 1e-9
+# SCF stop test per system: |dE| < energy_tol and ||rho_out - rho_in|| < rho_tol.
+# The 64-k-point system's partly occupied top bands converge slowly in the
+# Davidson steps of each SCF step (24 bands, no spare band above them), so
+# its energy keeps falling by ~1e-8 Ha per step (0.9x per step) long after
+# the density has settled; it stops at |dE| < 1e-8 Ha.
+TOLS = {"gamma108": (1e-11, 1e-8), "kmesh72": (1e-8, 1e-6)}
 NSAMPLE = 2048
 # Davidson steps per SCF step (first step, later steps).  The converged SCF
 # does not depend on them; the 64-k-point system's partly occupied top bands
@@ -64,14 +69,15 @@ def scf_fixture(name):
     print(f"{name}: dims {s.dims}, npw {[len(i) for i in s.fft_index][:4]}..., setup {time.time() - t0:.1f} s",
           flush=True)
     nd1, nd = N_DAVID[name]
-    r = oscf.scf(s, max_scf=80, energy_tol=ENERGY_TOL, rho_tol=RHO_TOL, final_res_tol=RES_TOL, n_david_first=nd1,
+    energy_tol, rho_tol = TOLS[name]
+    r = oscf.scf(s, max_scf=80, energy_tol=energy_tol, rho_tol=rho_tol, final_res_tol=RES_TOL, n_david_first=nd1,
                  n_david=nd, log=lambda m: print(m, flush=True))
     if not r["converged"]:
         raise SystemExit(f"{name}: oracle SCF did not converge")
     stride = s.N // NSAMPLE
     idx = np.arange(0, s.N, stride)[:NSAMPLE]
     out = dict(system=name, dims=list(s.dims), nk=s.nk, nbands=s.nbands, nelec=s.nelec, kT=s.kT,
-               energy_tol=ENERGY_TOL, rho_tol=RHO_TOL, res_tol=RES_TOL, n_david_first=nd1, n_david=nd,
+               energy_tol=energy_tol, rho_tol=rho_tol, res_tol=RES_TOL, n_david_first=nd1, n_david=nd,
                iterations=r["iterations"],
                etot=float(r["etot"]), mu=float(r["mu"]), eigs_shape=list(r["eigs_final"].shape),
                eigs=floats(r["eigs_final"]), occ=floats(r["occ"]),
