@@ -187,7 +187,7 @@ struct HeevWork {
   double2* sub;  // [n]  T(k+1,k) before the phase change
   double2* ph;   // [n]  phases
   double2* vb;   // [2n] reflector vectors of the wide reduction, double-buffered by k&1
-  double2* uu;   // [n]  A v of the wide reduction
+  double2* uu;   // [2n] A v of the wide reduction, double-buffered by k&1
   double* d;     // [n]  diagonal
   double* e;     // [n]  |sub|
   double* lam;   // [nev]
@@ -201,7 +201,7 @@ struct HeevWork {
     ph = sub + n;
     vb = ph + n;
     uu = vb + 2 * n;
-    d = reinterpret_cast<double*>(uu + n);
+    d = reinterpret_cast<double*>(uu + 2 * n);
     e = d + n;
     lam = e + n;
     scal = lam + nev;
@@ -214,14 +214,14 @@ struct HeevWork {
 
 int64_t heev_work_doubles(int n, int nev) {
   // rounded to 256 bytes so every matrix of a batch starts aligned
-  const int64_t w = 12 * (int64_t)n + nev + 8 + (int64_t)n * nev + (n + nev + 2 + 1) / 2 + 8;
+  const int64_t w = 14 * (int64_t)n + nev + 8 + (int64_t)n * nev + (n + nev + 2 + 1) / 2 + 8;
   return (w + 31) / 32 * 32;
 }
 
 // ---- 1. Householder tridiagonalisation + phases
-// Columns 0 .. n-99 (n > 98) take two multi-workgroup launches each
-// (k_wide_gemv: u = A v; k_wide_update: rank-2 update, and the wavefront of
-// the next column forms its reflector); the trailing block of at most 98
+// Columns 0 .. n-99 (n > 98) take one multi-workgroup launch each
+// (k_wide_step: the rank-2 update of column k's reflector, fused with the
+// next reflector and u = A v for it); the trailing block of at most 98
 // columns is copied to LDS and reduced by one workgroup per matrix
 // (k_tridiag), where the wavefront holding column k+1 forms the next
 // reflector while it applies the rank-2 update to that column.
@@ -432,13 +432,13 @@ __global__ __launch_bounds__(DT) void k_tridiag(double2* __restrict__ Aall, int 
   tri_phases(W, n, As);
 }
 
-// ---- 1'. wide tridiagonalisation (matrices that do not fit LDS): two
-// launches per column, the trailing matrix spread over many workgroups (one
-// wavefront per column).  Launch k computes u = A22 v; launch k' applies the
-// rank-2 update, and the wavefront that updates column k+1 forms the next
-// reflector from the values it just wrote.  Reflector k is kept in vb
-// (buffer k&1, scaled to unit norm) with its scalars in scal; A's column k
-// receives it for the back-transformation.
+// ---- 1'. wide tridiagonalisation (matrices that do not fit LDS): one
+// launch per column, the trailing matrix spread over many workgroups (one
+// wavefront per column).  k_wide_first and k_wide_gemv form reflector 0 and
+// u_0 = A22 v_0; launch k (k_wide_step) applies the rank-2 update of step k
+// and forms reflector k+1 and u_{k+1}.  Reflector k is kept in vb (buffer
+// k&1, scaled to unit norm) with its scalars in scal; A's column k receives
+// it for the back-transformation.
 constexpr int WT = 256;       // threads per workgroup
 constexpr int WC = WT / 64;   // columns per workgroup
 
@@ -522,57 +522,107 @@ __global__ __launch_bounds__(WT) void k_wide_gemv(double2* __restrict__ Aall, in
   if (lane == 0) W.uu[i] = make_double2(sr, si);
 }
 
-// A22 -= v w^H + w v^H with w = 2u - 2 (v^H u) v; the wavefront of column
-// k+1 then forms reflector k+1 from the updated column.
-__global__ __launch_bounds__(WT) void k_wide_update(double2* __restrict__ Aall, int lda, int64_t strideA, int n,
-                                                    int nev, int k, double* __restrict__ work, int64_t strideWork) {
+// Step k of the wide reduction in one launch.  Reflector k (vb[k&1], scal)
+// and u_k = A22 v_k (uu[k&1]) come from the previous launch; with
+// w = 2u - 2 (v^H u) v every wavefront applies A22 -= v w^H + w v^H to its
+// column j.  Every workgroup also updates column k+1 on the fly and forms
+// reflector k+1 from it in LDS (the column itself is not written back: its
+// rows below k+1 become that reflector, and the other workgroups read its
+// old values); the wavefront of each column j > k+1 then forms
+// u_{k+1}(j) = (updated column j)^H v_{k+1} into uu[(k+1)&1], so the next
+// step needs no separate product launch.  Workgroup 0 records
+// reflector k+1 (vb, scal, skip, sub), the updated diagonal entry of column
+// k+1, and reflector k into column k for the back-transformation.  On the
+// last wide step (`last`) column k+1 is updated in place like the others,
+// for the in-LDS reduction that follows.
+__global__ __launch_bounds__(WT) void k_wide_step(double2* __restrict__ Aall, int lda, int64_t strideA, int n,
+                                                  int nev, int k, int last, double* __restrict__ work,
+                                                  int64_t strideWork) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double2* vn = reinterpret_cast<double2*>(smem);  // [n] reflector k+1 (rows > k+1)
   const int mat = blockIdx.y;
   HeevWork W(work + mat * strideWork, n, nev);
   double2* A = Aall + mat * strideA;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int j = k + 1 + blockIdx.x * WC + wid;
-  const bool next = (j == k + 1) && (k + 3 < n);  // this wavefront forms reflector k+1
-  if (W.scal[4 * (k & 1)] != 0.0) {               // column k needed no reflector
-    if (next) wide_reflector(A, lda, k + 1, n, W, lane);
-    return;
-  }
-  if (j >= n) return;
+  const bool skip = W.scal[4 * (k & 1)] != 0.0;  // column k needed no reflector
   const double2* v = W.vb + (size_t)(k & 1) * n;
-  const double2* u = W.uu;
-  double gp = 0.0;
-  for (int i = k + 1 + lane; i < n; i += 64) gp += zconjmul(v[i], u[i]).x;
-  const double g2 = 2.0 * wave_sum(gp);
-  const double2 vj = v[j], uj = u[j];
-  const double2 wj = make_double2(2.0 * uj.x - g2 * vj.x, 2.0 * uj.y - g2 * vj.y);
-  double2* col = A + (int64_t)j * lda;
-  double part = 0.0;
-  double2 x0 = make_double2(0.0, 0.0);
-  for (int i = k + 1 + lane; i < n; i += 64) {
+  const double2* u = W.uu + (size_t)(k & 1) * n;
+  double g2 = 0.0;
+  if (!skip) {
+    double gp = 0.0;
+    for (int i = k + 1 + lane; i < n; i += 64) gp += zconjmul(v[i], u[i]).x;
+    g2 = 2.0 * wave_sum(gp);
+  }
+  auto wof = [=](int i) {
     const double2 vi = v[i], ui = u[i];
-    const double2 wi = make_double2(2.0 * ui.x - g2 * vi.x, 2.0 * ui.y - g2 * vi.y);
+    return make_double2(2.0 * ui.x - g2 * vi.x, 2.0 * ui.y - g2 * vi.y);
+  };
+  // a - v_i conj(w_j) - w_i conj(v_j)
+  auto upd = [](double2 a, double2 vi, double2 wi, double2 vj, double2 wj) {
     const double2 t1 = zmul(vi, make_double2(wj.x, -wj.y));
     const double2 t2 = zmul(wi, make_double2(vj.x, -vj.y));
-    const double2 a = col[i];
-    const double2 r = make_double2(a.x - t1.x - t2.x, a.y - t1.y - t2.y);
-    col[i] = r;
-    if (i > j) part += r.x * r.x + r.y * r.y;
-    if (i == j + 1) x0 = r;
-  }
-  if (next) {
-    part = wave_sum(part);
-    x0 = make_double2(wave_sum(x0.x), wave_sum(x0.y));  // one lane holds it
-    int skip;
-    double iv;
-    double2 v0;
-    wide_reflector_scalars(lane, k + 1, part, x0, W, skip, iv, v0);
-    double2* vn = W.vb + (size_t)((k + 1) & 1) * n;
-    // same row-to-lane mapping as the update loop: each lane reads back
-    // only the rows it wrote
-    for (int i = k + 1 + lane; i < n; i += 64) {
-      if (i <= j) continue;
-      const double2 a = i == j + 1 ? v0 : col[i];
-      vn[i] = make_double2(a.x * iv, a.y * iv);
+    return make_double2(a.x - t1.x - t2.x, a.y - t1.y - t2.y);
+  };
+  if (blockIdx.x == 0 && wid == 0 && !skip)
+    for (int i = k + 1 + lane; i < n; i += 64) A[i + (int64_t)k * lda] = v[i];
+  // reflector k+1 from column k+1 after step k (rows > k+1), formed by the
+  // whole workgroup: the unscaled values in vn, the norm from per-wavefront
+  // partial sums; the scale and the first element enter the products below
+  __shared__ double red[WC];
+  int skip1 = 1;
+  double iv1 = 0.0;
+  double2 v01 = make_double2(0.0, 0.0);
+  if (!last) {
+    const double2* c1 = A + (int64_t)(k + 1) * lda;
+    const double2 v1 = skip ? make_double2(0.0, 0.0) : v[k + 1];
+    const double2 w1 = skip ? make_double2(0.0, 0.0) : wof(k + 1);
+    double part = 0.0;
+    for (int i = k + 2 + (int)threadIdx.x; i < n; i += WT) {  // thread 0 holds row k+2
+      const double2 c = skip ? c1[i] : upd(c1[i], v[i], wof(i), v1, w1);
+      vn[i] = c;
+      part += c.x * c.x + c.y * c.y;
     }
+    part = wave_sum(part);
+    if (lane == 0) red[wid] = part;
+    __syncthreads();
+    part = 0.0;
+    for (int q = 0; q < WC; ++q) part += red[q];
+    // every thread derives the scalars; thread 0 of workgroup 0 records them
+    wide_reflector_scalars(blockIdx.x == 0 && threadIdx.x == 0 ? 0 : 1, k + 1, part, vn[k + 2], W, skip1, iv1, v01);
+    if (blockIdx.x == 0) {
+      double2* vb1 = W.vb + (size_t)((k + 1) & 1) * n;
+      for (int i = k + 2 + (int)threadIdx.x; i < n; i += WT) {
+        const double2 c = i == k + 2 ? v01 : vn[i];
+        vb1[i] = make_double2(c.x * iv1, c.y * iv1);
+      }
+      if (threadIdx.x == 0 && !skip) {
+        double2* dkk = A + (k + 1) + (int64_t)(k + 1) * lda;
+        *dkk = upd(*dkk, v1, w1, v1, w1);
+      }
+    }
+  }
+  const int j = k + 1 + blockIdx.x * WC + wid;
+  if (j >= n || (!last && j == k + 1)) return;
+  double2* col = A + (int64_t)j * lda;
+  const double2 vj = skip ? make_double2(0.0, 0.0) : v[j];
+  const double2 wj = skip ? make_double2(0.0, 0.0) : wof(j);
+  double sr = 0.0, si = 0.0;
+  for (int i = k + 1 + lane; i < n; i += 64) {
+    double2 a = col[i];
+    if (!skip) {
+      a = upd(a, v[i], wof(i), vj, wj);
+      col[i] = a;
+    }
+    if (!last && i > k + 1) {
+      const double2 t = zconjmul(a, i == k + 2 ? v01 : vn[i]);
+      sr += t.x;
+      si += t.y;
+    }
+  }
+  if (!last) {
+    sr = wave_sum(sr) * iv1;
+    si = wave_sum(si) * iv1;
+    if (lane == 0) W.uu[(size_t)((k + 1) & 1) * n + j] = make_double2(sr, si);
   }
 }
 
@@ -874,10 +924,12 @@ void heev_gpu(double2* A, int lda, int64_t strideA, int n, int nev, double* Wv, 
     // trailing matrix is larger than that, then the rest in one workgroup
     if (k0 > 0) {
       hipLaunchKernelGGL(k_wide_first, dim3(batch), dim3(64), 0, s, A, lda, strideA, n, nev, work, strideWork);
+      hipLaunchKernelGGL(k_wide_gemv, dim3((unsigned)ceil_div(n - 1, WC), batch), dim3(WT), 0, s, A, lda, strideA, n,
+                         nev, 0, work, strideWork);
       for (int k = 0; k < k0; ++k) {
         dim3 grid((unsigned)ceil_div(n - k - 1, WC), batch);
-        hipLaunchKernelGGL(k_wide_gemv, grid, dim3(WT), 0, s, A, lda, strideA, n, nev, k, work, strideWork);
-        hipLaunchKernelGGL(k_wide_update, grid, dim3(WT), 0, s, A, lda, strideA, n, nev, k, work, strideWork);
+        hipLaunchKernelGGL(k_wide_step, grid, dim3(WT), sizeof(double2) * (size_t)n, s, A, lda, strideA, n, nev, k,
+                           k == k0 - 1 ? 1 : 0, work, strideWork);
       }
     }
     allow_dynamic_lds(reinterpret_cast<const void*>(k_tridiag), kTridiagLds);
