@@ -1060,6 +1060,13 @@ int32_t derek_hip_get_occupations(derek_ctx* c, double* occ) {
   });
 }
 
+int32_t derek_hip_get_residuals(derek_ctx* c, double* res) {
+  return guard([&] {
+    DEREK_REQUIRE(c && res, "null argument");
+    scf_get(c, 4, res);
+  });
+}
+
 int32_t derek_hip_get_density(derek_ctx* c, double* rho) {
   return guard([&] {
     DEREK_REQUIRE(c && rho, "null argument");

@@ -751,7 +751,7 @@ void scf_get(derek_ctx* c, int what, double* out) {
     if (!c->vloc_set) throw StateError("no potential set yet");
     DEREK_HIP(hipMemcpyAsync(out, c->vloc, c->N * sizeof(double), hipMemcpyDeviceToHost, c->s));
   } else {
-    const double* src = what == 0 ? E.eps : E.occ;
+    const double* src = what == 0 ? E.eps : (what == 1 ? E.occ : E.rn);
     DEREK_HIP(hipMemcpyAsync(out, src, (size_t)c->nk * c->nb * sizeof(double), hipMemcpyDeviceToHost, c->s));
   }
   c->sync();

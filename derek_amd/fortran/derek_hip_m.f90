@@ -425,6 +425,12 @@ module derek_hip_m
        real(c_double), dimension(*) :: eig
      end function derek_hip_get_eigenvalues
 
+     integer(c_int32_t) function derek_hip_get_residuals(ctx, res) bind(C, name='derek_hip_get_residuals')
+       import :: c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       real(c_double), dimension(*) :: res
+     end function derek_hip_get_residuals
+
      integer(c_int32_t) function derek_hip_get_occupations(ctx, occ) bind(C, name='derek_hip_get_occupations')
        import :: c_int32_t, c_double, c_ptr
        type(c_ptr), value :: ctx

@@ -30,7 +30,7 @@ EXPORTS = (
     "derek_hip_rotate",
     "derek_hip_density", "derek_hip_lda", "derek_hip_hartree", "derek_hip_heev", "derek_hip_hegv",
     "derek_hip_scf_init", "derek_hip_scf_step", "derek_hip_scf_polish", "derek_hip_nscf", "derek_hip_get_eigenvalues",
-    "derek_hip_get_occupations", "derek_hip_get_density", "derek_hip_get_potential", "derek_hip_get_psi", "derek_hip_put_psi",
+    "derek_hip_get_residuals", "derek_hip_get_occupations", "derek_hip_get_density", "derek_hip_get_potential", "derek_hip_get_psi", "derek_hip_put_psi",
     "derek_hip_apply_h_resident", "derek_hip_sync", "derek_hip_stream",
     "derek_hip_profile_enable", "derek_hip_profile_reset", "derek_hip_profile_read", "derek_hip_get_tuning",
     "derek_hip_set_launch", "derek_hip_set_gemm", "derek_hip_set_tiles",
@@ -555,6 +555,12 @@ class Context:
     def eigenvalues(self):
         e = np.zeros((self.nk, self.nbands))
         check(self.lib.derek_hip_get_eigenvalues(self.h, ptr(e)))
+        return e
+
+    def residuals(self):
+        """Residual norms per k-point and band after the last Davidson step."""
+        e = np.zeros((self.nk, self.nbands))
+        check(self.lib.derek_hip_get_residuals(self.h, ptr(e)))
         return e
 
     def occupations(self):

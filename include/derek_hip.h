@@ -294,6 +294,12 @@ int32_t derek_hip_scf_polish(derek_ctx* ctx, int32_t max_iter, double res_tol, d
 int32_t derek_hip_nscf(derek_ctx* ctx, int32_t max_iter, double res_tol, double* max_res);
 int32_t derek_hip_get_eigenvalues(derek_ctx* ctx, double* eig);
 int32_t derek_hip_get_occupations(derek_ctx* ctx, double* occ);
+/* Residual norms |H psi - eps psi| of every band at every k-point after the
+ * last Davidson step (n_k x nbands doubles, band fastest), as the eigenvalues
+ * are laid out.  No counterpart in the reference (its SCF reports no
+ * residuals); a caller that converges only the lowest bands of a block reads
+ * them here. */
+int32_t derek_hip_get_residuals(derek_ctx* ctx, double* res);
 /* rho_out of the last step (N_g doubles) */
 int32_t derek_hip_get_density(derek_ctx* ctx, double* rho);
 /* The total local potential V_ext + V_H + V_xc (fine grid, Ha) that the next

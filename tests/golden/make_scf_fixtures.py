@@ -14,9 +14,11 @@ minutes for all three):
                       64 wells: the 300 lowest eigenvalues at the first
                       k-point of the first SCF step's potential (uniform
                       density: V_ext + V_H[n0] + V_xc[n0], n0 = N_e/Omega),
-                      residual 1e-9.  The full oracle SCF of this system does
-                      not fit a test budget; these eigenvalues depend on the
-                      Hamiltonian only, not on the algorithm that finds them.
+                      residual 1e-9, from a 320-band Davidson (20 spare
+                      bands above them).  The full oracle SCF of this system
+                      does not fit a test budget; these eigenvalues depend on
+                      the Hamiltonian only, not on the algorithm that finds
+                      them.
 
 Each fixture holds the energies and eigenvalues as 17-digit decimals and, for
 the SCF systems, the density at every (N // 2048)-th grid point.
@@ -92,11 +94,21 @@ def scf_fixture(name):
           flush=True)
 
 
-def first_step_eigs(name="kmesh144", k=0, chunk=24):
+def first_step_eigs(name="kmesh144", k=0, chunk=24, extra=20, steps=10):
+    """The lowest spec.nbands eigenpairs at k-point k of the first step's
+    potential, from a Davidson on spec.nbands + extra bands: the top one of
+    a block converges at the rate its gap to the next eigenvalue outside the
+    block allows (for 300 bands here about 1e3 steps to residual 1e-9), so
+    the block carries spare bands and the stop test looks at the lowest
+    spec.nbands only.  (20 spare bands keep the GPU engine's 2 x nbands
+    subspace within its 640 columns.)"""
     spec = config.workload(name)
+    nb = spec.nbands
     t0 = time.time()
-    s = system(spec)
-    print(f"{name}: dims {s.dims}, npw {len(s.fft_index[k])}, setup {time.time() - t0:.1f} s", flush=True)
+    s = oscf.System(spec.lengths, spec.ecut, spec.g_fine_scale, spec.mp, spec.nelec, nb + extra, spec.kT,
+                    spec.centres, spec.heights, spec.widths, spec.cos_amp, spec.seed)
+    print(f"{name}: dims {s.dims}, npw {len(s.fft_index[k])}, {nb} + {extra} bands, setup {time.time() - t0:.1f} s",
+          flush=True)
     rho0 = np.full(s.N, s.nelec / s.volume)
     vh, _ = oks.hartree(rho0, s.gsq, s.dims, s.volume)
     vxc, _, _ = oks.xc_on_grid(rho0, s.dims, s.volume)
@@ -108,13 +120,20 @@ def first_step_eigs(name="kmesh144", k=0, chunk=24):
                           for i in range(0, P.shape[0], chunk)])
 
     X = s.initial_psi(k)
-    X, _, eps, rn = oscf.davidson(X, apply(X), s.kin[k], apply, 2000, res_tol=RES_TOL)
-    print(f"{name}: largest residual {rn.max():.2e}, {time.time() - t0:.0f} s", flush=True)
-    if rn.max() >= RES_TOL:
-        raise SystemExit(f"{name}: Davidson stopped at residual {rn.max():.2e}")
+    HX = apply(X)
+    done = 0
+    while True:
+        X, HX, eps, rn = oscf.davidson(X, HX, s.kin[k], apply, steps)
+        done += steps
+        print(f"{name}: {done} Davidson steps, largest residual of the lowest {nb}: {rn[:nb].max():.2e} "
+              f"(all: {rn.max():.2e}), {time.time() - t0:.0f} s", flush=True)
+        if rn[:nb].max() < RES_TOL:
+            break
+        if done >= 600:
+            raise SystemExit(f"{name}: Davidson stopped at residual {rn[:nb].max():.2e}")
     out = dict(system=name, k=k, kfrac=floats(s.kfrac[k]), dims=list(s.dims), npw=len(s.fft_index[k]),
-               nbands=s.nbands, res_tol=RES_TOL, max_residual=float(rn.max()), eigs=floats(eps),
-               v_mean=float(np.mean(v)), seconds=time.time() - t0)
+               nbands=nb, extra_bands=extra, res_tol=RES_TOL, max_residual=float(rn[:nb].max()),
+               davidson_steps=done, eigs=floats(eps[:nb]), v_mean=float(np.mean(v)), seconds=time.time() - t0)
     path = os.path.join(HERE, f"eigs_{name}_k{k}.json")
     with open(path, "w") as f:
         json.dump(out, f, indent=0)

@@ -20,6 +20,7 @@ Both sides converge to the fixture's stop test (|dE| < 1e-11 Ha and
 with bounds to match, see there) before the comparison, then polish to
 residual 1e-9.
 """
+import dataclasses
 import json
 import os
 
@@ -114,10 +115,14 @@ def test_kmesh72_matches_oracle_fixture(mode):
 def test_kmesh144_first_step_eigenvalues_match_oracle_fixture():
     """300 bands at 161214 plane waves: the first SCF step's potential
     (uniform density) taken from the device, held fixed, and the lowest 300
-    eigenvalues at the first k-point converged to residual 1e-9."""
+    eigenvalues at the first k-point converged to residual 1e-9 in a block
+    with the fixture's spare bands above them (the top band of a bare
+    300-band block converges at the rate its gap to band 301 allows, about
+    1e3 Davidson steps to 1e-9)."""
     print("kmesh144 eigenvalues at k0", flush=True)
     fx = fixture("eigs_kmesh144_k0.json")
-    spec = config.workload("kmesh144")
+    nb = fx["nbands"]
+    spec = dataclasses.replace(config.workload("kmesh144"), nbands=nb + fx.get("extra_bands", 0))
     kfrac = np.array([fx["kfrac"]])
     su = gscf.Setup(spec, kfrac=kfrac)
     assert su.npw[0] == fx["npw"]
@@ -128,12 +133,17 @@ def test_kmesh144_first_step_eigenvalues_match_oracle_fixture():
     print(f"mean potential {np.mean(v):.12f} (oracle {fx['v_mean']:.12f})")
     assert abs(np.mean(v) - fx["v_mean"]) < 1e-10
     ctx.set_vloc(v)
-    rmax = ctx.nscf(max_iter=400, res_tol=1e-9)
-    eg = ctx.eigenvalues()[0]
+    rmax = 1.0
+    for _ in range(10):  # the spare bands need not converge: stop on the lowest nb
+        ctx.nscf(max_iter=50, res_tol=1e-9)
+        rmax = ctx.residuals()[0][:nb].max()
+        if rmax < 1e-9:
+            break
+    eg = ctx.eigenvalues()[0][:nb]
     ctx.close()
     ref = np.array(fx["eigs"])
     d = np.abs(eg - ref)
-    print(f"residual {rmax:.2e}; max |d eps| {d.max():.2e} Ha at band {d.argmax()}")
+    print(f"residual of the lowest {nb}: {rmax:.2e}; max |d eps| {d.max():.2e} Ha at band {d.argmax()}")
     assert rmax < 1e-9
     assert d.max() < 1e-8
 
