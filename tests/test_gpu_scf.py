@@ -21,6 +21,8 @@ def run_both(spec, max_scf=60):
     res = gscf.run(spec, max_scf=max_scf, energy_tol=1e-11, rho_tol=1e-8, setup=g, ctx=ctx, log=print)
     rmax = ctx.scf_polish(1000, 1e-9)
     print(f"polished residual {rmax:.2e}")
+    # the per-band residuals behind the returned maximum
+    assert ctx.residuals().max() == rmax
     eg = ctx.eigenvalues()
     rho = ctx.get_density()
     ctx.close()
