@@ -9,7 +9,9 @@ minutes for all three):
                       to residual 1e-9 at the final potential.
   scf_kmesh72.json    16 bohr, Ecut 20 Ha, 4x4x4 mesh (64 k), 72^3, 32
                       electrons, 24 bands, 8 wells, kT 0.01 Ha: the same with
-                      |dE| < 1e-8 Ha and ||rho_out - rho_in|| < 1e-6 (TOLS).
+                      |dE| < 1e-8 Ha and ||rho_out - rho_in|| < 1e-6 (TOLS),
+                      polished with 8 spare bands (SPARE); also the last SCF
+                      step's own eigenvalues (eigs_scf).
   eigs_kmesh144_k0.json  30 bohr, Ecut 25 Ha, 2x2x2 mesh, 144^3, 300 bands,
                       64 wells: the 300 lowest eigenvalues at the first
                       k-point of the first SCF step's potential (uniform
@@ -53,6 +55,11 @@ NSAMPLE = 2048
 # does not depend on them; the 64-k-point system's partly occupied top bands
 # converge slowly at two steps per SCF step, so it takes four.
 N_DAVID = {"gamma108": (4, 2), "kmesh72": (4, 4)}
+# Spare bands for the final polish at the converged potential: the 64-k-point
+# system's top bands converge to residual 1e-9 only with a few bands above
+# them in the block (about 1e3 Davidson steps without); the polished
+# eigenvalues are those of the lowest spec.nbands.
+SPARE = {"kmesh72": 8}
 
 
 def system(spec):
@@ -64,6 +71,35 @@ def floats(a):
     return [float(x) for x in np.asarray(a, dtype=np.float64).ravel()]
 
 
+def polish_with_spares(s, X, v, spare, steps=10, chunk=24):
+    """Eigenvalues of the lowest s.nbands bands at potential v, every k-point:
+    the SCF's vectors plus `spare` seeded random ones (orthonormalised), in
+    Davidson chunks of `steps` until the lowest s.nbands reach RES_TOL."""
+    nb = s.nbands
+    eps_all = np.zeros((s.nk, nb))
+    for k in range(s.nk):
+        def apply(P, k=k):
+            P = np.atleast_2d(P)
+            return np.vstack([oks.apply_h(P[i:i + chunk], s.fft_index[k], s.kin[k], v, s.dims)
+                              for i in range(0, P.shape[0], chunk)])
+        rng = np.random.default_rng(1000 + k)
+        npw = len(s.kin[k])
+        W = (rng.normal(size=(spare, npw)) + 1j * rng.normal(size=(spare, npw))) / (1.0 + s.kin[k])
+        Y = oscf.orthonormalise(np.vstack([X[k], W]))
+        HY = apply(Y)
+        done = 0
+        while True:
+            Y, HY, eps, rn = oscf.davidson(Y, HY, s.kin[k], apply, steps)
+            done += steps
+            if rn[:nb].max() < RES_TOL:
+                break
+            if done >= 400:
+                raise SystemExit(f"k-point {k}: polish stopped at residual {rn[:nb].max():.2e}")
+        eps_all[k] = eps[:nb]
+        print(f"polish k {k}: {done} Davidson steps, residual {rn[:nb].max():.2e}", flush=True)
+    return eps_all
+
+
 def scf_fixture(name):
     spec = config.workload(name)
     t0 = time.time()
@@ -72,17 +108,20 @@ def scf_fixture(name):
           flush=True)
     nd1, nd = N_DAVID[name]
     energy_tol, rho_tol = TOLS[name]
-    r = oscf.scf(s, max_scf=80, energy_tol=energy_tol, rho_tol=rho_tol, final_res_tol=RES_TOL, n_david_first=nd1,
-                 n_david=nd, log=lambda m: print(m, flush=True))
+    spare = SPARE.get(name, 0)
+    r = oscf.scf(s, max_scf=80, energy_tol=energy_tol, rho_tol=rho_tol, final_res_tol=None if spare else RES_TOL,
+                 n_david_first=nd1, n_david=nd, log=lambda m: print(m, flush=True))
     if not r["converged"]:
         raise SystemExit(f"{name}: oracle SCF did not converge")
+    if spare:
+        r["eigs_final"] = polish_with_spares(s, r["X"], r["v_in"], spare)
     stride = s.N // NSAMPLE
     idx = np.arange(0, s.N, stride)[:NSAMPLE]
     out = dict(system=name, dims=list(s.dims), nk=s.nk, nbands=s.nbands, nelec=s.nelec, kT=s.kT,
                energy_tol=energy_tol, rho_tol=rho_tol, res_tol=RES_TOL, n_david_first=nd1, n_david=nd,
                iterations=r["iterations"],
                etot=float(r["etot"]), mu=float(r["mu"]), eigs_shape=list(r["eigs_final"].shape),
-               eigs=floats(r["eigs_final"]), occ=floats(r["occ"]),
+               eigs=floats(r["eigs_final"]), eigs_scf=floats(r["eigs"]), spare_bands=spare, occ=floats(r["occ"]),
                energies=[float(h["etot"]) for h in r["history"]],
                rho_index=[int(i) for i in idx], rho=floats(r["rho_out"][idx]),
                rho_integral=float(np.sum(r["rho_out"]) * s.volume / s.N),

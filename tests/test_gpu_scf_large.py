@@ -43,16 +43,48 @@ def fixture(name):
         return json.load(f)
 
 
+def polish_with_spares(spec, v, spare):
+    """The lowest spec.nbands eigenvalues at potential v on every k-point,
+    from a context with `spare` more bands, the lowest converged to residual
+    1e-9 (the top band of a bare block converges at the rate its gap to the
+    first band outside allows)."""
+    nb = spec.nbands
+    su = gscf.Setup(dataclasses.replace(spec, nbands=nb + spare))
+    ctx = su.context(lib.DEREK_FP64)
+    ctx.scf_init(spec.nelec, kT=spec.kT, seed=spec.seed)
+    ctx.set_vloc(v)
+    r = 1.0
+    for _ in range(20):
+        ctx.nscf(max_iter=50, res_tol=1e-9)
+        r = ctx.residuals()[:, :nb].max()
+        if r < 1e-9:
+            break
+    eg = ctx.eigenvalues()[:, :nb]
+    ctx.close()
+    return eg, r
+
+
 def converge(spec, precision, fx, **kw):
-    """The SCF to the fixture's stop test, with its Davidson steps per SCF step."""
+    """The SCF to the fixture's stop test, with its Davidson steps per SCF
+    step, then the eigenvalues polished as the fixture's were (in place, or
+    with the fixture's spare bands at the final potential)."""
     su = gscf.Setup(spec)
     ctx = su.context(precision)
     res = gscf.run(spec, max_scf=80, energy_tol=fx["energy_tol"], rho_tol=fx["rho_tol"], setup=su, ctx=ctx,
                    log=print, n_david_first=fx.get("n_david_first", 4), n_david=fx.get("n_david", 2), **kw)
-    rmax = ctx.scf_polish(1000, 1e-9)
-    eg = ctx.eigenvalues()
     rho = ctx.get_density()
-    ctx.close()
+    spare = fx.get("spare_bands", 0)
+    if spare:
+        if "eigs_scf" in fx and res["iterations"] == fx["iterations"]:
+            d = np.abs(ctx.eigenvalues() - np.array(fx["eigs_scf"]).reshape(fx["eigs_shape"])).max()
+            print(f"last SCF step's eigenvalues against the oracle's (same step count): {d:.2e} Ha")
+        v = ctx.get_potential()
+        ctx.close()
+        eg, rmax = polish_with_spares(spec, v, spare)
+    else:
+        rmax = ctx.scf_polish(1000, 1e-9)
+        eg = ctx.eigenvalues()
+        ctx.close()
     print(f"converged {res['converged']} in {res['iterations']} steps ({res['fp32_steps']} FP32), "
           f"polished residual {rmax:.2e}")
     return res, eg, rho, rmax
