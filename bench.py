@@ -276,6 +276,10 @@ def main():
     ap.add_argument("--tiles", default=None,
                     help="ZS,ROUNDS[,TILE]: time H|psi> with these FFT/GEMM tiles (sticks per z-pass workgroup "
                          "16/32, overlap chunk rounds 1/2/4, BF16 overlap tile edge 64/128) instead of autotuned ones")
+    ap.add_argument("--heev", default="",
+                    help="N1,N2,...: also run the dense eigensolver (lowest n/2 eigenpairs, three calls each) on "
+                         "seeded random Hermitian matrices of these orders and report the eigenvalue error against "
+                         "numpy (for kernel traces of the solver)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -468,6 +472,21 @@ def main():
                  "complex64": items_sweep(gscf, lib, config, args.items_sweep, [64, 128, 256, 512, 0],
                                           lib.DEREK_MIXED)}
 
+    heev_info = {}
+    if args.heev:
+        hc = lib.Context((8, 8, 8), 1, 4, 1.0, 1)  # a minimal context: the solver needs no basis
+        rng = np.random.default_rng(0)
+        for n in [int(x) for x in args.heev.split(",") if x.strip()]:
+            a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+            A = 0.5 * (a + a.conj().T)
+            t0 = time.perf_counter()
+            for _ in range(3):
+                w, _z = hc.heev(A, n // 2)
+            heev_info[str(n)] = {"nev": n // 2, "ms_per_call_with_copies": round((time.perf_counter() - t0) / 3e-3, 3),
+                                 "max_eig_err": float(np.abs(w - np.linalg.eigvalsh(A)[:n // 2]).max())}
+            log(f"# heev n={n}: {heev_info[str(n)]}")
+        hc.close()
+
     out = {
         "metric": METRIC, "value": round(bands_per_s, 2), "unit": "H|psi> bands/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
@@ -490,7 +509,7 @@ def main():
                                         "gemm_tile": tune32["gemm_tile"]},
                       "kernels": kernels32},
         "scf": scf_info, "cpu_baseline": cpu, "cpu_baseline_all_cores": cpu_all, "cpu_baseline_scf": cpu_scf,
-        "items_sweep": sweep,
+        "items_sweep": sweep, "heev": heev_info,
     }
     print(json.dumps(out))
 

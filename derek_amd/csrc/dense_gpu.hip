@@ -2,12 +2,13 @@
 // matrix (blockIdx.x selects the matrix of a batch).
 //
 // k_heev: lowest nev eigenpairs of a Hermitian n x n matrix.
-//   1. Householder reduction to real tridiagonal form, the matrix kept in
-//      global memory (L2 resident): per column, a norm reduction, the
-//      reflector in LDS, u = A22 v with one wavefront per column (coalesced
-//      column reads), gamma = v^H u, and the rank-2 update A22 -= v w^H + w v^H.
-//      The reflectors overwrite the eliminated columns.
-//   2. Phases that make the sub-diagonal real and non-negative.
+//   1. Householder reduction to real tridiagonal form: while the trailing
+//      matrix exceeds LDS, one multi-workgroup launch per column on the matrix
+//      in global memory (L2 resident); then the last <= 98 columns in LDS in
+//      one workgroup (lower triangle, look-ahead reflector, two barriers per
+//      column).  The reflectors overwrite the eliminated columns.
+//   2. Phases that make the sub-diagonal real and non-negative (a prefix
+//      product over the workgroup).
 //   3. Sturm-sequence multisection for the lowest nev eigenvalues, one
 //      wavefront each (64 points per step).
 //   4. Inverse iteration for the eigenvectors of the tridiagonal matrix with
@@ -222,225 +223,54 @@ int64_t heev_work_doubles(int n, int nev) {
 // Columns 0 .. n-99 (n > 98) take one multi-workgroup launch each
 // (k_wide_step: the rank-2 update of column k's reflector, fused with the
 // next reflector and u = A v for it); the trailing block of at most 98
-// columns is copied to LDS and reduced by one workgroup per matrix
-// (k_tridiag), where the wavefront holding column k+1 forms the next
-// reflector while it applies the rank-2 update to that column.
-__device__ __forceinline__ void tri_reflector(int tid, int k, int n, double xn2, double2 x0, int* s_skip,
-                                              double* s_scal, HeevWork& W) {
-  if (tid != 0) return;
-  const double ax0 = sqrt(x0.x * x0.x + x0.y * x0.y);
-  const double rest = xn2 - ax0 * ax0;
-  const int skip = (xn2 == 0.0 || rest <= 1e-300) ? 1 : 0;
-  *s_skip = skip;
-  W.skip[k] = skip;
-  if (skip) {
-    W.sub[k] = x0;
-  } else {
-    const double xn = sqrt(xn2);
-    const double2 phs = ax0 > 0.0 ? make_double2(x0.x / ax0, x0.y / ax0) : make_double2(1.0, 0.0);
-    const double2 alpha = make_double2(-phs.x * xn, -phs.y * xn);
-    const double2 v0 = make_double2(x0.x - alpha.x, x0.y - alpha.y);
-    const double vn2 = rest + v0.x * v0.x + v0.y * v0.y;
-    W.sub[k] = alpha;
-    s_scal[0] = 1.0 / sqrt(vn2);
-    s_scal[1] = v0.x;
-    s_scal[2] = v0.y;
-  }
-}
+// columns is copied to LDS (lower triangle) and reduced by one workgroup
+// per matrix (k_tridiag).
 
-// e = |sub| and the phases ph[0] = 1, ph[k+1] = ph[k] sub_k/|sub_k| that make
-// D^H T D real.  The sub-diagonal is staged in LDS (s_sub, n entries) by the
-// whole workgroup first, so that the one-thread product runs at LDS latency
-// instead of one global round trip per step.
-__device__ void tri_phases(HeevWork& W, int n, double2* s_sub) {
-  for (int k = threadIdx.x; k + 1 < n; k += blockDim.x) {
-    const double2 sk = W.sub[k];
-    s_sub[k] = sk;
-    W.e[k] = sqrt(sk.x * sk.x + sk.y * sk.y);
-  }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  W.e[n - 1] = 0.0;
-  double2 p = make_double2(1.0, 0.0);
-  W.ph[0] = p;
-  for (int k = 0; k + 1 < n; ++k) {
-    const double2 sk = s_sub[k];
-    const double a = sqrt(sk.x * sk.x + sk.y * sk.y);
-    if (a > 0.0) p = zmul(p, make_double2(sk.x / a, sk.y / a));
-    W.ph[k + 1] = p;
-  }
-}
-
-// Householder reduction of the trailing block A[k0:, k0:] (nb = n - k0 rows
-// and columns) in LDS, one workgroup per matrix: the whole matrix for k0 = 0,
-// or the tail after the multi-workgroup steps 0..k0-1.  The LDS arrays are
-// addressed with global indices through offset bases.
-__global__ __launch_bounds__(DT) void k_tridiag(double2* __restrict__ Aall, int lda_g, int64_t strideA, int n,
-                                               int k0, int nev, double* __restrict__ work, int64_t strideWork) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int nb = n - k0;
-  double2* vs = reinterpret_cast<double2*>(smem);  // [nb]
-  double2* us = vs + nb;                           // [nb]
-  double2* As = us + nb;                           // [nb * nb]
-  double2* v = vs - k0;                            // v[i], i >= k0
-  double2* u = us - k0;
-  // reflector scalars, double-buffered by the parity of the step: step k
-  // reads slot k&1 while wave 0 writes slot (k+1)&1 for the next step
-  __shared__ int s_skip[2];
-  __shared__ double s_scal[2][4];
-  const int mat = blockIdx.x;
-  double2* Ag = Aall + mat * strideA;
-  HeevWork W(work + mat * strideWork, n, nev);
+// e = |sub| and the phases ph[0] = 1, ph[k+1] = ph[k] f_k, f_k = sub_k/|sub_k|
+// (1 where sub_k = 0), that make D^H T D real.  The phases are a prefix
+// product of unit complex numbers: each thread takes two consecutive f_k
+// (n <= 2048 = 2 DT), the threads' products are scanned across the lanes of
+// each wavefront and then across the wavefronts (s_wave: DW entries).
+__device__ void tri_phases(HeevWork& W, int n, double2* s_wave) {
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int lda = nb;
-  double2* A = As - k0 - (int64_t)k0 * lda;  // A[i + j*lda], i, j >= k0
-  for (int q = tid; q < nb * nb; q += DT) As[q] = Ag[(k0 + q % nb) + (int64_t)(k0 + q / nb) * lda_g];
-  __syncthreads();
-  if (nb > 2 && wid == 0) {
-    const double2* c0 = A + (int64_t)k0 * lda;
-    double part = 0.0;
-    for (int i = k0 + 1 + lane; i < n; i += 64) {
-      const double2 a = c0[i];
-      part += a.x * a.x + a.y * a.y;
+  const double2 one = make_double2(1.0, 0.0);
+  double2 f[2], q = one;
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int k = 2 * tid + r;
+    f[r] = one;
+    if (k + 1 < n) {
+      const double2 sk = W.sub[k];
+      const double a = sqrt(sk.x * sk.x + sk.y * sk.y);
+      W.e[k] = a;
+      if (a > 0.0) f[r] = make_double2(sk.x / a, sk.y / a);
     }
-    part = wave_sum(part);
-    tri_reflector(tid, k0, n, part, c0[k0 + 1], &s_skip[k0 & 1], s_scal[k0 & 1], W);
+    q = zmul(q, f[r]);
   }
+  // inclusive scan of the thread products over the wavefront
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double2 t = make_double2(__shfl_up(q.x, o, 64), __shfl_up(q.y, o, 64));
+    if (lane >= o) q = zmul(t, q);
+  }
+  if (lane == 63) s_wave[wid] = q;
   __syncthreads();
-  for (int k = k0; k + 2 < n; ++k) {
-    const bool last = k + 3 >= n;  // no reflector for column k+1 after this step
-    const int cur = k & 1, nxt = cur ^ 1;
-    const int m = n - k - 1;  // trailing rows and columns
-    if (!s_skip[cur]) {
-      const double iv = s_scal[cur][0];
-      for (int i = k + 1 + tid; i < n; i += DT) {
-        const double2 a = i == k + 1 ? make_double2(s_scal[cur][1], s_scal[cur][2]) : A[i + (int64_t)k * lda];
-        v[i] = make_double2(a.x * iv, a.y * iv);
-      }
-      __syncthreads();
-      // u_i = sum_j A(i,j) v_j along row i: one thread per row (consecutive
-      // rows across lanes, v_j a broadcast), no cross-lane reductions
-      for (int i = k + 1 + tid; i < n; i += DT) {
-        // eight columns per iteration, all loads issued before the first use
-        // (LDS latency paid once per eight), two accumulators
-        double sr = 0.0, si = 0.0, tr = 0.0, ti = 0.0;
-        const double2* row = A + i;
-        int j = k + 1;
-        for (; j + 7 < n; j += 8) {
-          double2 a[8], b[8];
+  double2 p = one;
+  for (int w = 0; w < wid; ++w) p = zmul(p, s_wave[w]);
+  double2 ex = make_double2(__shfl_up(q.x, 1, 64), __shfl_up(q.y, 1, 64));
+  if (lane == 0) ex = one;
+  p = zmul(p, ex);  // ph[2 tid]
 #pragma unroll
-          for (int q = 0; q < 8; ++q) {
-            a[q] = row[(int64_t)(j + q) * lda];
-            b[q] = v[j + q];
-          }
-#pragma unroll
-          for (int q = 0; q < 8; q += 2) {
-            sr += a[q].x * b[q].x - a[q].y * b[q].y;
-            si += a[q].x * b[q].y + a[q].y * b[q].x;
-            tr += a[q + 1].x * b[q + 1].x - a[q + 1].y * b[q + 1].y;
-            ti += a[q + 1].x * b[q + 1].y + a[q + 1].y * b[q + 1].x;
-          }
-        }
-        for (; j < n; ++j) {
-          const double2 a0 = row[(int64_t)j * lda], v0 = v[j];
-          sr += a0.x * v0.x - a0.y * v0.y;
-          si += a0.x * v0.y + a0.y * v0.x;
-        }
-        u[i] = make_double2(sr + tr, si + ti);
-      }
-      __syncthreads();
-      // gamma = v^H u, every wave on its own; w = 2u - 2 gamma v
-      double gp = 0.0;
-      for (int i = k + 1 + lane; i < n; i += 64) gp += zconjmul(v[i], u[i]).x;
-      const double g2 = 2.0 * wave_sum(gp);
-      // A22 -= v w^H + w v^H: thread (row t, group g) updates row k+1+t of
-      // the columns k+1+g, k+1+g+cg, ...; group 0 also keeps the reflector in
-      // column k and, for the next reflector, the norm of column k+1 below row k+1
-      const int cg = DT / m;
-      const int t = tid % m, g = tid / m;
-      double part = 0.0;
-      double2 x0 = make_double2(0.0, 0.0);
-      if (g < cg) {
-        const int i = k + 1 + t;
-        const double2 vi = v[i], ui = u[i];
-        const double2 wi = make_double2(2.0 * ui.x - g2 * vi.x, 2.0 * ui.y - g2 * vi.y);
-        // two columns per iteration with their loads issued first
-        auto upd = [&](int j, double2 a, double2 vj, double2 uj) {
-          const double2 wj = make_double2(2.0 * uj.x - g2 * vj.x, 2.0 * uj.y - g2 * vj.y);
-          const double2 t1 = zmul(vi, make_double2(wj.x, -wj.y));
-          const double2 t2 = zmul(wi, make_double2(vj.x, -vj.y));
-          const double2 r = make_double2(a.x - t1.x - t2.x, a.y - t1.y - t2.y);
-          A[i + (int64_t)j * lda] = r;
-          if (j == k + 1) {
-            if (i > j) part = r.x * r.x + r.y * r.y;
-            if (i == j + 1) x0 = r;
-          }
-        };
-        int j = k + 1 + g;
-        for (; j + cg < n; j += 2 * cg) {
-          const int j1 = j + cg;
-          const double2 a0 = A[i + (int64_t)j * lda], a1 = A[i + (int64_t)j1 * lda];
-          const double2 v0 = v[j], u0 = u[j], v1 = v[j1], u1 = u[j1];
-          upd(j, a0, v0, u0);
-          upd(j1, a1, v1, u1);
-        }
-        if (j < n) upd(j, A[i + (int64_t)j * lda], v[j], u[j]);
-        if (g == 0) A[i + (int64_t)k * lda] = vi;  // keep the reflector
-      }
-      if (!last) {
-        // column k+1 lies in wave 0 when m <= 64; otherwise it spans two waves
-        if (m > 64) __syncthreads();
-        if (wid == 0) {
-          if (m > 64) {
-            part = 0.0;
-            x0 = make_double2(0.0, 0.0);
-            for (int i = k + 2 + lane; i < n; i += 64) {
-              const double2 r = A[i + (int64_t)(k + 1) * lda];
-              part += r.x * r.x + r.y * r.y;
-              if (i == k + 2) x0 = r;
-            }
-          }
-          part = wave_sum(part);
-          x0 = make_double2(wave_sum(x0.x), wave_sum(x0.y));  // only one lane holds it
-          tri_reflector(lane, k + 1, n, part, x0, &s_skip[nxt], s_scal[nxt], W);
-        }
-      }
-    } else if (wid == 0 && !last) {
-      double part = 0.0;
-      for (int i = k + 2 + lane; i < n; i += 64) {
-        const double2 a = A[i + (int64_t)(k + 1) * lda];
-        part += a.x * a.x + a.y * a.y;
-      }
-      part = wave_sum(part);
-      tri_reflector(lane, k + 1, n, part, A[(k + 2) + (int64_t)(k + 1) * lda], &s_skip[nxt], s_scal[nxt], W);
-    }
-    __syncthreads();
+  for (int r = 0; r < 2; ++r) {
+    const int k = 2 * tid + r;
+    p = zmul(p, f[r]);
+    if (k + 1 < n) W.ph[k + 1] = p;
   }
   if (tid == 0) {
-    if (n >= 2) {
-      W.sub[n - 2] = A[(n - 1) + (int64_t)(n - 2) * lda];
-      W.skip[n - 2] = 1;
-    }
-    W.skip[n - 1] = 1;
+    W.ph[0] = one;
+    W.e[n - 1] = 0.0;
   }
-  // the diagonal: rows < k0 were final after their own step
-  for (int i = tid; i < n; i += DT) W.d[i] = i < k0 ? Ag[i + (int64_t)i * lda_g].x : A[i + (int64_t)i * lda].x;
-  // the reflectors (strict lower triangle) are read by the back-transformation
-  for (int q = tid; q < nb * nb; q += DT) Ag[(k0 + q % nb) + (int64_t)(k0 + q / nb) * lda_g] = As[q];
-  __syncthreads();
-  // phases: T' = D^H T D real with off-diagonal |sub| (As is free now, nb^2 >= n)
-  tri_phases(W, n, As);
 }
-
-// ---- 1'. wide tridiagonalisation (matrices that do not fit LDS): one
-// launch per column, the trailing matrix spread over many workgroups (one
-// wavefront per column).  k_wide_first and k_wide_gemv form reflector 0 and
-// u_0 = A22 v_0; launch k (k_wide_step) applies the rank-2 update of step k
-// and forms reflector k+1 and u_{k+1}.  Reflector k is kept in vb (buffer
-// k&1, scaled to unit norm) with its scalars in scal; A's column k receives
-// it for the back-transformation.
-constexpr int WT = 256;       // threads per workgroup
-constexpr int WC = WT / 64;   // columns per workgroup
 
 // Reflector of column k from sum |A(i,k)|^2 (i > k) and x0 = A(k+1,k), both
 // wave-uniform; every lane derives the scalars, lane 0 records them.
@@ -469,6 +299,317 @@ __device__ __forceinline__ void wide_reflector_scalars(int lane, int k, double x
     sc[3] = v0.y;
   }
 }
+
+// LDS pitch (in 16-byte elements) of the trailing block in k_tridiag: 2 mod 4,
+// so that the lanes of a column-wise read (eight columns of one row, stride =
+// pitch) and of the update's mirrored rows (stride = pitch + 1) fall on
+// distinct banks
+__host__ __device__ constexpr int tri_pitch(int nb) { return nb + (6 - nb % 4) % 4; }
+
+// LDS bytes of k_tridiag for a trailing block of nb columns: the block, u and
+// the per-column reflector scalars (at least the DW entries of the phase scan)
+static size_t tri_lds(int nb) {
+  return std::max(sizeof(double2) * ((size_t)tri_pitch(nb) * nb + 2 * (size_t)nb) + 8 * (size_t)nb,
+                  sizeof(double2) * DW);
+}
+
+// 1/sqrt(x) for x > 0 within a few ulps: the hardware estimate refined by two
+// Newton steps, a shorter dependent chain than a square root and a division
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double e = fma(-x * y, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  e = fma(-x * y, y, 1.0);
+  return fma(0.5 * y, e, y);
+}
+
+__device__ __forceinline__ double read_lane(double x, int l) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffull), l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Householder scalars of column k from xn2 = sum_{i>k} |A(i,k)|^2 and
+// x0 = A(k+1,k) (both wave-uniform): alpha = -(x0/|x0|) ||x|| (the new
+// sub-diagonal), v0 = x0 - alpha and iv = 1/||v|| with
+// ||v||^2 = 2 ||x|| (||x|| + |x0|); skip when x is already reduced.  When rec
+// is set the caller's lane records alpha and the skip flag.
+__device__ __forceinline__ void tri_scalars(bool rec, int k, double xn2, double2 x0, HeevWork& W, int& skip,
+                                            double& iv, double2& v0) {
+  const double a2 = x0.x * x0.x + x0.y * x0.y;
+  skip = (xn2 == 0.0 || xn2 - a2 <= 1e-300) ? 1 : 0;
+  iv = 0.0;
+  v0 = make_double2(0.0, 0.0);
+  double2 alpha = x0;
+  if (!skip) {
+    const double xn = xn2 * rsqrt_nr(xn2);
+    double ax0 = 0.0;
+    double2 phs = make_double2(1.0, 0.0);
+    if (a2 > 0.0) {
+      const double ra = rsqrt_nr(a2);
+      ax0 = a2 * ra;
+      phs = make_double2(x0.x * ra, x0.y * ra);
+    }
+    alpha = make_double2(-phs.x * xn, -phs.y * xn);
+    v0 = make_double2(x0.x + phs.x * xn, x0.y + phs.y * xn);
+    iv = rsqrt_nr(2.0 * xn * (xn + ax0));
+  }
+  if (rec) {
+    W.skip[k] = skip;
+    W.sub[k] = alpha;
+  }
+}
+
+// Reflector of column c of the LDS block from its values below the diagonal
+// (one wavefront); lane 0 records it and stores its scalars in sc.
+__device__ __forceinline__ void tri_col_reflector(const double2* __restrict__ As, int lda, int nb, int k0, int c,
+                                                  int lane, HeevWork& W, double* sc) {
+  const double2* col = As + c * lda;
+  double part = 0.0;
+  for (int i = c + 1 + lane; i < nb; i += 64) part += col[i].x * col[i].x + col[i].y * col[i].y;
+  part = wave_sum(part);
+  int skip;
+  double iv;
+  double2 v0;
+  tri_scalars(lane == 0, k0 + c, part, col[c + 1], W, skip, iv, v0);
+  if (lane == 0) {
+    sc[0] = skip;
+    sc[1] = iv;
+    sc[2] = v0.x;
+    sc[3] = v0.y;
+  }
+}
+
+// Householder reduction of the trailing block A[k0:, k0:] (nb = n - k0 rows
+// and columns) in LDS, one workgroup per matrix: the whole matrix for k0 = 0,
+// or the tail after the multi-workgroup steps 0..k0-1.  Only the lower
+// triangle is kept; indices below are local to the block.  Column c keeps x
+// while it is reduced; v = x iv (x_{c+1} replaced by v0) is formed where it
+// is read.  Per column c, with its reflector's scalars already in LDS:
+//   1. u = A v with A(i,j) = conj(A(j,i)) above the diagonal: each wavefront
+//      takes eight rows (lane >> 3) times eight column classes (lane & 7,
+//      columns c+1+cl, c+9+cl, ...), summed within the eight lanes by DPP;
+//      the wavefronts' parts of v^H u go to LDS;
+//   2. A -= v w^H + w v^H (w = 2u - 2 gamma v): wavefront 0 updates column
+//      c+1 and forms its reflector (look-ahead) while the other wavefronts
+//      update the lower triangle of rows and columns c+2.., each thread a row
+//      a and its mirror row m-1-a so that the threads share the work evenly.
+// Two barriers per column.  The reflectors are scaled to unit norm when the
+// block is written back.
+__global__ __launch_bounds__(DT) void k_tridiag(double2* __restrict__ Aall, int lda_g, int64_t strideA, int n,
+                                               int k0, int nev, double* __restrict__ work, int64_t strideWork) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int nb = n - k0, lda = tri_pitch(nb);
+  double2* As = reinterpret_cast<double2*>(smem);     // [lda * nb] A(i, j) = As[i + j lda], i >= j
+  double2* us = As + (size_t)lda * nb;                // [nb] u = A v
+  double2* rv0 = us + nb;                             // [nb] v0 iv of column c
+  double* riv = reinterpret_cast<double*>(rv0 + nb);  // [nb] iv of column c (0 without a reflector)
+  __shared__ double s_sc[2][4];                       // skip, iv, v0 of columns c (slot c & 1) and c + 1
+  __shared__ double s_g[DW];                          // the wavefronts' parts of v^H u
+  const int mat = blockIdx.x;
+  double2* Ag = Aall + mat * strideA + k0 + (int64_t)k0 * lda_g;
+  HeevWork W(work + mat * strideWork, n, nev);
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int nn = nb * nb;
+  for (int q0 = tid; q0 < nn; q0 += 4 * DT) {
+    double2 t[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int q = q0 + r * DT, i = q % nb, j = q / nb;
+      t[r] = make_double2(0.0, 0.0);
+      if (q < nn && i >= j) t[r] = Ag[i + (int64_t)j * lda_g];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int q = q0 + r * DT, i = q % nb, j = q / nb;
+      if (q < nn && i >= j) As[i + j * lda] = t[r];
+    }
+  }
+  __syncthreads();
+  if (nb > 2 && wid == 0) tri_col_reflector(As, lda, nb, k0, 0, lane, W, s_sc[0]);
+  __syncthreads();
+  for (int c = 0; c + 2 < nb; ++c) {
+    const int m = nb - c - 1;       // trailing rows and columns
+    const bool more = c + 3 < nb;  // column c+1 takes a reflector too
+    const double* sc = s_sc[c & 1];
+    const bool skip = sc[0] != 0.0;
+    const double iv = sc[1];
+    const double2 v0 = make_double2(sc[2], sc[3]);
+    if (tid == 0) {
+      rv0[c] = make_double2(v0.x * iv, v0.y * iv);
+      riv[c] = iv;
+    }
+    if (skip) {  // nothing changes; column c+1 is read as it stands
+      if (more && wid == 0) tri_col_reflector(As, lda, nb, k0, c + 1, lane, W, s_sc[(c + 1) & 1]);
+      __syncthreads();
+      continue;
+    }
+    const double2* col = As + c * lda;
+    auto vat = [=](int j) {
+      const double2 x = j == c + 1 ? v0 : col[j];
+      return make_double2(x.x * iv, x.y * iv);
+    };
+    // 1. u = A v and the wavefront's part of v^H u
+    double gp = 0.0;
+    for (int i0 = c + 1 + wid * 8; i0 < nb; i0 += DT / 8) {
+      const int i = i0 + (lane >> 3);
+      double sr = 0.0, si = 0.0, tr = 0.0, ti = 0.0;
+      if (i < nb) {
+        for (int j0 = c + 1 + (lane & 7); j0 < nb; j0 += 64) {
+          double2 a[8], b[8];
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const int j = j0 + 8 * q;
+            a[q] = make_double2(0.0, 0.0);
+            b[q] = make_double2(0.0, 0.0);
+            if (j < nb) {
+              const double2 e = As[j <= i ? i + j * lda : j + i * lda];
+              a[q] = j <= i ? e : make_double2(e.x, -e.y);
+              b[q] = vat(j);
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < 8; q += 2) {
+            sr += a[q].x * b[q].x - a[q].y * b[q].y;
+            si += a[q].x * b[q].y + a[q].y * b[q].x;
+            tr += a[q + 1].x * b[q + 1].x - a[q + 1].y * b[q + 1].y;
+            ti += a[q + 1].x * b[q + 1].y + a[q + 1].y * b[q + 1].x;
+          }
+        }
+      }
+      sr += tr;
+      si += ti;
+      sr += dpp_move<0xb1>(sr);  // quad_perm [1,0,3,2]
+      si += dpp_move<0xb1>(si);
+      sr += dpp_move<0x4e>(sr);  // quad_perm [2,3,0,1]
+      si += dpp_move<0x4e>(si);
+      sr += dpp_move<0x114>(sr);  // row_shr:4: lanes 4..7 of each eight hold the row's sum
+      si += dpp_move<0x114>(si);
+      if ((lane & 7) == 4 && i < nb) {
+        us[i] = make_double2(sr, si);
+        const double2 vi = vat(i);
+        gp += vi.x * sr + vi.y * si;  // Re(conj(v_i) u_i)
+      }
+    }
+    gp = wave_sum(gp);
+    if (lane == 0) s_g[wid] = gp;
+    __syncthreads();
+    double gam = 0.0;
+#pragma unroll
+    for (int w = 0; w < DW; ++w) gam += s_g[w];
+    const double g2 = 2.0 * gam;
+    // 2. the rank-2 update
+    if (wid == 0) {
+      const int j = c + 1;
+      const double2 vj = vat(j), uj = us[j];
+      const double2 wj = make_double2(2.0 * uj.x - g2 * vj.x, 2.0 * uj.y - g2 * vj.y);
+      double part = 0.0;
+      double2 x0 = make_double2(0.0, 0.0);
+      for (int i = j + lane; i < nb; i += 64) {
+        const double2 vi = vat(i), ui = us[i];
+        const double2 wi = make_double2(2.0 * ui.x - g2 * vi.x, 2.0 * ui.y - g2 * vi.y);
+        const double2 t1 = zmul(vi, make_double2(wj.x, -wj.y));
+        const double2 t2 = zmul(wi, make_double2(vj.x, -vj.y));
+        double2& r = As[i + j * lda];
+        r = make_double2(r.x - t1.x - t2.x, r.y - t1.y - t2.y);
+        if (i > j) part += r.x * r.x + r.y * r.y;
+        if (i == j + 1) x0 = r;
+      }
+      if (more) {  // row c+2 of column c+1 lies in lane 1
+        part = wave_sum(part);
+        x0 = make_double2(read_lane(x0.x, 1), read_lane(x0.y, 1));
+        int skip1;
+        double iv1;
+        double2 v01;
+        tri_scalars(lane == 0, k0 + c + 1, part, x0, W, skip1, iv1, v01);
+        if (lane == 0) {
+          double* s = s_sc[(c + 1) & 1];
+          s[0] = skip1;
+          s[1] = iv1;
+          s[2] = v01.x;
+          s[3] = v01.y;
+        }
+      }
+    } else {
+      const int m2 = m - 1;  // rows and columns c+2 .. nb-1
+      const int P = (m2 + 1) / 2, cg = (DT - 64) / P;
+      const int t = tid - 64, p = t % P, g = t / P;
+      if (g < cg) {
+        const int ra = c + 2 + p, rb = c + 1 + m2 - p;  // rows a and m2-1-a of the block
+        const double2 va = vat(ra), ua = us[ra], vb = vat(rb), ub = us[rb];
+        const double2 wa = make_double2(2.0 * ua.x - g2 * va.x, 2.0 * ua.y - g2 * va.y);
+        const double2 wb = make_double2(2.0 * ub.x - g2 * vb.x, 2.0 * ub.y - g2 * vb.y);
+        const int ne = ra == rb ? p + 1 : m2 + 1;  // row a: p + 1 elements, row m2-1-a: m2 - p
+        for (int e0 = g; e0 < ne; e0 += 4 * cg) {
+          int idx[4];
+          double2 r[4], vj[4], uj[4];
+          bool fs[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int e = e0 + q * cg;
+            fs[q] = e <= p;
+            const int i = fs[q] ? ra : rb, j = c + 2 + (fs[q] ? e : e - p - 1);
+            idx[q] = e < ne ? i + j * lda : -1;
+            r[q] = vj[q] = uj[q] = make_double2(0.0, 0.0);
+            if (e < ne) {
+              r[q] = As[idx[q]];
+              vj[q] = vat(j);
+              uj[q] = us[j];
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            if (idx[q] < 0) continue;
+            const double2 vi = fs[q] ? va : vb, wi = fs[q] ? wa : wb;
+            const double2 wj = make_double2(2.0 * uj[q].x - g2 * vj[q].x, 2.0 * uj[q].y - g2 * vj[q].y);
+            const double2 t1 = zmul(vi, make_double2(wj.x, -wj.y));
+            const double2 t2 = zmul(wi, make_double2(vj[q].x, -vj[q].y));
+            As[idx[q]] = make_double2(r[q].x - t1.x - t2.x, r[q].y - t1.y - t2.y);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  // the reflectors (strict lower triangle, unit norm) for the back-transformation
+  for (int q0 = tid; q0 < nn; q0 += 4 * DT) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int q = q0 + r * DT, i = q % nb, j = q / nb;
+      if (q < nn && i > j && j + 2 < nb) {
+        const double2 x = As[i + j * lda];
+        Ag[i + (int64_t)j * lda_g] = i == j + 1 ? rv0[j] : make_double2(x.x * riv[j], x.y * riv[j]);
+      }
+    }
+  }
+  // the diagonal (rows before k0 were final after their own step) and the
+  // last sub-diagonal element
+  for (int i = tid; i < n; i += DT)
+    W.d[i] = i < k0 ? Aall[mat * strideA + i + (int64_t)i * lda_g].x : As[(i - k0) * (lda + 1)].x;
+  if (tid == 0) {
+    if (nb >= 2) {
+      W.sub[n - 2] = As[(nb - 1) + (nb - 2) * lda];
+      W.skip[n - 2] = 1;
+    }
+    W.skip[n - 1] = 1;
+  }
+  __syncthreads();
+  // phases: T' = D^H T D real with off-diagonal |sub| (As is free now)
+  tri_phases(W, n, As);
+}
+
+// ---- 1'. wide tridiagonalisation (matrices that do not fit LDS): one
+// launch per column, the trailing matrix spread over many workgroups (one
+// wavefront per column).  k_wide_first and k_wide_gemv form reflector 0 and
+// u_0 = A22 v_0; launch k (k_wide_step) applies the rank-2 update of step k
+// and forms reflector k+1 and u_{k+1}.  Reflector k is kept in vb (buffer
+// k&1, scaled to unit norm) with its scalars in scal; A's column k receives
+// it for the back-transformation.
+constexpr int WT = 256;       // threads per workgroup
+constexpr int WC = WT / 64;   // columns per workgroup
+
 
 // column k's reflector from its current values (one wavefront)
 __device__ __forceinline__ void wide_reflector(const double2* __restrict__ A, int lda, int k, int n, HeevWork& W,
@@ -915,10 +1056,9 @@ void heev_gpu(double2* A, int lda, int64_t strideA, int n, int nev, double* Wv, 
   DEREK_REQUIRE(n >= 1 && nev >= 1 && nev <= n, "heev: need 1 <= nev <= n");
   DEREK_REQUIRE(n <= 2048, "heev: n must be <= 2048");
   {
-    // the largest trailing block the LDS kernel holds (A, v and u)
-    auto lds_for = [](int nb) { return sizeof(double2) * ((size_t)nb * nb + 2 * (size_t)nb); };
+    // the largest trailing block the LDS kernel holds
     int nbl = std::min(n, 98);
-    while (nbl > 1 && lds_for(nbl) > kTridiagLds) --nbl;
+    while (nbl > 1 && tri_lds(nbl) > kTridiagLds) --nbl;
     const int k0 = n - nbl;
     // steps 0..k0-1 with two multi-workgroup launches per column while the
     // trailing matrix is larger than that, then the rest in one workgroup
@@ -933,7 +1073,7 @@ void heev_gpu(double2* A, int lda, int64_t strideA, int n, int nev, double* Wv, 
       }
     }
     allow_dynamic_lds(reinterpret_cast<const void*>(k_tridiag), kTridiagLds);
-    hipLaunchKernelGGL(k_tridiag, dim3(batch), dim3(DT), lds_for(nbl), s, A, lda, strideA, n, k0, nev, work,
+    hipLaunchKernelGGL(k_tridiag, dim3(batch), dim3(DT), tri_lds(nbl), s, A, lda, strideA, n, k0, nev, work,
                        strideWork);
     DEREK_HIP(hipGetLastError());
   }
