@@ -98,6 +98,19 @@ __device__ double block_max_all(double x, double* red) {
   return t;
 }
 
+// 1/q for q != 0: the hardware estimate refined by two Newton steps, fewer
+// issue cycles than the division sequence (the Sturm recurrence below is
+// bound by its wavefront's issue rate).  q is clamped to the finite range
+// first, so an infinite q gives 0 as the division would.
+__device__ __forceinline__ double recip_nr(double q) {
+  const double qc = fmin(fmax(q, -DBL_MAX), DBL_MAX);
+  double y = __builtin_amdgcn_rcp(qc);
+  double e = fma(-qc, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-qc, y, 1.0);
+  return fma(y, e, y);
+}
+
 // number of eigenvalues of the tridiagonal (d, e) smaller than x
 __device__ int sturm_count(const double* d, const double* e2, int n, double x, double pivmin) {
   int c = 0;
@@ -105,7 +118,7 @@ __device__ int sturm_count(const double* d, const double* e2, int n, double x, d
   if (fabs(q) < pivmin) q = -pivmin;
   if (q < 0.0) ++c;
   for (int i = 1; i < n; ++i) {
-    q = d[i] - x - e2[i - 1] / q;
+    q = d[i] - x - e2[i - 1] * recip_nr(q);
     if (fabs(q) < pivmin) q = -pivmin;
     if (q < 0.0) ++c;
   }
@@ -116,9 +129,10 @@ __device__ int sturm_count(const double* d, const double* e2, int n, double x, d
 // off-diagonal e.  u0/u1/u2/l/pv are scratch of length n; b is overwritten by x.
 // LU factorisation with partial pivoting of T - lambda I (T symmetric
 // tridiagonal: diagonal d, off-diagonal e), pivots below pivmin raised to it;
-// U has the three diagonals u0, u1, u2, L the multipliers l with row swaps pv.
-// The arrays are separate LDS regions (restrict), so the loads of the next
-// row are issued while the recurrence runs in registers.
+// U has the three diagonals (u0 holds the reciprocals of the pivots, so the
+// solves multiply), L the multipliers l with row swaps pv.  The arrays are
+// separate LDS regions (restrict), so the loads of the next row are issued
+// while the recurrence runs in registers.
 __device__ void tri_factor(const double* __restrict__ d, const double* __restrict__ e, int n, double lambda,
                            double pivmin, double* __restrict__ u0, double* __restrict__ u1, double* __restrict__ u2,
                            double* __restrict__ l, double* __restrict__ pv) {
@@ -129,8 +143,9 @@ __device__ void tri_factor(const double* __restrict__ d, const double* __restric
     const double b1 = i + 2 < n ? e[i + 1] : 0.0;
     if (fabs(cd) >= fabs(c)) {
       const double piv = fabs(cd) < pivmin ? copysign(pivmin, cd == 0.0 ? 1.0 : cd) : cd;
-      const double li = c / piv;
-      u0[i] = piv;
+      const double rp = recip_nr(piv);
+      const double li = c * rp;
+      u0[i] = rp;
       u1[i] = cs;
       u2[i] = cs2;
       l[i] = li;
@@ -139,8 +154,9 @@ __device__ void tri_factor(const double* __restrict__ d, const double* __restric
       cs = b1 - li * cs2;
       cs2 = 0.0;
     } else {
-      const double li = cd / c;
-      u0[i] = c;
+      const double rc = recip_nr(c);
+      const double li = cd * rc;
+      u0[i] = rc;
       u1[i] = a1;
       u2[i] = b1;
       l[i] = li;
@@ -150,33 +166,52 @@ __device__ void tri_factor(const double* __restrict__ d, const double* __restric
       cs2 = 0.0;
     }
   }
-  u0[n - 1] = fabs(cd) < pivmin ? copysign(pivmin, cd == 0.0 ? 1.0 : cd) : cd;
+  u0[n - 1] = recip_nr(fabs(cd) < pivmin ? copysign(pivmin, cd == 0.0 ? 1.0 : cd) : cd);
 }
 
 // b <- (T - lambda I)^-1 b from tri_factor's factors; the values carried from
-// row to row stay in registers.
+// row to row stay in registers and the next row's operands are loaded one
+// step ahead, so each step waits only on its own arithmetic.
 __device__ void tri_solve(int n, const double* __restrict__ u0, const double* __restrict__ u1,
                           const double* __restrict__ u2, const double* __restrict__ l, const double* __restrict__ pv,
                           double* __restrict__ b) {
   double bi = b[0];
-  for (int i = 0; i + 1 < n; ++i) {
-    double bn = b[i + 1];
-    if (pv[i] != 0.0) {
-      const double t = bi;
+  if (n > 1) {
+    double nb = b[1], nl = l[0], np = pv[0];
+    for (int i = 0; i + 1 < n; ++i) {
+      double bn = nb;
+      const double li = nl, pi = np;
+      if (i + 2 < n) {
+        nb = b[i + 2];
+        nl = l[i + 1];
+        np = pv[i + 1];
+      }
+      if (pi != 0.0) {
+        const double t = bi;
+        bi = bn;
+        bn = t;
+      }
+      bn -= li * bi;
+      b[i] = bi;
       bi = bn;
-      bn = t;
     }
-    bn -= l[i] * bi;
-    b[i] = bi;
-    bi = bn;
   }
-  double b1 = bi / u0[n - 1];  // b[i+1]
+  double b1 = bi * u0[n - 1];  // b[i+1]
   b[n - 1] = b1;
   if (n < 2) return;
-  double b0 = (b[n - 2] - u1[n - 2] * b1) / u0[n - 2];  // b[i+1] after the step below
+  double b0 = (b[n - 2] - u1[n - 2] * b1) * u0[n - 2];  // b[i+1] after the step below
   b[n - 2] = b0;
+  if (n < 3) return;
+  double cb = b[n - 3], c1 = u1[n - 3], c2 = u2[n - 3], cr = u0[n - 3];
   for (int i = n - 3; i >= 0; --i) {
-    const double bi2 = (b[i] - u1[i] * b0 - u2[i] * b1) / u0[i];
+    const double xb = cb, x1 = c1, x2 = c2, xr = cr;
+    if (i > 0) {
+      cb = b[i - 1];
+      c1 = u1[i - 1];
+      c2 = u2[i - 1];
+      cr = u0[i - 1];
+    }
+    const double bi2 = (xb - x1 * b0 - x2 * b1) * xr;
     b[i] = bi2;
     b1 = b0;
     b0 = bi2;
@@ -868,7 +903,7 @@ __global__ __launch_bounds__(IT) void k_inviter(int n, int nev, int cpw, double*
   __syncthreads();
   const int c = blockIdx.x * cpw + threadIdx.x;
   if (threadIdx.x >= cpw || c >= *W.ncl) return;
-  double* mine = ee + n + (size_t)threadIdx.x * 6 * n;
+  double* mine = ee + n + (size_t)threadIdx.x * (6 * n + 1);  // odd stride: the lanes' arrays on distinct banks
   double* u0 = mine;
   double* u1 = u0 + n;
   double* u2 = u1 + n;
@@ -1087,9 +1122,10 @@ void heev_gpu(double2* A, int lda, int64_t strideA, int n, int nev, double* Wv, 
     DEREK_HIP(hipGetLastError());
   }
   {
-    // clusters per workgroup: LDS holds d, e and 6 n doubles per cluster thread
-    const int cpw = std::max(1, std::min(IT, (int)((60 * 1024 / sizeof(double) - 2 * (size_t)n) / (6 * (size_t)n))));
-    const size_t lds = sizeof(double) * (2 * (size_t)n + 6 * (size_t)n * cpw);
+    // clusters per workgroup: LDS holds d, e and 6 n + 1 doubles per cluster thread
+    const int cpw =
+        std::max(1, std::min(IT, (int)((60 * 1024 / sizeof(double) - 2 * (size_t)n) / (6 * (size_t)n + 1))));
+    const size_t lds = sizeof(double) * (2 * (size_t)n + (6 * (size_t)n + 1) * cpw);
     DEREK_REQUIRE(lds <= 64 * 1024, "heev: inverse-iteration scratch exceeds the LDS budget");
     dim3 grid((unsigned)ceil_div(nev, cpw), batch);
     hipLaunchKernelGGL(k_inviter, grid, dim3(IT), lds, s, n, nev, cpw, work, strideWork);
