@@ -145,8 +145,10 @@ def test_set_launch_rejects_bad_shapes():
 @pytest.mark.parametrize("n,precision", [(32, lib.DEREK_FP64), (72, lib.DEREK_FP64), (72, lib.DEREK_MIXED)])
 def test_fused_scf_matches_separate(n, precision):
     """Per-step energies of the SCF with the fused pass equal those with the
-    separate passes (FP64: to rounding; mixed: to 1e-6 Ha, since the two paths
-    round V / N to complex64 differently).  The mixed runs use the FP32
+    separate passes (FP64: to rounding; mixed: to 5e-6 Ha, about four float32
+    ulps of the ~9 Ha total energy, since the two paths round V / N and their
+    sums to complex64 differently and the SCF carries that rounding from step
+    to step; observed 4e-7 to 1.7e-6 Ha).  The mixed runs use the FP32
     matrix cores: the BF16 split GEMMs carry ~1e-5 relative overlap error,
     which moves FP32-phase energies at first order and would hide the
     comparison of the two FFT paths."""
@@ -161,5 +163,5 @@ def test_fused_scf_matches_separate(n, precision):
         res = gscf.run(spec, max_scf=6, energy_tol=0.0, setup=su, ctx=ctx, log=None)
         out.append(np.array([h["etot"] for h in res["history"]]))
         ctx.close()
-    tol = 1e-10 if precision == lib.DEREK_FP64 else 1e-6
+    tol = 1e-10 if precision == lib.DEREK_FP64 else 5e-6
     assert np.max(np.abs(out[1] - out[0])) < tol
