@@ -68,20 +68,23 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uin
 }
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
-template <typename C> __device__ __forceinline__ C buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+// CP is the cache policy: 0 for ordinary accesses, kNt (the non-temporal bit)
+// for streaming data that a kernel reads or writes once.
+constexpr int kNt = 2;
+template <typename C, int CP = 0> __device__ __forceinline__ C buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
   if constexpr (sizeof(C) == 16) {
-    return __builtin_bit_cast(C, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
+    return __builtin_bit_cast(C, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, CP));
   } else {
     static_assert(sizeof(C) == 8, "8- or 16-byte elements");
-    return __builtin_bit_cast(C, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+    return __builtin_bit_cast(C, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, CP));
   }
 }
-template <typename C> __device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t off, C v) {
+template <typename C, int CP = 0> __device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t off, C v) {
   if constexpr (sizeof(C) == 16) {
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, v), r, (int)off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, v), r, (int)off, 0, CP);
   } else {
     static_assert(sizeof(C) == 8, "8- or 16-byte elements");
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, v), r, (int)off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, v), r, (int)off, 0, CP);
   }
 }
 #endif
