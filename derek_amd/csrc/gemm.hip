@@ -427,16 +427,15 @@ __global__ __launch_bounds__(RD_E * RD_G) void k_overlap_reduce(const double2* _
   const int el = threadIdx.x % RD_E, grp = threadIdx.x / RD_E;
   for (int64_t e0 = (int64_t)blockIdx.x * RD_E; e0 < mn; e0 += (int64_t)gridDim.x * RD_E) {
     const int64_t e = e0 + el;
-    const bool ok = e < mn;
-    const int i = ok ? (int)(e % g.m) : 0, j = ok ? (int)(e / g.m) : 0;
-    // Hermitian: the strict lower triangle is the conjugate of the upper one
-    // (exactly Hermitian output, real diagonal)
-    const bool mirror = g.herm && i > j;
-    const int64_t src = mirror ? (int64_t)i * g.m + j : (int64_t)j * g.m + i;
+    const int i = e < mn ? (int)(e % g.m) : 0, j = e < mn ? (int)(e / g.m) : 0;
+    // Hermitian: only the upper triangle's partials are summed (read in
+    // order, so the loads stay coalesced); each sum also gives its mirror
+    // element, the conjugate (exactly Hermitian output, real diagonal)
+    const bool ok = e < mn && !(g.herm && i > j);
     double sr = 0.0, si = 0.0;
     if (ok)
       for (int c = grp; c < g.nchunk; c += RD_G) {
-        const double2 v = pk[(int64_t)c * mn + src];
+        const double2 v = pk[(int64_t)c * mn + e];
         sr += v.x;
         si += v.y;
       }
@@ -448,7 +447,8 @@ __global__ __launch_bounds__(RD_E * RD_G) void k_overlap_reduce(const double2* _
         si += red[q][el].y;
       }
       if (g.herm && i == j) si = 0.0;
-      Ck[(int64_t)j * ldc + i] = mirror ? make_double2(sr, -si) : make_double2(sr, si);
+      Ck[(int64_t)j * ldc + i] = make_double2(sr, si);
+      if (g.herm && i < j) Ck[(int64_t)i * ldc + j] = make_double2(sr, -si);
     }
     __syncthreads();
   }

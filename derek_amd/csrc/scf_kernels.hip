@@ -388,21 +388,25 @@ __global__ __launch_bounds__(NT) void k_pulay_dots(const int* __restrict__ st, i
 // then the coefficients, oldest slot first as in the reference ordering,
 // from a = A^-1 1 / (1^T A^-1 1) with A scaled by its largest diagonal and
 // 1e-12 added to the diagonal; st[0] += 1.
-__global__ void k_pulay_solve(int* __restrict__ st, int H, const double* __restrict__ partial, int nblk,
-                              double* __restrict__ A, double* __restrict__ coef) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// The partial sums are added by the 64 lanes of the one wavefront (lane-strided
+// blocks, then a butterfly; lane 0's total is used), the small system by lane 0.
+__global__ __launch_bounds__(64) void k_pulay_solve(int* __restrict__ st, int H, const double* __restrict__ partial,
+                                                   int nblk, double* __restrict__ A, double* __restrict__ coef) {
+  const int lane = threadIdx.x;
   const int np = st[0];
   const int slot = np % H;
   const int cnt = np + 1 < H ? np + 1 : H;
   for (int j = 0; j < H; ++j) {
     double t = 0.0;
-    for (int b = 0; b < nblk; ++b) t += partial[(int64_t)j * nblk + b];
+    for (int b = lane; b < nblk; b += 64) t += partial[(int64_t)j * nblk + b];
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
     const bool filled = j == slot || j < cnt;  // slots 0..cnt-1 hold data
-    if (filled) {
+    if (lane == 0 && filled) {
       A[slot * H + j] = t;
       A[j * H + slot] = t;
     }
   }
+  if (lane != 0) return;
   int ord[16];
   for (int i = 0; i < cnt; ++i) ord[i] = (np + 1 - cnt + i) % H;
   double M[16][17];
