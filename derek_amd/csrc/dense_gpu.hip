@@ -1385,19 +1385,36 @@ __global__ void k_scale_mask(const double2* __restrict__ H, const double* __rest
 
 // Dropped diagonal entries of A <- 2 max_i sum_j |A_ij| + 1 (above every
 // eigenvalue of the kept block), so the lowest eigenpairs are the kept ones.
+// Row sums: R = min(m, DT) rows per pass over consecutive threads (coalesced
+// column reads), the columns split over DT / R thread groups whose parts are
+// added through LDS.
 __global__ __launch_bounds__(DT) void k_big_diag(double2* __restrict__ Aall, const int* __restrict__ keepall, int m) {
   __shared__ double red[DW];
+  __shared__ double part[DT];
   const int kz = blockIdx.x;
   double2* A = Aall + (int64_t)kz * m * m;
   const int* keep = keepall + (int64_t)kz * m;
+  const int R = min(m, DT), cg = DT / R;
+  const int t = threadIdx.x % R, g = threadIdx.x / R;
   double rmax = 0.0;
-  for (int i = threadIdx.x; i < m; i += DT) {
+  for (int i0 = 0; i0 < m; i0 += R) {
+    const int i = i0 + t;
     double r = 0.0;
-    for (int j = 0; j < m; ++j) {
-      const double2 a = A[i + (int64_t)j * m];
-      r += sqrt(a.x * a.x + a.y * a.y);
+    if (g < cg && i < m) {
+#pragma unroll 4
+      for (int j = g; j < m; j += cg) {
+        const double2 a = A[i + (int64_t)j * m];
+        r += sqrt(a.x * a.x + a.y * a.y);
+      }
     }
-    rmax = fmax(rmax, r);
+    if (g < cg) part[g * R + t] = r;
+    __syncthreads();
+    if (threadIdx.x < R && i0 + (int)threadIdx.x < m) {
+      double sum = 0.0;
+      for (int q = 0; q < cg; ++q) sum += part[q * R + threadIdx.x];
+      rmax = fmax(rmax, sum);
+    }
+    __syncthreads();
   }
   const double big = 2.0 * block_max_all(rmax, red) + 1.0;
   for (int i = threadIdx.x; i < m; i += DT)
