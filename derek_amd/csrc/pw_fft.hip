@@ -1415,6 +1415,8 @@ __global__ __launch_bounds__(256) void k_slab_rho_sum(const double* __restrict__
     double s = 0.0;
     if (z < nz && x < nx) {
       const double* p = part + ((int64_t)z * ny + y) * nx + x;
+      // in order, with the loads of eight partials in flight
+#pragma unroll 8
       for (int j = 0; j < np; ++j) s += p[j * ng];
     }
     tile[i][tx] = s;
