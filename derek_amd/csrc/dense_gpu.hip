@@ -307,33 +307,6 @@ __device__ void tri_phases(HeevWork& W, int n, double2* s_wave) {
   }
 }
 
-// Reflector of column k from sum |A(i,k)|^2 (i > k) and x0 = A(k+1,k), both
-// wave-uniform; every lane derives the scalars, lane 0 records them.
-__device__ __forceinline__ void wide_reflector_scalars(int lane, int k, double xn2, double2 x0, HeevWork& W,
-                                                       int& skip, double& iv, double2& v0) {
-  const double ax0 = sqrt(x0.x * x0.x + x0.y * x0.y);
-  const double rest = xn2 - ax0 * ax0;
-  skip = (xn2 == 0.0 || rest <= 1e-300) ? 1 : 0;
-  iv = 0.0;
-  v0 = make_double2(0.0, 0.0);
-  double2 alpha = x0;
-  if (!skip) {
-    const double xn = sqrt(xn2);
-    const double2 phs = ax0 > 0.0 ? make_double2(x0.x / ax0, x0.y / ax0) : make_double2(1.0, 0.0);
-    alpha = make_double2(-phs.x * xn, -phs.y * xn);
-    v0 = make_double2(x0.x - alpha.x, x0.y - alpha.y);
-    iv = 1.0 / sqrt(rest + v0.x * v0.x + v0.y * v0.y);
-  }
-  if (lane == 0) {
-    W.skip[k] = skip;
-    W.sub[k] = alpha;
-    double* sc = W.scal + 4 * (k & 1);
-    sc[0] = skip;
-    sc[1] = iv;
-    sc[2] = v0.x;
-    sc[3] = v0.y;
-  }
-}
 
 // LDS pitch (in 16-byte elements) of the trailing block in k_tridiag: 2 mod 4,
 // so that the lanes of a column-wise read (eight columns of one row, stride =
@@ -393,6 +366,21 @@ __device__ __forceinline__ void tri_scalars(bool rec, int k, double xn2, double2
   if (rec) {
     W.skip[k] = skip;
     W.sub[k] = alpha;
+  }
+}
+
+// Reflector of column k from sum |A(i,k)|^2 (i > k) and x0 = A(k+1,k), both
+// wave-uniform, for the multi-workgroup steps: every lane derives the
+// scalars, lane 0 records them (with the scalars for the next launch).
+__device__ __forceinline__ void wide_reflector_scalars(int lane, int k, double xn2, double2 x0, HeevWork& W,
+                                                       int& skip, double& iv, double2& v0) {
+  tri_scalars(lane == 0, k, xn2, x0, W, skip, iv, v0);
+  if (lane == 0) {
+    double* sc = W.scal + 4 * (k & 1);
+    sc[0] = skip;
+    sc[1] = iv;
+    sc[2] = v0.x;
+    sc[3] = v0.y;
   }
 }
 
@@ -720,15 +708,28 @@ __global__ __launch_bounds__(WT) void k_wide_step(double2* __restrict__ Aall, in
   HeevWork W(work + mat * strideWork, n, nev);
   double2* A = Aall + mat * strideA;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  // the wavefront's column j and the thread's rows of column k+1 are loaded
+  // before anything is stored (the stores into A below would otherwise keep
+  // these loads, a second round trip, behind them)
+  constexpr int PF = 4;  // rows per lane of column j held from entry (n - k - 1 <= 256)
+  const int j = k + 1 + blockIdx.x * WC + wid;
+  const bool colj = j < n && (last || j != k + 1);
+  double2 pre[PF];
+#pragma unroll
+  for (int q = 0; q < PF; ++q) {
+    const int i = k + 1 + lane + 64 * q;
+    pre[q] = colj && i < n ? A[i + (int64_t)j * lda] : make_double2(0.0, 0.0);
+  }
+  const int i1 = k + 2 + (int)threadIdx.x;  // this thread's first row of column k+1
+  const double2 c1pre = !last && i1 < n ? A[i1 + (int64_t)(k + 1) * lda] : make_double2(0.0, 0.0);
   const bool skip = W.scal[4 * (k & 1)] != 0.0;  // column k needed no reflector
   const double2* v = W.vb + (size_t)(k & 1) * n;
   const double2* u = W.uu + (size_t)(k & 1) * n;
-  double g2 = 0.0;
-  if (!skip) {
-    double gp = 0.0;
-    for (int i = k + 1 + lane; i < n; i += 64) gp += zconjmul(v[i], u[i]).x;
-    g2 = 2.0 * wave_sum(gp);
-  }
+  // v and u are read whether or not column k took a reflector (the values are
+  // then unused), so these loads need not wait for the flag
+  double gp = 0.0;
+  for (int i = k + 1 + lane; i < n; i += 64) gp += zconjmul(v[i], u[i]).x;
+  const double g2 = skip ? 0.0 : 2.0 * wave_sum(gp);
   auto wof = [=](int i) {
     const double2 vi = v[i], ui = u[i];
     return make_double2(2.0 * ui.x - g2 * vi.x, 2.0 * ui.y - g2 * vi.y);
@@ -753,8 +754,9 @@ __global__ __launch_bounds__(WT) void k_wide_step(double2* __restrict__ Aall, in
     const double2 v1 = skip ? make_double2(0.0, 0.0) : v[k + 1];
     const double2 w1 = skip ? make_double2(0.0, 0.0) : wof(k + 1);
     double part = 0.0;
-    for (int i = k + 2 + (int)threadIdx.x; i < n; i += WT) {  // thread 0 holds row k+2
-      const double2 c = skip ? c1[i] : upd(c1[i], v[i], wof(i), v1, w1);
+    for (int i = i1; i < n; i += WT) {  // thread 0 holds row k+2
+      const double2 ci = i == i1 ? c1pre : c1[i];
+      const double2 c = skip ? ci : upd(ci, v[i], wof(i), v1, w1);
       vn[i] = c;
       part += c.x * c.x + c.y * c.y;
     }
@@ -777,14 +779,12 @@ __global__ __launch_bounds__(WT) void k_wide_step(double2* __restrict__ Aall, in
       }
     }
   }
-  const int j = k + 1 + blockIdx.x * WC + wid;
-  if (j >= n || (!last && j == k + 1)) return;
+  if (!colj) return;
   double2* col = A + (int64_t)j * lda;
   const double2 vj = skip ? make_double2(0.0, 0.0) : v[j];
   const double2 wj = skip ? make_double2(0.0, 0.0) : wof(j);
   double sr = 0.0, si = 0.0;
-  for (int i = k + 1 + lane; i < n; i += 64) {
-    double2 a = col[i];
+  auto row = [&](int i, double2 a) {
     if (!skip) {
       a = upd(a, v[i], wof(i), vj, wj);
       col[i] = a;
@@ -794,7 +794,13 @@ __global__ __launch_bounds__(WT) void k_wide_step(double2* __restrict__ Aall, in
       sr += t.x;
       si += t.y;
     }
+  };
+#pragma unroll
+  for (int q = 0; q < PF; ++q) {
+    const int i = k + 1 + lane + 64 * q;
+    if (i < n) row(i, pre[q]);
   }
+  for (int i = k + 1 + lane + 64 * PF; i < n; i += 64) row(i, col[i]);
   if (!last) {
     sr = wave_sum(sr) * iv1;
     si = wave_sum(si) * iv1;
