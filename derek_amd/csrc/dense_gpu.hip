@@ -956,7 +956,10 @@ __global__ __launch_bounds__(IT) void k_inviter(int n, int nev, int cpw, double*
 }
 
 // ---- 4. back-transformation, one wavefront per eigenvector
-constexpr int BW = 16;  // waves per back-transform workgroup
+// waves per back-transform workgroup: four, so that the eigenvectors spread
+// over many CUs (a wave's reflector steps are bound by its own FP64 issue: 16
+// waves per workgroup put 80 vectors on five CUs, 130 us at n = 160)
+constexpr int BW = 4;
 
 __global__ __launch_bounds__(BW * 64) void k_backtransform(const double2* __restrict__ Aall, int lda,
                                                           int64_t strideA, int n, int nev,
@@ -1150,7 +1153,9 @@ void heev_gpu(double2* A, int lda, int64_t strideA, int n, int nev, double* Wv, 
     else if (n <= 128) BT_(2);
     else if (n <= 192) BT_(3);
     else if (n <= 256) BT_(4);
-    else if (n <= 384) BT_(6);  // more rows per lane would not fit 128 registers
+    else if (n <= 384) BT_(6);
+    else if (n <= 512) BT_(8);
+    else if (n <= 624) BT_(10);  // LDS: two 8-column panels of n rows, 260 n bytes <= 160 KiB
     else
       hipLaunchKernelGGL(k_backtransform, grid, dim3(BW * 64), 0, s, A, lda, strideA, n, nev, work, strideWork, Wv,
                          strideW, Z, ldz, strideZ);
