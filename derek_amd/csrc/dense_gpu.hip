@@ -1131,11 +1131,15 @@ void heev_gpu(double2* A, int lda, int64_t strideA, int n, int nev, double* Wv, 
     DEREK_HIP(hipGetLastError());
   }
   {
-    // clusters per workgroup: LDS holds d, e and 6 n + 1 doubles per cluster thread
+    // clusters per workgroup: LDS holds d, e and 6 n + 1 doubles per cluster
+    // thread, up to 156 KiB (at n = 600 that is five clusters per workgroup,
+    // so the 300 clusters take one round of the CUs instead of two)
+    constexpr size_t kInvLds = 156 * 1024;
     const int cpw =
-        std::max(1, std::min(IT, (int)((60 * 1024 / sizeof(double) - 2 * (size_t)n) / (6 * (size_t)n + 1))));
+        std::max(1, std::min(IT, (int)((kInvLds / sizeof(double) - 2 * (size_t)n) / (6 * (size_t)n + 1))));
     const size_t lds = sizeof(double) * (2 * (size_t)n + (6 * (size_t)n + 1) * cpw);
-    DEREK_REQUIRE(lds <= 64 * 1024, "heev: inverse-iteration scratch exceeds the LDS budget");
+    DEREK_REQUIRE(lds <= kInvLds, "heev: inverse-iteration scratch exceeds the LDS budget");
+    if (lds > 65536) allow_dynamic_lds(reinterpret_cast<const void*>(k_inviter), kInvLds);
     dim3 grid((unsigned)ceil_div(nev, cpw), batch);
     hipLaunchKernelGGL(k_inviter, grid, dim3(IT), lds, s, n, nev, cpw, work, strideWork);
     DEREK_HIP(hipGetLastError());
