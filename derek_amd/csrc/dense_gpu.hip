@@ -1233,7 +1233,7 @@ __global__ __launch_bounds__(DT) void k_chol_drop(const double2* __restrict__ Sa
       double2* pc = P + jj * m;
       const double d = pc[j].x;
       if (d > tol * sd[j]) {
-        const double lj = sqrt(d), idj = 1.0 / lj;
+        const double idj = rsqrt_nr(d), lj = d * idj;
         double2* lcol = Lf + (int64_t)j * m;
         for (int i = j + 1 + tid; i < m; i += DT) {
           const double2 l = make_double2(pc[i].x * idj, pc[i].y * idj);
