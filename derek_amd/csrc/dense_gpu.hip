@@ -811,10 +811,16 @@ __global__ __launch_bounds__(WT) void k_wide_step(double2* __restrict__ Aall, in
 
 // ---- 2. eigenvalues by multisection: one wavefront per eigenvalue; the 64
 // lanes count eigenvalues below 64 interior points, so every step narrows the
-// bracket 65-fold.  Then the clusters (one thread).
+// bracket 65-fold.  When the eigenvalues of a launch fit one round of the CUs,
+// four wavefronts share an eigenvalue (256 points, 257-fold).  Then the
+// clusters (one thread).
 constexpr int BSW = 4;  // wavefronts (eigenvalues) per workgroup
 
-__global__ __launch_bounds__(BSW * 64) void k_bisect(int n, int nev, double* __restrict__ work, int64_t strideWork) {
+// With `shared` set, the BSW wavefronts of a workgroup work on one eigenvalue
+// together (64 BSW points per step, one wavefront per SIMD): for launches with
+// few eigenvalues, where the CUs would otherwise idle.
+__global__ __launch_bounds__(BSW * 64) void k_bisect(int n, int nev, int shared, double* __restrict__ work,
+                                                    int64_t strideWork) {
   extern __shared__ __align__(16) unsigned char smem[];
   double* dd = reinterpret_cast<double*>(smem);  // [n]
   double* e2 = dd + n;                            // [n]
@@ -848,6 +854,35 @@ __global__ __launch_bounds__(BSW * 64) void k_bisect(int n, int nev, double* __r
   const double span = ghi - glo;
   glo -= 2.0 * DBL_EPSILON * tnorm * n + 1e-300 + 1e-12 * span;
   ghi += 2.0 * DBL_EPSILON * tnorm * n + 1e-300 + 1e-12 * span;
+  if (shared) {
+    // invariant as below; point k (1..NP) sits at lo + (hi - lo) k / (NP + 1),
+    // every thread recomputes the new bracket from the same formula
+    __shared__ int s_p[BSW];
+    constexpr int NP = BSW * 64;
+    const int t = blockIdx.x;
+    auto point = [](double lo, double hi, int k) {
+      return fmin(fmax(lo + (hi - lo) * ((double)k / (double)(NP + 1)), lo), hi);
+    };
+    double lo = glo, hi = ghi;
+    for (int it = 0; it < 64; ++it) {
+      if (hi - lo <= 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + pivmin) break;
+      const int cnt = sturm_count(dd, e2, n, point(lo, hi, wid * 64 + lane + 1), pivmin);
+      const uint64_t below = __ballot(cnt <= t);
+      if (lane == 0) s_p[wid] = below ? wid * 64 + 64 - __clzll(below) : 0;
+      __syncthreads();
+      int p = 0;
+#pragma unroll
+      for (int w = 0; w < BSW; ++w) p = max(p, s_p[w]);
+      __syncthreads();
+      const double newlo = p > 0 ? point(lo, hi, p) : lo;
+      const double newhi = p < NP ? point(lo, hi, p + 1) : hi;
+      if (!(newlo > lo) && !(newhi < hi)) break;
+      lo = newlo;
+      hi = newhi;
+    }
+    if (threadIdx.x == 0) W.lam[t] = 0.5 * (lo + hi);
+    return;
+  }
   const int t = blockIdx.x * BSW + wid;
   if (t >= nev) return;
   // invariant: count(lo) <= t < count(hi); the points increase with the lane
@@ -1123,8 +1158,10 @@ void heev_gpu(double2* A, int lda, int64_t strideA, int n, int nev, double* Wv, 
   }
   {
     const size_t lds = sizeof(double) * 2 * (size_t)n;
-    hipLaunchKernelGGL(k_bisect, dim3((unsigned)ceil_div(nev, BSW), batch), dim3(BSW * 64), lds, s, n, nev, work,
-                       strideWork);
+    // one eigenvalue per workgroup while they fit one round of the CUs
+    const int shared = (int64_t)nev * batch <= 256 ? 1 : 0;
+    hipLaunchKernelGGL(k_bisect, dim3((unsigned)(shared ? nev : ceil_div(nev, BSW)), batch), dim3(BSW * 64), lds, s, n,
+                       nev, shared, work, strideWork);
     DEREK_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_clusters, dim3(batch), dim3(64), sizeof(double) * ((size_t)n + nev), s, n, nev, work,
                        strideWork);
